@@ -1,0 +1,1039 @@
+/*
+ * ORACLE — TEST INFRASTRUCTURE ONLY (see swe_oracle.h).
+ *
+ * FP64 restatement of PeanoClaw's SWE patch path.  Build with
+ * -O2 -ffp-contract=off and without fast-math so that every +,-,*,/ and sqrt
+ * is one IEEE-754 correctly rounded operation in the order written here.
+ *
+ * Parity status per function:
+ *   or_fwave, or_fwave_speed  restated from the published f-wave method
+ *       (Bale, LeVeque, Mitran, Rossmanith, SIAM J. Sci. Comput. 24 (2002))
+ *       with Einfeldt (HLLE) wave-speed estimates.  The reference calls
+ *       SWE's computeNetUpdates (tools/SWE_WaveAccumulationBlock.cpp:154,193)
+ *       whose source is absent, so this is pinned by invariants only:
+ *       lake at rest, equal states, exact h-flux conservation per edge,
+ *       mirror symmetry, Roe limit, wall (reflection) edges — see
+ *       tests/test_oracle_invariants.py.
+ *   or_block_fluxes/_max_timestep/_update  follow
+ *       tools/SWE_WaveAccumulationBlock.cpp:84-296 line by line; pinned by
+ *       decomposition invariance (9x9 patches == one 144^2 block, bitwise),
+ *       mass conservation with walls, x<->y symmetry.
+ *   or_fill_boundary_layer  SWEKernel.cpp:176-237; pinned by the reflect
+ *       invariants (wall edge carries zero h-flux).
+ *   or_transfer_ghostlayer  DefaultTransfer.cpph:15-101; pinned by exact copy
+ *       at time factor 1 and linear time interpolation checks.
+ *   or_interpolate_default  DefaultInterpolation.cpph:29-270; pinned by the
+ *       reference's golden values 65/9, 67/9, 73/9, 25/3, 29/9, 31/9, 37/9,
+ *       13/3 (GhostLayerCompositorTest.cpp:207-214, 298-305) and bilinear
+ *       exactness on linear fields.
+ *   or_restrict_default, region functions  DefaultRestriction.cpph:16-150,
+ *       Region.cpp:174-315; pinned by GridLevelTransferTest overlap areas
+ *       (2.25, 0, 0) and GhostLayerCompositorTest region tests, plus
+ *       conservation of the area-weighted average.
+ *   compositor + traversals  pinned by the traversal call-order tests and the
+ *       time-stepping veto test (GhostLayerCompositorTest.cpp:59-124).
+ *   BreakingDam, demanded mesh width, refinement decision: BreakingDam.cpp:
+ *       53-139, SolveTimestep.cpp:753-797; pinned by closed forms on the
+ *       dam-break initial data.
+ */
+#include "swe_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* tarch::la comparisons use Peano's default tolerance (third-party constant,
+ * Peano3 tarch/la; only matters at exact ties). */
+#define LA_EPS 1e-8
+static int la_greater(double a, double b) { return a - b > LA_EPS; }
+static int la_smaller(double a, double b) { return b - a > LA_EPS; }
+static int la_equals(double a, double b) { return fabs(a - b) <= LA_EPS; }
+static int la_one_greater(const double a[2], const double b[2]) {
+  return la_greater(a[0], b[0]) || la_greater(a[1], b[1]);
+}
+static int la_one_greater_equals(const double a[2], const double b[2]) {
+  return (a[0] - b[0] >= -LA_EPS) || (a[1] - b[1] >= -LA_EPS);
+}
+static int la_all_greater(const double a[2], const double b[2]) {
+  return la_greater(a[0], b[0]) && la_greater(a[1], b[1]);
+}
+static double dmin(double a, double b) { return b < a ? b : a; }
+static double dmax(double a, double b) { return a < b ? b : a; }
+static int imin(int a, int b) { return a < b ? a : b; }
+static int imax(int a, int b) { return a > b ? a : b; }
+
+/* ------------------------------------------------------------------ */
+/* layout (grid/LinearizationZYXQ.h:24-82)                              */
+/* ------------------------------------------------------------------ */
+int or_unew_index(const or_patch *p, int q, int x, int y) {
+  return q * p->nx * p->ny + x * p->ny + y;
+}
+int or_uold_index(const or_patch *p, int q, int x, int y) {
+  int sx = p->nx + 2 * p->gw, sy = p->ny + 2 * p->gw;
+  return q * sx * sy + (x + p->gw) * sy + (y + p->gw);
+}
+
+void or_patch_init(or_patch *p, int nq, int npw, int nx, int ny, int gw,
+                   double px, double py, double sx, double sy, int level,
+                   double time, double dt, double min_neighbor_time,
+                   double *unew, double *uold, double *pwg) {
+  memset(p, 0, sizeof(*p));
+  p->nq = nq; p->npw = npw; p->nx = nx; p->ny = ny; p->gw = gw;
+  p->pos[0] = px; p->pos[1] = py; p->size[0] = sx; p->size[1] = sy;
+  p->subcell[0] = sx / (double)nx;     /* Patch.cpp:38 */
+  p->subcell[1] = sy / (double)ny;
+  p->level = level;
+  p->is_leaf = 1; p->is_virtual = 0; p->is_valid = 1;
+  p->time = time; p->dt = dt; p->est_next = 0.0;
+  /* Helper.cpp:40 stores minimalNeighborTime into the cell description. */
+  p->min_neighbor_time = min_neighbor_time;
+  p->max_neighbor_dt = 0.0;
+  p->min_neighbor_time_constraint = DBL_MAX;
+  p->min_leaf_neighbor_time_constraint = DBL_MAX;
+  p->lower_ghost_bounds[0] = p->lower_ghost_bounds[1] = DBL_MAX;
+  p->upper_ghost_bounds[0] = p->upper_ghost_bounds[1] = -DBL_MAX;
+  p->unew = unew; p->uold = uold; p->pwg = pwg;
+}
+
+/* ------------------------------------------------------------------ */
+/* f-wave edge solver                                                   */
+/* ------------------------------------------------------------------ */
+/* Dry handling (one dry side is replaced by the mirrored wet state, so the
+ * edge acts as a wall; both dry gives no update) and speeds
+ *   s1 = min(uL - sqrt(g hL), uRoe - sqrt(g hRoe))
+ *   s2 = max(uR + sqrt(g hR), uRoe + sqrt(g hRoe))
+ * with uRoe = (uL sqrt(hL) + uR sqrt(hR)) / (sqrt(hL) + sqrt(hR)),
+ * hRoe = (hL + hR)/2.  Flux jump with bathymetry source:
+ *   d0 = huR - huL
+ *   d1 = (huR uR + g/2 hR^2) - (huL uL + g/2 hL^2) + g/2 (hL+hR)(bR-bL)
+ * decomposed into beta1 (1,s1) + beta2 (1,s2).  A wave with s < -zeroTol
+ * goes to the left cell, s > zeroTol to the right, otherwise half each.   */
+void or_fwave(double hL, double hR, double huL, double huR, double bL,
+              double bR, double out[5]) {
+  const double g = OR_GRAVITY;
+  const double halfg = 0.5 * OR_GRAVITY;
+  int leftDry = 0, rightDry = 0;
+  if (hL >= OR_DRY_TOL) {
+    if (hR < OR_DRY_TOL) {
+      hR = hL; huR = -huL; bR = bL; rightDry = 1;
+    }
+  } else if (hR >= OR_DRY_TOL) {
+    hL = hR; huL = -huR; bL = bR; leftDry = 1;
+  } else {
+    out[0] = out[1] = out[2] = out[3] = out[4] = 0.0;
+    return;
+  }
+  double uL = huL / hL;
+  double uR = huR / hR;
+  double sqL = sqrt(hL), sqR = sqrt(hR);
+  double cL = sqrt(g * hL), cR = sqrt(g * hR);
+  double fL = huL * uL + (halfg * hL) * hL;
+  double fR = huR * uR + (halfg * hR) * hR;
+
+  double hRoe = 0.5 * (hL + hR);
+  double uRoe = (uL * sqL + uR * sqR) / (sqL + sqR);
+  double cRoe = sqrt(g * hRoe);
+  double s1 = dmin(uL - cL, uRoe - cRoe);
+  double s2 = dmax(uR + cR, uRoe + cRoe);
+
+  double d0 = huR - huL;
+  double d1 = (fR - fL) + (halfg * (hL + hR)) * (bR - bL);
+  double inv = 1.0 / (s2 - s1);
+  double beta1 = (s2 * d0 - d1) * inv;
+  double beta2 = (d1 - s1 * d0) * inv;
+
+  double dhL = 0.0, dhR = 0.0, dqL = 0.0, dqR = 0.0;
+  double z1 = beta1 * s1, z2 = beta2 * s2;
+  if (s1 < -OR_ZERO_TOL) {
+    dhL += beta1; dqL += z1;
+  } else if (s1 > OR_ZERO_TOL) {
+    dhR += beta1; dqR += z1;
+  } else {
+    dhL += 0.5 * beta1; dqL += 0.5 * z1;
+    dhR += 0.5 * beta1; dqR += 0.5 * z1;
+  }
+  if (s2 < -OR_ZERO_TOL) {
+    dhL += beta2; dqL += z2;
+  } else if (s2 > OR_ZERO_TOL) {
+    dhR += beta2; dqR += z2;
+  } else {
+    dhL += 0.5 * beta2; dqL += 0.5 * z2;
+    dhR += 0.5 * beta2; dqR += 0.5 * z2;
+  }
+  if (leftDry) { dhL = 0.0; dqL = 0.0; }
+  if (rightDry) { dhR = 0.0; dqR = 0.0; }
+  out[0] = dhL; out[1] = dhR; out[2] = dqL; out[3] = dqR;
+  out[4] = dmax(fabs(s1), fabs(s2));
+}
+
+double or_fwave_speed(double hL, double hR, double huL, double huR) {
+  double out[5];
+  or_fwave(hL, hR, huL, huR, 0.0, 0.0, out);
+  return out[4];
+}
+
+/* ------------------------------------------------------------------ */
+/* SWE block (tools/SWE_WaveAccumulationBlock.cpp:84-296)               */
+/* arrays are (nx+2)*(ny+2), index i*(ny+2)+j, interior i,j in [1,n]    */
+/* ------------------------------------------------------------------ */
+double or_block_fluxes(int nx, int ny, double dx, double dy,
+                       const double *h, const double *hu, const double *hv,
+                       const double *b, double *netH, double *netHu,
+                       double *netHv) {
+  const int sy = ny + 2;
+  double dx_inv = 1.0 / dx;
+  double dy_inv = 1.0 / dy;
+  double maxWaveSpeed = 0.0;
+  double o[5];
+  /* vertical edges (:142-175) */
+  for (int i = 1; i < nx + 2; i++) {
+    for (int j = 1; j < ny + 1; j++) {
+      int l = (i - 1) * sy + j, r = i * sy + j;
+      or_fwave(h[l], h[r], hu[l], hu[r], b[l], b[r], o);
+      netH[l] += dx_inv * o[0];
+      netHu[l] += dx_inv * o[2];
+      netH[r] += dx_inv * o[1];
+      netHu[r] += dx_inv * o[3];
+      maxWaveSpeed = dmax(maxWaveSpeed, o[4]);
+    }
+  }
+  /* horizontal edges (:182-214) */
+  for (int i = 1; i < nx + 1; i++) {
+    for (int j = 1; j < ny + 2; j++) {
+      int d = i * sy + (j - 1), u = i * sy + j;
+      or_fwave(h[d], h[u], hv[d], hv[u], b[d], b[u], o);
+      netH[d] += dy_inv * o[0];
+      netHv[d] += dy_inv * o[2];
+      netH[u] += dy_inv * o[1];
+      netHv[u] += dy_inv * o[3];
+      maxWaveSpeed = dmax(maxWaveSpeed, o[4]);
+    }
+  }
+  return maxWaveSpeed;
+}
+
+/* :225-239 — the build uses DBL_MAX where the float reference uses FLT_MAX */
+double or_block_max_timestep(double maxWaveSpeed, double dx, double dy) {
+  if (maxWaveSpeed > 0.00001) {
+    double t = dmin(dx / maxWaveSpeed, dy / maxWaveSpeed);
+    t *= OR_CFL;
+    return t;
+  }
+  return DBL_MAX;
+}
+
+void or_block_update(int nx, int ny, double dt, double *h, double *hu,
+                     double *hv, double *netH, double *netHu, double *netHv) {
+  const int sy = ny + 2;
+  for (int i = 1; i < nx + 1; i++) {
+    for (int j = 1; j < ny + 1; j++) {
+      int k = i * sy + j;
+      h[k] -= dt * netH[k];
+      hu[k] -= dt * netHu[k];
+      hv[k] -= dt * netHv[k];
+      netH[k] = 0.0; netHu[k] = 0.0; netHv[k] = 0.0;
+      if (h[k] < 0.1) { hu[k] = 0.0; hv[k] = 0.0; }
+      if (h[k] < 0.0) h[k] = 0.0;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* TimeIntervals (grid/TimeIntervals.cpp)                               */
+/* ------------------------------------------------------------------ */
+double or_time_uold(const or_patch *p) {
+  return p->is_leaf ? p->time : p->min_neighbor_time;
+}
+double or_time_unew(const or_patch *p) {
+  return p->is_leaf ? p->time + p->dt
+                    : p->min_neighbor_time + p->max_neighbor_dt;
+}
+int or_is_blocked_by_neighbors(const or_patch *p) {
+  return la_greater(p->time + p->dt, p->min_neighbor_time_constraint);
+}
+void or_reset_min_neighbor_time_constraint(or_patch *p) {
+  p->min_neighbor_time_constraint = DBL_MAX;
+  p->min_leaf_neighbor_time_constraint = DBL_MAX;
+}
+/* UpdateNeighborTimeFunctor (GhostlayerCompositorFunctors.cpp:226-260) */
+void or_update_neighbor_time(or_patch *u, const or_patch *n) {
+  if (!(u->is_valid && n->is_valid)) return;
+  double c = n->time + n->dt; /* getTimeConstraint */
+  if (la_smaller(c, u->min_neighbor_time_constraint))
+    u->min_neighbor_time_constraint = c;
+  /* updateMaximalNeighborTimeInterval */
+  if (n->time + n->dt > u->min_neighbor_time + u->max_neighbor_dt)
+    u->max_neighbor_dt = n->time + n->dt - u->min_neighbor_time;
+  if (n->time < u->min_neighbor_time) {
+    u->max_neighbor_dt = u->max_neighbor_dt + u->min_neighbor_time - n->time;
+    u->min_neighbor_time = n->time;
+  }
+  if (n->is_leaf && la_smaller(c, u->min_leaf_neighbor_time_constraint))
+    u->min_leaf_neighbor_time_constraint = c;
+}
+
+/* ------------------------------------------------------------------ */
+/* SWEKernel                                                            */
+/* ------------------------------------------------------------------ */
+static void block_arrays(or_patch *p, double **h, double **hu, double **hv,
+                         double **b) {
+  int area = (p->nx + 2 * p->gw) * (p->ny + 2 * p->gw);
+  *h = p->uold; *hu = p->uold + area; *hv = p->uold + 2 * area; *b = p->pwg;
+}
+
+void or_swap_unew_uold(or_patch *p) {
+  for (int q = 0; q < p->nq; q++)
+    for (int x = 0; x < p->nx; x++)
+      for (int y = 0; y < p->ny; y++) {
+        int a = or_unew_index(p, q, x, y), b = or_uold_index(p, q, x, y);
+        double t = p->unew[a];
+        p->unew[a] = p->uold[b];
+        p->uold[b] = t;
+      }
+}
+
+void or_copy_unew_to_uold(or_patch *p) {
+  for (int q = 0; q < p->nq; q++)
+    for (int x = 0; x < p->nx; x++)
+      for (int y = 0; y < p->ny; y++)
+        p->uold[or_uold_index(p, q, x, y)] = p->unew[or_unew_index(p, q, x, y)];
+}
+
+/* SWEKernel::solveTimestep -> advanceBlockInTime (SWEKernel.cpp:50-88) */
+void or_solve_timestep(or_patch *p, double maximumTimestepSize) {
+  double *h, *hu, *hv, *b;
+  block_arrays(p, &h, &hu, &hv, &b);
+  int area = (p->nx + 2) * (p->ny + 2);
+  double *net = (double *)calloc(3 * (size_t)area, sizeof(double));
+  double dx = p->subcell[0], dy = p->subcell[1];
+  double smax = or_block_fluxes(p->nx, p->ny, dx, dy, h, hu, hv, b, net,
+                                net + area, net + 2 * area);
+  double maxTimestep = or_block_max_timestep(smax, dx, dy);
+  double dt = fmin(maxTimestep, maximumTimestepSize);
+  double estimatedNext = maxTimestep;
+  or_block_update(p->nx, p->ny, dt, h, hu, hv, net, net + area, net + 2 * area);
+  free(net);
+  or_swap_unew_uold(p);
+  if (la_greater(dt, 0.0)) {
+    p->time = p->time + p->dt;
+    p->dt = dt;
+  }
+  p->est_next = estimatedNext;
+}
+
+/* SWEKernel::fillBoundaryLayer (SWEKernel.cpp:176-237): reflective wall */
+void or_fill_boundary_layer(or_patch *p, int dimension, int setUpper) {
+  if (dimension == 0) {
+    for (int yi = -1; yi < p->ny + 1; yi++) {
+      int xi = setUpper ? p->nx : -1;
+      int sx = xi + (setUpper ? -1 : +1);
+      for (int q = 0; q < p->nq; q++) {
+        double v = p->uold[or_uold_index(p, q, sx, yi)];
+        p->uold[or_uold_index(p, q, xi, yi)] = (q == dimension + 1) ? -v : v;
+      }
+    }
+  } else {
+    for (int xi = -1; xi < p->nx + 1; xi++) {
+      int yi = setUpper ? p->ny : -1;
+      int sy = yi + (setUpper ? -1 : +1);
+      for (int q = 0; q < p->nq; q++) {
+        double v = p->uold[or_uold_index(p, q, xi, sy)];
+        p->uold[or_uold_index(p, q, xi, yi)] = (q == dimension + 1) ? -v : v;
+      }
+    }
+  }
+}
+
+/* DefaultTransfer::transferGhostlayer (DefaultTransfer.cpph:15-101) */
+void or_transfer_ghostlayer(const int size[2], const int srcOff[2],
+                            const int dstOff[2], or_patch *src, or_patch *dst) {
+  double timeFactor;
+  double dstTimeUNew = or_time_unew(dst);
+  if (src->is_virtual) {
+    timeFactor = (dstTimeUNew - 0.0) / 1.0;
+  } else {
+    double sNew = or_time_unew(src), sOld = or_time_uold(src);
+    if (la_greater(sNew - sOld, 0.0))
+      timeFactor = (dstTimeUNew - sOld) / (sNew - sOld);
+    else
+      timeFactor = 1.0;
+  }
+  double inverse = 1.0 - timeFactor;
+  for (int x = 0; x < size[0]; x++)
+    for (int y = 0; y < size[1]; y++)
+      for (int q = 0; q < src->nq; q++) {
+        double a = src->unew[or_unew_index(src, q, srcOff[0] + x, srcOff[1] + y)];
+        double o = src->uold[or_uold_index(src, q, srcOff[0] + x, srcOff[1] + y)];
+        dst->uold[or_uold_index(dst, q, dstOff[0] + x, dstOff[1] + y)] =
+            a * timeFactor + o * inverse;
+      }
+}
+
+/* SWEKernel::transformWaterHeight (SWEKernel.cpp:21-48) */
+void or_transform_water_height(or_patch *p, const int off[2], const int size[2],
+                               int modifyUOld, int absoluteToAboveSeaFloor) {
+  double sign = absoluteToAboveSeaFloor ? -1 : +1;
+  for (int x = 0; x < size[0]; x++)
+    for (int y = 0; y < size[1]; y++) {
+      int cx = off[0] + x, cy = off[1] + y;
+      double bv = p->pwg[or_uold_index(p, 0, cx, cy)];
+      if (modifyUOld) {
+        int k = or_uold_index(p, 0, cx, cy);
+        p->uold[k] = p->uold[k] + sign * bv;
+      } else {
+        int k = or_unew_index(p, 0, cx, cy);
+        p->unew[k] = p->unew[k] + sign * bv;
+      }
+    }
+}
+
+static void subcell_center(const or_patch *p, int x, int y, double c[2]) {
+  /* Patch::getSubcellCenter (Patch.cpp:429-435) */
+  double r0 = p->subcell[0] * (double)x, r1 = p->subcell[1] * (double)y;
+  c[0] = p->pos[0] + r0 + p->subcell[0] * 0.5;
+  c[1] = p->pos[1] + r1 + p->subcell[1] * 0.5;
+}
+
+/* DefaultInterpolationTemplate::interpolateSolution
+ * (DefaultInterpolation.cpph:29-270) */
+void or_interpolate_default(const int dstSize[2], const int dstOff[2],
+                            or_patch *src, or_patch *dst, int toUOld,
+                            int toCurrentTime, int useTimeUNewOrUOld) {
+  double timeFactor;
+  if (la_equals(src->dt, 0.0)) {
+    timeFactor = 1.0;
+  } else if (useTimeUNewOrUOld) {
+    double sOld = or_time_uold(src), sNew = or_time_unew(src);
+    if (toCurrentTime)
+      timeFactor = (or_time_uold(dst) - sOld) / (sNew - sOld);
+    else
+      timeFactor = (or_time_unew(dst) - sOld) / (sNew - sOld);
+  } else {
+    if (toCurrentTime)
+      timeFactor = (dst->time - src->time) / (src->time + src->dt - src->time);
+    else
+      timeFactor = (dst->time + dst->dt - src->time) /
+                   (src->time + src->dt - src->time);
+  }
+  timeFactor = dmax(0.0, dmin(1.0, timeFactor));
+
+  /* clear: the reference always zeroes uOld over the region (:80-83) */
+  for (int x = 0; x < dstSize[0]; x++)
+    for (int y = 0; y < dstSize[1]; y++)
+      for (int q = 0; q < dst->nq; q++)
+        dst->uold[or_uold_index(dst, q, dstOff[0] + x, dstOff[1] + y)] = 0.0;
+
+  double inv[2] = {1.0 / src->subcell[0], 1.0 / src->subcell[1]};
+  double base[2] = {src->pos[0] + src->subcell[0] * 0.5,
+                    src->pos[1] + src->subcell[1] * 0.5};
+  for (int x = 0; x < dstSize[0]; x++)
+    for (int y = 0; y < dstSize[1]; y++) {
+      int dx_ = dstOff[0] + x, dy_ = dstOff[1] + y;
+      double c[2];
+      subcell_center(dst, dx_, dy_, c);
+      int si[2];
+      for (int d = 0; d < 2; d++) {
+        double loc = c[d] - base[d];
+        si[d] = (int)floor(loc * inv[d]);
+      }
+      int sn[2] = {src->nx, src->ny};
+      for (int d = 0; d < 2; d++) si[d] = imax(0, imin(sn[d] - 2, si[d]));
+      for (int k = 0; k < 4; k++) {
+        int off[2] = {k & 1, (k >> 1) & 1}; /* dfor2: x fastest */
+        int nb[2] = {si[0] + off[0], si[1] + off[1]};
+        double np_[2];
+        subcell_center(src, nb[0], nb[1], np_);
+        double sf = 1.0;
+        for (int d = 0; d < 2; d++) {
+          double sign = (double)(1 - 2 * off[d]);
+          sf *= 1.0 - (sign * (c[d] - np_[d])) * inv[d];
+        }
+        for (int q = 0; q < src->nq; q++) {
+          double so = src->uold[or_uold_index(src, q, nb[0], nb[1])];
+          double sn_ = src->unew[or_unew_index(src, q, nb[0], nb[1])];
+          if (toUOld) {
+            int t = or_uold_index(dst, q, dx_, dy_);
+            dst->uold[t] = dst->uold[t] +
+                           sf * (so * (1.0 - timeFactor) + sn_ * timeFactor);
+          } else {
+            int t = or_unew_index(dst, q, dx_, dy_);
+            dst->unew[t] = dst->unew[t] + sf * so * (1.0 - timeFactor) +
+                           sf * sn_ * timeFactor;
+          }
+        }
+      }
+    }
+}
+
+/* Region::mapToPatch (Region.cpp:174-206) */
+void or_region_map_to_patch(const int off[2], const int size[2],
+                            const or_patch *src, const or_patch *dst,
+                            double epsilon, int outOff[2], int outSize[2]) {
+  for (int d = 0; d < 2; d++) {
+    double position = src->subcell[d] * (double)off[d];
+    position += src->pos[d] - dst->pos[d];
+    double ot = position + epsilon;
+    ot /= dst->subcell[d];
+    outOff[d] = (int)ot;
+    double sz = src->subcell[d] * (double)(off[d] + size[d]);
+    sz += src->pos[d] - dst->pos[d];
+    double st = sz - epsilon;
+    st /= dst->subcell[d];
+    st -= (double)outOff[d];
+    st += 1.0;
+    outSize[d] = (int)st;
+  }
+}
+
+/* SWEKernel::interpolateSolution (SWEKernel.cpp:239-286) */
+void or_interpolate_swe(const int dstSize[2], const int dstOff[2],
+                        or_patch *src, or_patch *dst, int toUOld,
+                        int toCurrentTime, int useTimeUNewOrUOld) {
+  int so[2], ss[2];
+  or_region_map_to_patch(dstOff, dstSize, dst, src, 1e-12, so, ss);
+  int sn[2] = {src->nx, src->ny};
+  for (int d = 0; d < 2; d++) {
+    if (so[d] > 0) {
+      so[d] = so[d] - 1;
+      ss[d] = imin(sn[d], ss[d] + 2);
+    } else {
+      ss[d] = imin(sn[d], ss[d] + 1);
+    }
+  }
+  or_transform_water_height(src, so, ss, 1, 0);
+  or_transform_water_height(src, so, ss, 0, 0);
+  or_interpolate_default(dstSize, dstOff, src, dst, toUOld, toCurrentTime,
+                         useTimeUNewOrUOld);
+  or_transform_water_height(src, so, ss, 1, 1);
+  or_transform_water_height(src, so, ss, 0, 1);
+  or_transform_water_height(dst, dstOff, dstSize, toUOld, 1);
+}
+
+/* DefaultRestriction::calculateOverlappingRegion (DefaultRestriction.h:44-62) */
+double or_overlapping_region(const double p1[2], const double s1[2],
+                             const double p2[2], const double s2[2]) {
+  double region = 1.0;
+  for (int d = 0; d < 2; d++) {
+    double iv = dmin(p1[d] + s1[d], p2[d] + s2[d]) - dmax(p1[d], p2[d]);
+    region *= iv;
+    region = dmax(region, 0.0);
+  }
+  return region;
+}
+
+/* Region::mapCellToPatch (Region.cpp:208-244) */
+static void region_map_cell(const int rOff[2], const int rSize[2],
+                            const double finePos[2], const double fineSub[2],
+                            const double coarseSub[2],
+                            const double coarsePos[2], double eps,
+                            int outOff[2], int outSize[2]) {
+  for (int d = 0; d < 2; d++) {
+    double ot = (coarsePos[d] - finePos[d] + eps);
+    ot /= fineSub[d];
+    int o = (int)ot;
+    double st = coarsePos[d] + coarseSub[d] - finePos[d] - eps;
+    st /= fineSub[d];
+    st -= ot;
+    st += 1.0;
+    int s = (int)st;
+    int upper = o + s, rUpper = rOff[d] + rSize[d];
+    outOff[d] = imax(o, rOff[d]);
+    outSize[d] = imin(upper, rUpper) - outOff[d];
+  }
+}
+
+/* Region::getRegionsOverlappedByNeighboringGhostlayers (Region.cpp:247-315) */
+int or_regions_overlapped_by_ghostlayers(const double lower[2],
+                                         const double upper[2],
+                                         const double srcPos[2],
+                                         const double srcSize[2],
+                                         const double srcSubcell[2],
+                                         const int srcN[2], int off[8],
+                                         int size[8]) {
+  double up_plus[2] = {upper[0] + srcSubcell[0], upper[1] + srcSubcell[1]};
+  double pend[2] = {srcPos[0] + srcSize[0], srcPos[1] + srcSize[1]};
+  memset(off, 0, 8 * sizeof(int));
+  memset(size, 0, 8 * sizeof(int));
+  if (la_one_greater(up_plus, lower) || la_one_greater_equals(upper, pend) ||
+      !la_all_greater(lower, srcPos)) {
+    size[0] = srcN[0]; size[1] = srcN[1];
+    return 1;
+  }
+  double epsilon = 1e-12;
+  int lo[2], hi[2];
+  for (int d = 0; d < 2; d++) {
+    double dl = (lower[d] - srcPos[d]) * (1.0 / srcSubcell[d]) + epsilon;
+    double du = (upper[d] - srcPos[d]) * (1.0 / srcSubcell[d]) - epsilon;
+    dl = dmin((double)2147483647, dl);
+    du = dmax((double)(-2147483647 - 1), du);
+    lo[d] = imax(imin((int)floor(dl), srcN[d]), -1);
+    hi[d] = imax(imin((int)floor(du), srcN[d]), -1);
+  }
+  for (int d = 0; d < 2; d++) {
+    /* region r = 2d+k stores (offset, size) at off/size[2*r + dim] */
+    int r0 = 2 * d, r1 = 2 * d + 1;
+    size[2 * r0 + d] = hi[d] + 1;
+    off[2 * r0 + d] = 0;
+    size[2 * r1 + d] = srcN[d] - lo[d];
+    off[2 * r1 + d] = lo[d];
+    for (int i = 0; i < d; i++) {
+      size[2 * r0 + i] = -hi[i] - 1 + lo[i];
+      off[2 * r0 + i] = hi[i] + 1;
+      size[2 * r1 + i] = size[2 * r0 + i];
+      off[2 * r1 + i] = off[2 * r0 + i];
+    }
+    for (int i = d + 1; i < 2; i++) {
+      size[2 * r0 + i] = srcN[i]; off[2 * r0 + i] = 0;
+      size[2 * r1 + i] = srcN[i]; off[2 * r1 + i] = 0;
+    }
+  }
+  return 4;
+}
+
+/* DefaultRestrictionTemplate::restrictRegion (DefaultRestriction.cpph:16-106) */
+static void restrict_region(or_patch *src, or_patch *dst, const int rOff[2],
+                            const int rSize[2]) {
+  const double epsilon = 1e-12;
+  const double dstTimeUOld = 0.0, dstTimeUNew = 1.0;
+  double srcSub[2], dstSub[2];
+  for (int d = 0; d < 2; d++) {
+    int sn = d ? src->ny : src->nx, dn = d ? dst->ny : dst->nx;
+    srcSub[d] = src->size[d] * (1.0 / (double)sn);
+    dstSub[d] = dst->size[d] * (1.0 / (double)dn);
+  }
+  double dstRegion = dstSub[0] * dstSub[1];
+  double tfOld = 1.0, tfNew = 1.0;
+  if (la_greater(src->dt, 0.0)) {
+    double sOld = or_time_uold(src), sNew = or_time_unew(src);
+    tfOld = (dstTimeUOld - sOld) / (sNew - sOld);
+    tfNew = (dstTimeUNew - sOld) / (sNew - sOld);
+  }
+  int dOff[2], dSize[2];
+  or_region_map_to_patch(rOff, rSize, src, dst, epsilon, dOff, dSize);
+  for (int x = 0; x < dSize[0]; x++)
+    for (int y = 0; y < dSize[1]; y++) {
+      int di[2] = {dOff[0] + x, dOff[1] + y};
+      double dpos[2] = {dst->pos[0] + (double)di[0] * dstSub[0],
+                        dst->pos[1] + (double)di[1] * dstSub[1]};
+      int cOff[2], cSize[2];
+      region_map_cell(rOff, rSize, src->pos, srcSub, dstSub, dpos, epsilon,
+                      cOff, cSize);
+      for (int sx = 0; sx < cSize[0]; sx++)
+        for (int sy = 0; sy < cSize[1]; sy++) {
+          int si[2] = {cOff[0] + sx, cOff[1] + sy};
+          double spos[2] = {src->pos[0] + (double)si[0] * srcSub[0],
+                            src->pos[1] + (double)si[1] * srcSub[1]};
+          double overlap = or_overlapping_region(dpos, dstSub, spos, srcSub);
+          for (int q = 0; q < src->nq; q++) {
+            double vo = src->uold[or_uold_index(src, q, si[0], si[1])];
+            double vn = src->unew[or_unew_index(src, q, si[0], si[1])];
+            double dOld = (vo * (1.0 - tfOld) + vn * tfOld) * overlap / dstRegion;
+            double dNew = (vo * (1.0 - tfNew) + vn * tfNew) * overlap / dstRegion;
+            int a = or_uold_index(dst, q, di[0], di[1]);
+            int b = or_unew_index(dst, q, di[0], di[1]);
+            dst->uold[a] = dst->uold[a] + dOld;
+            dst->unew[b] = dst->unew[b] + dNew;
+          }
+        }
+    }
+}
+
+void or_restrict_default(or_patch *src, or_patch *dst, int onlyOverlapped) {
+  int sn[2] = {src->nx, src->ny};
+  if (onlyOverlapped) {
+    int off[8], size[8];
+    int nr = or_regions_overlapped_by_ghostlayers(
+        dst->lower_ghost_bounds, dst->upper_ghost_bounds, src->pos, src->size,
+        src->subcell, sn, off, size);
+    for (int r = 0; r < nr; r++) {
+      if (size[2 * r] > 0 && size[2 * r + 1] > 0)
+        restrict_region(src, dst, off + 2 * r, size + 2 * r);
+    }
+  } else {
+    int z[2] = {0, 0};
+    restrict_region(src, dst, z, sn);
+  }
+}
+
+/* SWEKernel::restrictSolution / postProcessRestriction (SWEKernel.cpp:288-316) */
+void or_restrict_swe(or_patch *src, or_patch *dst, int onlyOverlapped) {
+  int z[2] = {0, 0}, sn[2] = {src->nx, src->ny};
+  or_transform_water_height(src, z, sn, 1, 0);
+  or_transform_water_height(src, z, sn, 0, 0);
+  or_restrict_default(src, dst, onlyOverlapped);
+  or_transform_water_height(src, z, sn, 1, 1);
+  or_transform_water_height(src, z, sn, 0, 1);
+}
+
+void or_post_process_restriction_swe(or_patch *dst, int onlyOverlapped) {
+  (void)onlyOverlapped;
+  int z[2] = {0, 0}, dn[2] = {dst->nx, dst->ny};
+  or_transform_water_height(dst, z, dn, 1, 1);
+  or_transform_water_height(dst, z, dn, 0, 1);
+}
+
+/* ------------------------------------------------------------------ */
+/* GhostLayerCompositor (vertex with patches "1 0 / 3 2")               */
+/* ------------------------------------------------------------------ */
+/* FaceAdjacentPatchTraversal.cpph:10-34 (Dim2 branch) */
+int or_face_traversal_order(or_traversal_call o[8]) {
+  static const int t[8][4] = {{3, 2, 1, 0},  {3, 1, 0, 1},  {2, 3, -1, 0},
+                              {2, 0, 0, 1},  {1, 0, 1, 0},  {1, 3, 0, -1},
+                              {0, 1, -1, 0}, {0, 2, 0, -1}};
+  for (int i = 0; i < 8; i++) {
+    o[i].src = t[i][0]; o[i].dst = t[i][1];
+    o[i].dir[0] = t[i][2]; o[i].dir[1] = t[i][3];
+  }
+  return 8;
+}
+
+/* EdgeAdjacentPatchTraversal.cpph:15-33 (Dim2): the diagonal neighbour */
+int or_edge_traversal_order(or_traversal_call o[4]) {
+  int k = 0;
+  for (int s = 3; s >= 0; s--) {
+    int lin = 3 - s;
+    int sp[2] = {lin & 1, (lin >> 1) & 1};
+    int dp[2] = {(sp[0] + 1) % 2, (sp[1] + 1) % 2};
+    int d = 3 - (dp[0] + 2 * dp[1]);
+    o[k].src = s; o[k].dst = d;
+    o[k].dir[0] = dp[0] - sp[0]; o[k].dir[1] = dp[1] - sp[1];
+    k++;
+  }
+  return k;
+}
+
+/* GhostLayerCompositor::shouldTransferGhostlayerData (:26-42) */
+int or_should_transfer(const or_patch *s, const or_patch *d) {
+  if (!s->is_valid) return 0;
+  int holds = s->is_virtual || s->is_leaf;
+  int blocked = la_greater(d->time + d->dt, d->min_neighbor_time_constraint);
+  return d->is_valid && d->is_leaf &&
+         ((holds && !la_greater(d->time + d->dt, s->time + s->dt)) ||
+          (s->is_leaf && !blocked));
+}
+
+static void fill_region(or_patch *s, or_patch *d, int level, const int size[2],
+                        const int so[2], const int dof[2], int kind) {
+  if (s->level == d->level && s->level == level) {
+    or_transfer_ghostlayer(size, so, dof, s, d);
+  } else if (s->level < d->level && d->level == level && s->is_leaf) {
+    if (kind == 1)
+      or_interpolate_swe(size, dof, s, d, 1, 0, 1);
+    else
+      or_interpolate_default(size, dof, s, d, 1, 0, 1);
+  }
+}
+
+/* FillGhostlayerFaceFunctor (GhostlayerCompositorFunctors.cpp:23-77) */
+static void face_functor(or_patch *s, or_patch *d, const int dir[2], int level,
+                         int kind) {
+  int dim = -1, offset = 0;
+  for (int k = 0; k < 2; k++)
+    if (abs(dir[k]) == 1) { dim = k; offset = dir[k]; }
+  int n[2] = {s->nx, s->ny};
+  int size[2] = {n[0], n[1]};
+  size[dim] = d->gw;
+  int dof[2] = {0, 0};
+  dof[dim] = (offset == 1) ? -d->gw : n[dim];
+  int so[2] = {0, 0};
+  so[dim] = (offset == 1) ? (n[dim] - s->gw) : 0;
+  fill_region(s, d, level, size, so, dof, kind);
+}
+
+/* FillGhostlayerEdgeFunctor (GhostlayerCompositorFunctors.cpp:87-160) */
+static void edge_functor(or_patch *s, or_patch *d, const int dir[2], int level,
+                         int kind) {
+  int gw = d->gw;
+  int n[2] = {s->nx, s->ny};
+  int size[2] = {gw, gw}, so[2] = {0, 0}, dof[2] = {0, 0};
+  for (int k = 0; k < 2; k++) {
+    if (dir[k] == 0) {
+      size[k] = n[k];
+    } else if (dir[k] == 1) {
+      so[k] = n[k] - gw; dof[k] = -gw;
+    } else {
+      dof[k] = n[k];
+    }
+  }
+  fill_region(s, d, level, size, so, dof, kind);
+}
+
+/* fillGhostLayersAndUpdateNeighborTimes (GhostLayerCompositor.cpp:280-320),
+ * without the (disabled by default) dimensional-splitting extrapolation:
+ * for dimensionality 0 (corners in 2D) then 1 (faces): fill, then update
+ * neighbour times over the same manifolds.                                 */
+void or_fill_ghostlayers_and_update_neighbor_times(or_patch *pt[4], int level,
+                                                   int dest, int kind) {
+  or_traversal_call f[8], e[4];
+  or_face_traversal_order(f);
+  or_edge_traversal_order(e);
+  for (int dimensionality = 0; dimensionality < 2; dimensionality++) {
+    or_traversal_call *calls = dimensionality == 1 ? f : e;
+    int nc = dimensionality == 1 ? 8 : 4;
+    for (int c = 0; c < nc; c++) {
+      or_patch *s = pt[calls[c].src], *d = pt[calls[c].dst];
+      if ((dest == -1 || dest == calls[c].dst) && or_should_transfer(s, d)) {
+        if (dimensionality == 1)
+          face_functor(s, d, calls[c].dir, level, kind);
+        else
+          edge_functor(s, d, calls[c].dir, level, kind);
+      }
+    }
+    for (int c = 0; c < nc; c++)
+      or_update_neighbor_time(pt[calls[c].dst], pt[calls[c].src]);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* BreakingDam scenario (native/scenarios/BreakingDam.cpp)              */
+/* ------------------------------------------------------------------ */
+void or_breaking_dam_init(or_patch *p, double cx, double cy, double radius,
+                          double hIn, double hOut) {
+  for (int yi = 0; yi < p->ny; yi++)
+    for (int xi = 0; xi < p->nx; xi++) {
+      double X = p->pos[0] + (xi + 0.5) * p->subcell[0];
+      double Y = p->pos[1] + (yi + 0.5) * p->subcell[1];
+      double r = sqrt((X - cx) * (X - cx) + (Y - cy) * (Y - cy));
+      double h = hIn * (r <= radius) + hOut * (r > radius);
+      p->unew[or_unew_index(p, 0, xi, yi)] = h;
+      p->unew[or_unew_index(p, 1, xi, yi)] = 0.0;
+      p->unew[or_unew_index(p, 2, xi, yi)] = 0.0;
+      p->pwg[or_uold_index(p, 0, xi, yi)] = 0.0;
+    }
+}
+
+/* BreakingDamSWEScenario::computeDemandedMeshWidth (BreakingDam.cpp:85-139) */
+void or_demanded_mesh_width(const or_patch *p, double threshold,
+                            double minimalWidth, double maximalWidth,
+                            double out[2]) {
+  if (la_equals(minimalWidth, maximalWidth)) {
+    out[0] = out[1] = minimalWidth;
+    return;
+  }
+  double max_gradient = 0.0;
+  for (int yi = 0; yi < p->ny - 1; yi++)
+    for (int xi = 0; xi < p->nx - 1; xi++) {
+      double q0 = p->unew[or_unew_index(p, 0, xi, yi)];
+      double q0x = (p->unew[or_unew_index(p, 0, xi + 1, yi)] - q0) / p->subcell[0];
+      double q0y = (p->unew[or_unew_index(p, 0, xi, yi + 1)] - q0) / p->subcell[1];
+      max_gradient = fmax(max_gradient,
+                          sqrt(fabs(q0x) * fabs(q0x) + fabs(q0y) * fabs(q0y)));
+    }
+  if (max_gradient > threshold) {
+    out[0] = out[1] = minimalWidth;
+  } else if (max_gradient < threshold) {
+    out[0] = out[1] = maximalWidth;
+  } else {
+    out[0] = p->subcell[0]; out[1] = p->subcell[1];
+  }
+}
+
+/* SolveTimestep::leaveCell (SolveTimestep.cpp:765-786) */
+int or_refinement_decision(const double subcell[2], const double demanded[2]) {
+  if (la_one_greater(subcell, demanded)) return 1;
+  double s3[2] = {subcell[0] * 3.0, subcell[1] * 3.0};
+  if (!la_one_greater(s3, demanded)) return 0;
+  return 2;
+}
+
+/* ------------------------------------------------------------------ */
+/* Global-dt drivers                                                    */
+/* ------------------------------------------------------------------ */
+/* Edge speeds only (used for the global dt): identical function of the
+ * edge's states as the speed output of or_fwave. */
+static double tile_max_speed(int nx, int ny, const double *h, const double *hu,
+                             const double *hv) {
+  const int sy = ny + 2;
+  double m = 0.0;
+  for (int i = 1; i < nx + 2; i++)
+    for (int j = 1; j < ny + 1; j++) {
+      int l = (i - 1) * sy + j, r = i * sy + j;
+      m = dmax(m, or_fwave_speed(h[l], h[r], hu[l], hu[r]));
+    }
+  for (int i = 1; i < nx + 1; i++)
+    for (int j = 1; j < ny + 2; j++) {
+      int d = i * sy + (j - 1), u = i * sy + j;
+      m = dmax(m, or_fwave_speed(h[d], h[u], hv[d], hv[u]));
+    }
+  return m;
+}
+
+void or_pool_fill_ghosts(or_pool *pl, const or_ghost_entry *e, long long ne) {
+  int c = pl->cur;
+  for (long long k = 0; k < ne; k++) {
+    long long d = pl->off[e[k].dst] + e[k].dst_cell;
+    long long s = pl->off[e[k].src] + e[k].src_cell;
+    double h = pl->h[c][s], hu = pl->hu[c][s], hv = pl->hv[c][s];
+    pl->h[c][d] = h;
+    pl->hu[c][d] = e[k].negate == 1 ? -hu : hu;
+    pl->hv[c][d] = e[k].negate == 2 ? -hv : hv;
+  }
+}
+
+void or_pool_max_speeds(const or_pool *pl, int state, double *smax) {
+  for (int p = 0; p < pl->np; p++) {
+    long long o = pl->off[p];
+    smax[p] = tile_max_speed(pl->nx[p], pl->ny[p], pl->h[state] + o,
+                             pl->hu[state] + o, pl->hv[state] + o);
+  }
+}
+
+/* One global-dt step over all patches: per patch, the block of
+ * SWE_WaveAccumulationBlock on the current state (ghosts already filled),
+ * dt = min over patches of the per-patch CFL step, clipped by maxDt; new
+ * interior values go to the other state.  Returns dt. */
+double or_pool_step(or_pool *pl, double maxDt, int nthreads) {
+  int c = pl->cur, n = 1 - c;
+  double dtmin = DBL_MAX;
+  double *smax = (double *)malloc(sizeof(double) * (size_t)(pl->np ? pl->np : 1));
+  (void)nthreads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+  for (int p = 0; p < pl->np; p++) {
+    long long o = pl->off[p];
+    smax[p] = tile_max_speed(pl->nx[p], pl->ny[p], pl->h[c] + o, pl->hu[c] + o,
+                             pl->hv[c] + o);
+  }
+  for (int p = 0; p < pl->np; p++)
+    dtmin = dmin(dtmin, or_block_max_timestep(smax[p], pl->dx[p], pl->dy[p]));
+  double dt = fmin(dtmin, maxDt);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+  for (int p = 0; p < pl->np; p++) {
+    int nx = pl->nx[p], ny = pl->ny[p], area = (nx + 2) * (ny + 2);
+    long long o = pl->off[p];
+    double *h = (double *)malloc(sizeof(double) * 6 * (size_t)area);
+    double *hu = h + area, *hv = h + 2 * area, *net = h + 3 * area;
+    memcpy(h, pl->h[c] + o, sizeof(double) * (size_t)area);
+    memcpy(hu, pl->hu[c] + o, sizeof(double) * (size_t)area);
+    memcpy(hv, pl->hv[c] + o, sizeof(double) * (size_t)area);
+    memset(net, 0, sizeof(double) * 3 * (size_t)area);
+    or_block_fluxes(nx, ny, pl->dx[p], pl->dy[p], h, hu, hv, pl->b + o, net,
+                    net + area, net + 2 * area);
+    or_block_update(nx, ny, dt, h, hu, hv, net, net + area, net + 2 * area);
+    for (int i = 1; i <= nx; i++)
+      for (int j = 1; j <= ny; j++) {
+        int k = i * (ny + 2) + j;
+        pl->h[n][o + k] = h[k];
+        pl->hu[n][o + k] = hu[k];
+        pl->hv[n][o + k] = hv[k];
+      }
+    free(h);
+  }
+  free(smax);
+  pl->cur = n;
+  return dt;
+}
+
+/* Uniform grid of px x py patches of n x n cells on a reflective box:
+ * ghost fill = same-level copy (DefaultTransfer at time factor 1) from the
+ * four face neighbours and the four diagonal neighbours, then reflect at
+ * the domain walls in the order left, top, right, bottom
+ * (SolveTimestep.cpp:87-142). */
+void or_grid_fill_ghosts(or_grid *gr) {
+  int n = gr->n, s = n + 2, c = gr->cur;
+  int np = gr->px * gr->py;
+  for (int P = 0; P < np; P++) {
+    int px = P / gr->py, py = P % gr->py;
+    double *H = gr->h[c] + (long long)P * s * s;
+    double *HU = gr->hu[c] + (long long)P * s * s;
+    double *HV = gr->hv[c] + (long long)P * s * s;
+    for (int di = -1; di <= 1; di++)
+      for (int dj = -1; dj <= 1; dj++) {
+        if (!di && !dj) continue;
+        int qx = px + di, qy = py + dj;
+        if (qx < 0 || qy < 0 || qx >= gr->px || qy >= gr->py) continue;
+        long long Q = (long long)qx * gr->py + qy;
+        const double *SH = gr->h[c] + Q * s * s;
+        const double *SHU = gr->hu[c] + Q * s * s;
+        const double *SHV = gr->hv[c] + Q * s * s;
+        int x0 = di < 0 ? 0 : (di > 0 ? n + 1 : 1), x1 = di ? x0 : n;
+        int y0 = dj < 0 ? 0 : (dj > 0 ? n + 1 : 1), y1 = dj ? y0 : n;
+        for (int x = x0; x <= x1; x++)
+          for (int y = y0; y <= y1; y++) {
+            int sx = di < 0 ? n : (di > 0 ? 1 : x);
+            int sy = dj < 0 ? n : (dj > 0 ? 1 : y);
+            H[x * s + y] = SH[sx * s + sy];
+            HU[x * s + y] = SHU[sx * s + sy];
+            HV[x * s + y] = SHV[sx * s + sy];
+          }
+      }
+    /* walls: left (dim0 lower), top (dim1 upper), right, bottom */
+    int wall[4][2] = {{0, 0}, {1, 1}, {0, 1}, {1, 0}};
+    for (int w = 0; w < 4; w++) {
+      int dim = wall[w][0], up = wall[w][1];
+      int onwall = dim == 0 ? (up ? px == gr->px - 1 : px == 0)
+                            : (up ? py == gr->py - 1 : py == 0);
+      if (!onwall) continue;
+      for (int k = 0; k < n + 2; k++) {
+        int dx_, dy_, sx, sy;
+        if (dim == 0) {
+          dx_ = up ? n + 1 : 0; sx = up ? n : 1; dy_ = sy = k;
+        } else {
+          dy_ = up ? n + 1 : 0; sy = up ? n : 1; dx_ = sx = k;
+        }
+        H[dx_ * s + dy_] = H[sx * s + sy];
+        double hu = HU[sx * s + sy], hv = HV[sx * s + sy];
+        HU[dx_ * s + dy_] = dim == 0 ? -hu : hu;
+        HV[dx_ * s + dy_] = dim == 1 ? -hv : hv;
+      }
+    }
+  }
+}
+
+double or_grid_max_speed(const or_grid *gr) {
+  int s = gr->n + 2, np = gr->px * gr->py, c = gr->cur;
+  double m = 0.0;
+  for (int P = 0; P < np; P++) {
+    long long o = (long long)P * s * s;
+    m = dmax(m, tile_max_speed(gr->n, gr->n, gr->h[c] + o, gr->hu[c] + o,
+                               gr->hv[c] + o));
+  }
+  return m;
+}
+
+double or_grid_step(or_grid *gr, double maxDt, double *out_dtcfl) {
+  int np = gr->px * gr->py, s = gr->n + 2;
+  int *nx = (int *)malloc(sizeof(int) * np * 2);
+  long long *off = (long long *)malloc(sizeof(long long) * np);
+  double *dx = (double *)malloc(sizeof(double) * np * 2);
+  for (int p = 0; p < np; p++) {
+    nx[p] = nx[np + p] = gr->n;
+    off[p] = (long long)p * s * s;
+    dx[p] = gr->dx; dx[np + p] = gr->dy;
+  }
+  or_pool pl = {np, nx, nx + np, off, dx, dx + np,
+                {gr->h[0], gr->h[1]}, {gr->hu[0], gr->hu[1]},
+                {gr->hv[0], gr->hv[1]}, gr->b, gr->cur};
+  if (out_dtcfl)
+    *out_dtcfl = or_block_max_timestep(or_grid_max_speed(gr), gr->dx, gr->dy);
+  double dt = or_pool_step(&pl, maxDt, 1);
+  gr->cur = pl.cur;
+  free(nx); free(off); free(dx);
+  return dt;
+}
+
+/* TimeIntervals::resetMinimalFineGridTimeInterval /
+ * updateMinimalFineGridTimeInterval (grid/TimeIntervals.cpp:199-227) and
+ * the switch of the coarse interval to it (Patch::
+ * switchValuesAndTimeIntervalToMinimalFineGridTimeInterval): returns the
+ * coarse (time, timestep) after folding in n fine intervals. */
+void or_minimal_fine_grid_interval(const double *t, const double *dt, int n,
+                                   double out[2]) {
+  double maxT = -1.0, minDt = DBL_MAX;
+  for (int i = 0; i < n; i++) {
+    if (t[i] > maxT) {
+      minDt = minDt - (t[i] - maxT);
+      maxT = t[i];
+    }
+    minDt = dmin(minDt, t[i] + dt[i] - maxT);
+  }
+  out[0] = maxT;
+  out[1] = minDt;
+}
