@@ -1,0 +1,210 @@
+/*
+ * peanoclaw_amd — C ABI of the MI355X (gfx950) shallow-water patch path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.
+ * Every call runs on the HIP stream owned by the pool; calls on one pool are
+ * thread-compatible (not thread-safe), different pools are independent.
+ *
+ * The per-patch entry points mirror peanoclaw::Numerics
+ * (src/peanoclaw/Numerics.h:30-204) as implemented by the SWE plugin
+ * (src/peanoclaw/native/SWEKernel.{h,cpp}) with the default transfer /
+ * interpolation / restriction strategies wired by
+ * NumericsFactory::createSWENumerics (src/peanoclaw/NumericsFactory.cpp:27-51).
+ * A `Patch&` of the reference becomes a slot id returned by
+ * pc_patch_register.  The batched entry points (pc_*_all, pc_graph_*) have
+ * no single counterpart: they run the whole per-step sequence of
+ * mappings/SolveTimestep.cpp:591-751 for every registered patch at once.
+ *
+ * Data layout of the host arrays exchanged with a slot is the reference's
+ * own (grid/LinearizationZYXQ.h:24-82): unknown q slowest, then x, then y
+ * fastest; uNew holds 3*nx*ny values, uOld and b hold (nx+2)*(ny+2) per
+ * field including the ghost layer of width 1.
+ */
+#ifndef PEANOCLAW_AMD_H
+#define PEANOCLAW_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define PC_OK 0
+#define PC_E_INVALID 1   /* bad handle, slot, size or argument            */
+#define PC_E_OOM 2       /* request exceeds the pool's HBM budget          */
+#define PC_E_HIP 3       /* a HIP runtime call failed                      */
+#define PC_E_NONFINITE 4 /* a non-finite value was found in a patch        */
+#define PC_E_NODEVICE 5  /* no usable gfx950 device                        */
+
+typedef struct pc_pool pc_pool;
+
+/* Mirrors the fields of dastgen/CellDescription.def:10-201 that the SWE
+ * path reads or writes. */
+typedef struct {
+  double position[2];
+  double size[2];
+  int32_t subdivision_factor[2];     /* nx, ny (each 2..64)                */
+  int32_t ghostlayer_width;          /* must be 1 (SWEKernel.h:133)        */
+  int32_t level;
+  int32_t is_virtual;
+  int32_t scenario;                  /* independent problem id (>= 0)      */
+  double time;
+  double timestep_size;
+  double estimated_next_timestep_size;
+  double minimal_neighbor_time_constraint;
+  double demanded_mesh_width[2];
+  /* neighbouring ghost-layer bounds of a virtual patch
+   * (CellDescription restrictionLowerBounds / restrictionUpperBounds,
+   * Patch.cpp:916-941), used by restrictSolution(onlyOverlapped). */
+  double restriction_lower_bounds[2];
+  double restriction_upper_bounds[2];
+} pc_cell_description;
+
+/* Error text of the last failing call on this thread. */
+const char *pc_last_error_string(void);
+
+/* ---- pool lifecycle (replaces Peano's DataHeap/CellDescriptionHeap) ---- */
+int pc_pool_create(int device, size_t hbm_budget_bytes, pc_pool **out);
+int pc_pool_destroy(pc_pool *pool);
+int pc_pool_bytes_in_use(pc_pool *pool, size_t *out);
+int pc_pool_synchronize(pc_pool *pool);
+
+/* Register a patch; returns its slot (replaces CellDescription.uIndex,
+ * Patch.cpp:742-769).  Data starts zeroed like the reference's heap vector. */
+int pc_patch_register(pc_pool *pool, const pc_cell_description *desc,
+                      int *out_slot);
+int pc_patch_release(pc_pool *pool, int slot);
+int pc_patch_get_description(pc_pool *pool, int slot, pc_cell_description *out);
+int pc_patch_set_description(pc_pool *pool, int slot,
+                             const pc_cell_description *desc);
+/* Host <-> device copies of a slot's arrays (any pointer may be NULL). */
+int pc_patch_upload(pc_pool *pool, int slot, const double *unew,
+                    const double *uold_with_ghost, const double *b_with_ghost);
+int pc_patch_download(pc_pool *pool, int slot, double *unew,
+                      double *uold_with_ghost, double *b_with_ghost);
+
+/* Bulk access for large pools: the tile offset of a slot in the field
+ * arrays, the state (0/1) holding a scenario's current values, and a copy of
+ * cells [begin, begin+count) of state `state` (and of b when b != NULL)
+ * between host arrays and the pool (to_device = 1 uploads). */
+int pc_patch_offset(pc_pool *pool, int slot, long long *offset);
+int pc_scenario_current_state(pc_pool *pool, int scenario, int *state);
+int pc_pool_copy_fields(pc_pool *pool, int to_device, int state,
+                        long long begin, long long count, double *h,
+                        double *hu, double *hv, double *b);
+
+/* ---- Numerics-mirroring per-patch calls ---- */
+/* Numerics::solveTimestep (Numerics.h:147-152; SWEKernel.cpp:125-167).
+ * boundary_flags[4] is accepted for interface parity and unused, as in the
+ * reference.  Outputs the applied dt and estimatedNextTimestepSize. */
+int pc_solve_timestep(pc_pool *pool, int slot, double maximum_timestep_size,
+                      const int boundary_flags[4], double *out_dt,
+                      double *out_estimated_next_dt);
+/* Numerics::fillBoundaryLayer (Numerics.h:131-135; SWEKernel.cpp:176-237) */
+int pc_fill_boundary_layer(pc_pool *pool, int slot, int dimension,
+                           int set_upper);
+/* Numerics::transferGhostlayer (Numerics.h:64-70; DefaultTransfer.cpph:15-101) */
+int pc_transfer_ghostlayer(pc_pool *pool, const int size[2],
+                           const int source_offset[2],
+                           const int destination_offset[2], int source_slot,
+                           int destination_slot);
+/* Numerics::interpolateSolution (Numerics.h:77-85; SWEKernel.cpp:239-286 over
+ * DefaultInterpolation.cpph:29-270) */
+int pc_interpolate_solution(pc_pool *pool, const int destination_size[2],
+                            const int destination_offset[2], int source_slot,
+                            int destination_slot, int interpolate_to_uold,
+                            int interpolate_to_current_time,
+                            int use_time_unew_or_uold);
+/* Numerics::restrictSolution (Numerics.h:92-96; SWEKernel.cpp:288-307 over
+ * DefaultRestriction.cpph:108-150).  The destination must be virtual. */
+int pc_restrict_solution(pc_pool *pool, int source_slot, int destination_slot,
+                         int restrict_only_overlapped_regions);
+/* Numerics::postProcessRestriction (Numerics.h:103-106; SWEKernel.cpp:309-316) */
+int pc_post_process_restriction(pc_pool *pool, int destination_slot,
+                                int restrict_only_overlapped_regions);
+/* DefaultTransfer::copyUNewToUOld / swapUNewAndUOld (DefaultTransfer.cpph:103-127) */
+int pc_copy_unew_to_uold(pc_pool *pool, int slot);
+int pc_swap_unew_and_uold(pc_pool *pool, int slot);
+/* Numerics::getDemandedMeshWidth -> BreakingDamSWEScenario::
+ * computeDemandedMeshWidth (BreakingDam.cpp:85-139) with the gradient
+ * threshold and minimal/maximal widths of the scenario. */
+int pc_demanded_mesh_width(pc_pool *pool, int slot, double threshold,
+                           double minimal_width, double maximal_width,
+                           double out[2]);
+/* Numerics constants (SWEKernel.h:124-133) */
+int pc_get_number_of_unknowns_per_cell(void);
+int pc_get_number_of_parameter_fields_without_ghostlayer(void);
+int pc_get_number_of_parameter_fields_with_ghostlayer(void);
+int pc_get_ghostlayer_width(void);
+
+/* ---- batched step (all patches, one scenario clock per scenario) ---- */
+/* One ghost strip: `length` cells copied from source cells
+ * (src_slot, src_cell + k*src_stride) to (dst_slot, dst_cell + k*dst_stride),
+ * cell indices into the slot's (nx+2)*(ny+2) tile; negate is a bit mask:
+ * bit 0 negates hu, bit 1 negates hv (reflective walls; a corner reflected
+ * by two walls has both).  Strips must be independent (no strip reads a
+ * ghost cell written by another).                                         */
+typedef struct {
+  int32_t dst_slot, dst_cell, dst_stride;
+  int32_t src_slot, src_cell, src_stride;
+  int32_t length, negate;
+} pc_ghost_strip;
+/* Replaces the device neighbour table; called only when the grid changes. */
+int pc_set_ghost_strips(pc_pool *pool, const pc_ghost_strip *strips, int n);
+/* Ghost strips of a px x py grid of equal-size patches (slot_of[i*py + j]
+ * is patch (i, j), i along x; -1 marks a hole).  Sides facing a hole or the
+ * border are reflective walls.  The strips reproduce the reference's fill
+ * order: same-level transfers from the 8 neighbours (GhostLayerCompositor,
+ * Vertex.cpp:61-145), then walls left, top, right, bottom including the
+ * corner cells (SolveTimestep.cpp:87-142).  Writes up to `cap` strips to
+ * `out` and their number to *n_out (call with out = NULL to count). */
+int pc_grid_strips(pc_pool *pool, int px, int py, const int *slot_of,
+                   pc_ghost_strip *out, int cap, int *n_out);
+/* Per scenario clock and global end time; every step uses
+ * maximumTimestepSize = end_time - (time + timestep_size)
+ * (SolveTimestep.cpp:623-625); a scenario whose bound is <= 0 does not
+ * advance. */
+int pc_scenario_set(pc_pool *pool, int scenario, double time,
+                    double timestep_size, double end_time);
+int pc_scenario_get(pc_pool *pool, int scenario, double *time,
+                    double *timestep_size, double *estimated_next_dt,
+                    long long *steps);
+/* Execution options: tasks per launch (0 = one task per workgroup,
+ * > 0 = persistent grid of that many workgroups), and whether the next-step
+ * edge speeds are computed inside the update kernel ("fused"). */
+int pc_set_launch_config(pc_pool *pool, int persistent_workgroups, int fused,
+                         int patches_per_task_cap);
+/* Run k global-dt steps of every scenario that has patches. */
+int pc_step_all(pc_pool *pool, int k);
+/* Plain per-patch-launch form of the same step (reference call pattern:
+ * ghost fill, per-patch CFL estimate, per-patch update, one launch each). */
+int pc_step_all_per_patch(pc_pool *pool, int k);
+/* Capture one step as a hipGraph and replay it k times. */
+int pc_graph_capture(pc_pool *pool);
+int pc_graph_replay(pc_pool *pool, int k);
+/* Refinement flags of all patches (1 refine, 0 coarsening allowed, 2 keep),
+ * SolveTimestep::leaveCell (SolveTimestep.cpp:753-797). */
+int pc_refinement_flags(pc_pool *pool, double threshold, double minimal_width,
+                        double maximal_width, int32_t *flags_out);
+/* Total water volume sum(h)*dx*dy of one scenario (compensated sum). */
+int pc_scenario_mass(pc_pool *pool, int scenario, double *out);
+/* Device time (ms) of the last pc_step_all / pc_graph_replay call measured
+ * with HIP events on the pool stream, and the update kernel's share. */
+int pc_last_timing(pc_pool *pool, double *total_ms, double *update_ms);
+
+/* Diagnostic: run the device f-wave edge solver on n host edges
+ * in[6n] = (hL, hR, huL, huR, bL, bR) -> out[5n] = (dhL, dhR, dqL, dqR,
+ * speed); the per-edge computeNetUpdates call of
+ * tools/SWE_WaveAccumulationBlock.cpp:154-159. */
+int pc_edge_batch(pc_pool *pool, long long n, const double *in, double *out);
+
+/* Raw device pointers of the pool (for zero-copy interop); state = 0/1. */
+int pc_pool_device_fields(pc_pool *pool, int state, double **h, double **hu,
+                          double **hv, double **b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

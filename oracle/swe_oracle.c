@@ -869,8 +869,8 @@ void or_pool_fill_ghosts(or_pool *pl, const or_ghost_entry *e, long long ne) {
     long long s = pl->off[e[k].src] + e[k].src_cell;
     double h = pl->h[c][s], hu = pl->hu[c][s], hv = pl->hv[c][s];
     pl->h[c][d] = h;
-    pl->hu[c][d] = e[k].negate == 1 ? -hu : hu;
-    pl->hv[c][d] = e[k].negate == 2 ? -hv : hv;
+    pl->hu[c][d] = (e[k].negate & 1) ? -hu : hu;
+    pl->hv[c][d] = (e[k].negate & 2) ? -hv : hv;
   }
 }
 
@@ -936,23 +936,26 @@ double or_pool_step(or_pool *pl, double maxDt, int nthreads) {
  * four face neighbours and the four diagonal neighbours, then reflect at
  * the domain walls in the order left, top, right, bottom
  * (SolveTimestep.cpp:87-142). */
-void or_grid_fill_ghosts(or_grid *gr) {
-  int n = gr->n, s = n + 2, c = gr->cur;
-  int np = gr->px * gr->py;
-  for (int P = 0; P < np; P++) {
-    int px = P / gr->py, py = P % gr->py;
-    double *H = gr->h[c] + (long long)P * s * s;
-    double *HU = gr->hu[c] + (long long)P * s * s;
-    double *HV = gr->hv[c] + (long long)P * s * s;
+/* Same, for a px x py grid whose patch (i, j) has pool offset
+ * off[slot_of[i*py+j]] (slot_of < 0: hole; sides facing a hole are walls). */
+void or_grid_fill(int gpx, int gpy, int n, const int *slot_of,
+                  const long long *off, double *h, double *hu, double *hv) {
+  int s = n + 2;
+  for (int P = 0; P < gpx * gpy; P++) {
+    if (slot_of[P] < 0) continue;
+    int px = P / gpy, py = P % gpy;
+    double *H = h + off[slot_of[P]];
+    double *HU = hu + off[slot_of[P]];
+    double *HV = hv + off[slot_of[P]];
+#define OR_AT(i, j) (((i) < 0 || (j) < 0 || (i) >= gpx || (j) >= gpy) ? -1 : slot_of[(i) * gpy + (j)])
     for (int di = -1; di <= 1; di++)
       for (int dj = -1; dj <= 1; dj++) {
         if (!di && !dj) continue;
-        int qx = px + di, qy = py + dj;
-        if (qx < 0 || qy < 0 || qx >= gr->px || qy >= gr->py) continue;
-        long long Q = (long long)qx * gr->py + qy;
-        const double *SH = gr->h[c] + Q * s * s;
-        const double *SHU = gr->hu[c] + Q * s * s;
-        const double *SHV = gr->hv[c] + Q * s * s;
+        int q = OR_AT(px + di, py + dj);
+        if (q < 0) continue;
+        const double *SH = h + off[q];
+        const double *SHU = hu + off[q];
+        const double *SHV = hv + off[q];
         int x0 = di < 0 ? 0 : (di > 0 ? n + 1 : 1), x1 = di ? x0 : n;
         int y0 = dj < 0 ? 0 : (dj > 0 ? n + 1 : 1), y1 = dj ? y0 : n;
         for (int x = x0; x <= x1; x++)
@@ -968,8 +971,8 @@ void or_grid_fill_ghosts(or_grid *gr) {
     int wall[4][2] = {{0, 0}, {1, 1}, {0, 1}, {1, 0}};
     for (int w = 0; w < 4; w++) {
       int dim = wall[w][0], up = wall[w][1];
-      int onwall = dim == 0 ? (up ? px == gr->px - 1 : px == 0)
-                            : (up ? py == gr->py - 1 : py == 0);
+      int onwall = dim == 0 ? OR_AT(px + (up ? 1 : -1), py) < 0
+                            : OR_AT(px, py + (up ? 1 : -1)) < 0;
       if (!onwall) continue;
       for (int k = 0; k < n + 2; k++) {
         int dx_, dy_, sx, sy;
@@ -984,7 +987,22 @@ void or_grid_fill_ghosts(or_grid *gr) {
         HV[dx_ * s + dy_] = dim == 1 ? -hv : hv;
       }
     }
+#undef OR_AT
   }
+}
+
+void or_grid_fill_ghosts(or_grid *gr) {
+  int np = gr->px * gr->py, s = gr->n + 2;
+  int *slot = (int *)malloc(sizeof(int) * np);
+  long long *off = (long long *)malloc(sizeof(long long) * np);
+  for (int p = 0; p < np; p++) {
+    slot[p] = p;
+    off[p] = (long long)p * s * s;
+  }
+  or_grid_fill(gr->px, gr->py, gr->n, slot, off, gr->h[gr->cur], gr->hu[gr->cur],
+               gr->hv[gr->cur]);
+  free(slot);
+  free(off);
 }
 
 double or_grid_max_speed(const or_grid *gr) {
@@ -1036,4 +1054,11 @@ void or_minimal_fine_grid_interval(const double *t, const double *dt, int n,
   }
   out[0] = maxT;
   out[1] = minDt;
+}
+
+/* or_fwave over n edges: in[6n] = (hL, hR, huL, huR, bL, bR), out[5n]. */
+void or_fwave_batch(long long n, const double *in, double *out) {
+  for (long long e = 0; e < n; e++) or_fwave(in[6 * e], in[6 * e + 1], in[6 * e + 2],
+                                             in[6 * e + 3], in[6 * e + 4], in[6 * e + 5],
+                                             out + 5 * e);
 }

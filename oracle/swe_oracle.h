@@ -72,6 +72,7 @@ void or_patch_init(or_patch *p, int nq, int npw, int nx, int ny, int gw,
 void or_fwave(double hL, double hR, double huL, double huR, double bL, double bR,
               double out[5]); /* dhL, dhR, dqL, dqR, speed */
 double or_fwave_speed(double hL, double hR, double huL, double huR);
+void or_fwave_batch(long long n, const double *in, double *out);
 
 /* ---- SWE block on one (nx+2)x(ny+2) tile, ghost width 1 ---- */
 double or_block_fluxes(int nx, int ny, double dx, double dy,
@@ -152,6 +153,8 @@ typedef struct {
   int cur;
 } or_grid;
 void or_grid_fill_ghosts(or_grid *gr);
+void or_grid_fill(int px, int py, int n, const int *slot_of,
+                  const long long *off, double *h, double *hu, double *hv);
 double or_grid_step(or_grid *gr, double maxDt, double *out_dtcfl);
 double or_grid_max_speed(const or_grid *gr);
 
@@ -162,7 +165,7 @@ double or_grid_max_speed(const or_grid *gr);
 typedef struct {
   int dst, dst_cell;   /* patch id, linear index into the (n+2)^2 tile     */
   int src, src_cell;   /* patch id, linear index into the (n+2)^2 tile     */
-  int negate;          /* 0 none, 1 negate hu, 2 negate hv                  */
+  int negate;          /* bit 0 negates hu, bit 1 negates hv               */
 } or_ghost_entry;
 typedef struct {
   int np;

@@ -1,0 +1,54 @@
+"""Build the gfx950 shared library of the SWE patch path in-tree.
+
+    python -m peanoclaw_amd.build
+
+produces peanoclaw_amd/libpeanoclaw_amd.so (hipcc, --offload-arch=gfx950).
+FP64 arithmetic is compiled with -ffp-contract=off so no multiply-add is
+fused behind the source's back; packed FP32 instructions are disabled.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+SRC = os.path.join(PKG, "csrc", "pc_abi.hip")
+DEPS = [os.path.join(PKG, "csrc", f) for f in
+        ("pc_abi.hip", "pc_kernels.hpp", "pc_patch_kernels.hpp", "swe_device.hpp")] + \
+       [os.path.join(ROOT, "include", "peanoclaw_amd.h")]
+LIB = os.path.join(PKG, "libpeanoclaw_amd.so")
+ARCH = "gfx950"
+
+
+def hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def flags():
+    return [
+        f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+        "-ffp-contract=off", "-fno-fast-math",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-Wall", "-Wno-unused-result",
+    ]
+
+
+def build(verbose=False):
+    tmp = LIB + ".tmp"
+    cmd = [hipcc()] + flags() + ["-o", tmp, SRC]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed for " + SRC)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,1437 @@
+// C ABI of the MI355X SWE patch path (declared in include/peanoclaw_amd.h).
+//
+// Host side of the patch pool: HBM allocation of the structure-of-arrays
+// fields, slot registry mirroring CellDescription, scenario clocks, the
+// ghost-strip table, the work list of the batched update, hipGraph capture
+// of one step, and the per-patch Numerics-mirroring calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/peanoclaw_amd.h"
+#include "pc_kernels.hpp"
+#include "pc_patch_kernels.hpp"
+
+using namespace pcamd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                            \
+  do {                                                                       \
+    hipError_t e_ = (x);                                                     \
+    if (e_ != hipSuccess)                                                    \
+      return fail(PC_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(x)              \
+  do {                      \
+    int r_ = (x);           \
+    if (r_ != PC_OK) return r_; \
+  } while (0)
+
+constexpr int kNumFields = 7;  // h0 h1 hu0 hu1 hv0 hv1 b
+constexpr size_t kBytesPerCell = kNumFields * sizeof(double);
+
+struct Slot {
+  bool live = false;
+  pc_cell_description desc{};
+  long long off = 0;
+  long long cells = 0;
+  bool clock_from_scenario = false;
+};
+
+bool la_greater(double a, double b) { return a - b > kLaEps; }
+
+}  // namespace
+
+struct pc_pool {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t budget = 0;
+  long long cap_cells = 0;
+  long long top = 0;
+  std::vector<std::pair<long long, long long>> free_ranges;
+  double* f[kNumFields] = {};
+  std::vector<Slot> slots;
+  std::vector<ScnDev> scn;  // host mirror, indexed by scenario id
+  bool scn_host_stale = false;
+  bool scn_dev_dirty = true;
+  bool topo_dirty = true;
+  // device arrays
+  PatchDev* d_pd = nullptr;
+  size_t pd_cap = 0;
+  double* d_smax = nullptr;
+  double* d_est = nullptr;
+  ScnDev* d_scn = nullptr;
+  size_t scn_cap = 0;
+  Task* d_tasks = nullptr;
+  int ntasks = 0;
+  std::vector<Task> h_tasks;
+  int* d_plist = nullptr;
+  int* d_plist_scn = nullptr;
+  size_t plist_cap = 0;
+  std::vector<int> h_plist;
+  std::vector<int> h_live;  // live slots (for flags / sums)
+  int* d_live = nullptr;
+  size_t live_cap = 0;
+  GhostStrip* d_strips = nullptr;
+  int nstrips = 0;
+  size_t strips_cap = 0;
+  std::vector<GhostStrip> h_strips;
+  std::vector<int> strip_begin;  // per slot (for the per-patch form)
+  int* d_counter = nullptr;
+  double* d_result = nullptr;
+  double* d_aux = nullptr;
+  size_t aux_cap = 0;
+  int* d_flags = nullptr;
+  size_t flags_cap = 0;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_valid = false;
+  // launch config
+  int persistent_wgs = 0;
+  int fused = 0;
+  int task_cap = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  float last_total_ms = 0.f, last_update_ms = 0.f;
+};
+
+namespace {
+
+Fields fields_of(pc_pool* p) {
+  Fields F;
+  F.h[0] = p->f[0]; F.h[1] = p->f[1];
+  F.hu[0] = p->f[2]; F.hu[1] = p->f[3];
+  F.hv[0] = p->f[4]; F.hv[1] = p->f[5];
+  F.b = p->f[6];
+  return F;
+}
+
+template <class T>
+int ensure_dev(T** ptr, size_t* cap, size_t need) {
+  if (need <= *cap && *ptr) return PC_OK;
+  size_t n = std::max<size_t>(need, std::max<size_t>(16, *cap * 2));
+  if (*ptr) HIPCHK(hipFree(*ptr));
+  *ptr = nullptr;
+  HIPCHK(hipMalloc(reinterpret_cast<void**>(ptr), n * sizeof(T)));
+  *cap = n;
+  return PC_OK;
+}
+
+int invalidate_graph(pc_pool* p) {
+  if (p->gexec) HIPCHK(hipGraphExecDestroy(p->gexec));
+  if (p->graph) HIPCHK(hipGraphDestroy(p->graph));
+  p->gexec = nullptr;
+  p->graph = nullptr;
+  p->graph_valid = false;
+  return PC_OK;
+}
+
+int grow_fields(pc_pool* p, long long need_cells) {
+  if (need_cells <= p->cap_cells) return PC_OK;
+  long long ncap = std::max<long long>(need_cells, p->cap_cells * 2);
+  ncap = std::max<long long>(ncap, 1 << 16);
+  if ((size_t)ncap * kBytesPerCell > p->budget) {
+    ncap = need_cells;
+    if ((size_t)ncap * kBytesPerCell > p->budget)
+      return fail(PC_E_OOM, "patch pool exceeds its HBM budget");
+  }
+  for (int k = 0; k < kNumFields; ++k) {
+    double* nf = nullptr;
+    HIPCHK(hipMalloc(&nf, (size_t)ncap * sizeof(double)));
+    HIPCHK(hipMemsetAsync(nf, 0, (size_t)ncap * sizeof(double), p->stream));
+    if (p->f[k]) {
+      HIPCHK(hipMemcpyAsync(nf, p->f[k], (size_t)p->cap_cells * sizeof(double),
+                            hipMemcpyDeviceToDevice, p->stream));
+      HIPCHK(hipStreamSynchronize(p->stream));
+      HIPCHK(hipFree(p->f[k]));
+    }
+    p->f[k] = nf;
+  }
+  p->cap_cells = ncap;
+  CHK(invalidate_graph(p));
+  return PC_OK;
+}
+
+int alloc_cells(pc_pool* p, long long n, long long* off) {
+  for (size_t i = 0; i < p->free_ranges.size(); ++i) {
+    auto& r = p->free_ranges[i];
+    if (r.second >= n) {
+      *off = r.first;
+      r.first += n;
+      r.second -= n;
+      if (r.second == 0) p->free_ranges.erase(p->free_ranges.begin() + i);
+      return PC_OK;
+    }
+  }
+  CHK(grow_fields(p, p->top + n));
+  *off = p->top;
+  p->top += n;
+  return PC_OK;
+}
+
+void free_cells(pc_pool* p, long long off, long long n) {
+  p->free_ranges.push_back({off, n});
+  std::sort(p->free_ranges.begin(), p->free_ranges.end());
+  std::vector<std::pair<long long, long long>> m;
+  for (auto& r : p->free_ranges) {
+    if (!m.empty() && m.back().first + m.back().second == r.first)
+      m.back().second += r.second;
+    else
+      m.push_back(r);
+  }
+  if (!m.empty() && m.back().first + m.back().second == p->top) {
+    p->top = m.back().first;
+    m.pop_back();
+  }
+  p->free_ranges = m;
+}
+
+bool valid_slot(pc_pool* p, int s) {
+  return p && s >= 0 && s < (int)p->slots.size() && p->slots[s].live;
+}
+
+int sync_scn_host(pc_pool* p) {
+  if (!p->scn_host_stale) return PC_OK;
+  HIPCHK(hipMemcpyAsync(p->scn.data(), p->d_scn, p->scn.size() * sizeof(ScnDev),
+                        hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  p->scn_host_stale = false;
+  return PC_OK;
+}
+
+int upload_scn(pc_pool* p) {
+  CHK(ensure_dev(&p->d_scn, &p->scn_cap, std::max<size_t>(p->scn.size(), 1)));
+  if (!p->scn.empty())
+    HIPCHK(hipMemcpyAsync(p->d_scn, p->scn.data(), p->scn.size() * sizeof(ScnDev),
+                          hipMemcpyHostToDevice, p->stream));
+  p->scn_dev_dirty = false;
+  return PC_OK;
+}
+
+int cur_of(pc_pool* p, int slot) {
+  return p->scn[p->slots[slot].desc.scenario].stepc & 1;
+}
+
+// Clock of a slot: the scenario clock while the batched path drives it.
+void slot_clock(pc_pool* p, int slot, double* t, double* dt) {
+  Slot& s = p->slots[slot];
+  if (s.clock_from_scenario) {
+    const ScnDev& S = p->scn[s.desc.scenario];
+    *t = S.time;
+    *dt = S.dt;
+  } else {
+    *t = s.desc.time;
+    *dt = s.desc.timestep_size;
+  }
+}
+
+// TimeIntervals::getTimeUOld / getTimeUNew (grid/TimeIntervals.cpp:95-115).
+// A virtual patch reports its neighbour interval; the pool keeps that
+// interval in (time, timestep_size) of its description.
+double time_uold(pc_pool* p, int slot) {
+  double t, dt;
+  slot_clock(p, slot, &t, &dt);
+  return t;
+}
+double time_unew(pc_pool* p, int slot) {
+  double t, dt;
+  slot_clock(p, slot, &t, &dt);
+  return t + dt;
+}
+
+PatchDev make_patchdev(const Slot& s) {
+  PatchDev d{};
+  d.off = s.off;
+  d.nx = s.desc.subdivision_factor[0];
+  d.ny = s.desc.subdivision_factor[1];
+  d.scn = s.desc.scenario;
+  d.dx = s.desc.size[0] / (double)d.nx;
+  d.dy = s.desc.size[1] / (double)d.ny;
+  d.dxinv = 1.0 / d.dx;
+  d.dyinv = 1.0 / d.dy;
+  return d;
+}
+
+// Work list: tasks of same-size patches of one scenario, large sizes first.
+int rebuild_topology(pc_pool* p) {
+  if (!p->topo_dirty) return PC_OK;
+  const int ns = (int)p->slots.size();
+  std::vector<PatchDev> pd(std::max(ns, 1));
+  for (int i = 0; i < ns; ++i)
+    if (p->slots[i].live) pd[i] = make_patchdev(p->slots[i]);
+  CHK(ensure_dev(&p->d_pd, &p->pd_cap, pd.size()));
+  HIPCHK(hipMemcpyAsync(p->d_pd, pd.data(), pd.size() * sizeof(PatchDev),
+                        hipMemcpyHostToDevice, p->stream));
+  // per-slot arrays
+  {
+    size_t cap = p->pd_cap;
+    if (p->d_smax) HIPCHK(hipFree(p->d_smax));
+    if (p->d_est) HIPCHK(hipFree(p->d_est));
+    HIPCHK(hipMalloc(&p->d_smax, cap * sizeof(double)));
+    HIPCHK(hipMalloc(&p->d_est, cap * sizeof(double)));
+    HIPCHK(hipMemsetAsync(p->d_smax, 0, cap * sizeof(double), p->stream));
+    HIPCHK(hipMemsetAsync(p->d_est, 0, cap * sizeof(double), p->stream));
+  }
+  // group live slots by (nx, ny, scenario)
+  std::vector<int> live;
+  for (int i = 0; i < ns; ++i)
+    if (p->slots[i].live) live.push_back(i);
+  p->h_live = live;
+  std::vector<int> order = live;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const auto& da = p->slots[a].desc;
+    const auto& db = p->slots[b].desc;
+    long long ca = (long long)da.subdivision_factor[0] * da.subdivision_factor[1];
+    long long cb = (long long)db.subdivision_factor[0] * db.subdivision_factor[1];
+    if (ca != cb) return ca > cb;
+    if (da.subdivision_factor[0] != db.subdivision_factor[0])
+      return da.subdivision_factor[0] > db.subdivision_factor[0];
+    return da.scenario < db.scenario;
+  });
+  p->h_tasks.clear();
+  p->h_plist.clear();
+  size_t i = 0;
+  while (i < order.size()) {
+    const auto& d0 = p->slots[order[i]].desc;
+    const int nx = d0.subdivision_factor[0], ny = d0.subdivision_factor[1];
+    int W = 1;
+    while (W < ny) W <<= 1;
+    int G = std::min(64 / W, 128 / nx);
+    if (p->task_cap > 0) G = std::min(G, p->task_cap);
+    G = std::max(G, 1);
+    Task t;
+    t.first = (int)p->h_plist.size();
+    t.nseg = 0;
+    t.nx = nx;
+    t.ny = ny;
+    while (i < order.size() && t.nseg < G) {
+      const auto& d = p->slots[order[i]].desc;
+      if (d.subdivision_factor[0] != nx || d.subdivision_factor[1] != ny ||
+          d.scenario != d0.scenario)
+        break;
+      p->h_plist.push_back(order[i]);
+      ++t.nseg;
+      ++i;
+    }
+    p->h_tasks.push_back(t);
+  }
+  p->ntasks = (int)p->h_tasks.size();
+  // scenario patch lists
+  int nscn = 0;
+  for (int s : live) nscn = std::max(nscn, p->slots[s].desc.scenario + 1);
+  if ((int)p->scn.size() < nscn) p->scn.resize(nscn);
+  std::vector<int> plist_scn;
+  for (size_t k = 0; k < p->scn.size(); ++k) {
+    p->scn[k].pbeg = (int)plist_scn.size();
+    for (int s : live)
+      if (p->slots[s].desc.scenario == (int)k) plist_scn.push_back(s);
+    p->scn[k].pend = (int)plist_scn.size();
+  }
+  size_t need = std::max<size_t>({p->h_plist.size(), plist_scn.size(), 1});
+  if (need > p->plist_cap) {
+    if (p->d_plist) HIPCHK(hipFree(p->d_plist));
+    if (p->d_plist_scn) HIPCHK(hipFree(p->d_plist_scn));
+    p->plist_cap = need * 2;
+    HIPCHK(hipMalloc(&p->d_plist, p->plist_cap * sizeof(int)));
+    HIPCHK(hipMalloc(&p->d_plist_scn, p->plist_cap * sizeof(int)));
+  }
+  if (!p->h_plist.empty())
+    HIPCHK(hipMemcpyAsync(p->d_plist, p->h_plist.data(), p->h_plist.size() * sizeof(int),
+                          hipMemcpyHostToDevice, p->stream));
+  if (!plist_scn.empty())
+    HIPCHK(hipMemcpyAsync(p->d_plist_scn, plist_scn.data(), plist_scn.size() * sizeof(int),
+                          hipMemcpyHostToDevice, p->stream));
+  if (p->d_tasks) HIPCHK(hipFree(p->d_tasks));
+  HIPCHK(hipMalloc(&p->d_tasks, std::max<size_t>(p->h_tasks.size(), 1) * sizeof(Task)));
+  if (!p->h_tasks.empty())
+    HIPCHK(hipMemcpyAsync(p->d_tasks, p->h_tasks.data(), p->h_tasks.size() * sizeof(Task),
+                          hipMemcpyHostToDevice, p->stream));
+  CHK(ensure_dev(&p->d_live, &p->live_cap, std::max<size_t>(live.size(), 1)));
+  if (!live.empty())
+    HIPCHK(hipMemcpyAsync(p->d_live, live.data(), live.size() * sizeof(int),
+                          hipMemcpyHostToDevice, p->stream));
+  CHK(upload_scn(p));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  p->topo_dirty = false;
+  CHK(invalidate_graph(p));
+  return PC_OK;
+}
+
+int prepare(pc_pool* p) {
+  CHK(sync_scn_host(p));
+  CHK(rebuild_topology(p));
+  if (p->scn_dev_dirty) CHK(upload_scn(p));
+  return PC_OK;
+}
+
+int launch_step(pc_pool* p, bool timed_update) {
+  Fields F = fields_of(p);
+  if (p->nstrips > 0) {
+    hipLaunchKernelGGL(k_gather, dim3((p->nstrips + 3) / 4), dim3(256), 0, p->stream,
+                       p->d_strips, p->nstrips, p->d_pd, F, p->d_scn);
+  }
+  if (p->ntasks > 0) {
+    hipLaunchKernelGGL(k_speed, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
+                       p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, p->d_smax);
+  }
+  const int nscn = (int)p->scn.size();
+  if (nscn > 0) {
+    hipLaunchKernelGGL(k_dt, dim3(nscn), dim3(256), 0, p->stream, p->d_scn,
+                       p->d_plist_scn, p->d_pd, p->d_smax, p->d_est, p->d_counter);
+  }
+  if (p->ntasks > 0) {
+    if (timed_update) HIPCHK(hipEventRecord(p->ev2, p->stream));
+    const int grid = p->persistent_wgs > 0 ? p->persistent_wgs : p->ntasks;
+    hipLaunchKernelGGL(k_update, dim3(grid), dim3(64), 0, p->stream, p->d_tasks,
+                       p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, p->d_counter,
+                       p->persistent_wgs > 0 ? 1 : 0);
+    if (timed_update) HIPCHK(hipEventRecord(p->ev3, p->stream));
+  }
+  HIPCHK(hipGetLastError());
+  return PC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pc_last_error_string(void) { return g_err.c_str(); }
+
+int pc_pool_create(int device, size_t hbm_budget_bytes, pc_pool** out) {
+  if (!out) return fail(PC_E_INVALID, "out is NULL");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0)
+    return fail(PC_E_NODEVICE, "no HIP device with that index");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(PC_E_NODEVICE, std::string("device is ") + prop.gcnArchName +
+                                   ", this library is built for gfx950");
+  HIPCHK(hipSetDevice(device));
+  pc_pool* p = new pc_pool();
+  p->device = device;
+  p->budget = hbm_budget_bytes ? hbm_budget_bytes : (size_t)prop.totalGlobalMem / 10 * 9;
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete p;
+    return fail(PC_E_HIP, "hipStreamCreate failed");
+  }
+  HIPCHK(hipMalloc(&p->d_counter, sizeof(int)));
+  HIPCHK(hipMemsetAsync(p->d_counter, 0, sizeof(int), p->stream));
+  HIPCHK(hipMalloc(&p->d_result, 8 * sizeof(double)));
+  HIPCHK(hipEventCreate(&p->ev0));
+  HIPCHK(hipEventCreate(&p->ev1));
+  HIPCHK(hipEventCreate(&p->ev2));
+  HIPCHK(hipEventCreate(&p->ev3));
+  *out = p;
+  return PC_OK;
+}
+
+int pc_pool_destroy(pc_pool* p) {
+  if (!p) return fail(PC_E_INVALID, "pool is NULL");
+  hipSetDevice(p->device);
+  hipStreamSynchronize(p->stream);
+  invalidate_graph(p);
+  for (int k = 0; k < kNumFields; ++k)
+    if (p->f[k]) hipFree(p->f[k]);
+  void* ptrs[] = {p->d_pd, p->d_smax, p->d_est, p->d_scn, p->d_tasks, p->d_plist,
+                  p->d_plist_scn, p->d_live, p->d_strips, p->d_counter,
+                  p->d_result, p->d_aux, p->d_flags};
+  for (void* q : ptrs)
+    if (q) hipFree(q);
+  hipEventDestroy(p->ev0);
+  hipEventDestroy(p->ev1);
+  hipEventDestroy(p->ev2);
+  hipEventDestroy(p->ev3);
+  hipStreamDestroy(p->stream);
+  delete p;
+  return PC_OK;
+}
+
+int pc_pool_bytes_in_use(pc_pool* p, size_t* out) {
+  if (!p || !out) return fail(PC_E_INVALID, "NULL argument");
+  long long freec = 0;
+  for (auto& r : p->free_ranges) freec += r.second;
+  *out = (size_t)(p->top - freec) * kBytesPerCell;
+  return PC_OK;
+}
+
+int pc_pool_synchronize(pc_pool* p) {
+  if (!p) return fail(PC_E_INVALID, "pool is NULL");
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return PC_OK;
+}
+
+int pc_patch_register(pc_pool* p, const pc_cell_description* d, int* out_slot) {
+  if (!p || !d || !out_slot) return fail(PC_E_INVALID, "NULL argument");
+  const int nx = d->subdivision_factor[0], ny = d->subdivision_factor[1];
+  if (nx < 2 || ny < 2 || nx > 64 || ny > 64)
+    return fail(PC_E_INVALID, "subdivision factor must be in [2, 64]");
+  if (d->ghostlayer_width != 1)
+    return fail(PC_E_INVALID, "the SWE kernel uses ghost layer width 1");
+  if (d->scenario < 0 || d->scenario > (1 << 20))
+    return fail(PC_E_INVALID, "scenario id out of range");
+  if (!(d->size[0] > 0) || !(d->size[1] > 0))
+    return fail(PC_E_INVALID, "patch size must be positive");
+  hipSetDevice(p->device);
+  CHK(sync_scn_host(p));
+  Slot s;
+  s.live = true;
+  s.desc = *d;
+  s.cells = (long long)(nx + 2) * (ny + 2);
+  CHK(alloc_cells(p, s.cells, &s.off));
+  for (int k = 0; k < kNumFields; ++k)
+    HIPCHK(hipMemsetAsync(p->f[k] + s.off, 0, (size_t)s.cells * sizeof(double), p->stream));
+  int slot = -1;
+  for (size_t i = 0; i < p->slots.size(); ++i)
+    if (!p->slots[i].live) {
+      slot = (int)i;
+      break;
+    }
+  if (slot < 0) {
+    slot = (int)p->slots.size();
+    p->slots.push_back(s);
+  } else {
+    p->slots[slot] = s;
+  }
+  if ((int)p->scn.size() <= d->scenario) {
+    size_t old = p->scn.size();
+    p->scn.resize(d->scenario + 1);
+    for (size_t k = old; k < p->scn.size(); ++k) {
+      ScnDev z{};
+      z.end_time = INFINITY;
+      p->scn[k] = z;
+    }
+    p->scn[d->scenario].time = d->time;
+    p->scn[d->scenario].dt = d->timestep_size;
+  }
+  p->scn_dev_dirty = true;
+  p->topo_dirty = true;
+  *out_slot = slot;
+  return PC_OK;
+}
+
+int pc_patch_release(pc_pool* p, int slot) {
+  if (!valid_slot(p, slot)) return fail(PC_E_INVALID, "invalid slot");
+  free_cells(p, p->slots[slot].off, p->slots[slot].cells);
+  p->slots[slot].live = false;
+  p->topo_dirty = true;
+  return PC_OK;
+}
+
+int pc_patch_get_description(pc_pool* p, int slot, pc_cell_description* out) {
+  if (!valid_slot(p, slot) || !out) return fail(PC_E_INVALID, "invalid slot");
+  CHK(sync_scn_host(p));
+  *out = p->slots[slot].desc;
+  if (p->slots[slot].clock_from_scenario) {
+    const ScnDev& S = p->scn[out->scenario];
+    out->time = S.time;
+    out->timestep_size = S.dt;
+    double e = 0.0;
+    if (p->d_est && !p->topo_dirty) {
+      HIPCHK(hipMemcpyAsync(&e, p->d_est + slot, sizeof(double), hipMemcpyDeviceToHost,
+                            p->stream));
+      HIPCHK(hipStreamSynchronize(p->stream));
+    }
+    out->estimated_next_timestep_size = e;
+  }
+  return PC_OK;
+}
+
+int pc_patch_set_description(pc_pool* p, int slot, const pc_cell_description* d) {
+  if (!valid_slot(p, slot) || !d) return fail(PC_E_INVALID, "invalid slot");
+  Slot& s = p->slots[slot];
+  if (d->subdivision_factor[0] != s.desc.subdivision_factor[0] ||
+      d->subdivision_factor[1] != s.desc.subdivision_factor[1] ||
+      d->scenario != s.desc.scenario || d->ghostlayer_width != 1)
+    return fail(PC_E_INVALID, "size, ghost width and scenario are fixed per slot");
+  bool geom = d->size[0] != s.desc.size[0] || d->size[1] != s.desc.size[1];
+  s.desc = *d;
+  s.clock_from_scenario = false;
+  if (geom) p->topo_dirty = true;
+  return PC_OK;
+}
+
+int pc_patch_upload(pc_pool* p, int slot, const double* unew, const double* uold,
+                    const double* b) {
+  if (!valid_slot(p, slot)) return fail(PC_E_INVALID, "invalid slot");
+  hipSetDevice(p->device);
+  CHK(sync_scn_host(p));
+  const Slot& s = p->slots[slot];
+  const int nx = s.desc.subdivision_factor[0], ny = s.desc.subdivision_factor[1];
+  const int cur = cur_of(p, slot);
+  const size_t area = (size_t)(nx + 2) * (ny + 2);
+  if (unew) {
+    for (int q = 0; q < 3; ++q) {
+      double* dst = p->f[2 * q + cur] + s.off + (ny + 2) + 1;
+      HIPCHK(hipMemcpy2DAsync(dst, (ny + 2) * sizeof(double), unew + (size_t)q * nx * ny,
+                              ny * sizeof(double), ny * sizeof(double), nx,
+                              hipMemcpyHostToDevice, p->stream));
+    }
+  }
+  if (uold) {
+    for (int q = 0; q < 3; ++q)
+      HIPCHK(hipMemcpyAsync(p->f[2 * q + 1 - cur] + s.off, uold + q * area,
+                            area * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  }
+  if (b)
+    HIPCHK(hipMemcpyAsync(p->f[6] + s.off, b, area * sizeof(double), hipMemcpyHostToDevice,
+                          p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return PC_OK;
+}
+
+int pc_patch_download(pc_pool* p, int slot, double* unew, double* uold, double* b) {
+  if (!valid_slot(p, slot)) return fail(PC_E_INVALID, "invalid slot");
+  hipSetDevice(p->device);
+  CHK(sync_scn_host(p));
+  const Slot& s = p->slots[slot];
+  const int nx = s.desc.subdivision_factor[0], ny = s.desc.subdivision_factor[1];
+  const int cur = cur_of(p, slot);
+  const size_t area = (size_t)(nx + 2) * (ny + 2);
+  if (unew) {
+    for (int q = 0; q < 3; ++q) {
+      const double* src = p->f[2 * q + cur] + s.off + (ny + 2) + 1;
+      HIPCHK(hipMemcpy2DAsync(unew + (size_t)q * nx * ny, ny * sizeof(double), src,
+                              (ny + 2) * sizeof(double), ny * sizeof(double), nx,
+                              hipMemcpyDeviceToHost, p->stream));
+    }
+  }
+  if (uold) {
+    for (int q = 0; q < 3; ++q)
+      HIPCHK(hipMemcpyAsync(uold + q * area, p->f[2 * q + 1 - cur] + s.off,
+                            area * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  }
+  if (b)
+    HIPCHK(hipMemcpyAsync(b, p->f[6] + s.off, area * sizeof(double), hipMemcpyDeviceToHost,
+                          p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return PC_OK;
+}
+
+int pc_patch_offset(pc_pool* p, int slot, long long* off) {
+  if (!valid_slot(p, slot) || !off) return fail(PC_E_INVALID, "invalid slot");
+  *off = p->slots[slot].off;
+  return PC_OK;
+}
+
+int pc_scenario_current_state(pc_pool* p, int scenario, int* state) {
+  if (!p || !state || scenario < 0) return fail(PC_E_INVALID, "invalid argument");
+  CHK(sync_scn_host(p));
+  if (scenario >= (int)p->scn.size()) return fail(PC_E_INVALID, "unknown scenario");
+  *state = p->scn[scenario].stepc & 1;
+  return PC_OK;
+}
+
+int pc_pool_copy_fields(pc_pool* p, int to_device, int state, long long begin,
+                        long long count, double* h, double* hu, double* hv, double* b) {
+  if (!p || (state != 0 && state != 1) || begin < 0 || count < 0 ||
+      begin + count > p->cap_cells)
+    return fail(PC_E_INVALID, "invalid range");
+  if (count == 0) return PC_OK;
+  hipSetDevice(p->device);
+  double* host[4] = {h, hu, hv, b};
+  double* dev[4] = {p->f[0 + state], p->f[2 + state], p->f[4 + state], p->f[6]};
+  for (int k = 0; k < 4; ++k) {
+    if (!host[k]) continue;
+    if (to_device)
+      HIPCHK(hipMemcpyAsync(dev[k] + begin, host[k], (size_t)count * sizeof(double),
+                            hipMemcpyHostToDevice, p->stream));
+    else
+      HIPCHK(hipMemcpyAsync(host[k], dev[k] + begin, (size_t)count * sizeof(double),
+                            hipMemcpyDeviceToHost, p->stream));
+  }
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return PC_OK;
+}
+
+int pc_get_number_of_unknowns_per_cell(void) { return 3; }
+int pc_get_number_of_parameter_fields_without_ghostlayer(void) { return 0; }
+int pc_get_number_of_parameter_fields_with_ghostlayer(void) { return 1; }
+int pc_get_ghostlayer_width(void) { return 1; }
+
+int pc_solve_timestep(pc_pool* p, int slot, double maxDt, const int flags[4],
+                      double* out_dt, double* out_est) {
+  (void)flags;
+  if (!valid_slot(p, slot)) return fail(PC_E_INVALID, "invalid slot");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  Slot& s = p->slots[slot];
+  if (s.clock_from_scenario) {
+    slot_clock(p, slot, &s.desc.time, &s.desc.timestep_size);
+    s.clock_from_scenario = false;
+  }
+  hipLaunchKernelGGL(k_patch_solve, dim3(1), dim3(64), 0, p->stream, slot, p->d_pd,
+                     fields_of(p), cur_of(p, slot), maxDt, p->d_smax, p->d_result);
+  HIPCHK(hipGetLastError());
+  double res[2];
+  HIPCHK(hipMemcpyAsync(res, p->d_result, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                        p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  // SWEKernel.cpp:83-87
+  if (la_greater(res[0], 0.0)) {
+    s.desc.time = s.desc.time + s.desc.timestep_size;
+    s.desc.timestep_size = res[0];
+  }
+  s.desc.estimated_next_timestep_size = res[1];
+  if (out_dt) *out_dt = res[0];
+  if (out_est) *out_est = res[1];
+  return PC_OK;
+}
+
+int pc_fill_boundary_layer(pc_pool* p, int slot, int dim, int upper) {
+  if (!valid_slot(p, slot) || (dim != 0 && dim != 1))
+    return fail(PC_E_INVALID, "invalid slot or dimension");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  hipLaunchKernelGGL(k_fill_boundary, dim3(1), dim3(128), 0, p->stream, p->d_pd, slot,
+                     fields_of(p), 1 - cur_of(p, slot), dim, upper ? 1 : 0);
+  HIPCHK(hipGetLastError());
+  return PC_OK;
+}
+
+int pc_transfer_ghostlayer(pc_pool* p, const int size[2], const int so[2],
+                           const int dof[2], int src, int dst) {
+  if (!valid_slot(p, src) || !valid_slot(p, dst) || !size || !so || !dof)
+    return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  const auto& sd = p->slots[src].desc;
+  const auto& dd = p->slots[dst].desc;
+  for (int d = 0; d < 2; ++d) {
+    if (so[d] < 0 || so[d] + size[d] > sd.subdivision_factor[d] || size[d] < 0 ||
+        dof[d] < -1 || dof[d] + size[d] > dd.subdivision_factor[d] + 1)
+      return fail(PC_E_INVALID, "transfer region out of bounds");
+  }
+  // DefaultTransfer.cpph:27-51
+  double tf;
+  const double dstTimeUNew = time_unew(p, dst);
+  if (sd.is_virtual) {
+    tf = (dstTimeUNew - 0.0) / 1.0;
+  } else {
+    const double sNew = time_unew(p, src), sOld = time_uold(p, src);
+    if (la_greater(sNew - sOld, 0.0))
+      tf = (dstTimeUNew - sOld) / (sNew - sOld);
+    else
+      tf = 1.0;
+  }
+  const double inv = 1.0 - tf;
+  const int n = size[0] * size[1];
+  if (n == 0) return PC_OK;
+  hipLaunchKernelGGL(k_transfer, dim3((n + 255) / 256), dim3(256), 0, p->stream, p->d_pd,
+                     src, dst, fields_of(p), cur_of(p, src), cur_of(p, dst), size[0], size[1],
+                     so[0], so[1], dof[0], dof[1], tf, inv);
+  HIPCHK(hipGetLastError());
+  return PC_OK;
+}
+
+static int transform(pc_pool* p, int slot, const int off[2], const int size[2], int uold,
+                     int to_above_sea_floor) {
+  const int n = size[0] * size[1];
+  if (n <= 0) return PC_OK;
+  const int cur = cur_of(p, slot);
+  hipLaunchKernelGGL(k_transform, dim3((n + 255) / 256), dim3(256), 0, p->stream, p->d_pd,
+                     slot, fields_of(p), uold ? 1 - cur : cur, off[0], off[1], size[0],
+                     size[1], to_above_sea_floor ? -1.0 : 1.0);
+  HIPCHK(hipGetLastError());
+  return PC_OK;
+}
+
+// Region::mapToPatch (Region.cpp:174-206)
+static void map_to_patch(const int off[2], const int size[2], const PatchDev& src,
+                         const pc_cell_description& sd, const PatchDev& dst,
+                         const pc_cell_description& dd, double eps, int o[2], int s[2]) {
+  const double ssub[2] = {src.dx, src.dy}, dsub[2] = {dst.dx, dst.dy};
+  for (int d = 0; d < 2; ++d) {
+    double position = ssub[d] * (double)off[d];
+    position += sd.position[d] - dd.position[d];
+    double ot = position + eps;
+    ot /= dsub[d];
+    o[d] = (int)ot;
+    double sz = ssub[d] * (double)(off[d] + size[d]);
+    sz += sd.position[d] - dd.position[d];
+    double st = sz - eps;
+    st /= dsub[d];
+    st -= (double)o[d];
+    st += 1.0;
+    s[d] = (int)st;
+  }
+}
+
+int pc_interpolate_solution(pc_pool* p, const int dsize[2], const int doff[2], int src,
+                            int dst, int toUOld, int toCurrentTime, int useTimeUNewOrUOld) {
+  if (!valid_slot(p, src) || !valid_slot(p, dst) || !dsize || !doff)
+    return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  const auto& sd = p->slots[src].desc;
+  const auto& dd = p->slots[dst].desc;
+  const PatchDev S = make_patchdev(p->slots[src]), D = make_patchdev(p->slots[dst]);
+  for (int d = 0; d < 2; ++d)
+    if (doff[d] < -1 || dsize[d] < 0 || doff[d] + dsize[d] > dd.subdivision_factor[d] + 1 ||
+        (!toUOld && (doff[d] < 0 || doff[d] + dsize[d] > dd.subdivision_factor[d])))
+      return fail(PC_E_INVALID, "interpolation region out of bounds");
+  // SWEKernel.cpp:252-263: source region grown by one cell
+  int so[2], ss[2];
+  map_to_patch(doff, dsize, D, dd, S, sd, 1e-12, so, ss);
+  const int sn[2] = {sd.subdivision_factor[0], sd.subdivision_factor[1]};
+  for (int d = 0; d < 2; ++d) {
+    if (so[d] > 0) {
+      so[d] = so[d] - 1;
+      ss[d] = std::min(sn[d], ss[d] + 2);
+    } else {
+      ss[d] = std::min(sn[d], ss[d] + 1);
+    }
+    if (so[d] < 0 || so[d] + ss[d] > sn[d])
+      return fail(PC_E_INVALID, "destination region is not covered by the source patch");
+  }
+  // DefaultInterpolation.cpph:45-67 time factor
+  double st, sdt, dt_, ddt;
+  slot_clock(p, src, &st, &sdt);
+  slot_clock(p, dst, &dt_, &ddt);
+  double tf;
+  if (std::fabs(sdt - 0.0) <= kLaEps) {
+    tf = 1.0;
+  } else if (useTimeUNewOrUOld) {
+    const double sOld = time_uold(p, src), sNew = time_unew(p, src);
+    tf = toCurrentTime ? (time_uold(p, dst) - sOld) / (sNew - sOld)
+                       : (time_unew(p, dst) - sOld) / (sNew - sOld);
+  } else {
+    tf = toCurrentTime ? (dt_ - st) / (st + sdt - st) : (dt_ + ddt - st) / (st + sdt - st);
+  }
+  tf = std::max(0.0, std::min(1.0, tf));
+  CHK(transform(p, src, so, ss, 1, 0));
+  CHK(transform(p, src, so, ss, 0, 0));
+  InterpArgs a{};
+  a.srcSlot = src; a.dstSlot = dst;
+  a.srcCur = cur_of(p, src); a.dstCur = cur_of(p, dst);
+  a.sx = dsize[0]; a.sy = dsize[1]; a.ox = doff[0]; a.oy = doff[1];
+  a.toUOld = toUOld ? 1 : 0;
+  a.tf = tf;
+  a.spos[0] = sd.position[0]; a.spos[1] = sd.position[1];
+  a.ssub[0] = S.dx; a.ssub[1] = S.dy;
+  a.sinv[0] = 1.0 / S.dx; a.sinv[1] = 1.0 / S.dy;
+  a.dpos[0] = dd.position[0]; a.dpos[1] = dd.position[1];
+  a.dsub[0] = D.dx; a.dsub[1] = D.dy;
+  a.sn[0] = sn[0]; a.sn[1] = sn[1];
+  const int n = dsize[0] * dsize[1];
+  if (n > 0) {
+    hipLaunchKernelGGL(k_interpolate, dim3((n + 127) / 128), dim3(128), 0, p->stream,
+                       p->d_pd, fields_of(p), a);
+    HIPCHK(hipGetLastError());
+  }
+  CHK(transform(p, src, so, ss, 1, 1));
+  CHK(transform(p, src, so, ss, 0, 1));
+  CHK(transform(p, dst, doff, dsize, toUOld, 1));
+  return PC_OK;
+}
+
+// Region::getRegionsOverlappedByNeighboringGhostlayers (Region.cpp:247-315)
+static int overlapped_regions(const double lower[2], const double upper[2],
+                              const double pos[2], const double size[2],
+                              const double sub[2], const int n[2], int off[8], int sz[8]) {
+  auto gt = [](double a, double b) { return a - b > kLaEps; };
+  std::memset(off, 0, 8 * sizeof(int));
+  std::memset(sz, 0, 8 * sizeof(int));
+  const bool oneGreater = gt(upper[0] + sub[0], lower[0]) || gt(upper[1] + sub[1], lower[1]);
+  const bool oneGE = (upper[0] - (pos[0] + size[0]) >= -kLaEps) ||
+                     (upper[1] - (pos[1] + size[1]) >= -kLaEps);
+  const bool allGreater = gt(lower[0], pos[0]) && gt(lower[1], pos[1]);
+  if (oneGreater || oneGE || !allGreater) {
+    sz[0] = n[0];
+    sz[1] = n[1];
+    return 1;
+  }
+  const double eps = 1e-12;
+  int lo[2], hi[2];
+  for (int d = 0; d < 2; ++d) {
+    double dl = (lower[d] - pos[d]) * (1.0 / sub[d]) + eps;
+    double du = (upper[d] - pos[d]) * (1.0 / sub[d]) - eps;
+    dl = std::min((double)2147483647, dl);
+    du = std::max((double)(-2147483647 - 1), du);
+    lo[d] = std::max(std::min((int)std::floor(dl), n[d]), -1);
+    hi[d] = std::max(std::min((int)std::floor(du), n[d]), -1);
+  }
+  for (int d = 0; d < 2; ++d) {
+    const int r0 = 2 * d, r1 = 2 * d + 1;
+    sz[2 * r0 + d] = hi[d] + 1;
+    off[2 * r0 + d] = 0;
+    sz[2 * r1 + d] = n[d] - lo[d];
+    off[2 * r1 + d] = lo[d];
+    for (int i = 0; i < d; ++i) {
+      sz[2 * r0 + i] = -hi[i] - 1 + lo[i];
+      off[2 * r0 + i] = hi[i] + 1;
+      sz[2 * r1 + i] = sz[2 * r0 + i];
+      off[2 * r1 + i] = off[2 * r0 + i];
+    }
+    for (int i = d + 1; i < 2; ++i) {
+      sz[2 * r0 + i] = n[i];
+      off[2 * r0 + i] = 0;
+      sz[2 * r1 + i] = n[i];
+      off[2 * r1 + i] = 0;
+    }
+  }
+  return 4;
+}
+
+int pc_restrict_solution(pc_pool* p, int src, int dst, int onlyOverlapped) {
+  if (!valid_slot(p, src) || !valid_slot(p, dst))
+    return fail(PC_E_INVALID, "invalid slot");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  const auto& sd = p->slots[src].desc;
+  const auto& dd = p->slots[dst].desc;
+  if (!dd.is_virtual) return fail(PC_E_INVALID, "restriction destination must be virtual");
+  const PatchDev S = make_patchdev(p->slots[src]), D = make_patchdev(p->slots[dst]);
+  const int sn[2] = {sd.subdivision_factor[0], sd.subdivision_factor[1]};
+  const int dn[2] = {dd.subdivision_factor[0], dd.subdivision_factor[1]};
+  const int z[2] = {0, 0};
+  // SWEKernel.cpp:294-297
+  CHK(transform(p, src, z, sn, 1, 0));
+  CHK(transform(p, src, z, sn, 0, 0));
+  RestrictArgs a{};
+  a.srcSlot = src; a.dstSlot = dst;
+  a.srcCur = cur_of(p, src); a.dstCur = cur_of(p, dst);
+  a.eps = 1e-12;
+  for (int d = 0; d < 2; ++d) {
+    a.srcSub[d] = sd.size[d] * (1.0 / (double)sn[d]);
+    a.dstSub[d] = dd.size[d] * (1.0 / (double)dn[d]);
+    a.spos[d] = sd.position[d];
+    a.dpos[d] = dd.position[d];
+  }
+  a.dstRegion = a.dstSub[0] * a.dstSub[1];
+  double st, sdt;
+  slot_clock(p, src, &st, &sdt);
+  a.tfOld = 1.0;
+  a.tfNew = 1.0;
+  if (la_greater(sdt, 0.0)) {
+    const double sOld = time_uold(p, src), sNew = time_unew(p, src);
+    a.tfOld = (0.0 - sOld) / (sNew - sOld);
+    a.tfNew = (1.0 - sOld) / (sNew - sOld);
+  }
+  int roff[8], rsz[8];
+  int nr;
+  if (onlyOverlapped) {
+    // DefaultRestriction.cpph:121-142
+    const double spos[2] = {sd.position[0], sd.position[1]};
+    const double ssz[2] = {sd.size[0], sd.size[1]};
+    const double ssub[2] = {S.dx, S.dy};
+    nr = overlapped_regions(dd.restriction_lower_bounds, dd.restriction_upper_bounds, spos,
+                            ssz, ssub, sn, roff, rsz);
+  } else {
+    roff[0] = roff[1] = 0;
+    rsz[0] = sn[0];
+    rsz[1] = sn[1];
+    nr = 1;
+  }
+  for (int r = 0; r < nr; ++r) {
+    if (rsz[2 * r] <= 0 || rsz[2 * r + 1] <= 0) continue;
+    a.rOff[0] = roff[2 * r]; a.rOff[1] = roff[2 * r + 1];
+    a.rSize[0] = rsz[2 * r]; a.rSize[1] = rsz[2 * r + 1];
+    map_to_patch(a.rOff, a.rSize, S, sd, D, dd, a.eps, a.dOff, a.dSize);
+    for (int d = 0; d < 2; ++d)
+      if (a.dOff[d] < 0 || a.dOff[d] + a.dSize[d] > dn[d])
+        return fail(PC_E_INVALID, "restriction source is not inside the destination");
+    const int n = a.dSize[0] * a.dSize[1];
+    if (n > 0) {
+      hipLaunchKernelGGL(k_restrict, dim3((n + 63) / 64), dim3(64), 0, p->stream, p->d_pd,
+                         fields_of(p), a);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  CHK(transform(p, src, z, sn, 1, 1));
+  CHK(transform(p, src, z, sn, 0, 1));
+  return PC_OK;
+}
+
+int pc_post_process_restriction(pc_pool* p, int dst, int onlyOverlapped) {
+  (void)onlyOverlapped;
+  if (!valid_slot(p, dst)) return fail(PC_E_INVALID, "invalid slot");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  const int z[2] = {0, 0};
+  const int dn[2] = {p->slots[dst].desc.subdivision_factor[0],
+                     p->slots[dst].desc.subdivision_factor[1]};
+  CHK(transform(p, dst, z, dn, 1, 1));
+  CHK(transform(p, dst, z, dn, 0, 1));
+  return PC_OK;
+}
+
+int pc_copy_unew_to_uold(pc_pool* p, int slot) {
+  if (!valid_slot(p, slot)) return fail(PC_E_INVALID, "invalid slot");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  hipLaunchKernelGGL(k_copy_swap, dim3(4), dim3(256), 0, p->stream, p->d_pd, slot,
+                     fields_of(p), cur_of(p, slot), 0);
+  HIPCHK(hipGetLastError());
+  return PC_OK;
+}
+
+int pc_swap_unew_and_uold(pc_pool* p, int slot) {
+  if (!valid_slot(p, slot)) return fail(PC_E_INVALID, "invalid slot");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  hipLaunchKernelGGL(k_copy_swap, dim3(4), dim3(256), 0, p->stream, p->d_pd, slot,
+                     fields_of(p), cur_of(p, slot), 1);
+  HIPCHK(hipGetLastError());
+  return PC_OK;
+}
+
+static int refine_flags_slots(pc_pool* p, const std::vector<int>& slots, double thr,
+                              double minW, double maxW, int32_t* flags, double* demanded) {
+  const int n = (int)slots.size();
+  if (n == 0) return PC_OK;
+  CHK(ensure_dev(&p->d_flags, &p->flags_cap, (size_t)n + slots.size()));
+  CHK(ensure_dev(&p->d_aux, &p->aux_cap, 2 * (size_t)n));
+  int* d_sl = p->d_flags + n;
+  HIPCHK(hipMemcpyAsync(d_sl, slots.data(), n * sizeof(int), hipMemcpyHostToDevice,
+                        p->stream));
+  hipLaunchKernelGGL(k_refine_flags, dim3(n), dim3(64), 0, p->stream, d_sl, n, p->d_pd,
+                     fields_of(p), p->d_scn, thr, minW, maxW, p->d_flags, p->d_aux);
+  HIPCHK(hipGetLastError());
+  if (flags)
+    HIPCHK(hipMemcpyAsync(flags, p->d_flags, n * sizeof(int), hipMemcpyDeviceToHost,
+                          p->stream));
+  if (demanded)
+    HIPCHK(hipMemcpyAsync(demanded, p->d_aux, 2 * n * sizeof(double), hipMemcpyDeviceToHost,
+                          p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return PC_OK;
+}
+
+int pc_demanded_mesh_width(pc_pool* p, int slot, double thr, double minW, double maxW,
+                           double out[2]) {
+  if (!valid_slot(p, slot) || !out) return fail(PC_E_INVALID, "invalid slot");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  int32_t f;
+  std::vector<int> one{slot};
+  CHK(refine_flags_slots(p, one, thr, minW, maxW, &f, out));
+  return PC_OK;
+}
+
+int pc_refinement_flags(pc_pool* p, double thr, double minW, double maxW, int32_t* flags) {
+  if (!p || !flags) return fail(PC_E_INVALID, "NULL argument");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  // flags are returned per slot index (dead slots get -1)
+  std::vector<int32_t> f(p->h_live.size());
+  CHK(refine_flags_slots(p, p->h_live, thr, minW, maxW, f.data(), nullptr));
+  for (size_t i = 0; i < p->slots.size(); ++i) flags[i] = -1;
+  for (size_t k = 0; k < p->h_live.size(); ++k) flags[p->h_live[k]] = f[k];
+  return PC_OK;
+}
+
+int pc_scenario_mass(pc_pool* p, int scenario, double* out) {
+  if (!p || !out || scenario < 0) return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  std::vector<int> sl;
+  for (int s : p->h_live)
+    if (p->slots[s].desc.scenario == scenario) sl.push_back(s);
+  const int n = (int)sl.size();
+  *out = 0.0;
+  if (!n) return PC_OK;
+  CHK(ensure_dev(&p->d_flags, &p->flags_cap, (size_t)n));
+  CHK(ensure_dev(&p->d_aux, &p->aux_cap, (size_t)n));
+  HIPCHK(hipMemcpyAsync(p->d_flags, sl.data(), n * sizeof(int), hipMemcpyHostToDevice,
+                        p->stream));
+  hipLaunchKernelGGL(k_patch_sums, dim3(n), dim3(64), 0, p->stream, p->d_flags, n, p->d_pd,
+                     fields_of(p), p->d_scn, p->d_aux);
+  HIPCHK(hipGetLastError());
+  std::vector<double> sums(n);
+  HIPCHK(hipMemcpyAsync(sums.data(), p->d_aux, n * sizeof(double), hipMemcpyDeviceToHost,
+                        p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  // Neumaier summation of the per-patch volumes
+  double s = 0.0, c = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const PatchDev d = make_patchdev(p->slots[sl[k]]);
+    const double v = sums[k] * d.dx * d.dy;
+    const double t = s + v;
+    c += std::fabs(s) >= std::fabs(v) ? (s - t) + v : (v - t) + s;
+    s = t;
+  }
+  *out = s + c;
+  return PC_OK;
+}
+
+int pc_set_ghost_strips(pc_pool* p, const pc_ghost_strip* strips, int n) {
+  if (!p || n < 0 || (n > 0 && !strips)) return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(p->device);
+  std::vector<GhostStrip> v(n);
+  for (int i = 0; i < n; ++i) {
+    const pc_ghost_strip& g = strips[i];
+    if (!valid_slot(p, g.dst_slot) || !valid_slot(p, g.src_slot) || g.length < 0 ||
+        g.negate < 0 || g.negate > 3)
+      return fail(PC_E_INVALID, "invalid ghost strip");
+    const auto& dd = p->slots[g.dst_slot].desc;
+    const auto& sd = p->slots[g.src_slot].desc;
+    if (dd.scenario != sd.scenario)
+      return fail(PC_E_INVALID, "ghost strips must stay within one scenario");
+    const long long dcells = (long long)(dd.subdivision_factor[0] + 2) * (dd.subdivision_factor[1] + 2);
+    const long long scells = (long long)(sd.subdivision_factor[0] + 2) * (sd.subdivision_factor[1] + 2);
+    if (g.length > 0) {
+      const long long d0 = g.dst_cell, d1 = g.dst_cell + (long long)(g.length - 1) * g.dst_stride;
+      const long long s0 = g.src_cell, s1 = g.src_cell + (long long)(g.length - 1) * g.src_stride;
+      if (std::min(d0, d1) < 0 || std::max(d0, d1) >= dcells || std::min(s0, s1) < 0 ||
+          std::max(s0, s1) >= scells)
+        return fail(PC_E_INVALID, "ghost strip leaves its tile");
+    }
+    v[i] = GhostStrip{g.dst_slot, g.dst_cell, g.dst_stride, g.src_slot, g.src_cell,
+                      g.src_stride, g.length, g.negate};
+  }
+  std::stable_sort(v.begin(), v.end(),
+                   [](const GhostStrip& a, const GhostStrip& b) { return a.dst_slot < b.dst_slot; });
+  p->h_strips = v;
+  p->strip_begin.assign(p->slots.size() + 1, 0);
+  for (auto& g : v) p->strip_begin[g.dst_slot + 1]++;
+  for (size_t i = 1; i < p->strip_begin.size(); ++i) p->strip_begin[i] += p->strip_begin[i - 1];
+  CHK(ensure_dev(&p->d_strips, &p->strips_cap, std::max<size_t>(v.size(), 1)));
+  if (n > 0)
+    HIPCHK(hipMemcpyAsync(p->d_strips, v.data(), v.size() * sizeof(GhostStrip),
+                          hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  p->nstrips = n;
+  CHK(invalidate_graph(p));
+  return PC_OK;
+}
+
+namespace {
+// Symbolic source of one ghost cell: the value it holds after the
+// reference's fill sequence, as (slot, cell x, cell y, negate mask).
+struct GhostSrc {
+  int slot = -1;  // -1: not written (keeps its old value)
+  int x = 0, y = 0;
+  int neg = 0;
+  bool ghost = false;  // refers to a ghost cell of its own patch (stale read)
+};
+}  // namespace
+
+int pc_grid_strips(pc_pool* p, int px, int py, const int* slot_of, pc_ghost_strip* out,
+                   int cap, int* n_out) {
+  if (!p || px <= 0 || py <= 0 || !slot_of || !n_out)
+    return fail(PC_E_INVALID, "invalid argument");
+  int nx = -1, ny = -1;
+  for (int k = 0; k < px * py; ++k) {
+    if (slot_of[k] < 0) continue;
+    if (!valid_slot(p, slot_of[k])) return fail(PC_E_INVALID, "invalid slot in grid");
+    const auto& d = p->slots[slot_of[k]].desc;
+    if (nx < 0) {
+      nx = d.subdivision_factor[0];
+      ny = d.subdivision_factor[1];
+    } else if (nx != d.subdivision_factor[0] || ny != d.subdivision_factor[1]) {
+      return fail(PC_E_INVALID, "grid patches must share their subdivision factor");
+    }
+  }
+  auto at = [&](int i, int j) -> int {
+    if (i < 0 || j < 0 || i >= px || j >= py) return -1;
+    return slot_of[i * py + j];
+  };
+  std::vector<pc_ghost_strip> res;
+  const int sy = ny + 2;
+  auto cell = [&](int x, int y) { return (x + 1) * sy + (y + 1); };
+  for (int i = 0; i < px; ++i)
+    for (int j = 0; j < py; ++j) {
+      const int me = at(i, j);
+      if (me < 0) continue;
+      // ghost ring indexed by tile coordinates
+      std::vector<GhostSrc> g((size_t)(nx + 2) * (ny + 2));
+      auto G = [&](int x, int y) -> GhostSrc& { return g[(size_t)(x + 1) * sy + (y + 1)]; };
+      auto is_ghost = [&](int x, int y) { return x < 0 || y < 0 || x >= nx || y >= ny; };
+      // phase 1: same-level transfers from face and diagonal neighbours
+      for (int di = -1; di <= 1; ++di)
+        for (int dj = -1; dj <= 1; ++dj) {
+          if (!di && !dj) continue;
+          const int nb = at(i + di, j + dj);
+          if (nb < 0) continue;
+          const int x0 = di < 0 ? -1 : (di > 0 ? nx : 0), x1 = di ? x0 : nx - 1;
+          const int y0 = dj < 0 ? -1 : (dj > 0 ? ny : 0), y1 = dj ? y0 : ny - 1;
+          for (int x = x0; x <= x1; ++x)
+            for (int y = y0; y <= y1; ++y) {
+              GhostSrc s;
+              s.slot = nb;
+              s.x = di < 0 ? nx - 1 : (di > 0 ? 0 : x);
+              s.y = dj < 0 ? ny - 1 : (dj > 0 ? 0 : y);
+              G(x, y) = s;
+            }
+        }
+      // phase 2: walls left, top, right, bottom (SolveTimestep.cpp:94-131)
+      const int walls[4][2] = {{0, 0}, {1, 1}, {0, 1}, {1, 0}};
+      for (auto& w : walls) {
+        const int dim = w[0], up = w[1];
+        const bool wall = dim == 0 ? at(i + (up ? 1 : -1), j) < 0 : at(i, j + (up ? 1 : -1)) < 0;
+        if (!wall) continue;
+        const int len = (dim == 0 ? ny : nx) + 2;
+        for (int k = -1; k < len - 1; ++k) {
+          int dx, dy, sx, sy_;
+          if (dim == 0) {
+            dx = up ? nx : -1; sx = dx + (up ? -1 : 1); dy = sy_ = k;
+          } else {
+            dy = up ? ny : -1; sy_ = dy + (up ? -1 : 1); dx = sx = k;
+          }
+          GhostSrc s;
+          if (!is_ghost(sx, sy_)) {
+            s.slot = me; s.x = sx; s.y = sy_;
+          } else if (G(sx, sy_).slot >= 0) {
+            s = G(sx, sy_);
+          } else {
+            s.slot = me; s.x = sx; s.y = sy_; s.ghost = true;
+          }
+          s.neg ^= (dim == 0 ? 1 : 2);
+          G(dx, dy) = s;
+        }
+      }
+      // emit the ring in a fixed order: bottom row, top row, then columns
+      std::vector<std::pair<int, int>> ring;
+      for (int x = -1; x <= nx; ++x) ring.push_back({x, -1});
+      for (int x = -1; x <= nx; ++x) ring.push_back({x, ny});
+      for (int y = 0; y < ny; ++y) ring.push_back({-1, y});
+      for (int y = 0; y < ny; ++y) ring.push_back({nx, y});
+      for (auto& c : ring) {
+        const GhostSrc& s = G(c.first, c.second);
+        if (s.slot < 0 || s.ghost) continue;  // unwritten, or a stale-only read
+        pc_ghost_strip e;
+        e.dst_slot = me;
+        e.dst_cell = cell(c.first, c.second);
+        e.src_slot = s.slot;
+        e.src_cell = cell(s.x, s.y);
+        e.length = 1;
+        e.dst_stride = 0;
+        e.src_stride = 0;
+        e.negate = s.neg;
+        // merge with the previous strip when strides continue
+        if (!res.empty()) {
+          pc_ghost_strip& b = res.back();
+          if (b.dst_slot == e.dst_slot && b.src_slot == e.src_slot && b.negate == e.negate) {
+            const int ds = e.dst_cell - (b.dst_cell + (b.length - 1) * b.dst_stride);
+            const int ss = e.src_cell - (b.src_cell + (b.length - 1) * b.src_stride);
+            if (b.length == 1 && ds != 0) {
+              b.dst_stride = ds;
+              b.src_stride = ss;
+              b.length = 2;
+              continue;
+            }
+            if (b.length > 1 && ds == b.dst_stride && ss == b.src_stride) {
+              b.length += 1;
+              continue;
+            }
+          }
+        }
+        res.push_back(e);
+      }
+    }
+  *n_out = (int)res.size();
+  if (out) {
+    if (cap < (int)res.size()) return fail(PC_E_INVALID, "output capacity too small");
+    std::copy(res.begin(), res.end(), out);
+  }
+  return PC_OK;
+}
+
+int pc_scenario_set(pc_pool* p, int scenario, double time, double dt, double end_time) {
+  if (!p || scenario < 0) return fail(PC_E_INVALID, "invalid scenario");
+  CHK(sync_scn_host(p));
+  if ((int)p->scn.size() <= scenario) {
+    size_t old = p->scn.size();
+    p->scn.resize(scenario + 1);
+    for (size_t k = old; k < p->scn.size(); ++k) {
+      ScnDev z{};
+      z.end_time = INFINITY;
+      p->scn[k] = z;
+    }
+  }
+  p->scn[scenario].time = time;
+  p->scn[scenario].dt = dt;
+  p->scn[scenario].end_time = end_time;
+  p->scn_dev_dirty = true;
+  return PC_OK;
+}
+
+int pc_scenario_get(pc_pool* p, int scenario, double* time, double* dt, double* est,
+                    long long* steps) {
+  if (!p || scenario < 0) return fail(PC_E_INVALID, "invalid scenario");
+  CHK(sync_scn_host(p));
+  if (scenario >= (int)p->scn.size()) return fail(PC_E_INVALID, "unknown scenario");
+  const ScnDev& S = p->scn[scenario];
+  if (time) *time = S.time;
+  if (dt) *dt = S.dt;
+  if (est) *est = S.est;
+  if (steps) *steps = S.steps;
+  return PC_OK;
+}
+
+int pc_set_launch_config(pc_pool* p, int persistent_wgs, int fused, int cap) {
+  if (!p || persistent_wgs < 0 || cap < 0) return fail(PC_E_INVALID, "invalid config");
+  if (fused) return fail(PC_E_INVALID, "the fused next-step speed variant is not built");
+  p->persistent_wgs = persistent_wgs;
+  p->fused = fused;
+  if (cap != p->task_cap) p->topo_dirty = true;
+  p->task_cap = cap;
+  CHK(invalidate_graph(p));
+  return PC_OK;
+}
+
+static void mark_batched_clock(pc_pool* p) {
+  for (int s : p->h_live) p->slots[s].clock_from_scenario = true;
+}
+
+int pc_step_all(pc_pool* p, int k) {
+  if (!p || k < 0) return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  HIPCHK(hipEventRecord(p->ev0, p->stream));
+  for (int i = 0; i < k; ++i) CHK(launch_step(p, i == k - 1));
+  HIPCHK(hipEventRecord(p->ev1, p->stream));
+  HIPCHK(hipEventSynchronize(p->ev1));
+  HIPCHK(hipEventElapsedTime(&p->last_total_ms, p->ev0, p->ev1));
+  if (k > 0 && p->ntasks > 0) HIPCHK(hipEventElapsedTime(&p->last_update_ms, p->ev2, p->ev3));
+  p->scn_host_stale = true;
+  mark_batched_clock(p);
+  return PC_OK;
+}
+
+int pc_step_all_per_patch(pc_pool* p, int k) {
+  if (!p || k < 0) return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  // per-patch tasks (one patch per launch)
+  std::vector<Task> single;
+  for (int s : p->h_live) {
+    const auto& d = p->slots[s].desc;
+    single.push_back(Task{0, 1, d.subdivision_factor[0], d.subdivision_factor[1]});
+  }
+  const int n = (int)single.size();
+  Task* d_single = nullptr;
+  int* d_sl = nullptr;
+  HIPCHK(hipMalloc(&d_single, std::max(n, 1) * sizeof(Task)));
+  HIPCHK(hipMalloc(&d_sl, std::max(n, 1) * sizeof(int)));
+  for (int i = 0; i < n; ++i) single[i].first = i;
+  if (n) {
+    HIPCHK(hipMemcpyAsync(d_single, single.data(), n * sizeof(Task), hipMemcpyHostToDevice,
+                          p->stream));
+    HIPCHK(hipMemcpyAsync(d_sl, p->h_live.data(), n * sizeof(int), hipMemcpyHostToDevice,
+                          p->stream));
+  }
+  Fields F = fields_of(p);
+  HIPCHK(hipEventRecord(p->ev0, p->stream));
+  for (int it = 0; it < k; ++it) {
+    // ghost fill, one launch per destination patch
+    for (int i = 0; i < n; ++i) {
+      const int s = p->h_live[i];
+      if ((size_t)s + 1 >= p->strip_begin.size()) continue;
+      const int b = p->strip_begin[s], e = p->strip_begin[s + 1];
+      if (e > b)
+        hipLaunchKernelGGL(k_gather, dim3((e - b + 3) / 4), dim3(256), 0, p->stream,
+                           p->d_strips + b, e - b, p->d_pd, F, p->d_scn);
+    }
+    // per-patch CFL estimate, one launch per patch
+    for (int i = 0; i < n; ++i)
+      hipLaunchKernelGGL(k_speed, dim3(1), dim3(64), 0, p->stream, d_single + i, 1, d_sl,
+                         p->d_pd, F, p->d_scn, p->d_smax);
+    hipLaunchKernelGGL(k_dt, dim3((int)p->scn.size()), dim3(256), 0, p->stream, p->d_scn,
+                       p->d_plist_scn, p->d_pd, p->d_smax, p->d_est, p->d_counter);
+    // per-patch update, one launch per patch
+    for (int i = 0; i < n; ++i)
+      hipLaunchKernelGGL(k_update, dim3(1), dim3(64), 0, p->stream, d_single + i, 1, d_sl,
+                         p->d_pd, F, p->d_scn, p->d_counter, 0);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(p->ev1, p->stream));
+  HIPCHK(hipEventSynchronize(p->ev1));
+  HIPCHK(hipEventElapsedTime(&p->last_total_ms, p->ev0, p->ev1));
+  p->last_update_ms = 0.f;
+  HIPCHK(hipFree(d_single));
+  HIPCHK(hipFree(d_sl));
+  p->scn_host_stale = true;
+  mark_batched_clock(p);
+  return PC_OK;
+}
+
+int pc_graph_capture(pc_pool* p) {
+  if (!p) return fail(PC_E_INVALID, "pool is NULL");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  CHK(invalidate_graph(p));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+  int rc = launch_step(p, false);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(p->stream, &g);
+  if (rc != PC_OK) return rc;
+  if (e != hipSuccess) return fail(PC_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+  p->graph = g;
+  HIPCHK(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0));
+  p->graph_valid = true;
+  return PC_OK;
+}
+
+int pc_graph_replay(pc_pool* p, int k) {
+  if (!p || k < 0) return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(p->device);
+  if (p->topo_dirty || !p->graph_valid || p->scn_dev_dirty)
+    return fail(PC_E_INVALID, "no valid captured graph (grid or scenarios changed)");
+  CHK(sync_scn_host(p));
+  HIPCHK(hipEventRecord(p->ev0, p->stream));
+  for (int i = 0; i < k; ++i) HIPCHK(hipGraphLaunch(p->gexec, p->stream));
+  HIPCHK(hipEventRecord(p->ev1, p->stream));
+  HIPCHK(hipEventSynchronize(p->ev1));
+  HIPCHK(hipEventElapsedTime(&p->last_total_ms, p->ev0, p->ev1));
+  p->scn_host_stale = true;
+  mark_batched_clock(p);
+  return PC_OK;
+}
+
+int pc_last_timing(pc_pool* p, double* total_ms, double* update_ms) {
+  if (!p) return fail(PC_E_INVALID, "pool is NULL");
+  if (total_ms) *total_ms = p->last_total_ms;
+  if (update_ms) *update_ms = p->last_update_ms;
+  return PC_OK;
+}
+
+int pc_edge_batch(pc_pool* p, long long n, const double* in, double* out) {
+  if (!p || n < 0 || (n > 0 && (!in || !out))) return fail(PC_E_INVALID, "invalid argument");
+  if (n == 0) return PC_OK;
+  hipSetDevice(p->device);
+  double *din = nullptr, *dout = nullptr;
+  HIPCHK(hipMalloc(&din, (size_t)n * 6 * sizeof(double)));
+  HIPCHK(hipMalloc(&dout, (size_t)n * 5 * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(din, in, (size_t)n * 6 * sizeof(double), hipMemcpyHostToDevice,
+                        p->stream));
+  const long long blocks = std::min<long long>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_edge_batch, dim3((unsigned)blocks), dim3(256), 0, p->stream, n, din,
+                     dout);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout, (size_t)n * 5 * sizeof(double), hipMemcpyDeviceToHost,
+                        p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  HIPCHK(hipFree(din));
+  HIPCHK(hipFree(dout));
+  return PC_OK;
+}
+
+int pc_pool_device_fields(pc_pool* p, int state, double** h, double** hu, double** hv,
+                          double** b) {
+  if (!p || (state != 0 && state != 1)) return fail(PC_E_INVALID, "invalid argument");
+  if (h) *h = p->f[0 + state];
+  if (hu) *hu = p->f[2 + state];
+  if (hv) *hv = p->f[4 + state];
+  if (b) *b = p->f[6];
+  return PC_OK;
+}
+
+}  // extern "C"

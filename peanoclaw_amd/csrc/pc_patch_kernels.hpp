@@ -1,0 +1,330 @@
+// Per-patch kernels behind the Numerics-mirroring entry points and the
+// grid-level transfers (reflect, same-level transfer with time factor,
+// d-linear interpolation, area-weighted restriction, h <-> h+b transform,
+// refinement criterion).  All FP64, same operation order as the formulas
+// they restate (file:line given per kernel).
+#pragma once
+
+#include "pc_kernels.hpp"
+
+namespace pcamd {
+
+__device__ __forceinline__ long long tile_index(const PatchDev& P, int x, int y) {
+  // cell (x, y) with x, y in [-1, n]
+  return P.off + (long long)(x + 1) * (P.ny + 2) + (y + 1);
+}
+
+// SWEKernel::solveTimestep on one patch (SWEKernel.cpp:50-88): speed of
+// every edge of uOld, dt = fmin(CFL, maxDt), update, swap.
+__global__ void __launch_bounds__(64)
+k_patch_solve(int slot, const PatchDev* __restrict__ pd, Fields F, int cur,
+              double maxDt, double* __restrict__ smax, double* __restrict__ out) {
+  __shared__ MarchLds L;
+  __shared__ int plist[1];
+  if (threadIdx.x == 0) plist[0] = slot;
+  __syncthreads();
+  const PatchDev P = pd[slot];
+  Task t;
+  t.first = 0; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;
+  march_task<kSpeed>(t, plist, pd, F, 1 - cur, 0.0, smax, L);
+  __syncthreads();
+  const double s = smax[slot];
+  const double cfl = max_timestep(s, P.dx, P.dy);
+  const double dt = fmin(cfl, maxDt);
+  march_task<kUpdateSwap>(t, plist, pd, F, cur, dt, nullptr, L);
+  if (threadIdx.x == 0) {
+    out[0] = dt;
+    out[1] = cfl;
+  }
+}
+
+// SWEKernel::fillBoundaryLayer (SWEKernel.cpp:176-237) on uOld (state st).
+__global__ void k_fill_boundary(const PatchDev* __restrict__ pd, int slot, Fields F,
+                                int st, int dim, int upper) {
+  const PatchDev P = pd[slot];
+  const int n_other = dim == 0 ? P.ny : P.nx;
+  for (int k = -1 + (int)threadIdx.x; k < n_other + 1; k += blockDim.x) {
+    int dx, dy, sx, sy;
+    if (dim == 0) {
+      dx = upper ? P.nx : -1;
+      sx = dx + (upper ? -1 : 1);
+      dy = sy = k;
+    } else {
+      dy = upper ? P.ny : -1;
+      sy = dy + (upper ? -1 : 1);
+      dx = sx = k;
+    }
+    const long long d = tile_index(P, dx, dy), s = tile_index(P, sx, sy);
+    const double h = F.h[st][s], hu = F.hu[st][s], hv = F.hv[st][s];
+    F.h[st][d] = h;
+    F.hu[st][d] = dim == 0 ? -hu : hu;
+    F.hv[st][d] = dim == 1 ? -hv : hv;
+  }
+}
+
+// DefaultTransfer::transferGhostlayer (DefaultTransfer.cpph:15-101).
+__global__ void k_transfer(const PatchDev* __restrict__ pd, int srcSlot, int dstSlot,
+                           Fields F, int srcCur, int dstCur, int sx, int sy,
+                           int sox, int soy, int dox, int doy, double tf,
+                           double inv) {
+  const PatchDev S = pd[srcSlot], D = pd[dstSlot];
+  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < sx * sy;
+       k += blockDim.x * gridDim.x) {
+    const int x = k / sy, y = k - (k / sy) * sy;
+    const long long si = tile_index(S, sox + x, soy + y);
+    const long long di = tile_index(D, dox + x, doy + y);
+    const int so = 1 - srcCur, dOld = 1 - dstCur;
+    F.h[dOld][di] = F.h[srcCur][si] * tf + F.h[so][si] * inv;
+    F.hu[dOld][di] = F.hu[srcCur][si] * tf + F.hu[so][si] * inv;
+    F.hv[dOld][di] = F.hv[srcCur][si] * tf + F.hv[so][si] * inv;
+  }
+}
+
+// copyUNewToUOld (mode 0) / swapUNewAndUOld (mode 1), DefaultTransfer.cpph:103-127
+__global__ void k_copy_swap(const PatchDev* __restrict__ pd, int slot, Fields F,
+                            int cur, int mode) {
+  const PatchDev P = pd[slot];
+  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < P.nx * P.ny;
+       k += blockDim.x * gridDim.x) {
+    const int x = k / P.ny, y = k - (k / P.ny) * P.ny;
+    const long long i = tile_index(P, x, y);
+    double* a[3] = {F.h[cur], F.hu[cur], F.hv[cur]};
+    double* o[3] = {F.h[1 - cur], F.hu[1 - cur], F.hv[1 - cur]};
+    for (int q = 0; q < 3; ++q) {
+      const double vn = a[q][i];
+      if (mode == 1) a[q][i] = o[q][i];
+      o[q][i] = vn;
+    }
+  }
+}
+
+// SWEKernel::transformWaterHeight (SWEKernel.cpp:21-48) over a region.
+__global__ void k_transform(const PatchDev* __restrict__ pd, int slot, Fields F,
+                            int st, int ox, int oy, int sx, int sy, double sign) {
+  const PatchDev P = pd[slot];
+  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < sx * sy;
+       k += blockDim.x * gridDim.x) {
+    const int x = k / sy, y = k - (k / sy) * sy;
+    const long long i = tile_index(P, ox + x, oy + y);
+    F.h[st][i] = F.h[st][i] + sign * F.b[i];
+  }
+}
+
+struct InterpArgs {
+  int srcSlot, dstSlot, srcCur, dstCur;
+  int sx, sy, ox, oy;
+  int toUOld;
+  double tf;
+  double spos[2], ssub[2], sinv[2], dpos[2], dsub[2];
+  int sn[2];
+};
+
+// DefaultInterpolationTemplate::interpolateSolution
+// (DefaultInterpolation.cpph:29-270): one thread per destination cell.
+__global__ void k_interpolate(const PatchDev* __restrict__ pd, Fields F, InterpArgs a) {
+  const PatchDev S = pd[a.srcSlot], D = pd[a.dstSlot];
+  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < a.sx * a.sy;
+       k += blockDim.x * gridDim.x) {
+    const int x = a.ox + k / a.sy, y = a.oy + (k - (k / a.sy) * a.sy);
+    const long long di = tile_index(D, x, y);
+    const int dOld = 1 - a.dstCur;
+    double* dq[3];
+    if (a.toUOld) {
+      dq[0] = F.h[dOld]; dq[1] = F.hu[dOld]; dq[2] = F.hv[dOld];
+    } else {
+      dq[0] = F.h[a.dstCur]; dq[1] = F.hu[a.dstCur]; dq[2] = F.hv[a.dstCur];
+    }
+    F.h[dOld][di] = 0.0;
+    F.hu[dOld][di] = 0.0;
+    F.hv[dOld][di] = 0.0;
+    double c[2];
+    c[0] = a.dpos[0] + a.dsub[0] * (double)x + a.dsub[0] * 0.5;
+    c[1] = a.dpos[1] + a.dsub[1] * (double)y + a.dsub[1] * 0.5;
+    int si[2];
+    for (int d = 0; d < 2; ++d) {
+      const double base = a.spos[d] + a.ssub[d] * 0.5;
+      const double loc = c[d] - base;
+      int v = (int)floor(loc * a.sinv[d]);
+      v = v < a.sn[d] - 2 ? v : a.sn[d] - 2;
+      v = v > 0 ? v : 0;
+      si[d] = v;
+    }
+    const double tf = a.tf;
+    for (int o = 0; o < 4; ++o) {
+      const int off[2] = {o & 1, (o >> 1) & 1};
+      const int nb[2] = {si[0] + off[0], si[1] + off[1]};
+      double np[2];
+      np[0] = a.spos[0] + a.ssub[0] * (double)nb[0] + a.ssub[0] * 0.5;
+      np[1] = a.spos[1] + a.ssub[1] * (double)nb[1] + a.ssub[1] * 0.5;
+      double sf = 1.0;
+      for (int d = 0; d < 2; ++d) {
+        const double sign = (double)(1 - 2 * off[d]);
+        sf *= 1.0 - (sign * (c[d] - np[d])) * a.sinv[d];
+      }
+      const long long si_ = tile_index(S, nb[0], nb[1]);
+      const double* so[3] = {F.h[1 - a.srcCur], F.hu[1 - a.srcCur], F.hv[1 - a.srcCur]};
+      const double* sn[3] = {F.h[a.srcCur], F.hu[a.srcCur], F.hv[a.srcCur]};
+      for (int q = 0; q < 3; ++q) {
+        const double vo = so[q][si_], vn = sn[q][si_];
+        if (a.toUOld)
+          dq[q][di] = dq[q][di] + sf * (vo * (1.0 - tf) + vn * tf);
+        else
+          dq[q][di] = dq[q][di] + sf * vo * (1.0 - tf) + sf * vn * tf;
+      }
+    }
+  }
+}
+
+struct RestrictArgs {
+  int srcSlot, dstSlot, srcCur, dstCur;
+  int rOff[2], rSize[2];  // region in source cells
+  int dOff[2], dSize[2];  // mapped destination region
+  double tfOld, tfNew;
+  double srcSub[2], dstSub[2], dstRegion, eps;
+  double spos[2], dpos[2];
+};
+
+// DefaultRestrictionTemplate::restrictRegion (DefaultRestriction.cpph:16-106):
+// one thread per destination cell, source cells in the reference's order.
+__global__ void k_restrict(const PatchDev* __restrict__ pd, Fields F, RestrictArgs a) {
+  const PatchDev S = pd[a.srcSlot], D = pd[a.dstSlot];
+  const int n = a.dSize[0] * a.dSize[1];
+  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < n;
+       k += blockDim.x * gridDim.x) {
+    const int di0 = a.dOff[0] + k / a.dSize[1];
+    const int di1 = a.dOff[1] + (k - (k / a.dSize[1]) * a.dSize[1]);
+    double dp[2] = {a.dpos[0] + (double)di0 * a.dstSub[0],
+                    a.dpos[1] + (double)di1 * a.dstSub[1]};
+    int cOff[2], cSize[2];
+    for (int d = 0; d < 2; ++d) {
+      double ot = (dp[d] - a.spos[d] + a.eps);
+      ot /= a.srcSub[d];
+      const int o = (int)ot;
+      double st = dp[d] + a.dstSub[d] - a.spos[d] - a.eps;
+      st /= a.srcSub[d];
+      st -= ot;
+      st += 1.0;
+      const int s = (int)st;
+      const int upper = o + s, rUpper = a.rOff[d] + a.rSize[d];
+      cOff[d] = o > a.rOff[d] ? o : a.rOff[d];
+      cSize[d] = (upper < rUpper ? upper : rUpper) - cOff[d];
+    }
+    const long long di = tile_index(D, di0, di1);
+    const int dOldSt = 1 - a.dstCur, sOldSt = 1 - a.srcCur;
+    double accO[3] = {F.h[dOldSt][di], F.hu[dOldSt][di], F.hv[dOldSt][di]};
+    double accN[3] = {F.h[a.dstCur][di], F.hu[a.dstCur][di], F.hv[a.dstCur][di]};
+    for (int sx = 0; sx < cSize[0]; ++sx)
+      for (int sy = 0; sy < cSize[1]; ++sy) {
+        const int s0 = cOff[0] + sx, s1 = cOff[1] + sy;
+        const double sp[2] = {a.spos[0] + (double)s0 * a.srcSub[0],
+                              a.spos[1] + (double)s1 * a.srcSub[1]};
+        double region = 1.0;
+        for (int d = 0; d < 2; ++d) {
+          const double hi = dmin(dp[d] + a.dstSub[d], sp[d] + a.srcSub[d]);
+          const double lo = dmax(dp[d], sp[d]);
+          region *= hi - lo;
+          region = dmax(region, 0.0);
+        }
+        const long long si = tile_index(S, s0, s1);
+        const double vo[3] = {F.h[sOldSt][si], F.hu[sOldSt][si], F.hv[sOldSt][si]};
+        const double vn[3] = {F.h[a.srcCur][si], F.hu[a.srcCur][si], F.hv[a.srcCur][si]};
+        for (int q = 0; q < 3; ++q) {
+          const double dO = (vo[q] * (1.0 - a.tfOld) + vn[q] * a.tfOld) * region / a.dstRegion;
+          const double dN = (vo[q] * (1.0 - a.tfNew) + vn[q] * a.tfNew) * region / a.dstRegion;
+          accO[q] = accO[q] + dO;
+          accN[q] = accN[q] + dN;
+        }
+      }
+    F.h[dOldSt][di] = accO[0]; F.hu[dOldSt][di] = accO[1]; F.hv[dOldSt][di] = accO[2];
+    F.h[a.dstCur][di] = accN[0]; F.hu[a.dstCur][di] = accN[1]; F.hv[a.dstCur][di] = accN[2];
+  }
+}
+
+// BreakingDamSWEScenario::computeDemandedMeshWidth (BreakingDam.cpp:85-139)
+// and SolveTimestep::leaveCell's decision (SolveTimestep.cpp:765-786):
+// one wavefront per patch over uNew.
+__global__ void __launch_bounds__(64)
+k_refine_flags(const int* __restrict__ slots, int nslots, const PatchDev* __restrict__ pd,
+               Fields F, const ScnDev* __restrict__ scn, double threshold,
+               double minW, double maxW, int* __restrict__ flags,
+               double* __restrict__ demanded) {
+  const int k = blockIdx.x;
+  if (k >= nslots) return;
+  const int slot = slots[k];
+  const PatchDev P = pd[slot];
+  const int cur = scn[P.scn].stepc & 1;
+  const double* H = F.h[cur];
+  double g = 0.0;
+  const int nxy = (P.nx - 1) * (P.ny - 1);
+  for (int e = threadIdx.x; e < nxy; e += 64) {
+    const int xi = e / (P.ny - 1), yi = e - (e / (P.ny - 1)) * (P.ny - 1);
+    const double q0 = H[tile_index(P, xi, yi)];
+    const double q0x = (H[tile_index(P, xi + 1, yi)] - q0) / P.dx;
+    const double q0y = (H[tile_index(P, xi, yi + 1)] - q0) / P.dy;
+    g = fmax(g, sqrt(fabs(q0x) * fabs(q0x) + fabs(q0y) * fabs(q0y)));
+  }
+  for (int m = 32; m >= 1; m >>= 1) g = fmax(g, __shfl_xor(g, m));
+  if (threadIdx.x == 0) {
+    double dw0, dw1;
+    if (fabs(minW - maxW) <= kLaEps) {
+      dw0 = dw1 = minW;
+    } else if (g > threshold) {
+      dw0 = dw1 = minW;
+    } else if (g < threshold) {
+      dw0 = dw1 = maxW;
+    } else {
+      dw0 = P.dx; dw1 = P.dy;
+    }
+    int f;
+    if (P.dx - dw0 > kLaEps || P.dy - dw1 > kLaEps) {
+      f = 1;
+    } else if (!((P.dx * 3.0) - dw0 > kLaEps || (P.dy * 3.0) - dw1 > kLaEps)) {
+      f = 0;
+    } else {
+      f = 2;
+    }
+    flags[k] = f;
+    demanded[2 * k] = dw0;
+    demanded[2 * k + 1] = dw1;
+  }
+}
+
+// Per-patch sum of h over the interior of the current state.
+__global__ void __launch_bounds__(64)
+k_patch_sums(const int* __restrict__ slots, int nslots, const PatchDev* __restrict__ pd,
+             Fields F, const ScnDev* __restrict__ scn, double* __restrict__ out) {
+  const int k = blockIdx.x;
+  if (k >= nslots) return;
+  const PatchDev P = pd[slots[k]];
+  const int cur = scn[P.scn].stepc & 1;
+  double s = 0.0, comp = 0.0;
+  for (int e = threadIdx.x; e < P.nx * P.ny; e += 64) {
+    const double v = F.h[cur][tile_index(P, e / P.ny, e % P.ny)];
+    const double t = s + v;
+    comp += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
+    s = t;
+  }
+  s += comp;
+  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+  if (threadIdx.x == 0) out[k] = s;
+}
+
+__global__ void k_reset_counter(int* c) { *c = 0; }
+
+// The edge solver on a batch of independent edges (diagnostics/tests).
+__global__ void k_edge_batch(long long n, const double* __restrict__ in,
+                             double* __restrict__ out) {
+  for (long long e = threadIdx.x + (long long)blockIdx.x * blockDim.x; e < n;
+       e += (long long)blockDim.x * gridDim.x) {
+    const double* v = in + 6 * e;
+    const CellQ L = cell_dir(cell_base(v[0], v[4]), v[2]);
+    const CellQ R = cell_dir(cell_base(v[1], v[5]), v[3]);
+    double dhL, dhR, dqL, dqR;
+    edge_solve(L, R, dhL, dhR, dqL, dqR);
+    double* o = out + 5 * e;
+    o[0] = dhL; o[1] = dhR; o[2] = dqL; o[3] = dqR;
+    o[4] = edge_speed(L, R);
+  }
+}
+
+}  // namespace pcamd

@@ -1,0 +1,150 @@
+// Device-side arithmetic of the SWE patch update for gfx950 (FP64).
+//
+// The f-wave edge solver and the block accumulation follow the reference's
+// call sites (tools/SWE_WaveAccumulationBlock.cpp:142-214) and its update
+// (:247-296).  All arithmetic is FP64, compiled with -ffp-contract=off, so
+// every +,-,*,/ and sqrt is one correctly rounded operation in the order
+// written here; the CPU restatement performs the same operations in the
+// same order, which makes results bitwise reproducible between the two.
+//
+// Work per edge is reduced by computing the per-cell terms once per cell
+// (u = q/h, sqrt(h), sqrt(g h), q u + (g/2 h) h) instead of once per edge
+// side; these are pure functions of the cell's state, so reusing them does
+// not change any result bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace pcamd {
+
+constexpr double kGravity = 9.81;
+constexpr double kHalfG = 0.5 * 9.81;
+constexpr double kDryTol = 0.01;   // f-wave dry tolerance
+constexpr double kZeroTol = 1e-7;  // zero wave-speed tolerance
+constexpr double kCfl = 0.4;       // SWE_WaveAccumulationBlock.cpp:236
+constexpr double kMinSpeed = 0.00001;  // :225
+constexpr double kClampH = 0.1;    // :270 velocity clamp depth
+constexpr double kLaEps = 1e-8;    // tarch::la comparison tolerance
+
+// One cell as seen by one sweep direction (q = normal momentum).
+struct CellQ {
+  double h, q, b;
+  double u, sq, c, f;
+};
+
+__device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return a < b ? b : a; }
+
+// Per-cell terms shared by both sweeps.
+struct CellBase {
+  double h, b, sq, c, p;  // p = (g/2 h) h
+};
+
+__device__ __forceinline__ CellBase cell_base(double h, double b) {
+  CellBase r;
+  r.h = h;
+  r.b = b;
+  r.sq = sqrt(h);
+  r.c = sqrt(kGravity * h);
+  r.p = (kHalfG * h) * h;
+  return r;
+}
+
+__device__ __forceinline__ CellQ cell_dir(const CellBase& cb, double q) {
+  CellQ r;
+  r.h = cb.h;
+  r.q = q;
+  r.b = cb.b;
+  r.u = q / cb.h;
+  r.sq = cb.sq;
+  r.c = cb.c;
+  r.f = q * r.u + cb.p;
+  return r;
+}
+
+// Replace a dry side by the mirror image of the wet side (wall).
+__device__ __forceinline__ void apply_dry(CellQ& L, CellQ& R, bool& leftDry,
+                                          bool& rightDry, bool& bothDry) {
+  const bool lw = L.h >= kDryTol;
+  const bool rw = R.h >= kDryTol;
+  rightDry = lw && !rw;
+  leftDry = !lw && rw;
+  bothDry = !lw && !rw;
+  if (rightDry) {
+    R.h = L.h; R.q = -L.q; R.b = L.b; R.u = -L.u; R.sq = L.sq; R.c = L.c; R.f = L.f;
+  }
+  if (leftDry) {
+    L.h = R.h; L.q = -R.q; L.b = R.b; L.u = -R.u; L.sq = R.sq; L.c = R.c; L.f = R.f;
+  }
+}
+
+__device__ __forceinline__ void wave_speeds(const CellQ& L, const CellQ& R,
+                                            double& s1, double& s2) {
+  const double hRoe = 0.5 * (L.h + R.h);
+  const double uRoe = (L.u * L.sq + R.u * R.sq) / (L.sq + R.sq);
+  const double cRoe = sqrt(kGravity * hRoe);
+  s1 = dmin(L.u - L.c, uRoe - cRoe);
+  s2 = dmax(R.u + R.c, uRoe + cRoe);
+}
+
+// Edge speed only: max(|s1|, |s2|), 0 when both sides are dry.
+__device__ __forceinline__ double edge_speed(CellQ L, CellQ R) {
+  bool ld, rd, bd;
+  apply_dry(L, R, ld, rd, bd);
+  double s1, s2;
+  wave_speeds(L, R, s1, s2);
+  const double s = dmax(fabs(s1), fabs(s2));
+  return bd ? 0.0 : s;
+}
+
+// Full f-wave edge solve: net updates of the left and right cell.
+__device__ __forceinline__ void edge_solve(CellQ L, CellQ R, double& dhL,
+                                           double& dhR, double& dqL,
+                                           double& dqR) {
+  bool leftDry, rightDry, bothDry;
+  apply_dry(L, R, leftDry, rightDry, bothDry);
+  double s1, s2;
+  wave_speeds(L, R, s1, s2);
+  const double d0 = R.q - L.q;
+  const double d1 = (R.f - L.f) + (kHalfG * (L.h + R.h)) * (R.b - L.b);
+  const double inv = 1.0 / (s2 - s1);
+  const double beta1 = (s2 * d0 - d1) * inv;
+  const double beta2 = (d1 - s1 * d0) * inv;
+  const double z1 = beta1 * s1, z2 = beta2 * s2;
+  double a = 0.0, b = 0.0, c = 0.0, d = 0.0;  // dhL, dhR, dqL, dqR
+  if (s1 < -kZeroTol) {
+    a += beta1; c += z1;
+  } else if (s1 > kZeroTol) {
+    b += beta1; d += z1;
+  } else {
+    a += 0.5 * beta1; c += 0.5 * z1;
+    b += 0.5 * beta1; d += 0.5 * z1;
+  }
+  if (s2 < -kZeroTol) {
+    a += beta2; c += z2;
+  } else if (s2 > kZeroTol) {
+    b += beta2; d += z2;
+  } else {
+    a += 0.5 * beta2; c += 0.5 * z2;
+    b += 0.5 * beta2; d += 0.5 * z2;
+  }
+  if (leftDry || bothDry) { a = 0.0; c = 0.0; }
+  if (rightDry || bothDry) { b = 0.0; d = 0.0; }
+  dhL = a; dhR = b; dqL = c; dqR = d;
+}
+
+// SWE_WaveAccumulationBlock::computeNumericalFluxes (:225-239): CFL step of
+// one patch from its maximum edge speed (DBL_MAX where the float reference
+// uses FLT_MAX).
+__device__ __host__ __forceinline__ double max_timestep(double s, double dx,
+                                                        double dy) {
+  if (s > kMinSpeed) {
+    double a = dx / s, b = dy / s;
+    double t = b < a ? b : a;
+    t *= kCfl;
+    return t;
+  }
+  return 1.7976931348623157e308;
+}
+
+}  // namespace pcamd

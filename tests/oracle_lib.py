@@ -78,6 +78,7 @@ def lib():
         sig = {
             "or_fwave": (None, [ctypes.c_double] * 6 + [_dp]),
             "or_fwave_speed": (ctypes.c_double, [ctypes.c_double] * 4),
+            "or_fwave_batch": (None, [ctypes.c_longlong, _dp, _dp]),
             "or_block_fluxes": (ctypes.c_double, [ctypes.c_int, ctypes.c_int,
                                 ctypes.c_double, ctypes.c_double] + [_dp] * 7),
             "or_block_max_timestep": (ctypes.c_double, [ctypes.c_double] * 3),
@@ -120,6 +121,8 @@ def lib():
             "or_breaking_dam_init": (None, [P] + [ctypes.c_double] * 5),
             "or_demanded_mesh_width": (None, [P] + [ctypes.c_double] * 3 + [_dp]),
             "or_refinement_decision": (ctypes.c_int, [_dp, _dp]),
+            "or_grid_fill": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
+                             ctypes.POINTER(ctypes.c_longlong), _dp, _dp, _dp]),
             "or_pool_fill_ghosts": (None, [ctypes.POINTER(OrPool),
                                     ctypes.POINTER(OrGhostEntry),
                                     ctypes.c_longlong]),

@@ -1,0 +1,346 @@
+"""HIP path (through the C ABI) against the CPU oracle on the same inputs.
+
+Bar: bit-exact where the arithmetic is the same sequence of IEEE-754
+operations (all of the SWE path), checked as max relative difference
+<= 1e-12 per step and <= 1e-10 after many steps, with the exact-equality
+count printed for the record.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from oracle_lib import OrPool, Patch as OPatch, dptr as odptr, iarr, lib as olib
+
+pytestmark = pytest.mark.gpu
+
+REL_STEP = 1e-12
+REL_LONG = 1e-10
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import peanoclaw_amd  # noqa: F401
+    from peanoclaw_amd.pool import PatchPool
+    pool = PatchPool(0)
+    yield pool
+    pool.close()
+
+
+def new_pool():
+    from peanoclaw_amd.pool import PatchPool
+    return PatchPool(0)
+
+
+def relmax(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    d = np.abs(a - b)
+    s = np.maximum(np.abs(b), 1e-300)
+    m = np.where(d == 0, 0.0, d / np.maximum(s, 1.0))
+    return float(m.max()) if m.size else 0.0
+
+
+def random_edges(rng, n):
+    e = np.empty((n, 6))
+    e[:, 0:2] = rng.uniform(0.0, 3.0, (n, 2))
+    dry = rng.random((n, 2)) < 0.1
+    e[:, 0:2][dry] = rng.uniform(0.0, 0.02, dry.sum())
+    e[:, 2:4] = rng.uniform(-2.0, 2.0, (n, 2))
+    e[:, 4:6] = rng.uniform(-0.5, 0.5, (n, 2))
+    return e
+
+
+def test_edge_solver_bitwise(amd):
+    rng = np.random.default_rng(7)
+    e = random_edges(rng, 400000)
+    got = amd.edge_batch(e)
+    ref = np.empty((len(e), 5))
+    olib().or_fwave_batch(len(e), odptr(np.ascontiguousarray(e)), odptr(ref))
+    print("edge solver: exact", int((got == ref).all(axis=1).sum()), "of", len(e))
+    np.testing.assert_array_equal(got, ref)
+
+
+def _oracle_patch_from(unew, uold, b, pos, size, time, dt):
+    nx, ny = unew.shape[1:]
+    p = OPatch(3, 1, nx, 1, pos, size, 0, time, dt, ny=ny)
+    p.unew[:] = unew
+    p.uold[:] = uold
+    p.pwg[0] = b
+    return p
+
+
+@pytest.mark.parametrize("nx,ny", [(16, 16), (8, 12), (64, 64), (27, 27)])
+def test_solve_timestep_single_patch(amd, nx, ny):
+    from peanoclaw_amd.pool import make_description
+    rng = np.random.default_rng(nx * 100 + ny)
+    d = make_description((0.25, 0.5), (0.3, 0.2), nx, ny=ny, time=0.7, timestep_size=0.01)
+    s = amd.register(d)
+    unew = np.stack([rng.uniform(0.5, 2.0, (nx, ny)), rng.uniform(-0.2, 0.2, (nx, ny)),
+                     rng.uniform(-0.2, 0.2, (nx, ny))])
+    uold = np.stack([rng.uniform(0.5, 2.0, (nx + 2, ny + 2)),
+                     rng.uniform(-0.2, 0.2, (nx + 2, ny + 2)),
+                     rng.uniform(-0.2, 0.2, (nx + 2, ny + 2))])
+    uold[0, 3, 3] = 0.0  # a dry cell inside
+    b = rng.uniform(-0.3, 0.3, (nx + 2, ny + 2))
+    amd.upload(s, unew=unew, uold=uold, b=b)
+    from peanoclaw_amd.numerics import Patch, SWENumerics
+    num = SWENumerics(amd)
+    dt, est = num.solveTimestep(Patch(amd, s), 1e-3)
+    gn, go, gb = amd.download(s)
+    op = _oracle_patch_from(unew, uold, b, (0.25, 0.5), (0.3, 0.2), 0.7, 0.01)
+    olib().or_solve_timestep(op.ref, 1e-3)
+    assert dt == op.s.dt and est == op.s.est_next
+    np.testing.assert_array_equal(gn, op.unew)
+    np.testing.assert_array_equal(go, op.uold)
+    desc = amd.description(s)
+    assert desc.time == op.s.time and desc.timestep_size == op.s.dt
+    amd.release(s)
+
+
+def test_boundary_transfer_copy_swap(amd):
+    from peanoclaw_amd.numerics import Patch, SWENumerics
+    from peanoclaw_amd.pool import make_description
+    rng = np.random.default_rng(3)
+    n = 8
+    num = SWENumerics(amd)
+    sa = amd.register(make_description((0.0, 0.0), 0.5, n, time=0.2, timestep_size=0.1))
+    sb = amd.register(make_description((0.5, 0.0), 0.5, n, time=0.25, timestep_size=0.05))
+    data = {}
+    for s in (sa, sb):
+        unew = rng.uniform(0.5, 2, (3, n, n))
+        uold = rng.uniform(0.5, 2, (3, n + 2, n + 2))
+        b = rng.uniform(0, 0.1, (n + 2, n + 2))
+        amd.upload(s, unew=unew, uold=uold, b=b)
+        data[s] = (unew, uold, b)
+    oa = _oracle_patch_from(*data[sa], (0.0, 0.0), 0.5, 0.2, 0.1)
+    ob = _oracle_patch_from(*data[sb], (0.5, 0.0), 0.5, 0.25, 0.05)
+    # transfer a's right strip into b's left ghost layer (face functor)
+    num.transferGhostlayer((1, n), (n - 1, 0), (-1, 0), Patch(amd, sa), Patch(amd, sb))
+    olib().or_transfer_ghostlayer(iarr(1, n), iarr(n - 1, 0), iarr(-1, 0), oa.ref, ob.ref)
+    for dim in (0, 1):
+        for up in (0, 1):
+            num.fillBoundaryLayer(Patch(amd, sa), dim, up)
+            olib().or_fill_boundary_layer(oa.ref, dim, up)
+    num.copyUNewToUOld(Patch(amd, sb))
+    olib().or_copy_unew_to_uold(ob.ref)
+    num.swapUNewAndUOld(Patch(amd, sa))
+    olib().or_swap_unew_uold(oa.ref)
+    for s, o in ((sa, oa), (sb, ob)):
+        gn, go, gb = amd.download(s)
+        np.testing.assert_array_equal(gn, o.unew)
+        np.testing.assert_array_equal(go, o.uold)
+    amd.release(sa)
+    amd.release(sb)
+
+
+def test_interpolation_and_restriction_swe(amd):
+    from peanoclaw_amd.numerics import Patch, SWENumerics
+    from peanoclaw_amd.pool import make_description
+    rng = np.random.default_rng(11)
+    num = SWENumerics(amd)
+    nc, nf = 9, 9
+    # coarse leaf at [0,1]^2 and a fine patch (refinement factor 3) inside it
+    sc = amd.register(make_description((0.0, 0.0), 1.0, nc, time=0.3, timestep_size=0.02))
+    sf = amd.register(make_description((1.0 / 3, 1.0 / 3), 1.0 / 3, nf, level=1, time=0.3,
+                                       timestep_size=0.01))
+    cu = rng.uniform(0.8, 1.5, (3, nc, nc))
+    co = rng.uniform(0.8, 1.5, (3, nc + 2, nc + 2))
+    cb = rng.uniform(0.0, 0.3, (nc + 2, nc + 2))
+    fu = np.zeros((3, nf, nf))
+    fo = rng.uniform(0.8, 1.5, (3, nf + 2, nf + 2))
+    fb = rng.uniform(0.0, 0.3, (nf + 2, nf + 2))
+    amd.upload(sc, unew=cu, uold=co, b=cb)
+    amd.upload(sf, unew=fu, uold=fo, b=fb)
+    oc = _oracle_patch_from(cu, co, cb, (0.0, 0.0), 1.0, 0.3, 0.02)
+    of = _oracle_patch_from(fu, fo, fb, (1.0 / 3, 1.0 / 3), 1.0 / 3, 0.3, 0.01)
+    # fine left ghost layer from the coarse patch (face functor arguments)
+    num.interpolateSolution((1, nf), (-1, 0), Patch(amd, sc), Patch(amd, sf), True, False, True)
+    olib().or_interpolate_swe(iarr(1, nf), iarr(-1, 0), oc.ref, of.ref, 1, 0, 1)
+    # whole fine patch into uNew and uOld (Remesh::createCell, Remesh.cpp:399-418)
+    num.interpolateSolution((nf, nf), (0, 0), Patch(amd, sc), Patch(amd, sf), False, False,
+                            False)
+    olib().or_interpolate_swe(iarr(nf, nf), iarr(0, 0), oc.ref, of.ref, 0, 0, 0)
+    for s, o in ((sc, oc), (sf, of)):
+        gn, go, gb = amd.download(s)
+        assert relmax(gn, o.unew) == 0.0
+        assert relmax(go, o.uold) == 0.0
+    # restriction of the fine patch into a virtual coarse patch
+    dv = make_description((0.0, 0.0), 1.0, nc, time=0.0, timestep_size=0.0, is_virtual=True)
+    sv = amd.register(dv)
+    ov = OPatch(3, 1, nc, 1, 0.0, 1.0, 0, 0.0, 0.0)
+    ov.s.is_virtual = 1
+    ov.s.is_leaf = 0
+    vb = rng.uniform(0.0, 0.3, (nc + 2, nc + 2))
+    amd.upload(sv, b=vb)
+    ov.pwg[0] = vb
+    num.restrictSolution(Patch(amd, sf), Patch(amd, sv), False)
+    olib().or_restrict_swe(of.ref, ov.ref, 0)
+    num.postProcessRestriction(Patch(amd, sv), False)
+    olib().or_post_process_restriction_swe(ov.ref, 0)
+    gn, go, gb = amd.download(sv)
+    assert relmax(gn, ov.unew) == 0.0
+    assert relmax(go, ov.uold) == 0.0
+    gn, go, gb = amd.download(sf)
+    assert relmax(gn, of.unew) == 0.0 and relmax(go, of.uold) == 0.0
+    for s in (sc, sf, sv):
+        amd.release(s)
+
+
+def _oracle_pool_from_device(pool, wl):
+    """Oracle arrays in pool layout, from the device's current state."""
+    top = max(pool.offset(s) + (pool.dims(s)[0] + 2) * (pool.dims(s)[1] + 2) for s in wl.slots)
+    st = pool.current_state(wl.scenario)
+    H = [np.zeros(top), np.zeros(top)]
+    HU = [np.zeros(top), np.zeros(top)]
+    HV = [np.zeros(top), np.zeros(top)]
+    B = np.zeros(top)
+    pool.copy_fields(False, st, 0, H[0], HU[0], HV[0], B)
+    return H, HU, HV, B
+
+
+class OracleRun:
+    def __init__(self, pool, wl):
+        self.wl = wl
+        self.H, self.HU, self.HV, self.B = _oracle_pool_from_device(pool, wl)
+        nslots = max(wl.slots) + 1
+        self.off = np.zeros(nslots, dtype=np.int64)
+        nx = np.zeros(nslots, dtype=np.int32)
+        dx = np.zeros(nslots)
+        for s in wl.slots:
+            self.off[s] = pool.offset(s)
+            d = pool.description(s)
+            nx[s] = d.subdivision_factor[0]
+            dx[s] = d.size[0] / d.subdivision_factor[0]
+        self.nx, self.dx = nx, dx
+        self.pl = OrPool()
+        self.pl.np = nslots
+        self.pl.nx = nx.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        self.pl.ny = nx.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        self.pl.off = self.off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
+        self.pl.dx = dx.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        self.pl.dy = dx.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        for k in range(2):
+            self.pl.h[k] = odptr(self.H[k])
+            self.pl.hu[k] = odptr(self.HU[k])
+            self.pl.hv[k] = odptr(self.HV[k])
+        self.pl.b = odptr(self.B)
+        self.pl.cur = 0
+
+    def fill(self):
+        c = self.pl.cur
+        for L in self.wl.layouts:
+            so = np.ascontiguousarray(L.slot_of, dtype=np.int32)
+            olib().or_grid_fill(so.shape[0], so.shape[1], L.n,
+                                so.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                self.off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                                odptr(self.H[c]), odptr(self.HU[c]), odptr(self.HV[c]))
+
+    def step(self):
+        self.fill()
+        return olib().or_pool_step(ctypes.byref(self.pl), math.inf, 8)
+
+    def interior(self, pool, slot):
+        n = self.nx[slot]
+        o = self.off[slot]
+        c = self.pl.cur
+        t = lambda a: a[o:o + (n + 2) ** 2].reshape(n + 2, n + 2)[1:-1, 1:-1]
+        return np.stack([t(self.H[c]), t(self.HU[c]), t(self.HV[c])])
+
+
+def _compare(pool, orun, slots):
+    worst = 0.0
+    exact = 0
+    total = 0
+    for s in slots:
+        gn, _, _ = pool.download(s)
+        ref = orun.interior(pool, s)
+        worst = max(worst, relmax(gn, ref))
+        exact += int((gn == ref).sum())
+        total += gn.size
+    return worst, exact, total
+
+
+def test_dam_break_grid_200_steps():
+    from peanoclaw_amd.scenarios import dam_break_grid
+    pool = new_pool()
+    wl = dam_break_grid(pool, 9, 9, 16)
+    orun = OracleRun(pool, wl)
+    m0 = pool.mass()
+    dts = []
+    for it in range(200):
+        pool.step(1)
+        dts.append(orun.step())
+        if it in (0, 1, 9):
+            worst, exact, total = _compare(pool, orun, wl.slots)
+            print(f"step {it + 1}: max rel {worst:.3e}, exact {exact}/{total}")
+            assert worst <= REL_STEP
+    t, dt, est, nsteps = pool.scenario_get(0)
+    assert nsteps == 200
+    assert dt == dts[-1]
+    assert t == pytest.approx(sum(dts[:-1]), rel=1e-14)
+    worst, exact, total = _compare(pool, orun, wl.slots)
+    print(f"after 200 steps: max rel {worst:.3e}, exact {exact}/{total}")
+    assert worst <= REL_LONG
+    m1 = pool.mass()
+    assert abs(m1 - m0) / m0 < 1e-13
+    pool.close()
+
+
+def test_uneven_patches_parity_and_forms():
+    from peanoclaw_amd.scenarios import uneven_patches
+    pool = new_pool()
+    wl = uneven_patches(pool, num_patches=240, seed=5, field_seed=6)
+    orun = OracleRun(pool, wl)
+    for it in range(5):
+        pool.step(1)
+        orun.step()
+        worst, exact, total = _compare(pool, orun, wl.slots)
+        print(f"uneven step {it + 1}: max rel {worst:.3e}, exact {exact}/{total}")
+        assert worst <= REL_STEP
+    # the plain per-patch-launch form continues identically
+    pool.step_per_patch(3)
+    for _ in range(3):
+        orun.step()
+    worst, _, _ = _compare(pool, orun, wl.slots)
+    assert worst <= REL_STEP
+    # hipGraph replays continue identically (replayed several times with
+    # eager work and synchronisation between replays)
+    pool.graph_capture()
+    for _ in range(3):
+        pool.graph_replay(2)
+        pool.synchronize()
+        for _ in range(2):
+            orun.step()
+        worst, _, _ = _compare(pool, orun, wl.slots)
+        assert worst <= REL_STEP
+    # persistent launch over the device work list
+    pool.launch_config(persistent_workgroups=512)
+    pool.step(4)
+    for _ in range(4):
+        orun.step()
+    worst, _, _ = _compare(pool, orun, wl.slots)
+    assert worst <= REL_STEP
+    pool.close()
+
+
+def test_refinement_flags_bitexact():
+    from peanoclaw_amd.scenarios import dam_break_grid
+    pool = new_pool()
+    wl = dam_break_grid(pool, 27, 27, 16, centre=(0.52, 0.47))
+    pool.step(3)
+    flags = pool.refinement_flags(0.05, 1.0 / 1296, 1.0 / 432, max(wl.slots) + 1)
+    for s in wl.slots:
+        gn, _, _ = pool.download(s)
+        d = pool.description(s)
+        op = OPatch(3, 1, 16, 1, (d.position[0], d.position[1]), d.size[0], 0, 0.0, 0.0)
+        op.unew[:] = gn
+        out = (ctypes.c_double * 2)()
+        olib().or_demanded_mesh_width(op.ref, 0.05, 1.0 / 1296, 1.0 / 432, out)
+        sub = (ctypes.c_double * 2)(*op.s.subcell)
+        assert flags[s] == olib().or_refinement_decision(sub, out)
+    refined = float((flags[wl.slots] == 1).mean())
+    print("refined fraction of coarse patches:", refined)
+    assert 0.0 < refined < 1.0
+    pool.close()
