@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched FP64 SWE patch step on one MI355X.
+
+Workload: 50,000 patches with sizes drawn uniformly from {8,16,32,64}^2
+(seeded), h in [0.5, 2] with Gaussian bumps, small random momenta,
+Gaussian-hill bathymetry; each step = ghost fill + CFL reduction + update
+of every patch with one global dt.  One "step" advances all 68M cells once.
+
+Prints one JSON line: cell-updates/s of the batched path, its HBM roofline
+fraction (update kernel, HIP events around every launch of the timed
+steps), the plain per-patch-launch path on the same pool, the hipGraph
+replay rate, and the CPU oracle timed on the host cores on a sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 cell-updates/s + %HBM roofline, batched vs per-patch; p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_UPDATE = 56  # read h, hu, hv, b; write h, hu, hv (FP64)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_update_kernel_uneven50k.md")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per update launch from the committed PMC summary, if any."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(num_patches, seconds, threads):
+    """The oracle (FP64, C, OpenMP over patches) on a sample of the same
+    workload generator: ghost fill of every size-class grid + global-dt
+    step of every patch, repeated until `seconds` of wall time."""
+    import ctypes
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import OrPool, dptr, lib
+    from peanoclaw_amd.scenarios import SIZES, uneven_class_counts, uneven_fields
+
+    L = lib()
+    counts = uneven_class_counts(num_patches, 2)
+    rng = np.random.default_rng(3)
+    dx = 1.0 / 64
+    layouts, offs, nxs = [], [], []
+    chunks = {k: [] for k in ("h", "hu", "hv", "b")}
+    top = 0
+    slot = 0
+    for n in sorted(SIZES, reverse=True):
+        c = counts[n]
+        if not c:
+            continue
+        width = int(math.ceil(math.sqrt(c)))
+        h, hu, hv, b, rows = uneven_fields(n, c, width, rng, dx)
+        slot_of = -np.ones((width, rows), dtype=np.int32)
+        for k in range(c):
+            slot_of[k // rows, k % rows] = slot + k
+            offs.append(top + k * (n + 2) ** 2)
+            nxs.append(n)
+        for name, arr in (("h", h), ("hu", hu), ("hv", hv), ("b", b)):
+            chunks[name].append(arr.ravel())
+        layouts.append((n, slot_of))
+        top += c * (n + 2) ** 2
+        slot += c
+    H = [np.concatenate(chunks["h"]), np.zeros(top)]
+    HU = [np.concatenate(chunks["hu"]), np.zeros(top)]
+    HV = [np.concatenate(chunks["hv"]), np.zeros(top)]
+    B = np.concatenate(chunks["b"])
+    off = np.array(offs, dtype=np.int64)
+    nx = np.array(nxs, dtype=np.int32)
+    dxs = np.full(len(nxs), dx)
+    pl = OrPool()
+    pl.np = len(nxs)
+    pl.nx = nx.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    pl.ny = pl.nx
+    pl.off = off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
+    pl.dx = dxs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    pl.dy = pl.dx
+    for k in range(2):
+        pl.h[k], pl.hu[k], pl.hv[k] = dptr(H[k]), dptr(HU[k]), dptr(HV[k])
+    pl.b = dptr(B)
+    pl.cur = 0
+    cells = int(sum(n * n * counts[n] for n in SIZES))
+
+    def step():
+        c = pl.cur
+        for n, so in layouts:
+            so = np.ascontiguousarray(so)
+            L.or_grid_fill(so.shape[0], so.shape[1], n,
+                           so.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                           off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                           dptr(H[c]), dptr(HU[c]), dptr(HV[c]))
+        L.or_pool_step(ctypes.byref(pl), math.inf, threads)
+
+    step()  # warm-up
+    nsteps = 0
+    t0 = time.perf_counter()
+    while True:
+        step()
+        nsteps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or nsteps >= 200:
+            break
+    return {"value": cells * nsteps / el, "unit": "cell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{num_patches} patches of the same generator ({cells} cells), "
+                      f"{nsteps} global-dt steps incl. ghost fill, {el:.1f} s wall, "
+                      f"FP64 oracle in C (gcc -O2 -ffp-contract=off, OpenMP)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--patches", type=int, default=50000)
+    ap.add_argument("--per-patch-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-patches", type=int, default=4000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build measures the patch path on a "
+              "single MI355X (multi-GPU sharding is described in DESIGN.md, not run).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from peanoclaw_amd import autotune
+    from peanoclaw_amd.pool import PatchPool
+    from peanoclaw_amd.scenarios import uneven_patches
+
+    t_setup = time.perf_counter()
+    pool = PatchPool(0)
+    wl = uneven_patches(pool, num_patches=args.patches)
+    cells = wl.cells_per_step
+    log(f"setup: {args.patches} patches, {cells} cells/step, "
+        f"{pool.bytes_in_use() / 2**30:.2f} GiB pool, {time.perf_counter() - t_setup:.1f} s")
+    mass0 = pool.mass()
+
+    tuned = None
+    if not args.no_autotune:
+        tuned = autotune.tune(pool, steps=3, verbose=True)
+        log("autotune picked", {k: v for k, v in tuned.items() if k != "candidates"})
+
+    pool.step(args.warmup)
+    pool.synchronize()
+    t0 = time.perf_counter()
+    pool.step(args.steps)
+    pool.synchronize()
+    wall = time.perf_counter() - t0
+    ev_total_ms, upd_ms = pool.last_timing()
+    ms_per_step = 1e3 * wall / args.steps
+    value = cells * args.steps / wall
+    log(f"batched: {ms_per_step:.3f} ms/step (events {ev_total_ms / args.steps:.3f}), "
+        f"update kernel {upd_ms:.3f} ms, {value / 1e9:.2f} G cell-updates/s")
+
+    # hipGraph replay of {ghost fill, CFL reduction, dt, update}
+    pool.graph_capture()
+    pool.graph_replay(2)
+    pool.synchronize()
+    t0 = time.perf_counter()
+    pool.graph_replay(args.steps)
+    pool.synchronize()
+    gwall = time.perf_counter() - t0
+    graph_value = cells * args.steps / gwall
+    log(f"graph replay: {1e3 * gwall / args.steps:.3f} ms/step, {graph_value / 1e9:.2f} G/s")
+
+    # plain per-patch-launch form on the same pool (same results bitwise)
+    pool.step_per_patch(1)
+    pool.synchronize()
+    t0 = time.perf_counter()
+    pool.step_per_patch(args.per_patch_steps)
+    pool.synchronize()
+    pwall = time.perf_counter() - t0
+    per_patch_value = cells * args.per_patch_steps / pwall
+    log(f"per-patch launches: {1e3 * pwall / args.per_patch_steps:.1f} ms/step, "
+        f"{per_patch_value / 1e9:.3f} G/s")
+
+    mass1 = pool.mass()
+    t, dt, est, nsteps = pool.scenario_get(0)
+
+    achieved = cells * BYTES_PER_CELL_UPDATE / (upd_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic,
+                "kernel": "k_update (row-march f-wave update, all patches)",
+                "algorithmic_bytes_per_launch": cells * BYTES_PER_CELL_UPDATE,
+                "avg_launch_ms": round(upd_ms, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(args.cpu_patches, args.cpu_seconds, threads)
+        log(f"cpu baseline: {cpu['value'] / 1e6:.1f} M cell-updates/s on {threads} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "50k uneven SWE patches {8,16,32,64}^2 (seeded), FP64 f-wave, "
+                               "global dt; ghost fill + CFL reduction + update per step",
+                   "patches": args.patches, "cells_per_step": cells,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "hbm_frac_whole_step": round(value * BYTES_PER_CELL_UPDATE / (HBM_PEAK_GBS * 1e9), 4),
+        "graph_replay_value": graph_value,
+        "per_patch_launch_value": per_patch_value,
+        "batched_vs_per_patch": value / per_patch_value,
+        "autotune": None if tuned is None else {k: v for k, v in tuned.items()
+                                                if k != "candidates"},
+        "mass_rel_drift": abs(mass1 - mass0) / mass0,
+        "sim_time": t, "steps_done": nsteps,
+    }
+    print(json.dumps(out), flush=True)
+    pool.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -69,6 +69,9 @@ const char *pc_last_error_string(void);
 int pc_pool_create(int device, size_t hbm_budget_bytes, pc_pool **out);
 int pc_pool_destroy(pc_pool *pool);
 int pc_pool_bytes_in_use(pc_pool *pool, size_t *out);
+/* Pre-allocate field storage for `cells` tile cells (sum of (nx+2)(ny+2)),
+ * 56 bytes each; fails with PC_E_OOM beyond the budget. */
+int pc_pool_reserve(pc_pool *pool, long long cells);
 int pc_pool_synchronize(pc_pool *pool);
 
 /* Register a patch; returns its slot (replaces CellDescription.uIndex,
@@ -191,7 +194,9 @@ int pc_refinement_flags(pc_pool *pool, double threshold, double minimal_width,
 /* Total water volume sum(h)*dx*dy of one scenario (compensated sum). */
 int pc_scenario_mass(pc_pool *pool, int scenario, double *out);
 /* Device time (ms) of the last pc_step_all / pc_graph_replay call measured
- * with HIP events on the pool stream, and the update kernel's share. */
+ * with HIP events on the pool stream, and (pc_step_all only) the mean
+ * duration of one update-kernel launch over that call's steps, from an
+ * event pair recorded around every update launch. */
 int pc_last_timing(pc_pool *pool, double *total_ms, double *update_ms);
 
 /* Diagnostic: run the device f-wave edge solver on n host edges

@@ -68,6 +68,7 @@ _SIGS = {
     "pc_pool_bytes_in_use": (ctypes.c_int, [ctypes.c_void_p,
                                             ctypes.POINTER(ctypes.c_size_t)]),
     "pc_pool_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "pc_pool_reserve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
     "pc_patch_register": (ctypes.c_int, [ctypes.c_void_p,
                                          ctypes.POINTER(CellDescription), _ip]),
     "pc_patch_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),

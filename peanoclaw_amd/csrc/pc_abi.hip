@@ -108,6 +108,8 @@ struct pc_pool {
   int task_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   float last_total_ms = 0.f, last_update_ms = 0.f;
+  std::vector<hipEvent_t> upd_ev;  // per-step events around the update launch
+  long long hwm = 0;               // cells below this may hold old data
 };
 
 namespace {
@@ -379,7 +381,8 @@ int prepare(pc_pool* p) {
   return PC_OK;
 }
 
-int launch_step(pc_pool* p, bool timed_update) {
+int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
+                hipEvent_t ee = nullptr) {
   Fields F = fields_of(p);
   if (p->nstrips > 0) {
     hipLaunchKernelGGL(k_gather, dim3((p->nstrips + 3) / 4), dim3(256), 0, p->stream,
@@ -395,12 +398,12 @@ int launch_step(pc_pool* p, bool timed_update) {
                        p->d_plist_scn, p->d_pd, p->d_smax, p->d_est, p->d_counter);
   }
   if (p->ntasks > 0) {
-    if (timed_update) HIPCHK(hipEventRecord(p->ev2, p->stream));
+    if (timed_update) HIPCHK(hipEventRecord(eb ? eb : p->ev2, p->stream));
     const int grid = p->persistent_wgs > 0 ? p->persistent_wgs : p->ntasks;
     hipLaunchKernelGGL(k_update, dim3(grid), dim3(64), 0, p->stream, p->d_tasks,
                        p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, p->d_counter,
                        p->persistent_wgs > 0 ? 1 : 0);
-    if (timed_update) HIPCHK(hipEventRecord(p->ev3, p->stream));
+    if (timed_update) HIPCHK(hipEventRecord(ee ? ee : p->ev3, p->stream));
   }
   HIPCHK(hipGetLastError());
   return PC_OK;
@@ -453,12 +456,21 @@ int pc_pool_destroy(pc_pool* p) {
                   p->d_result, p->d_aux, p->d_flags};
   for (void* q : ptrs)
     if (q) hipFree(q);
+  for (hipEvent_t e : p->upd_ev) hipEventDestroy(e);
   hipEventDestroy(p->ev0);
   hipEventDestroy(p->ev1);
   hipEventDestroy(p->ev2);
   hipEventDestroy(p->ev3);
   hipStreamDestroy(p->stream);
   delete p;
+  return PC_OK;
+}
+
+int pc_pool_reserve(pc_pool* p, long long cells) {
+  if (!p || cells < 0) return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(p->device);
+  CHK(grow_fields(p, cells));
+  HIPCHK(hipStreamSynchronize(p->stream));
   return PC_OK;
 }
 
@@ -494,8 +506,11 @@ int pc_patch_register(pc_pool* p, const pc_cell_description* d, int* out_slot) {
   s.desc = *d;
   s.cells = (long long)(nx + 2) * (ny + 2);
   CHK(alloc_cells(p, s.cells, &s.off));
-  for (int k = 0; k < kNumFields; ++k)
-    HIPCHK(hipMemsetAsync(p->f[k] + s.off, 0, (size_t)s.cells * sizeof(double), p->stream));
+  // cells above the high-water mark are still zero from the allocation
+  if (s.off < p->hwm)
+    for (int k = 0; k < kNumFields; ++k)
+      HIPCHK(hipMemsetAsync(p->f[k] + s.off, 0, (size_t)s.cells * sizeof(double), p->stream));
+  p->hwm = std::max(p->hwm, s.off + s.cells);
   int slot = -1;
   for (size_t i = 0; i < p->slots.size(); ++i)
     if (!p->slots[i].live) {
@@ -1294,12 +1309,28 @@ int pc_step_all(pc_pool* p, int k) {
   if (!p || k < 0) return fail(PC_E_INVALID, "invalid argument");
   hipSetDevice(p->device);
   CHK(prepare(p));
+  // one event pair around every update launch: the update kernel's mean
+  // duration over exactly these k steps
+  while (p->upd_ev.size() < 2 * (size_t)k) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    p->upd_ev.push_back(e);
+  }
   HIPCHK(hipEventRecord(p->ev0, p->stream));
-  for (int i = 0; i < k; ++i) CHK(launch_step(p, i == k - 1));
+  for (int i = 0; i < k; ++i) CHK(launch_step(p, true, p->upd_ev[2 * i], p->upd_ev[2 * i + 1]));
   HIPCHK(hipEventRecord(p->ev1, p->stream));
   HIPCHK(hipEventSynchronize(p->ev1));
   HIPCHK(hipEventElapsedTime(&p->last_total_ms, p->ev0, p->ev1));
-  if (k > 0 && p->ntasks > 0) HIPCHK(hipEventElapsedTime(&p->last_update_ms, p->ev2, p->ev3));
+  p->last_update_ms = 0.f;
+  if (k > 0 && p->ntasks > 0) {
+    double sum = 0.0;
+    for (int i = 0; i < k; ++i) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, p->upd_ev[2 * i], p->upd_ev[2 * i + 1]));
+      sum += ms;
+    }
+    p->last_update_ms = (float)(sum / k);
+  }
   p->scn_host_stale = true;
   mark_batched_clock(p);
   return PC_OK;

@@ -172,6 +172,9 @@ class PatchPool:
                                            f.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
         return f
 
+    def reserve(self, cells):
+        check(self.lib.pc_pool_reserve(self.h, int(cells)))
+
     def synchronize(self):
         check(self.lib.pc_pool_synchronize(self.h))
 

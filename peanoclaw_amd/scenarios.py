@@ -130,6 +130,7 @@ def uneven_patches(pool, num_patches=50000, seed=2, field_seed=3, scenario=0, dx
     rng = np.random.default_rng(field_seed)
     layouts, slots = [], []
     cells = 0
+    pool.reserve(sum(c * (n + 2) ** 2 for n, c in counts.items()))
     pool.scenario_set(scenario, 0.0, 0.0, np.inf)
     state = pool.current_state(scenario)
     # large classes first: their tiles come first in the pool
