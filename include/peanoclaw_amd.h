@@ -165,6 +165,18 @@ int pc_set_ghost_strips(pc_pool *pool, const pc_ghost_strip *strips, int n);
  * `out` and their number to *n_out (call with out = NULL to count). */
 int pc_grid_strips(pc_pool *pool, int px, int py, const int *slot_of,
                    pc_ghost_strip *out, int cap, int *n_out);
+/* Fused ghost sources of the batched step for a px x py grid of equal-size
+ * patches (slot_of as for pc_grid_strips): every side of every patch reads
+ * its ghost values directly from the face neighbour (rows from the
+ * neighbour's boundary row, columns from its edge buffer) or, facing a hole
+ * or the border, reflects its own boundary (negated normal momentum).  The
+ * values equal those the reference's ghost fill writes into the ghost
+ * layer (DefaultTransfer at time factor 1, SWEKernel::fillBoundaryLayer);
+ * corner ghosts are not needed by the update (its x-edges use rows 1..ny,
+ * its y-edges columns 1..nx) and are not produced.  Patches without sides
+ * read their tile's ghost layer, filled by the ghost strips. */
+int pc_set_grid_sides(pc_pool *pool, int px, int py, const int *slot_of);
+int pc_clear_sides(pc_pool *pool);
 /* Per scenario clock and global end time; every step uses
  * maximumTimestepSize = end_time - (time + timestep_size)
  * (SolveTimestep.cpp:623-625); a scenario whose bound is <= 0 does not

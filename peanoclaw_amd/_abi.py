@@ -109,6 +109,8 @@ _SIGS = {
     "pc_set_ghost_strips": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "pc_grid_strips": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip,
                                       ctypes.c_void_p, ctypes.c_int, _ip]),
+    "pc_set_grid_sides": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip]),
+    "pc_clear_sides": (ctypes.c_int, [ctypes.c_void_p]),
     "pc_scenario_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double]),
     "pc_scenario_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp,

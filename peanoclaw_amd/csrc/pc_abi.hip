@@ -74,7 +74,17 @@ struct pc_pool {
   PatchDev* d_pd = nullptr;
   size_t pd_cap = 0;
   double* d_smax = nullptr;
+  double* d_smax_int = nullptr;
   double* d_est = nullptr;
+  double* d_ebuf[2] = {nullptr, nullptr};
+  double* d_bcol = nullptr;
+  SideDev* d_sides = nullptr;
+  std::vector<SideDev> h_sides;
+  bool sides_any = false;
+  bool edges_dirty = true;
+  bool fused_ready = false;
+  int* d_scn_of = nullptr;
+  int nentries = 0;
   ScnDev* d_scn = nullptr;
   size_t scn_cap = 0;
   Task* d_tasks = nullptr;
@@ -93,6 +103,7 @@ struct pc_pool {
   std::vector<GhostStrip> h_strips;
   std::vector<int> strip_begin;  // per slot (for the per-patch form)
   int* d_counter = nullptr;
+  int* d_slot1 = nullptr;  // one-entry patch list of pc_solve_timestep
   double* d_result = nullptr;
   double* d_aux = nullptr;
   size_t aux_cap = 0;
@@ -207,6 +218,13 @@ bool valid_slot(pc_pool* p, int s) {
   return p && s >= 0 && s < (int)p->slots.size() && p->slots[s].live;
 }
 
+// The state changed outside the batched step: edge buffers and the fused
+// interior speed maxima must be rebuilt before the next batched step.
+void mark_state_changed(pc_pool* p) {
+  p->edges_dirty = true;
+  p->fused_ready = false;
+}
+
 int sync_scn_host(pc_pool* p) {
   if (!p->scn_host_stale) return PC_OK;
   HIPCHK(hipMemcpyAsync(p->scn.data(), p->d_scn, p->scn.size() * sizeof(ScnDev),
@@ -276,19 +294,35 @@ int rebuild_topology(pc_pool* p) {
   std::vector<PatchDev> pd(std::max(ns, 1));
   for (int i = 0; i < ns; ++i)
     if (p->slots[i].live) pd[i] = make_patchdev(p->slots[i]);
+  const size_t old_cap = p->pd_cap;
   CHK(ensure_dev(&p->d_pd, &p->pd_cap, pd.size()));
   HIPCHK(hipMemcpyAsync(p->d_pd, pd.data(), pd.size() * sizeof(PatchDev),
                         hipMemcpyHostToDevice, p->stream));
   // per-slot arrays
-  {
-    size_t cap = p->pd_cap;
-    if (p->d_smax) HIPCHK(hipFree(p->d_smax));
-    if (p->d_est) HIPCHK(hipFree(p->d_est));
-    HIPCHK(hipMalloc(&p->d_smax, cap * sizeof(double)));
-    HIPCHK(hipMalloc(&p->d_est, cap * sizeof(double)));
-    HIPCHK(hipMemsetAsync(p->d_smax, 0, cap * sizeof(double), p->stream));
-    HIPCHK(hipMemsetAsync(p->d_est, 0, cap * sizeof(double), p->stream));
+  if (p->pd_cap != old_cap || !p->d_smax) {
+    const size_t cap = p->pd_cap;
+    void** arrs[] = {(void**)&p->d_smax, (void**)&p->d_smax_int, (void**)&p->d_est,
+                     (void**)&p->d_ebuf[0], (void**)&p->d_ebuf[1], (void**)&p->d_bcol,
+                     (void**)&p->d_sides};
+    const size_t sizes[] = {cap * sizeof(double), cap * sizeof(double), cap * sizeof(double),
+                            cap * 4 * kERows * sizeof(double), cap * 4 * kERows * sizeof(double),
+                            cap * 4 * kERows * sizeof(double), cap * sizeof(SideDev)};
+    for (int k = 0; k < 7; ++k) {
+      if (*arrs[k]) HIPCHK(hipFree(*arrs[k]));
+      *arrs[k] = nullptr;
+      HIPCHK(hipMalloc(arrs[k], sizes[k]));
+      HIPCHK(hipMemsetAsync(*arrs[k], 0, sizes[k], p->stream));
+    }
+    p->edges_dirty = true;
   }
+  p->h_sides.resize(ns);
+  bool any = false;
+  for (int i = 0; i < ns; ++i)
+    for (int k = 0; k < 4; ++k) any |= p->slots[i].live && p->h_sides[i].kind[k] != kSideTile;
+  p->sides_any = any;
+  if (ns > 0)
+    HIPCHK(hipMemcpyAsync(p->d_sides, p->h_sides.data(), ns * sizeof(SideDev),
+                          hipMemcpyHostToDevice, p->stream));
   // group live slots by (nx, ny, scenario)
   std::vector<int> live;
   for (int i = 0; i < ns; ++i)
@@ -313,7 +347,7 @@ int rebuild_topology(pc_pool* p) {
     const int nx = d0.subdivision_factor[0], ny = d0.subdivision_factor[1];
     int W = 1;
     while (W < ny) W <<= 1;
-    int G = std::min(64 / W, 128 / nx);
+    int G = std::min(64 / W, 64 / nx);
     if (p->task_cap > 0) G = std::min(G, p->task_cap);
     G = std::max(G, 1);
     Task t;
@@ -337,27 +371,36 @@ int rebuild_topology(pc_pool* p) {
   int nscn = 0;
   for (int s : live) nscn = std::max(nscn, p->slots[s].desc.scenario + 1);
   if ((int)p->scn.size() < nscn) p->scn.resize(nscn);
-  std::vector<int> plist_scn;
+  std::vector<int> plist_scn, scn_of;
   for (size_t k = 0; k < p->scn.size(); ++k) {
     p->scn[k].pbeg = (int)plist_scn.size();
     for (int s : live)
-      if (p->slots[s].desc.scenario == (int)k) plist_scn.push_back(s);
+      if (p->slots[s].desc.scenario == (int)k) {
+        plist_scn.push_back(s);
+        scn_of.push_back((int)k);
+      }
     p->scn[k].pend = (int)plist_scn.size();
   }
+  p->nentries = (int)plist_scn.size();
   size_t need = std::max<size_t>({p->h_plist.size(), plist_scn.size(), 1});
   if (need > p->plist_cap) {
     if (p->d_plist) HIPCHK(hipFree(p->d_plist));
     if (p->d_plist_scn) HIPCHK(hipFree(p->d_plist_scn));
+    if (p->d_scn_of) HIPCHK(hipFree(p->d_scn_of));
     p->plist_cap = need * 2;
     HIPCHK(hipMalloc(&p->d_plist, p->plist_cap * sizeof(int)));
     HIPCHK(hipMalloc(&p->d_plist_scn, p->plist_cap * sizeof(int)));
+    HIPCHK(hipMalloc(&p->d_scn_of, p->plist_cap * sizeof(int)));
   }
   if (!p->h_plist.empty())
     HIPCHK(hipMemcpyAsync(p->d_plist, p->h_plist.data(), p->h_plist.size() * sizeof(int),
                           hipMemcpyHostToDevice, p->stream));
-  if (!plist_scn.empty())
+  if (!plist_scn.empty()) {
     HIPCHK(hipMemcpyAsync(p->d_plist_scn, plist_scn.data(), plist_scn.size() * sizeof(int),
                           hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipMemcpyAsync(p->d_scn_of, scn_of.data(), scn_of.size() * sizeof(int),
+                          hipMemcpyHostToDevice, p->stream));
+  }
   if (p->d_tasks) HIPCHK(hipFree(p->d_tasks));
   HIPCHK(hipMalloc(&p->d_tasks, std::max<size_t>(p->h_tasks.size(), 1) * sizeof(Task)));
   if (!p->h_tasks.empty())
@@ -370,6 +413,7 @@ int rebuild_topology(pc_pool* p) {
   CHK(upload_scn(p));
   HIPCHK(hipStreamSynchronize(p->stream));
   p->topo_dirty = false;
+  p->fused_ready = false;
   CHK(invalidate_graph(p));
   return PC_OK;
 }
@@ -381,28 +425,93 @@ int prepare(pc_pool* p) {
   return PC_OK;
 }
 
+EdgeBufs edges_of(pc_pool* p) {
+  EdgeBufs E;
+  E.e[0] = p->d_ebuf[0];
+  E.e[1] = p->d_ebuf[1];
+  E.bcol = p->d_bcol;
+  return E;
+}
+
+// Everything the batched step needs that is derived from the current
+// state: edge buffers (after uploads / per-patch calls), and in fused mode
+// the interior speed maxima of the current state.
+int prepare_batched(pc_pool* p) {
+  CHK(prepare(p));
+  if (p->sides_any && p->edges_dirty && !p->h_live.empty()) {
+    hipLaunchKernelGGL(k_edges_init, dim3((unsigned)p->h_live.size()), dim3(64), 0, p->stream,
+                       p->d_live, (int)p->h_live.size(), p->d_pd, fields_of(p), p->d_scn,
+                       edges_of(p), p->d_bcol);
+    HIPCHK(hipGetLastError());
+    p->edges_dirty = false;
+    p->fused_ready = false;
+  }
+  if (p->fused && !p->fused_ready && p->ntasks > 0) {
+    if (p->nstrips > 0)
+      hipLaunchKernelGGL(k_gather, dim3((p->nstrips + 3) / 4), dim3(256), 0, p->stream,
+                         p->d_strips, p->nstrips, p->d_pd, fields_of(p), p->d_scn);
+    if (p->sides_any)
+      hipLaunchKernelGGL(k_speed<true>, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
+                         p->ntasks, p->d_plist, p->d_pd, fields_of(p), p->d_scn, p->d_sides,
+                         edges_of(p), p->d_smax_int);
+    else
+      hipLaunchKernelGGL(k_speed<false>, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
+                         p->ntasks, p->d_plist, p->d_pd, fields_of(p), p->d_scn, nullptr,
+                         edges_of(p), p->d_smax_int);
+    HIPCHK(hipGetLastError());
+    p->fused_ready = true;
+  }
+  return PC_OK;
+}
+
+// One step: [ghost strips] -> CFL speeds -> dt -> update.
 int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
                 hipEvent_t ee = nullptr) {
   Fields F = fields_of(p);
+  EdgeBufs E = edges_of(p);
+  const SideDev* sides = p->sides_any ? p->d_sides : nullptr;
+  const bool fused = p->fused && p->sides_any;
   if (p->nstrips > 0) {
     hipLaunchKernelGGL(k_gather, dim3((p->nstrips + 3) / 4), dim3(256), 0, p->stream,
                        p->d_strips, p->nstrips, p->d_pd, F, p->d_scn);
   }
   if (p->ntasks > 0) {
-    hipLaunchKernelGGL(k_speed, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
-                       p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, p->d_smax);
+    if (fused)
+      hipLaunchKernelGGL(k_boundary_speed, dim3((unsigned)p->h_live.size()), dim3(64), 0,
+                         p->stream, p->d_live, (int)p->h_live.size(), p->d_pd, F, p->d_scn,
+                         sides, E, p->d_smax_int, p->d_smax);
+    else if (sides)
+      hipLaunchKernelGGL(k_speed<true>, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
+                         p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E, p->d_smax);
+    else
+      hipLaunchKernelGGL(k_speed<false>, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
+                         p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E, p->d_smax);
   }
   const int nscn = (int)p->scn.size();
   if (nscn > 0) {
-    hipLaunchKernelGGL(k_dt, dim3(nscn), dim3(256), 0, p->stream, p->d_scn,
-                       p->d_plist_scn, p->d_pd, p->d_smax, p->d_est, p->d_counter);
+    if (p->nentries > 0)
+      hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
+                         p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
+                         p->d_pd, p->d_smax, p->d_est);
+    hipLaunchKernelGGL(k_dt_final, dim3((nscn + 63) / 64), dim3(64), 0, p->stream, p->d_scn,
+                       nscn, p->d_counter);
   }
   if (p->ntasks > 0) {
     if (timed_update) HIPCHK(hipEventRecord(eb ? eb : p->ev2, p->stream));
     const int grid = p->persistent_wgs > 0 ? p->persistent_wgs : p->ntasks;
-    hipLaunchKernelGGL(k_update, dim3(grid), dim3(64), 0, p->stream, p->d_tasks,
-                       p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, p->d_counter,
-                       p->persistent_wgs > 0 ? 1 : 0);
+    const int pers = p->persistent_wgs > 0 ? 1 : 0;
+    if (fused)
+      hipLaunchKernelGGL((k_update<true, true>), dim3(grid), dim3(64), 0, p->stream,
+                         p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
+                         p->d_smax_int, p->d_counter, pers);
+    else if (sides)
+      hipLaunchKernelGGL((k_update<false, true>), dim3(grid), dim3(64), 0, p->stream,
+                         p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
+                         p->d_smax_int, p->d_counter, pers);
+    else
+      hipLaunchKernelGGL((k_update<false, false>), dim3(grid), dim3(64), 0, p->stream,
+                         p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
+                         p->d_smax_int, p->d_counter, pers);
     if (timed_update) HIPCHK(hipEventRecord(ee ? ee : p->ev3, p->stream));
   }
   HIPCHK(hipGetLastError());
@@ -436,6 +545,7 @@ int pc_pool_create(int device, size_t hbm_budget_bytes, pc_pool** out) {
   HIPCHK(hipMalloc(&p->d_counter, sizeof(int)));
   HIPCHK(hipMemsetAsync(p->d_counter, 0, sizeof(int), p->stream));
   HIPCHK(hipMalloc(&p->d_result, 8 * sizeof(double)));
+  HIPCHK(hipMalloc(&p->d_slot1, sizeof(int)));
   HIPCHK(hipEventCreate(&p->ev0));
   HIPCHK(hipEventCreate(&p->ev1));
   HIPCHK(hipEventCreate(&p->ev2));
@@ -451,9 +561,10 @@ int pc_pool_destroy(pc_pool* p) {
   invalidate_graph(p);
   for (int k = 0; k < kNumFields; ++k)
     if (p->f[k]) hipFree(p->f[k]);
-  void* ptrs[] = {p->d_pd, p->d_smax, p->d_est, p->d_scn, p->d_tasks, p->d_plist,
-                  p->d_plist_scn, p->d_live, p->d_strips, p->d_counter,
-                  p->d_result, p->d_aux, p->d_flags};
+  void* ptrs[] = {p->d_pd, p->d_smax, p->d_smax_int, p->d_est, p->d_scn, p->d_tasks,
+                  p->d_plist, p->d_plist_scn, p->d_scn_of, p->d_live, p->d_strips,
+                  p->d_counter, p->d_slot1, p->d_result, p->d_aux, p->d_flags, p->d_ebuf[0],
+                  p->d_ebuf[1], p->d_bcol, p->d_sides};
   for (void* q : ptrs)
     if (q) hipFree(q);
   for (hipEvent_t e : p->upd_ev) hipEventDestroy(e);
@@ -529,6 +640,7 @@ int pc_patch_register(pc_pool* p, const pc_cell_description* d, int* out_slot) {
     for (size_t k = old; k < p->scn.size(); ++k) {
       ScnDev z{};
       z.end_time = INFINITY;
+      z.minbits = 0x7FEFFFFFFFFFFFFFull;  // DBL_MAX
       p->scn[k] = z;
     }
     p->scn[d->scenario].time = d->time;
@@ -544,6 +656,7 @@ int pc_patch_release(pc_pool* p, int slot) {
   if (!valid_slot(p, slot)) return fail(PC_E_INVALID, "invalid slot");
   free_cells(p, p->slots[slot].off, p->slots[slot].cells);
   p->slots[slot].live = false;
+  if ((size_t)slot < p->h_sides.size()) p->h_sides[slot] = SideDev{};
   p->topo_dirty = true;
   return PC_OK;
 }
@@ -607,6 +720,7 @@ int pc_patch_upload(pc_pool* p, int slot, const double* unew, const double* uold
     HIPCHK(hipMemcpyAsync(p->f[6] + s.off, b, area * sizeof(double), hipMemcpyHostToDevice,
                           p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
+  mark_state_changed(p);
   return PC_OK;
 }
 
@@ -671,6 +785,7 @@ int pc_pool_copy_fields(pc_pool* p, int to_device, int state, long long begin,
                             hipMemcpyDeviceToHost, p->stream));
   }
   HIPCHK(hipStreamSynchronize(p->stream));
+  if (to_device) mark_state_changed(p);
   return PC_OK;
 }
 
@@ -685,13 +800,16 @@ int pc_solve_timestep(pc_pool* p, int slot, double maxDt, const int flags[4],
   if (!valid_slot(p, slot)) return fail(PC_E_INVALID, "invalid slot");
   hipSetDevice(p->device);
   CHK(prepare(p));
+  mark_state_changed(p);
   Slot& s = p->slots[slot];
   if (s.clock_from_scenario) {
     slot_clock(p, slot, &s.desc.time, &s.desc.timestep_size);
     s.clock_from_scenario = false;
   }
-  hipLaunchKernelGGL(k_patch_solve, dim3(1), dim3(64), 0, p->stream, slot, p->d_pd,
-                     fields_of(p), cur_of(p, slot), maxDt, p->d_smax, p->d_result);
+  HIPCHK(hipMemcpyAsync(p->d_slot1, &slot, sizeof(int), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  hipLaunchKernelGGL(k_patch_solve, dim3(1), dim3(64), 0, p->stream, slot, p->d_slot1,
+                     p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_smax, p->d_result);
   HIPCHK(hipGetLastError());
   double res[2];
   HIPCHK(hipMemcpyAsync(res, p->d_result, 2 * sizeof(double), hipMemcpyDeviceToHost,
@@ -713,6 +831,7 @@ int pc_fill_boundary_layer(pc_pool* p, int slot, int dim, int upper) {
     return fail(PC_E_INVALID, "invalid slot or dimension");
   hipSetDevice(p->device);
   CHK(prepare(p));
+  mark_state_changed(p);
   hipLaunchKernelGGL(k_fill_boundary, dim3(1), dim3(128), 0, p->stream, p->d_pd, slot,
                      fields_of(p), 1 - cur_of(p, slot), dim, upper ? 1 : 0);
   HIPCHK(hipGetLastError());
@@ -725,6 +844,7 @@ int pc_transfer_ghostlayer(pc_pool* p, const int size[2], const int so[2],
     return fail(PC_E_INVALID, "invalid argument");
   hipSetDevice(p->device);
   CHK(prepare(p));
+  mark_state_changed(p);
   const auto& sd = p->slots[src].desc;
   const auto& dd = p->slots[dst].desc;
   for (int d = 0; d < 2; ++d) {
@@ -793,6 +913,7 @@ int pc_interpolate_solution(pc_pool* p, const int dsize[2], const int doff[2], i
     return fail(PC_E_INVALID, "invalid argument");
   hipSetDevice(p->device);
   CHK(prepare(p));
+  mark_state_changed(p);
   const auto& sd = p->slots[src].desc;
   const auto& dd = p->slots[dst].desc;
   const PatchDev S = make_patchdev(p->slots[src]), D = make_patchdev(p->slots[dst]);
@@ -908,6 +1029,7 @@ int pc_restrict_solution(pc_pool* p, int src, int dst, int onlyOverlapped) {
     return fail(PC_E_INVALID, "invalid slot");
   hipSetDevice(p->device);
   CHK(prepare(p));
+  mark_state_changed(p);
   const auto& sd = p->slots[src].desc;
   const auto& dd = p->slots[dst].desc;
   if (!dd.is_virtual) return fail(PC_E_INVALID, "restriction destination must be virtual");
@@ -978,6 +1100,7 @@ int pc_post_process_restriction(pc_pool* p, int dst, int onlyOverlapped) {
   if (!valid_slot(p, dst)) return fail(PC_E_INVALID, "invalid slot");
   hipSetDevice(p->device);
   CHK(prepare(p));
+  mark_state_changed(p);
   const int z[2] = {0, 0};
   const int dn[2] = {p->slots[dst].desc.subdivision_factor[0],
                      p->slots[dst].desc.subdivision_factor[1]};
@@ -990,6 +1113,7 @@ int pc_copy_unew_to_uold(pc_pool* p, int slot) {
   if (!valid_slot(p, slot)) return fail(PC_E_INVALID, "invalid slot");
   hipSetDevice(p->device);
   CHK(prepare(p));
+  mark_state_changed(p);
   hipLaunchKernelGGL(k_copy_swap, dim3(4), dim3(256), 0, p->stream, p->d_pd, slot,
                      fields_of(p), cur_of(p, slot), 0);
   HIPCHK(hipGetLastError());
@@ -1000,6 +1124,7 @@ int pc_swap_unew_and_uold(pc_pool* p, int slot) {
   if (!valid_slot(p, slot)) return fail(PC_E_INVALID, "invalid slot");
   hipSetDevice(p->device);
   CHK(prepare(p));
+  mark_state_changed(p);
   hipLaunchKernelGGL(k_copy_swap, dim3(4), dim3(256), 0, p->stream, p->d_pd, slot,
                      fields_of(p), cur_of(p, slot), 1);
   HIPCHK(hipGetLastError());
@@ -1267,6 +1392,7 @@ int pc_scenario_set(pc_pool* p, int scenario, double time, double dt, double end
     for (size_t k = old; k < p->scn.size(); ++k) {
       ScnDev z{};
       z.end_time = INFINITY;
+      z.minbits = 0x7FEFFFFFFFFFFFFFull;  // DBL_MAX
       p->scn[k] = z;
     }
   }
@@ -1292,9 +1418,9 @@ int pc_scenario_get(pc_pool* p, int scenario, double* time, double* dt, double* 
 
 int pc_set_launch_config(pc_pool* p, int persistent_wgs, int fused, int cap) {
   if (!p || persistent_wgs < 0 || cap < 0) return fail(PC_E_INVALID, "invalid config");
-  if (fused) return fail(PC_E_INVALID, "the fused next-step speed variant is not built");
   p->persistent_wgs = persistent_wgs;
-  p->fused = fused;
+  if ((fused != 0) != (p->fused != 0)) p->fused_ready = false;
+  p->fused = fused ? 1 : 0;
   if (cap != p->task_cap) p->topo_dirty = true;
   p->task_cap = cap;
   CHK(invalidate_graph(p));
@@ -1308,7 +1434,7 @@ static void mark_batched_clock(pc_pool* p) {
 int pc_step_all(pc_pool* p, int k) {
   if (!p || k < 0) return fail(PC_E_INVALID, "invalid argument");
   hipSetDevice(p->device);
-  CHK(prepare(p));
+  CHK(prepare_batched(p));
   // one event pair around every update launch: the update kernel's mean
   // duration over exactly these k steps
   while (p->upd_ev.size() < 2 * (size_t)k) {
@@ -1339,7 +1465,11 @@ int pc_step_all(pc_pool* p, int k) {
 int pc_step_all_per_patch(pc_pool* p, int k) {
   if (!p || k < 0) return fail(PC_E_INVALID, "invalid argument");
   hipSetDevice(p->device);
-  CHK(prepare(p));
+  const int fused = p->fused;
+  p->fused = 0;  // the plain form computes every patch's CFL step separately
+  int rc = prepare_batched(p);
+  p->fused = fused;
+  if (rc != PC_OK) return rc;
   // per-patch tasks (one patch per launch)
   std::vector<Task> single;
   for (int s : p->h_live) {
@@ -1359,9 +1489,12 @@ int pc_step_all_per_patch(pc_pool* p, int k) {
                           p->stream));
   }
   Fields F = fields_of(p);
+  EdgeBufs E = edges_of(p);
+  const SideDev* sides = p->sides_any ? p->d_sides : nullptr;
+  const int nscn = (int)p->scn.size();
   HIPCHK(hipEventRecord(p->ev0, p->stream));
   for (int it = 0; it < k; ++it) {
-    // ghost fill, one launch per destination patch
+    // ghost fill of tile-side patches, one launch per destination patch
     for (int i = 0; i < n; ++i) {
       const int s = p->h_live[i];
       if ((size_t)s + 1 >= p->strip_begin.size()) continue;
@@ -1371,15 +1504,32 @@ int pc_step_all_per_patch(pc_pool* p, int k) {
                            p->d_strips + b, e - b, p->d_pd, F, p->d_scn);
     }
     // per-patch CFL estimate, one launch per patch
-    for (int i = 0; i < n; ++i)
-      hipLaunchKernelGGL(k_speed, dim3(1), dim3(64), 0, p->stream, d_single + i, 1, d_sl,
-                         p->d_pd, F, p->d_scn, p->d_smax);
-    hipLaunchKernelGGL(k_dt, dim3((int)p->scn.size()), dim3(256), 0, p->stream, p->d_scn,
-                       p->d_plist_scn, p->d_pd, p->d_smax, p->d_est, p->d_counter);
+    for (int i = 0; i < n; ++i) {
+      if (sides)
+        hipLaunchKernelGGL(k_speed<true>, dim3(1), dim3(64), 0, p->stream, d_single + i, 1,
+                           d_sl, p->d_pd, F, p->d_scn, sides, E, p->d_smax);
+      else
+        hipLaunchKernelGGL(k_speed<false>, dim3(1), dim3(64), 0, p->stream, d_single + i, 1,
+                           d_sl, p->d_pd, F, p->d_scn, sides, E, p->d_smax);
+    }
+    if (p->nentries > 0)
+      hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
+                         p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
+                         p->d_pd, p->d_smax, p->d_est);
+    if (nscn > 0)
+      hipLaunchKernelGGL(k_dt_final, dim3((nscn + 63) / 64), dim3(64), 0, p->stream,
+                         p->d_scn, nscn, p->d_counter);
     // per-patch update, one launch per patch
-    for (int i = 0; i < n; ++i)
-      hipLaunchKernelGGL(k_update, dim3(1), dim3(64), 0, p->stream, d_single + i, 1, d_sl,
-                         p->d_pd, F, p->d_scn, p->d_counter, 0);
+    for (int i = 0; i < n; ++i) {
+      if (sides)
+        hipLaunchKernelGGL((k_update<false, true>), dim3(1), dim3(64), 0, p->stream,
+                           d_single + i, 1, d_sl, p->d_pd, F, p->d_scn, sides, E,
+                           p->d_smax_int, p->d_counter, 0);
+      else
+        hipLaunchKernelGGL((k_update<false, false>), dim3(1), dim3(64), 0, p->stream,
+                           d_single + i, 1, d_sl, p->d_pd, F, p->d_scn, sides, E,
+                           p->d_smax_int, p->d_counter, 0);
+    }
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(p->ev1, p->stream));
@@ -1389,14 +1539,58 @@ int pc_step_all_per_patch(pc_pool* p, int k) {
   HIPCHK(hipFree(d_single));
   HIPCHK(hipFree(d_sl));
   p->scn_host_stale = true;
+  p->fused_ready = false;
   mark_batched_clock(p);
+  return PC_OK;
+}
+
+int pc_set_grid_sides(pc_pool* p, int px, int py, const int* slot_of) {
+  if (!p || px <= 0 || py <= 0 || !slot_of) return fail(PC_E_INVALID, "invalid argument");
+  auto at = [&](int i, int j) -> int {
+    if (i < 0 || j < 0 || i >= px || j >= py) return -1;
+    return slot_of[i * py + j];
+  };
+  for (int k = 0; k < px * py; ++k)
+    if (slot_of[k] >= 0 && !valid_slot(p, slot_of[k]))
+      return fail(PC_E_INVALID, "invalid slot in grid");
+  if (p->h_sides.size() < p->slots.size()) p->h_sides.resize(p->slots.size());
+  for (int i = 0; i < px; ++i)
+    for (int j = 0; j < py; ++j) {
+      const int me = at(i, j);
+      if (me < 0) continue;
+      const auto& d = p->slots[me].desc;
+      const int nb[4] = {at(i - 1, j), at(i + 1, j), at(i, j - 1), at(i, j + 1)};
+      SideDev sd{};
+      for (int k = 0; k < 4; ++k) {
+        if (nb[k] >= 0) {
+          const auto& e = p->slots[nb[k]].desc;
+          if (e.subdivision_factor[0] != d.subdivision_factor[0] ||
+              e.subdivision_factor[1] != d.subdivision_factor[1] || e.scenario != d.scenario)
+            return fail(PC_E_INVALID, "grid neighbours must share size and scenario");
+          sd.kind[k] = kSideNeighbour;
+          sd.src[k] = nb[k];
+        } else {
+          sd.kind[k] = kSideWall;
+          sd.src[k] = me;
+        }
+      }
+      p->h_sides[me] = sd;
+    }
+  p->topo_dirty = true;
+  return PC_OK;
+}
+
+int pc_clear_sides(pc_pool* p) {
+  if (!p) return fail(PC_E_INVALID, "pool is NULL");
+  p->h_sides.assign(p->slots.size(), SideDev{});
+  p->topo_dirty = true;
   return PC_OK;
 }
 
 int pc_graph_capture(pc_pool* p) {
   if (!p) return fail(PC_E_INVALID, "pool is NULL");
   hipSetDevice(p->device);
-  CHK(prepare(p));
+  CHK(prepare_batched(p));
   CHK(invalidate_graph(p));
   HIPCHK(hipStreamSynchronize(p->stream));
   HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
@@ -1414,8 +1608,9 @@ int pc_graph_capture(pc_pool* p) {
 int pc_graph_replay(pc_pool* p, int k) {
   if (!p || k < 0) return fail(PC_E_INVALID, "invalid argument");
   hipSetDevice(p->device);
-  if (p->topo_dirty || !p->graph_valid || p->scn_dev_dirty)
-    return fail(PC_E_INVALID, "no valid captured graph (grid or scenarios changed)");
+  if (p->topo_dirty || !p->graph_valid || p->scn_dev_dirty ||
+      (p->sides_any && p->edges_dirty) || (p->fused && p->sides_any && !p->fused_ready))
+    return fail(PC_E_INVALID, "no valid captured graph (grid, scenarios or state changed)");
   CHK(sync_scn_host(p));
   HIPCHK(hipEventRecord(p->ev0, p->stream));
   for (int i = 0; i < k; ++i) HIPCHK(hipGraphLaunch(p->gexec, p->stream));

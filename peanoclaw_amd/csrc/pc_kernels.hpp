@@ -9,10 +9,20 @@
 // 64/W patches of the same size (W = ny rounded up to a power of two); lane
 // (seg, col) walks down column `col` of patch `seg`, keeps the previous row
 // in registers, solves the x-edge between the two rows and the y-edge to its
-// left neighbour lane (exchanged through LDS), and finalises the previous
-// row once both of its x-edges are known.  Each cell is read from HBM once
-// and written once.  The two y-edges per row that touch the ghost columns
-// are solved up front by a pre-pass in which the lanes take rows.
+// left neighbour lane (exchanged through LDS inside the wavefront), and
+// finalises the previous row once both of its x-edges are known.  Each cell
+// is read from HBM once and written once.  The two y-edges per row that
+// touch the ghost columns are solved up front by a pre-pass in which the
+// lanes take rows.
+//
+// Ghost values in the batched step come straight from where they live
+// (the ghost fill of the reference, DefaultTransfer at time factor 1 and
+// SWEKernel::fillBoundaryLayer, fused into the loads): a ghost row is read
+// from the neighbour's boundary row, a ghost column from the neighbour's
+// edge buffer (its first and last interior column, written contiguously by
+// the previous update), a wall side from the patch's own boundary with the
+// normal momentum negated.  Side kind 0 reads the tile's own ghost cells
+// (filled by the gather kernel, e.g. across refinement levels).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,6 +32,29 @@
 #include "swe_device.hpp"
 
 namespace pcamd {
+
+constexpr int kERows = 64;  // edge-buffer rows per side (max nx)
+
+// Global-memory accessors: loads and stores through address space 1, so the
+// compiler emits global_* instead of flat_* instructions (flat accesses
+// also count against the LDS wait counter, which would make every LDS
+// hand-off wait for the prefetched rows).
+template <class T>
+__device__ __forceinline__ T gl(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *(const __attribute__((address_space(1))) T*)p;
+#else
+  return *p;
+#endif
+}
+template <class T>
+__device__ __forceinline__ void gs(T* p, T v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  *(__attribute__((address_space(1))) T*)p = v;
+#else
+  *p = v;
+#endif
+}
 
 struct PatchDev {
   long long off;  // tile base in the field arrays
@@ -37,6 +70,29 @@ struct Fields {
   double* hu[2];
   double* hv[2];
   double* b;
+  // state s picked with a select: a run-time index into these arrays
+  // would place the argument struct in scratch memory
+  __device__ __forceinline__ double* H(int s) const { return s ? h[1] : h[0]; }
+  __device__ __forceinline__ double* HU(int s) const { return s ? hu[1] : hu[0]; }
+  __device__ __forceinline__ double* HV(int s) const { return s ? hv[1] : hv[0]; }
+};
+
+// Where the four ghost sides of a patch come from in the batched step:
+// 0 = x-lower (row 0), 1 = x-upper (row nx+1), 2 = y-lower (col 0),
+// 3 = y-upper (col ny+1).
+enum SideKind { kSideTile = 0, kSideNeighbour = 1, kSideWall = 2 };
+struct SideDev {
+  int kind[4];
+  int src[4];
+};
+
+// Edge buffers: h and hv of the first (side 0) and last (side 1) interior
+// column of every patch, per state: e[s][((p*2 + side)*2 + q)*kERows + row-1]
+// (q 0 = h, 1 = hv); bcol[(p*4 + k)*kERows + row-1] = b at columns
+// 0, 1, ny, ny+1 (static).
+struct EdgeBufs {
+  double* e[2];
+  const double* bcol;
 };
 
 struct Task {
@@ -52,6 +108,7 @@ struct ScnDev {
   double end_time;    // global timestep end time
   double dt_applied;  // dt of the current step, 0 if it did not advance
   long long steps;
+  unsigned long long minbits;  // running min of the patch CFL steps (bits)
   int stepc;          // parity counter: state (stepc & 1) is current
   int advance;        // current step advances this scenario
   int pbeg, pend;     // its patches in the scenario patch list
@@ -71,41 +128,68 @@ __device__ __forceinline__ int pow2ceil(int v) {
   return w;
 }
 
+// Intra-wavefront LDS hand-off: LDS operations of one wavefront complete in
+// order, so lanes only need the compiler not to move LDS accesses across
+// this point (no s_barrier, no wait on outstanding global loads).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // LDS scratch of one wavefront (64 lanes).
 struct MarchLds {
   double yc[7][64];      // y-sweep cell terms of the current row
+  double nc[5][64];      // y-sweep speed terms of the new row (fused mode)
   double ydow[2][64];    // y-edge results handed to the lower neighbour
-  double pre[4][128];    // boundary y-edges: [side*2 + k][seg*nx + row-1]
+  double pre[4][64];     // boundary y-edges: [side*2 + k][seg*nx + row-1]
 };
 
-// One task.  kSpeed: per-patch maximum edge speed of state `par` written
-// to smax[p].  kUpdate: new interior into state 1-par with step dt.
-// kUpdateSwap: the reference's in-place update of uOld followed by the
-// swap (SWEKernel.cpp:57-73): source = uOld tile (state 1-cur), new values
-// to uNew (state cur), the old uNew into uOld's interior.
-template <int MODE>
-__device__ void march_task(const Task& t, const int* __restrict__ plist,
-                           const PatchDev* __restrict__ pd, const Fields& F,
-                           int par, double dt, double* __restrict__ smax,
-                           MarchLds& L) {
+struct MarchArgs {
+  const int* plist;
+  const PatchDev* pd;
+  Fields F;
+  const SideDev* sides;  // nullptr: ghost cells from the tile
+  EdgeBufs E;
+  int par;
+  double dt;
+  double* smax;       // kSpeed: full per-patch max; fused: next-state interior max
+};
+
+template <int MODE, bool SIDES, bool FUSED>
+__device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, MarchLds& L) {
   const int lane = threadIdx.x & 63;
   const int nx = t.nx, ny = t.ny;
   const int W = pow2ceil(ny);
   const int seg = lane / W;
   const int col = lane - seg * W + 1;  // 1-based interior column
-  const bool active = (seg < t.nseg) && (col <= ny);
-  const int p = plist[t.first + (seg < t.nseg ? seg : 0)];
-  const PatchDev P = pd[p];
+  const bool segok = seg < t.nseg;
+  const bool active = segok && (col <= ny);
+  const int p = gl(A.plist + t.first + (segok ? seg : 0));
+  const PatchDev P = gl(A.pd + p);
   const int stride = ny + 2;
+  const int par = A.par;
   const int src = (MODE == kUpdateSwap) ? 1 - par : par;
+  const int dst = (MODE == kUpdateSwap) ? par : 1 - par;
+  const Fields& F = A.F;
   // kUpdateSwap writes into the tile it reads (rows already consumed), so
   // its source pointers must not be declared non-aliasing.
   using SrcPtr = typename std::conditional<MODE == kUpdateSwap, const double*,
                                            const double* __restrict__>::type;
-  SrcPtr Hs = F.h[src];
-  SrcPtr HUs = F.hu[src];
-  SrcPtr HVs = F.hv[src];
+  // state pointers picked with selects (a run-time index into the argument
+  // arrays would place them in scratch memory)
+  double* const Hsrc = src ? F.h[1] : F.h[0];
+  double* const HUsrc = src ? F.hu[1] : F.hu[0];
+  double* const HVsrc = src ? F.hv[1] : F.hv[0];
+  double* const Hdst = dst ? F.h[1] : F.h[0];
+  double* const HUdst = dst ? F.hu[1] : F.hu[0];
+  double* const HVdst = dst ? F.hv[1] : F.hv[0];
+  SrcPtr Hs = Hsrc;
+  SrcPtr HUs = HUsrc;
+  SrcPtr HVs = HVsrc;
   const double* __restrict__ Bs = F.b;
+  const double* Ecur = SIDES ? (par ? A.E.e[1] : A.E.e[0]) : nullptr;
+  double* const Edst = SIDES ? (dst ? A.E.e[1] : A.E.e[0]) : nullptr;
 
   // ---- pre-pass: the y-edges (row, 0|1) and (row, ny|ny+1) ----
   const int npre = t.nseg * nx * 2;
@@ -114,56 +198,132 @@ __device__ void march_task(const Task& t, const int* __restrict__ plist,
     const int rem = e - s * 2 * nx;
     const int row = (rem >> 1) + 1;
     const int side = rem & 1;
-    const PatchDev Q = pd[plist[t.first + s]];
+    const int ps = gl(A.plist + t.first + s);
+    const PatchDev Q = gl(A.pd + ps);
+    double hL, hvL, bL, hR, hvR, bR;
     const long long iL = Q.off + (long long)row * stride + (side ? ny : 0);
-    const long long iR = iL + 1;
-    const CellBase bl = cell_base(Hs[iL], MODE == kSpeed ? 0.0 : Bs[iL]);
-    const CellBase br = cell_base(Hs[iR], MODE == kSpeed ? 0.0 : Bs[iR]);
-    const CellQ cl = cell_dir(bl, HVs[iL]);
-    const CellQ cr = cell_dir(br, HVs[iR]);
+    if (SIDES) {
+      const int k = gl(&A.sides[ps].kind[2 + side]);
+      const double* Eo = Ecur + (long long)((ps * 2 + side) * 2) * kERows + row - 1;
+      const double* Bc = A.E.bcol + (long long)ps * 4 * kERows + row - 1;
+      const double ho = gl(Eo), hvo = gl(Eo + kERows);  // own (row,1) or (row,ny)
+      double hg, hvg;  // the ghost (row,0) or (row,ny+1)
+      if (k == kSideNeighbour) {
+        const int q = gl(&A.sides[ps].src[2 + side]);
+        const double* Eq = Ecur + (long long)((q * 2 + (1 - side)) * 2) * kERows + row - 1;
+        hg = gl(Eq);
+        hvg = gl(Eq + kERows);
+      } else if (k == kSideWall) {
+        hg = ho;
+        hvg = -hvo;
+      } else {
+        const long long ig = side ? iL + 1 : iL;
+        hg = gl(Hs + ig);
+        hvg = gl(HVs + ig);
+      }
+      if (side == 0) {
+        hL = hg; hvL = hvg; bL = gl(Bc);
+        hR = ho; hvR = hvo; bR = gl(Bc + kERows);
+      } else {
+        hL = ho; hvL = hvo; bL = gl(Bc + 2 * kERows);
+        hR = hg; hvR = hvg; bR = gl(Bc + 3 * kERows);
+      }
+    } else {
+      hL = gl(Hs + iL); hvL = gl(HVs + iL); bL = gl(Bs + iL);
+      hR = gl(Hs + iL + 1); hvR = gl(HVs + iL + 1); bR = gl(Bs + iL + 1);
+    }
+    if (MODE == kSpeed) { bL = 0.0; bR = 0.0; }
+    const CellQ cl = cell_dir(cell_base(hL, bL), hvL);
+    const CellQ cr = cell_dir(cell_base(hR, bR), hvR);
     const int slot = s * nx + row - 1;
     if (MODE == kSpeed) {
-      L.pre[side][slot] = edge_speed(cl, cr);
+      L.pre[side][slot] = edge_speed_any(cl, cr);
     } else {
       double dhL, dhR, dqL, dqR;
-      edge_solve(cl, cr, dhL, dhR, dqL, dqR);
+      edge_solve_any(cl, cr, dhL, dhR, dqL, dqR);
       // side 0: the right cell (row,1) takes the "upwards" part;
       // side 1: the left cell (row,ny) takes the "downwards" part.
       L.pre[side * 2 + 0][slot] = Q.dyinv * (side ? dhL : dhR);
       L.pre[side * 2 + 1][slot] = Q.dyinv * (side ? dqL : dqR);
     }
   }
-  __syncthreads();
+  wave_sync();
 
   const long long base = P.off + col;
   const double dxinv = P.dxinv, dyinv = P.dyinv;
+  const double dt = A.dt;
   double smaxl = 0.0;
+  // ghost-row sources (x sides), kept in scalars
+  int kx0 = kSideTile, kx1 = kSideTile;
+  long long gx0 = 0, gx1 = 0;  // index of the first interior cell of the ghost row source
+  bool neg0 = false, neg1 = false;
+  if (SIDES) {
+    kx0 = gl(&A.sides[p].kind[0]);
+    kx1 = gl(&A.sides[p].kind[1]);
+    if (kx0 == kSideNeighbour) {
+      const PatchDev Q = gl(A.pd + gl(&A.sides[p].src[0]));
+      gx0 = Q.off + (long long)Q.nx * stride + col;
+    } else if (kx0 == kSideWall) {
+      gx0 = base + stride;
+      neg0 = true;
+    } else {
+      gx0 = base;
+    }
+    if (kx1 == kSideNeighbour) {
+      const PatchDev Q = gl(A.pd + gl(&A.sides[p].src[1]));
+      gx1 = Q.off + stride + col;
+    } else if (kx1 == kSideWall) {
+      gx1 = base + (long long)nx * stride;
+      neg1 = true;
+    } else {
+      gx1 = base + (long long)(nx + 1) * stride;
+    }
+  }
+
+  // row loads (r = 0 and nx+1 are ghost rows; their hv is never used)
+  auto load_row = [&](int r, double& h, double& hu, double& hv, double& b) {
+    h = 1.0; hu = 0.0; hv = 0.0; b = 0.0;
+    if (!active) return;
+    const long long i = base + (long long)r * stride;
+    if (MODE != kSpeed) b = gl(Bs + i);
+    if (SIDES && (r == 0 || r == nx + 1)) {
+      const long long j = r == 0 ? gx0 : gx1;
+      h = gl(Hs + j);
+      hu = gl(HUs + j);
+      if (r == 0 ? neg0 : neg1) hu = -hu;
+      return;
+    }
+    h = gl(Hs + i);
+    hu = gl(HUs + i);
+    hv = gl(HVs + i);
+  };
 
   // previous row (x-sweep terms + raw values)
   CellQ px = {};
   double phv = 0.0;
   // accumulators of the previous row
   double accH = 0.0, accHu = 0.0, yUpH = 0.0, yUpHv = 0.0, yDnH = 0.0, yDnHv = 0.0;
+  // fused: x-sweep speed terms of the previous new row
+  CellQ pnx = {};
+  CellBase nbase = {};  // new row terms awaiting their y exchange
+  double nhv = 0.0;
+  bool have_new = false;
 
+  double h, hu, hv, b;
+  load_row(0, h, hu, hv, b);
   for (int r = 0; r <= nx + 1; ++r) {
-    const long long i = base + (long long)r * stride;
-    double h = 1.0, hu = 0.0, hv = 0.0, b = 0.0;
-    if (active) {
-      h = Hs[i];
-      hu = HUs[i];
-      hv = HVs[i];
-      if (MODE != kSpeed) b = Bs[i];
-    }
-    const CellBase cb = cell_base(h, b);
-    const CellQ cx = cell_dir(cb, hu);
+    const double ch = h, chu = hu, chv = hv, cbv = b;
+    if (r < nx + 1) load_row(r + 1, h, hu, hv, b);  // prefetch the next row
+    const CellBase cb = cell_base(ch, cbv);
+    const CellQ cx = cell_dir(cb, chu);
 
     if (r >= 1) {
       // x-edge (r-1 | r)
       if (MODE == kSpeed) {
-        smaxl = dmax(smaxl, edge_speed(px, cx));
+        smaxl = dmax(smaxl, edge_speed_any(px, cx));
       } else {
         double dhL, dhR, dqL, dqR;
-        edge_solve(px, cx, dhL, dhR, dqL, dqR);
+        edge_solve_any(px, cx, dhL, dhR, dqL, dqR);
         if (r >= 2) {
           // finalise row r-1: x-left (already in acc), x-right, y-up, y-down
           double nH = accH + dxinv * dhL;
@@ -178,17 +338,32 @@ __device__ void march_task(const Task& t, const int* __restrict__ plist,
           if (h1 < kClampH) { hu1 = 0.0; hv1 = 0.0; }
           if (h1 < 0.0) h1 = 0.0;
           if (active) {
-            const long long o = i - stride;
-            const int dst = (MODE == kUpdateSwap) ? par : 1 - par;
+            const long long o = base + (long long)(r - 1) * stride;
             if (MODE == kUpdateSwap) {
               // uOld interior <- old uNew, uNew <- new values
-              F.h[src][o] = F.h[dst][o];
-              F.hu[src][o] = F.hu[dst][o];
-              F.hv[src][o] = F.hv[dst][o];
+              Hsrc[o] = Hdst[o];
+              HUsrc[o] = HUdst[o];
+              HVsrc[o] = HVdst[o];
             }
-            F.h[dst][o] = h1;
-            F.hu[dst][o] = hu1;
-            F.hv[dst][o] = hv1;
+            gs(Hdst + o, h1);
+            gs(HUdst + o, hu1);
+            gs(HVdst + o, hv1);
+            if (SIDES && (col == 1 || col == ny)) {
+              double* Ed = Edst + (long long)((p * 2 + (col == 1 ? 0 : 1)) * 2) * kERows + r - 2;
+              gs(Ed, h1);
+              gs(Ed + kERows, hv1);
+            }
+          }
+          if (FUSED) {
+            nbase = cell_base(h1, 0.0);
+            const CellQ ncx = cell_dir(nbase, hu1);
+            if (r >= 3) {
+              const double sp = edge_speed_any(pnx, ncx);
+              if (active) smaxl = dmax(smaxl, sp);
+            }
+            pnx = ncx;
+            nhv = hv1;
+            have_new = true;
           }
         }
         accH = 0.0 + dxinv * dhR;
@@ -198,7 +373,7 @@ __device__ void march_task(const Task& t, const int* __restrict__ plist,
 
     if (r >= 1 && r <= nx) {
       // y-edges of row r: lane col solves (col-1 | col) for col >= 2
-      const CellQ cy = cell_dir(cb, hv);
+      const CellQ cy = cell_dir(cb, chv);
       L.yc[0][lane] = cy.h;
       L.yc[1][lane] = cy.q;
       L.yc[2][lane] = cy.b;
@@ -206,80 +381,117 @@ __device__ void march_task(const Task& t, const int* __restrict__ plist,
       L.yc[4][lane] = cy.sq;
       L.yc[5][lane] = cy.c;
       L.yc[6][lane] = cy.f;
-      __syncthreads();
+      CellQ ncy = {};
+      if (FUSED && have_new) {
+        ncy = cell_dir(nbase, nhv);
+        L.nc[0][lane] = ncy.h;
+        L.nc[1][lane] = ncy.q;
+        L.nc[2][lane] = ncy.u;
+        L.nc[3][lane] = ncy.sq;
+        L.nc[4][lane] = ncy.c;
+      }
+      wave_sync();
       const int slot = seg * nx + r - 1;
+      const int lm = lane > 0 ? lane - 1 : 0;
+      CellQ cl;
+      cl.h = L.yc[0][lm]; cl.q = L.yc[1][lm]; cl.b = L.yc[2][lm];
+      cl.u = L.yc[3][lm]; cl.sq = L.yc[4][lm]; cl.c = L.yc[5][lm]; cl.f = L.yc[6][lm];
+      if (col < 2) cl = cy;  // no left neighbour inside the patch: a dummy edge
       if (MODE == kSpeed) {
-        if (col >= 2 && active) {
-          CellQ cl;
-          cl.h = L.yc[0][lane - 1]; cl.q = L.yc[1][lane - 1]; cl.b = 0.0;
-          cl.u = L.yc[3][lane - 1]; cl.sq = L.yc[4][lane - 1];
-          cl.c = L.yc[5][lane - 1]; cl.f = L.yc[6][lane - 1];
-          smaxl = dmax(smaxl, edge_speed(cl, cy));
-        }
+        cl.b = 0.0;
+        const double sp = edge_speed_any(cl, cy);
+        if (col >= 2 && active) smaxl = dmax(smaxl, sp);
         if (active && col == 1) smaxl = dmax(smaxl, L.pre[0][slot]);
         if (active && col == ny) smaxl = dmax(smaxl, L.pre[1][slot]);
-        __syncthreads();
+        wave_sync();
       } else {
-        double up_h = 0.0, up_hv = 0.0;
+        double dhL, dhR, dqL, dqR;
+        edge_solve_any(cl, cy, dhL, dhR, dqL, dqR);
+        double up_h, up_hv;
         if (col >= 2) {
-          CellQ cl;
-          cl.h = L.yc[0][lane - 1]; cl.q = L.yc[1][lane - 1]; cl.b = L.yc[2][lane - 1];
-          cl.u = L.yc[3][lane - 1]; cl.sq = L.yc[4][lane - 1];
-          cl.c = L.yc[5][lane - 1]; cl.f = L.yc[6][lane - 1];
-          double dhL, dhR, dqL, dqR;
-          edge_solve(cl, cy, dhL, dhR, dqL, dqR);
           up_h = dyinv * dhR;
           up_hv = dyinv * dqR;
           if (active) {
             L.ydow[0][lane - 1] = dyinv * dhL;
             L.ydow[1][lane - 1] = dyinv * dqL;
           }
-        } else if (seg < t.nseg) {
-          up_h = L.pre[0][slot];
-          up_hv = L.pre[1][slot];
+        } else {
+          up_h = segok ? L.pre[0][slot] : 0.0;
+          up_hv = segok ? L.pre[1][slot] : 0.0;
         }
-        __syncthreads();
+        if (FUSED && have_new) {
+          CellQ nl;
+          nl.h = L.nc[0][lm]; nl.q = L.nc[1][lm]; nl.b = 0.0;
+          nl.u = L.nc[2][lm]; nl.sq = L.nc[3][lm]; nl.c = L.nc[4][lm]; nl.f = 0.0;
+          if (col < 2) nl = ncy;
+          const double sp = edge_speed_any(nl, ncy);
+          if (col >= 2 && active) smaxl = dmax(smaxl, sp);
+        }
+        wave_sync();
         double dn_h = 0.0, dn_hv = 0.0;
         if (col < ny) {
           dn_h = L.ydow[0][lane];
           dn_hv = L.ydow[1][lane];
-        } else if (seg < t.nseg) {
+        } else if (segok) {
           dn_h = L.pre[2][slot];
           dn_hv = L.pre[3][slot];
         }
         yUpH = up_h; yUpHv = up_hv; yDnH = dn_h; yDnHv = dn_hv;
-        __syncthreads();
+        wave_sync();
       }
     }
     px = cx;
-    phv = hv;
+    phv = chv;
   }
 
-  if (MODE == kSpeed) {
+  if (FUSED && MODE == kUpdate) {
+    // the y-edges of the last new row (row nx) are still open
+    const CellQ ncy = cell_dir(nbase, nhv);
+    L.nc[0][lane] = ncy.h;
+    L.nc[1][lane] = ncy.q;
+    L.nc[2][lane] = ncy.u;
+    L.nc[3][lane] = ncy.sq;
+    L.nc[4][lane] = ncy.c;
+    wave_sync();
+    const int lm = lane > 0 ? lane - 1 : 0;
+    CellQ nl;
+    nl.h = L.nc[0][lm]; nl.q = L.nc[1][lm]; nl.b = 0.0;
+    nl.u = L.nc[2][lm]; nl.sq = L.nc[3][lm]; nl.c = L.nc[4][lm]; nl.f = 0.0;
+    if (col < 2) nl = ncy;
+    const double sp = edge_speed_any(nl, ncy);
+    if (col >= 2 && active) smaxl = dmax(smaxl, sp);
+    wave_sync();
+  }
+
+  if (MODE == kSpeed || FUSED) {
     // segmented max over the W lanes of each patch
     double v = active ? smaxl : 0.0;
     for (int m = W >> 1; m >= 1; m >>= 1) v = dmax(v, __shfl_xor(v, m));
-    if (seg < t.nseg && col == 1) smax[p] = v;
+    if (segok && col == 1) gs(A.smax + p, v);
   }
 }
 
+template <bool SIDES>
 __global__ void __launch_bounds__(64)
 k_speed(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plist,
         const PatchDev* __restrict__ pd, Fields F, const ScnDev* __restrict__ scn,
-        double* __restrict__ smax) {
+        const SideDev* __restrict__ sides, EdgeBufs E, double* __restrict__ smax) {
   __shared__ MarchLds L;
   const int ti = blockIdx.x;
   if (ti >= ntasks) return;
-  const Task t = tasks[ti];
-  const int par = scn[pd[plist[t.first]].scn].stepc & 1;
-  march_task<kSpeed>(t, plist, pd, F, par, 0.0, smax, L);
+  const Task t = gl(tasks + ti);
+  const int par = gl(&scn[gl(&pd[gl(plist + t.first)].scn)].stepc) & 1;
+  MarchArgs A{plist, pd, F, sides, E, par, 0.0, smax};
+  march_task<kSpeed, SIDES, false>(t, A, L);
 }
 
 // One task per workgroup, or a persistent grid pulling tasks from a
-// device-side counter (reset by k_dt before every update launch).
+// device-side counter (reset by k_dt_final before every update launch).
+template <bool FUSED, bool SIDES>
 __global__ void __launch_bounds__(64)
 k_update(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plist,
          const PatchDev* __restrict__ pd, Fields F, const ScnDev* __restrict__ scn,
+         const SideDev* __restrict__ sides, EdgeBufs E, double* __restrict__ smax_next,
          int* __restrict__ counter, int persistent) {
   __shared__ MarchLds L;
   __shared__ int next;
@@ -292,61 +504,183 @@ k_update(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ pli
       __syncthreads();
     }
     if (ti >= ntasks) return;
-    const Task t = tasks[ti];
-    const ScnDev S = scn[pd[plist[t.first]].scn];
-    if (S.advance) {
-      // k_dt already moved the parity counter past this step
-      const int par = (S.stepc - 1) & 1;
-      march_task<kUpdate>(t, plist, pd, F, par, S.dt_applied, nullptr, L);
+    const Task t = gl(tasks + ti);
+    const ScnDev* S = scn + gl(&pd[gl(plist + t.first)].scn);
+    if (gl(&S->advance)) {
+      // k_dt_final already moved the parity counter past this step
+      MarchArgs A{plist, pd, F, sides, E, (gl(&S->stepc) - 1) & 1, gl(&S->dt_applied),
+                  smax_next};
+      march_task<kUpdate, SIDES, FUSED && SIDES>(t, A, L);
     }
     if (!persistent) return;
   }
 }
 
-// Per scenario: dt = min(min_p CFL(smax_p), end - (t + dt_prev)), the time
-// bookkeeping of SWEKernel.cpp:83-87 and the parity flip.
+// Edge speeds of the four boundary lines of every patch (ghost values from
+// the side sources), folded with the interior maximum of the same state
+// left by the fused update: smax[p] = max(smax_int[p], boundary max).
+__global__ void __launch_bounds__(64)
+k_boundary_speed(const int* __restrict__ slots, int nslots, const PatchDev* __restrict__ pd,
+                 Fields F, const ScnDev* __restrict__ scn, const SideDev* __restrict__ sides,
+                 EdgeBufs E, const double* __restrict__ smax_int, double* __restrict__ smax) {
+  const int k = blockIdx.x;
+  if (k >= nslots) return;
+  const int p = gl(slots + k);
+  const PatchDev P = gl(pd + p);
+  const int par = gl(&scn[P.scn].stepc) & 1;
+  const SideDev sd = gl(sides + p);
+  const int lane = threadIdx.x;
+  const int stride = P.ny + 2;
+  const double* H = F.H(par);
+  const double* HU = F.HU(par);
+  const double* HV = F.HV(par);
+  double m = 0.0;
+  // x sides: edges (0|1) and (nx|nx+1) along the ny columns
+  for (int side = 0; side < 2; ++side) {
+    const int kk = side ? sd.kind[1] : sd.kind[0];
+    const int srcp = side ? sd.src[1] : sd.src[0];
+    for (int c = lane + 1; c <= P.ny; c += 64) {
+      const int orow = side ? P.nx : 1;
+      const long long io = P.off + (long long)orow * stride + c;
+      const double ho = gl(H + io), huo = gl(HU + io);
+      double hg, hug;
+      if (kk == kSideNeighbour) {
+        const PatchDev Q = gl(pd + srcp);
+        const long long j = Q.off + (long long)(side ? 1 : Q.nx) * stride + c;
+        hg = gl(H + j);
+        hug = gl(HU + j);
+      } else if (kk == kSideWall) {
+        hg = ho;
+        hug = -huo;
+      } else {
+        const long long j = P.off + (long long)(side ? P.nx + 1 : 0) * stride + c;
+        hg = gl(H + j);
+        hug = gl(HU + j);
+      }
+      const CellQ co = cell_dir(cell_base(ho, 0.0), huo);
+      const CellQ cg = cell_dir(cell_base(hg, 0.0), hug);
+      m = dmax(m, side ? edge_speed(co, cg) : edge_speed(cg, co));
+    }
+  }
+  // y sides: edges (r,0|1) and (r,ny|ny+1) along the nx rows
+  for (int side = 0; side < 2; ++side) {
+    const int kk = side ? sd.kind[3] : sd.kind[2];
+    const int srcp = side ? sd.src[3] : sd.src[2];
+    for (int r = lane + 1; r <= P.nx; r += 64) {
+      const double* Eo = (par ? E.e[1] : E.e[0]) + (long long)((p * 2 + side) * 2) * kERows + r - 1;
+      const double ho = gl(Eo), hvo = gl(Eo + kERows);
+      double hg, hvg;
+      if (kk == kSideNeighbour) {
+        const double* Eq = (par ? E.e[1] : E.e[0]) + (long long)((srcp * 2 + (1 - side)) * 2) * kERows + r - 1;
+        hg = gl(Eq);
+        hvg = gl(Eq + kERows);
+      } else if (kk == kSideWall) {
+        hg = ho;
+        hvg = -hvo;
+      } else {
+        const long long j = P.off + (long long)r * stride + (side ? P.ny + 1 : 0);
+        hg = gl(H + j);
+        hvg = gl(HV + j);
+      }
+      const CellQ co = cell_dir(cell_base(ho, 0.0), hvo);
+      const CellQ cg = cell_dir(cell_base(hg, 0.0), hvg);
+      m = dmax(m, side ? edge_speed(co, cg) : edge_speed(cg, co));
+    }
+  }
+  for (int s = 32; s >= 1; s >>= 1) m = dmax(m, __shfl_xor(m, s));
+  if (lane == 0) smax[p] = dmax(smax_int[p], m);
+}
+
+// Edge buffers of the current state from the tiles (after uploads or
+// per-patch calls that changed the interior).
+__global__ void k_edges_init(const int* __restrict__ slots, int nslots,
+                             const PatchDev* __restrict__ pd, Fields F,
+                             const ScnDev* __restrict__ scn, EdgeBufs E, double* bcol) {
+  const int k = blockIdx.x;
+  if (k >= nslots) return;
+  const int p = slots[k];
+  const PatchDev P = pd[p];
+  const int par = scn[P.scn].stepc & 1;
+  const int stride = P.ny + 2;
+  for (int r = threadIdx.x + 1; r <= P.nx; r += blockDim.x) {
+    const long long row = P.off + (long long)r * stride;
+    for (int side = 0; side < 2; ++side) {
+      const long long i = row + (side ? P.ny : 1);
+      double* Ed = (par ? E.e[1] : E.e[0]) + (long long)((p * 2 + side) * 2) * kERows + r - 1;
+      Ed[0] = F.H(par)[i];
+      Ed[kERows] = F.HV(par)[i];
+    }
+    double* Bc = bcol + (long long)p * 4 * kERows + r - 1;
+    Bc[0] = F.b[row];
+    Bc[kERows] = F.b[row + 1];
+    Bc[2 * kERows] = F.b[row + P.ny];
+    Bc[3 * kERows] = F.b[row + P.ny + 1];
+  }
+}
+
+// Per-patch CFL steps and their minimum per scenario (first stage): the
+// scenario patch list is cut into chunks; a chunk inside one scenario is
+// reduced in LDS and folded with one atomic min on the bit pattern (all
+// values are positive), otherwise each entry folds its own value.
 __global__ void __launch_bounds__(256)
-k_dt(ScnDev* __restrict__ scn, const int* __restrict__ plist_scn,
-     const PatchDev* __restrict__ pd, const double* __restrict__ smax,
-     double* __restrict__ est_patch, int* __restrict__ counter) {
+k_dt_partial(ScnDev* __restrict__ scn, const int* __restrict__ plist_scn,
+             const int* __restrict__ scn_of_entry, int nentries,
+             const PatchDev* __restrict__ pd, const double* __restrict__ smax,
+             double* __restrict__ est_patch) {
   __shared__ double red[256];
-  ScnDev& S = scn[blockIdx.x];
+  const int beg = blockIdx.x * 1024;
+  const int end = min(beg + 1024, nentries);
+  if (beg >= end) return;
+  const int s0 = scn_of_entry[beg], s1 = scn_of_entry[end - 1];
   double m = 1.7976931348623157e308;
-  for (int k = S.pbeg + threadIdx.x; k < S.pend; k += blockDim.x) {
+  for (int k = beg + threadIdx.x; k < end; k += 256) {
     const int p = plist_scn[k];
     const double v = max_timestep(smax[p], pd[p].dx, pd[p].dy);
     est_patch[p] = v;
-    m = dmin(m, v);
+    if (s0 == s1)
+      m = dmin(m, v);
+    else
+      atomicMin(&scn[scn_of_entry[k]].minbits, (unsigned long long)__double_as_longlong(v));
   }
+  if (s0 != s1) return;
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] = dmin(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    if (blockIdx.x == 0 && counter) *counter = 0;
-    const double cfl = red[0];
-    const double maxDt = S.end_time - (S.time + S.dt);
-    int adv = 0;
-    double dt = 0.0;
-    if (S.pend > S.pbeg && maxDt - 0.0 > kLaEps) {
-      dt = fmin(cfl, maxDt);
-      adv = 1;
-      if (dt - 0.0 > kLaEps) {
-        S.time = S.time + S.dt;
-        S.dt = dt;
-      }
-      S.stepc = S.stepc + 1;
-      S.steps = S.steps + 1;
-    }
-    S.est = cfl;
-    S.dt_applied = dt;
-    S.advance = adv;
-  }
+  if (threadIdx.x == 0)
+    atomicMin(&scn[s0].minbits, (unsigned long long)__double_as_longlong(red[0]));
 }
 
-// Ghost fill of the current state: one wavefront per strip.
+// Second stage, one thread per scenario: dt = min(CFL, end - (t + dt_prev)),
+// the time bookkeeping of SWEKernel.cpp:83-87 and the parity flip.
+__global__ void k_dt_final(ScnDev* __restrict__ scn, int nscn, int* __restrict__ counter) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0 && counter) *counter = 0;
+  if (s >= nscn) return;
+  ScnDev& S = scn[s];
+  const double cfl = __longlong_as_double((long long)S.minbits);
+  S.minbits = (unsigned long long)__double_as_longlong(1.7976931348623157e308);
+  const double maxDt = S.end_time - (S.time + S.dt);
+  int adv = 0;
+  double dt = 0.0;
+  if (S.pend > S.pbeg && maxDt - 0.0 > kLaEps) {
+    dt = fmin(cfl, maxDt);
+    adv = 1;
+    if (dt - 0.0 > kLaEps) {
+      S.time = S.time + S.dt;
+      S.dt = dt;
+    }
+    S.stepc = S.stepc + 1;
+    S.steps = S.steps + 1;
+  }
+  S.est = cfl;
+  S.dt_applied = dt;
+  S.advance = adv;
+}
+
+// Ghost fill of the current state's tile ghost cells: one wavefront per strip.
 __global__ void __launch_bounds__(256)
 k_gather(const GhostStrip* __restrict__ strips, int nstrips,
          const PatchDev* __restrict__ pd, Fields F, const ScnDev* __restrict__ scn) {
@@ -360,12 +694,12 @@ k_gather(const GhostStrip* __restrict__ strips, int nstrips,
   for (int c = k; c < g.length; c += 64) {
     const long long di = D.off + g.dst_cell + (long long)c * g.dst_stride;
     const long long si = S.off + g.src_cell + (long long)c * g.src_stride;
-    const double h = F.h[par][si];
-    const double hu = F.hu[par][si];
-    const double hv = F.hv[par][si];
-    F.h[par][di] = h;
-    F.hu[par][di] = (g.negate & 1) ? -hu : hu;
-    F.hv[par][di] = (g.negate & 2) ? -hv : hv;
+    const double h = F.H(par)[si];
+    const double hu = F.HU(par)[si];
+    const double hv = F.HV(par)[si];
+    F.H(par)[di] = h;
+    F.HU(par)[di] = (g.negate & 1) ? -hu : hu;
+    F.HV(par)[di] = (g.negate & 2) ? -hv : hv;
   }
 }
 

@@ -16,22 +16,27 @@ __device__ __forceinline__ long long tile_index(const PatchDev& P, int x, int y)
 
 // SWEKernel::solveTimestep on one patch (SWEKernel.cpp:50-88): speed of
 // every edge of uOld, dt = fmin(CFL, maxDt), update, swap.
+// `plist` is a one-entry patch list in global memory holding `slot` (the
+// march reads its patch list with global loads).
 __global__ void __launch_bounds__(64)
-k_patch_solve(int slot, const PatchDev* __restrict__ pd, Fields F, int cur,
-              double maxDt, double* __restrict__ smax, double* __restrict__ out) {
+k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restrict__ pd,
+              Fields F, int cur, double maxDt, double* __restrict__ smax,
+              double* __restrict__ out) {
   __shared__ MarchLds L;
-  __shared__ int plist[1];
-  if (threadIdx.x == 0) plist[0] = slot;
-  __syncthreads();
   const PatchDev P = pd[slot];
   Task t;
   t.first = 0; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;
-  march_task<kSpeed>(t, plist, pd, F, 1 - cur, 0.0, smax, L);
+  EdgeBufs none{{nullptr, nullptr}, nullptr};
+  MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax};
+  march_task<kSpeed, false, false>(t, A, L);
   __syncthreads();
   const double s = smax[slot];
   const double cfl = max_timestep(s, P.dx, P.dy);
   const double dt = fmin(cfl, maxDt);
-  march_task<kUpdateSwap>(t, plist, pd, F, cur, dt, nullptr, L);
+  A.par = cur;
+  A.dt = dt;
+  A.smax = nullptr;
+  march_task<kUpdateSwap, false, false>(t, A, L);
   if (threadIdx.x == 0) {
     out[0] = dt;
     out[1] = cfl;
@@ -55,10 +60,10 @@ __global__ void k_fill_boundary(const PatchDev* __restrict__ pd, int slot, Field
       dx = sx = k;
     }
     const long long d = tile_index(P, dx, dy), s = tile_index(P, sx, sy);
-    const double h = F.h[st][s], hu = F.hu[st][s], hv = F.hv[st][s];
-    F.h[st][d] = h;
-    F.hu[st][d] = dim == 0 ? -hu : hu;
-    F.hv[st][d] = dim == 1 ? -hv : hv;
+    const double h = F.H(st)[s], hu = F.HU(st)[s], hv = F.HV(st)[s];
+    F.H(st)[d] = h;
+    F.HU(st)[d] = dim == 0 ? -hu : hu;
+    F.HV(st)[d] = dim == 1 ? -hv : hv;
   }
 }
 
@@ -74,9 +79,9 @@ __global__ void k_transfer(const PatchDev* __restrict__ pd, int srcSlot, int dst
     const long long si = tile_index(S, sox + x, soy + y);
     const long long di = tile_index(D, dox + x, doy + y);
     const int so = 1 - srcCur, dOld = 1 - dstCur;
-    F.h[dOld][di] = F.h[srcCur][si] * tf + F.h[so][si] * inv;
-    F.hu[dOld][di] = F.hu[srcCur][si] * tf + F.hu[so][si] * inv;
-    F.hv[dOld][di] = F.hv[srcCur][si] * tf + F.hv[so][si] * inv;
+    F.H(dOld)[di] = F.H(srcCur)[si] * tf + F.H(so)[si] * inv;
+    F.HU(dOld)[di] = F.HU(srcCur)[si] * tf + F.HU(so)[si] * inv;
+    F.HV(dOld)[di] = F.HV(srcCur)[si] * tf + F.HV(so)[si] * inv;
   }
 }
 
@@ -88,8 +93,8 @@ __global__ void k_copy_swap(const PatchDev* __restrict__ pd, int slot, Fields F,
        k += blockDim.x * gridDim.x) {
     const int x = k / P.ny, y = k - (k / P.ny) * P.ny;
     const long long i = tile_index(P, x, y);
-    double* a[3] = {F.h[cur], F.hu[cur], F.hv[cur]};
-    double* o[3] = {F.h[1 - cur], F.hu[1 - cur], F.hv[1 - cur]};
+    double* a[3] = {F.H(cur), F.HU(cur), F.HV(cur)};
+    double* o[3] = {F.H(1 - cur), F.HU(1 - cur), F.HV(1 - cur)};
     for (int q = 0; q < 3; ++q) {
       const double vn = a[q][i];
       if (mode == 1) a[q][i] = o[q][i];
@@ -106,7 +111,7 @@ __global__ void k_transform(const PatchDev* __restrict__ pd, int slot, Fields F,
        k += blockDim.x * gridDim.x) {
     const int x = k / sy, y = k - (k / sy) * sy;
     const long long i = tile_index(P, ox + x, oy + y);
-    F.h[st][i] = F.h[st][i] + sign * F.b[i];
+    F.H(st)[i] = F.H(st)[i] + sign * F.b[i];
   }
 }
 
@@ -130,13 +135,13 @@ __global__ void k_interpolate(const PatchDev* __restrict__ pd, Fields F, InterpA
     const int dOld = 1 - a.dstCur;
     double* dq[3];
     if (a.toUOld) {
-      dq[0] = F.h[dOld]; dq[1] = F.hu[dOld]; dq[2] = F.hv[dOld];
+      dq[0] = F.H(dOld); dq[1] = F.HU(dOld); dq[2] = F.HV(dOld);
     } else {
-      dq[0] = F.h[a.dstCur]; dq[1] = F.hu[a.dstCur]; dq[2] = F.hv[a.dstCur];
+      dq[0] = F.H(a.dstCur); dq[1] = F.HU(a.dstCur); dq[2] = F.HV(a.dstCur);
     }
-    F.h[dOld][di] = 0.0;
-    F.hu[dOld][di] = 0.0;
-    F.hv[dOld][di] = 0.0;
+    F.H(dOld)[di] = 0.0;
+    F.HU(dOld)[di] = 0.0;
+    F.HV(dOld)[di] = 0.0;
     double c[2];
     c[0] = a.dpos[0] + a.dsub[0] * (double)x + a.dsub[0] * 0.5;
     c[1] = a.dpos[1] + a.dsub[1] * (double)y + a.dsub[1] * 0.5;
@@ -162,8 +167,8 @@ __global__ void k_interpolate(const PatchDev* __restrict__ pd, Fields F, InterpA
         sf *= 1.0 - (sign * (c[d] - np[d])) * a.sinv[d];
       }
       const long long si_ = tile_index(S, nb[0], nb[1]);
-      const double* so[3] = {F.h[1 - a.srcCur], F.hu[1 - a.srcCur], F.hv[1 - a.srcCur]};
-      const double* sn[3] = {F.h[a.srcCur], F.hu[a.srcCur], F.hv[a.srcCur]};
+      const double* so[3] = {F.H(1 - a.srcCur), F.HU(1 - a.srcCur), F.HV(1 - a.srcCur)};
+      const double* sn[3] = {F.H(a.srcCur), F.HU(a.srcCur), F.HV(a.srcCur)};
       for (int q = 0; q < 3; ++q) {
         const double vo = so[q][si_], vn = sn[q][si_];
         if (a.toUOld)
@@ -211,8 +216,8 @@ __global__ void k_restrict(const PatchDev* __restrict__ pd, Fields F, RestrictAr
     }
     const long long di = tile_index(D, di0, di1);
     const int dOldSt = 1 - a.dstCur, sOldSt = 1 - a.srcCur;
-    double accO[3] = {F.h[dOldSt][di], F.hu[dOldSt][di], F.hv[dOldSt][di]};
-    double accN[3] = {F.h[a.dstCur][di], F.hu[a.dstCur][di], F.hv[a.dstCur][di]};
+    double accO[3] = {F.H(dOldSt)[di], F.HU(dOldSt)[di], F.HV(dOldSt)[di]};
+    double accN[3] = {F.H(a.dstCur)[di], F.HU(a.dstCur)[di], F.HV(a.dstCur)[di]};
     for (int sx = 0; sx < cSize[0]; ++sx)
       for (int sy = 0; sy < cSize[1]; ++sy) {
         const int s0 = cOff[0] + sx, s1 = cOff[1] + sy;
@@ -226,8 +231,8 @@ __global__ void k_restrict(const PatchDev* __restrict__ pd, Fields F, RestrictAr
           region = dmax(region, 0.0);
         }
         const long long si = tile_index(S, s0, s1);
-        const double vo[3] = {F.h[sOldSt][si], F.hu[sOldSt][si], F.hv[sOldSt][si]};
-        const double vn[3] = {F.h[a.srcCur][si], F.hu[a.srcCur][si], F.hv[a.srcCur][si]};
+        const double vo[3] = {F.H(sOldSt)[si], F.HU(sOldSt)[si], F.HV(sOldSt)[si]};
+        const double vn[3] = {F.H(a.srcCur)[si], F.HU(a.srcCur)[si], F.HV(a.srcCur)[si]};
         for (int q = 0; q < 3; ++q) {
           const double dO = (vo[q] * (1.0 - a.tfOld) + vn[q] * a.tfOld) * region / a.dstRegion;
           const double dN = (vo[q] * (1.0 - a.tfNew) + vn[q] * a.tfNew) * region / a.dstRegion;
@@ -235,8 +240,8 @@ __global__ void k_restrict(const PatchDev* __restrict__ pd, Fields F, RestrictAr
           accN[q] = accN[q] + dN;
         }
       }
-    F.h[dOldSt][di] = accO[0]; F.hu[dOldSt][di] = accO[1]; F.hv[dOldSt][di] = accO[2];
-    F.h[a.dstCur][di] = accN[0]; F.hu[a.dstCur][di] = accN[1]; F.hv[a.dstCur][di] = accN[2];
+    F.H(dOldSt)[di] = accO[0]; F.HU(dOldSt)[di] = accO[1]; F.HV(dOldSt)[di] = accO[2];
+    F.H(a.dstCur)[di] = accN[0]; F.HU(a.dstCur)[di] = accN[1]; F.HV(a.dstCur)[di] = accN[2];
   }
 }
 
@@ -253,7 +258,7 @@ k_refine_flags(const int* __restrict__ slots, int nslots, const PatchDev* __rest
   const int slot = slots[k];
   const PatchDev P = pd[slot];
   const int cur = scn[P.scn].stepc & 1;
-  const double* H = F.h[cur];
+  const double* H = F.H(cur);
   double g = 0.0;
   const int nxy = (P.nx - 1) * (P.ny - 1);
   for (int e = threadIdx.x; e < nxy; e += 64) {
@@ -299,7 +304,7 @@ k_patch_sums(const int* __restrict__ slots, int nslots, const PatchDev* __restri
   const int cur = scn[P.scn].stepc & 1;
   double s = 0.0, comp = 0.0;
   for (int e = threadIdx.x; e < P.nx * P.ny; e += 64) {
-    const double v = F.h[cur][tile_index(P, e / P.ny, e % P.ny)];
+    const double v = F.H(cur)[tile_index(P, e / P.ny, e % P.ny)];
     const double t = s + v;
     comp += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
     s = t;

@@ -133,6 +133,69 @@ __device__ __forceinline__ void edge_solve(CellQ L, CellQ R, double& dhL,
   dhL = a; dhR = b; dqL = c; dqR = d;
 }
 
+// Wave-uniform fast paths.  When every lane of the wavefront has two wet
+// cells (h >= dryTol) the dry substitution is the identity, and when every
+// lane's waves are subsonic (s1 < -zeroTol, s2 > zeroTol) the distribution
+// is "wave 1 left, wave 2 right".  The operations that remain are exactly
+// those the general path performs in that case, so results are unchanged
+// bit for bit; only the selects of the unused branches are skipped.
+__device__ __forceinline__ bool wave_all(bool c) { return __all(c ? 1 : 0) != 0; }
+
+__device__ __forceinline__ double edge_speed_wet(const CellQ& L, const CellQ& R) {
+  double s1, s2;
+  wave_speeds(L, R, s1, s2);
+  return dmax(fabs(s1), fabs(s2));
+}
+
+__device__ __forceinline__ void edge_solve_wet(const CellQ& L, const CellQ& R, double& dhL,
+                                               double& dhR, double& dqL, double& dqR) {
+  double s1, s2;
+  wave_speeds(L, R, s1, s2);
+  const double d0 = R.q - L.q;
+  const double d1 = (R.f - L.f) + (kHalfG * (L.h + R.h)) * (R.b - L.b);
+  const double inv = 1.0 / (s2 - s1);
+  const double beta1 = (s2 * d0 - d1) * inv;
+  const double beta2 = (d1 - s1 * d0) * inv;
+  const double z1 = beta1 * s1, z2 = beta2 * s2;
+  if (wave_all(s1 < -kZeroTol && s2 > kZeroTol)) {
+    dhL = 0.0 + beta1; dqL = 0.0 + z1;
+    dhR = 0.0 + beta2; dqR = 0.0 + z2;
+    return;
+  }
+  double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+  if (s1 < -kZeroTol) {
+    a += beta1; c += z1;
+  } else if (s1 > kZeroTol) {
+    b += beta1; d += z1;
+  } else {
+    a += 0.5 * beta1; c += 0.5 * z1;
+    b += 0.5 * beta1; d += 0.5 * z1;
+  }
+  if (s2 < -kZeroTol) {
+    a += beta2; c += z2;
+  } else if (s2 > kZeroTol) {
+    b += beta2; d += z2;
+  } else {
+    a += 0.5 * beta2; c += 0.5 * z2;
+    b += 0.5 * beta2; d += 0.5 * z2;
+  }
+  dhL = a; dhR = b; dqL = c; dqR = d;
+}
+
+// Dispatch on wave-wide wetness (all lanes must call it together).
+__device__ __forceinline__ void edge_solve_any(const CellQ& L, const CellQ& R, double& dhL,
+                                               double& dhR, double& dqL, double& dqR) {
+  if (wave_all(L.h >= kDryTol && R.h >= kDryTol))
+    edge_solve_wet(L, R, dhL, dhR, dqL, dqR);
+  else
+    edge_solve(L, R, dhL, dhR, dqL, dqR);
+}
+
+__device__ __forceinline__ double edge_speed_any(const CellQ& L, const CellQ& R) {
+  if (wave_all(L.h >= kDryTol && R.h >= kDryTol)) return edge_speed_wet(L, R);
+  return edge_speed(L, R);
+}
+
 // SWE_WaveAccumulationBlock::computeNumericalFluxes (:225-239): CFL step of
 // one patch from its maximum edge speed (DBL_MAX where the float reference
 // uses FLT_MAX).

@@ -124,6 +124,15 @@ class PatchPool:
                                       n.value, ctypes.byref(n)))
         return out
 
+    def set_grid_sides(self, slot_of):
+        slot_of = np.ascontiguousarray(slot_of, dtype=np.int32)
+        px, py = slot_of.shape
+        check(self.lib.pc_set_grid_sides(self.h, px, py,
+                                         slot_of.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
+    def clear_sides(self):
+        check(self.lib.pc_clear_sides(self.h))
+
     def set_ghost_strips(self, strips):
         strips = np.ascontiguousarray(strips, dtype=GHOST_STRIP_DTYPE)
         check(self.lib.pc_set_ghost_strips(self.h, strips.ctypes.data_as(ctypes.c_void_p),

@@ -18,6 +18,7 @@ import math
 
 import numpy as np
 
+from ._abi import GHOST_STRIP_DTYPE
 from .pool import make_description
 
 SIZES = (8, 16, 32, 64)
@@ -39,9 +40,17 @@ class Workload:
         self.cells_per_step = cells_per_step
         self.slots = slots
 
-    def build_ghost_table(self):
+    def build_ghost_table(self, fused_sides=True):
+        """Device neighbour table of the grid(s): fused ghost sources (the
+        fast path), or explicit ghost strips gathered into the tiles."""
+        if fused_sides:
+            self.pool.set_ghost_strips(np.zeros(0, dtype=GHOST_STRIP_DTYPE))
+            for L in self.layouts:
+                self.pool.set_grid_sides(L.slot_of)
+            return None
+        self.pool.clear_sides()
         tables = [self.pool.grid_strips(L.slot_of) for L in self.layouts]
-        strips = np.concatenate(tables) if tables else np.zeros(0)
+        strips = np.concatenate(tables) if tables else np.zeros(0, dtype=GHOST_STRIP_DTYPE)
         self.pool.set_ghost_strips(strips)
         return strips
 

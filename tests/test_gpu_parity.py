@@ -262,10 +262,28 @@ def _compare(pool, orun, slots):
     return worst, exact, total
 
 
-def test_dam_break_grid_200_steps():
+MODES = {
+    # ghost strips gathered into the tiles, separate CFL pass
+    "strips": dict(fused_sides=False, fused=0),
+    # ghost values read from neighbours / edge buffers, separate CFL pass
+    "sides": dict(fused_sides=True, fused=0),
+    # as "sides", next-step CFL speeds computed inside the update kernel
+    "sides_fused": dict(fused_sides=True, fused=1),
+}
+
+
+def _apply_mode(pool, wl, mode):
+    m = MODES[mode]
+    wl.build_ghost_table(fused_sides=m["fused_sides"])
+    pool.launch_config(fused=m["fused"])
+
+
+@pytest.mark.parametrize("mode", ["strips", "sides", "sides_fused"])
+def test_dam_break_grid_200_steps(mode):
     from peanoclaw_amd.scenarios import dam_break_grid
     pool = new_pool()
     wl = dam_break_grid(pool, 9, 9, 16)
+    _apply_mode(pool, wl, mode)
     orun = OracleRun(pool, wl)
     m0 = pool.mass()
     dts = []
@@ -288,10 +306,12 @@ def test_dam_break_grid_200_steps():
     pool.close()
 
 
-def test_uneven_patches_parity_and_forms():
+@pytest.mark.parametrize("mode", ["strips", "sides", "sides_fused"])
+def test_uneven_patches_parity_and_forms(mode):
     from peanoclaw_amd.scenarios import uneven_patches
     pool = new_pool()
     wl = uneven_patches(pool, num_patches=240, seed=5, field_seed=6)
+    _apply_mode(pool, wl, mode)
     orun = OracleRun(pool, wl)
     for it in range(5):
         pool.step(1)
@@ -316,7 +336,7 @@ def test_uneven_patches_parity_and_forms():
         worst, _, _ = _compare(pool, orun, wl.slots)
         assert worst <= REL_STEP
     # persistent launch over the device work list
-    pool.launch_config(persistent_workgroups=512)
+    pool.launch_config(persistent_workgroups=512, fused=MODES[mode]["fused"])
     pool.step(4)
     for _ in range(4):
         orun.step()
