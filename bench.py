@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 METRIC = "FP64 cell-updates/s + %HBM roofline, batched vs per-patch; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 56  # read h, hu, hv, b; write h, hu, hv (FP64)
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_update_kernel_uneven50k.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_update_uneven50k_pmc.md")
 
 
 def log(*a):
@@ -137,11 +137,16 @@ def main():
     ap.add_argument("--cpu-patches", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--config", default="",
+                    help="fixed launch config 'fused,persistent_workgroups,cap' (skips autotune)")
+    ap.add_argument("--copy-calibration", type=float, default=1.0,
+                    help="GiB copied by the plain copy-kernel probe (HBM rate, counter scale)")
+    ap.add_argument("--copy-reps", type=int, default=5)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build measures the patch path on a "
-              "single MI355X (multi-GPU sharding is described in DESIGN.md, not run).",
+        print("bench.py runs on one GPU only: it measures the patch path on a single "
+              "MI355X; sharding patches over several GPUs is not part of this benchmark.",
               file=sys.stderr)
         sys.exit(2)
 
@@ -160,7 +165,11 @@ def main():
     mass0 = pool.mass()
 
     tuned = None
-    if not args.no_autotune:
+    if args.config:
+        f, pw, cap = (int(v) for v in args.config.split(","))
+        pool.launch_config(persistent_workgroups=pw, fused=f, patches_per_task_cap=cap)
+        tuned = {"fused": f, "persistent_workgroups": pw, "patches_per_task_cap": cap}
+    elif not args.no_autotune:
         tuned = autotune.tune(pool, steps=3, verbose=True)
         log("autotune picked", {k: v for k, v in tuned.items() if k != "candidates"})
 
@@ -188,15 +197,24 @@ def main():
     log(f"graph replay: {1e3 * gwall / args.steps:.3f} ms/step, {graph_value / 1e9:.2f} G/s")
 
     # plain per-patch-launch form on the same pool (same results bitwise)
-    pool.step_per_patch(1)
-    pool.synchronize()
-    t0 = time.perf_counter()
-    pool.step_per_patch(args.per_patch_steps)
-    pool.synchronize()
-    pwall = time.perf_counter() - t0
-    per_patch_value = cells * args.per_patch_steps / pwall
-    log(f"per-patch launches: {1e3 * pwall / args.per_patch_steps:.1f} ms/step, "
-        f"{per_patch_value / 1e9:.3f} G/s")
+    per_patch_value = None
+    if args.per_patch_steps > 0:
+        pool.step_per_patch(1)
+        pool.synchronize()
+        t0 = time.perf_counter()
+        pool.step_per_patch(args.per_patch_steps)
+        pool.synchronize()
+        pwall = time.perf_counter() - t0
+        per_patch_value = cells * args.per_patch_steps / pwall
+        log(f"per-patch launches: {1e3 * pwall / args.per_patch_steps:.1f} ms/step, "
+            f"{per_patch_value / 1e9:.3f} G/s")
+
+    # a plain copy kernel of known size (k_copy_probe): the achievable HBM
+    # rate, and the byte scale of FETCH_SIZE / WRITE_SIZE in a profiled run
+    copy_bytes = int(args.copy_calibration * 2**30) // 16 * 16
+    copy_gbs = pool.copy_probe(copy_bytes, reps=args.copy_reps)
+    log(f"copy probe: {copy_bytes} bytes read + {copy_bytes} written per launch, "
+        f"best {copy_gbs:.0f} GB/s")
 
     mass1 = pool.mass()
     t, dt, est, nsteps = pool.scenario_get(0)
@@ -208,7 +226,9 @@ def main():
                 "traffic": traffic,
                 "kernel": "k_update (row-march f-wave update, all patches)",
                 "algorithmic_bytes_per_launch": cells * BYTES_PER_CELL_UPDATE,
-                "avg_launch_ms": round(upd_ms, 4)}
+                "avg_launch_ms": round(upd_ms, 4),
+                "copy_kernel_gbs": round(copy_gbs, 1),
+                "frac_of_copy_kernel": round(achieved / copy_gbs, 4)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -238,7 +258,7 @@ def main():
         "hbm_frac_whole_step": round(value * BYTES_PER_CELL_UPDATE / (HBM_PEAK_GBS * 1e9), 4),
         "graph_replay_value": graph_value,
         "per_patch_launch_value": per_patch_value,
-        "batched_vs_per_patch": value / per_patch_value,
+        "batched_vs_per_patch": value / per_patch_value if per_patch_value else None,
         "autotune": None if tuned is None else {k: v for k, v in tuned.items()
                                                 if k != "candidates"},
         "mass_rel_drift": abs(mass1 - mass0) / mass0,

@@ -217,6 +217,13 @@ int pc_last_timing(pc_pool *pool, double *total_ms, double *update_ms);
  * tools/SWE_WaveAccumulationBlock.cpp:154-159. */
 int pc_edge_batch(pc_pool *pool, long long n, const double *in, double *out);
 
+/* Diagnostic (no reference counterpart): device-to-device copy of `bytes`
+ * (rounded down to 16) between two scratch buffers, `reps` timed launches of
+ * one plain copy kernel (k_copy_probe) after a warm-up; *best_gbs = (read +
+ * written bytes) / fastest launch.  Gives the achievable HBM copy rate and
+ * the byte scale of hardware traffic counters. */
+int pc_copy_probe(pc_pool *pool, long long bytes, int reps, double *best_gbs);
+
 /* Raw device pointers of the pool (for zero-copy interop); state = 0/1. */
 int pc_pool_device_fields(pc_pool *pool, int state, double **h, double **hu,
                           double **hv, double **b);

@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libpeanoclaw_amd.so")
+# PEANOCLAW_AMD_LIB selects another build of the same library (e.g. a
+# variant compiled with different tuning macros)
+LIB_PATH = os.environ.get("PEANOCLAW_AMD_LIB") or os.path.join(PKG, "libpeanoclaw_amd.so")
 
 PC_OK = 0
 PC_E_INVALID = 1
@@ -126,6 +128,7 @@ _SIGS = {
     "pc_scenario_mass": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
     "pc_last_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "pc_edge_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp]),
+    "pc_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, _dp]),
     "pc_pool_device_fields": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] +
                               [ctypes.POINTER(_dp)] * 4),
 }

@@ -37,17 +37,20 @@ def flags():
     ]
 
 
-def build(verbose=False):
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + flags() + ["-o", tmp, SRC]
+def build(verbose=False, out=None, defines=()):
+    """Compile the library; `defines` are extra -D tuning macros (e.g.
+    PC_UPDATE_WAVES_PER_EU=4) and `out` another output path."""
+    lib = out or LIB
+    tmp = lib + ".tmp"
+    cmd = [hipcc()] + flags() + [f"-D{d}" for d in defines] + ["-o", tmp, SRC]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed for " + SRC)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

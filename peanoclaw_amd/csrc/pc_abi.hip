@@ -1650,6 +1650,39 @@ int pc_edge_batch(pc_pool* p, long long n, const double* in, double* out) {
   return PC_OK;
 }
 
+int pc_copy_probe(pc_pool* p, long long bytes, int reps, double* best_gbs) {
+  if (!p || bytes < 16 || reps < 1 || !best_gbs) return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(p->device);
+  const long long n2 = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, (size_t)n2 * 16));
+  if (hipMalloc(&b, (size_t)n2 * 16) != hipSuccess) {
+    hipFree(a);
+    return fail(PC_E_OOM, "copy probe: hipMalloc failed");
+  }
+  int ncu = 256;
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device);
+  const long long blocks = std::min<long long>((n2 + 255) / 256, (long long)ncu * 16);
+  float best = 1e30f;
+  hipError_t e = hipSuccess;
+  for (int r = 0; r <= reps && e == hipSuccess; ++r) {  // r = 0 is an untimed warm-up
+    if ((e = hipEventRecord(p->ev0, p->stream)) != hipSuccess) break;
+    hipLaunchKernelGGL(k_copy_probe, dim3((unsigned)blocks), dim3(256), 0, p->stream, n2,
+                       (const double2*)(r & 1 ? b : a), r & 1 ? a : b);
+    if ((e = hipGetLastError()) != hipSuccess) break;
+    if ((e = hipEventRecord(p->ev1, p->stream)) != hipSuccess) break;
+    if ((e = hipEventSynchronize(p->ev1)) != hipSuccess) break;
+    float ms = 0.f;
+    if ((e = hipEventElapsedTime(&ms, p->ev0, p->ev1)) != hipSuccess) break;
+    if (r > 0) best = std::min(best, ms);
+  }
+  hipFree(a);
+  hipFree(b);
+  if (e != hipSuccess) return fail(PC_E_HIP, std::string("copy probe: ") + hipGetErrorString(e));
+  *best_gbs = 2.0 * (double)n2 * 16 / (best * 1e-3) / 1e9;
+  return PC_OK;
+}
+
 int pc_pool_device_fields(pc_pool* p, int state, double** h, double** hu, double** hv,
                           double** b) {
   if (!p || (state != 0 && state != 1)) return fail(PC_E_INVALID, "invalid argument");

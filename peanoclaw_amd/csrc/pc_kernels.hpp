@@ -485,10 +485,17 @@ k_speed(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plis
   march_task<kSpeed, SIDES, false>(t, A, L);
 }
 
+// Occupancy target of the update kernel (waves per SIMD): the register
+// budget the compiler may use is 512 / this.
+#ifndef PC_UPDATE_WAVES_PER_EU
+#define PC_UPDATE_WAVES_PER_EU 3
+#endif
+
 // One task per workgroup, or a persistent grid pulling tasks from a
 // device-side counter (reset by k_dt_final before every update launch).
 template <bool FUSED, bool SIDES>
 __global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(PC_UPDATE_WAVES_PER_EU, 8)))
 k_update(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plist,
          const PatchDev* __restrict__ pd, Fields F, const ScnDev* __restrict__ scn,
          const SideDev* __restrict__ sides, EdgeBufs E, double* __restrict__ smax_next,

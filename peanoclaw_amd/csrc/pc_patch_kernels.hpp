@@ -332,4 +332,14 @@ __global__ void k_edge_batch(long long n, const double* __restrict__ in,
   }
 }
 
+// Plain device copy (16-byte lanes, grid-stride): the bandwidth probe whose
+// byte count is known exactly, n2 = number of double2 elements.
+__global__ void __launch_bounds__(256) k_copy_probe(long long n2,
+                                                    const double2* __restrict__ src,
+                                                    double2* __restrict__ dst) {
+  for (long long i = threadIdx.x + (long long)blockIdx.x * blockDim.x; i < n2;
+       i += (long long)blockDim.x * gridDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace pcamd

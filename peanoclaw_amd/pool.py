@@ -199,5 +199,11 @@ class PatchPool:
         check(self.lib.pc_edge_batch(self.h, n, dptr(edges), dptr(out)))
         return out
 
+    def copy_probe(self, nbytes, reps=5):
+        """Best device-to-device copy rate in GB/s (read + written bytes)."""
+        v = ctypes.c_double()
+        check(self.lib.pc_copy_probe(self.h, int(nbytes), int(reps), ctypes.byref(v)))
+        return v.value
+
 
 __all__ = ["PatchPool", "make_description", "iarr"]

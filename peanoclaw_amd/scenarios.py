@@ -95,6 +95,48 @@ def dam_break_grid(pool, px=9, py=9, n=16, scenario=0, centre=(0.5, 0.5), radius
     return wl
 
 
+def wet_dry_basin(X, Y, rng_params):
+    """Bathymetry of the wet/dry basin: a slope from -1 (x = 0) to +0.5
+    (x = 1) plus seeded Gaussian bumps of height <= 0.15."""
+    b = -1.0 + 1.5 * X
+    for cx, cy, s, a in rng_params:
+        b = b + a * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * s * s))
+    return b
+
+
+def wet_dry_grid(pool, px=9, py=9, n=16, seed=4, scenario=0):
+    """Wet/dry stress case on [0,1]^2: sloping basin (b from -1 to +0.5,
+    seeded bumps), still water at surface level 0 where b < 0, dry land
+    (h = 0) elsewhere, and a dam-break column of depth 1.5 over 10% of the
+    domain (x in [0.05, 0.35], y in [0.3, 0.6333]); reflective walls."""
+    rng = np.random.default_rng(seed)
+    params = [(rng.uniform(0.2, 0.9), rng.uniform(0.1, 0.9), rng.uniform(0.03, 0.1),
+               rng.uniform(-0.15, 0.15)) for _ in range(6)]
+    dx = 1.0 / (px * n)
+    slot_of = _register_grid(pool, px, py, n, dx, scenario)
+    pool.scenario_set(scenario, 0.0, 0.0, np.inf)
+    for i in range(px):
+        for j in range(py):
+            s = int(slot_of[i, j])
+            d = pool.description(s)
+            sub = d.size[0] / n
+            ii = np.arange(-1, n + 1)
+            X = d.position[0] + (ii[:, None] + 0.5) * sub
+            Y = d.position[1] + (ii[None, :] + 0.5) * sub
+            X, Y = np.broadcast_arrays(X, Y)
+            b = wet_dry_basin(X, Y, params)
+            h = np.maximum(0.0, 0.0 - b)
+            col = (X >= 0.05) & (X <= 0.35) & (Y >= 0.3) & (Y <= 0.3 + 0.1 / 0.3)
+            h = np.where(col, 1.5, h)
+            unew = np.zeros((3, n, n))
+            unew[0] = h[1:-1, 1:-1]
+            pool.upload(s, unew=unew, b=np.ascontiguousarray(b))
+    wl = Workload(pool, [GridLayout(n, slot_of)], scenario, px * py * n * n,
+                  [int(s) for s in slot_of.ravel()])
+    wl.build_ghost_table()
+    return wl
+
+
 def uneven_class_counts(num_patches, seed=2):
     rng = np.random.default_rng(seed)
     sizes = np.asarray(SIZES)[rng.integers(0, len(SIZES), num_patches)]
@@ -171,5 +213,6 @@ def uneven_patches(pool, num_patches=50000, seed=2, field_seed=3, scenario=0, dx
     return wl
 
 
-__all__ = ["dam_break_grid", "uneven_patches", "uneven_class_counts", "uneven_fields",
+__all__ = ["dam_break_grid", "wet_dry_grid", "uneven_patches", "uneven_class_counts",
+           "uneven_fields",
            "Workload", "GridLayout", "SIZES"]

@@ -345,6 +345,32 @@ def test_uneven_patches_parity_and_forms(mode):
     pool.close()
 
 
+@pytest.mark.parametrize("mode", ["strips", "sides_fused"])
+def test_wet_dry_basin_300_steps(mode):
+    """Dry land, a sloping basin and a dam column: the dry/wet edge
+    branches and the depth/velocity clamps, over 300 steps."""
+    from peanoclaw_amd.scenarios import wet_dry_grid
+    pool = new_pool()
+    wl = wet_dry_grid(pool, 9, 9, 16)
+    _apply_mode(pool, wl, mode)
+    orun = OracleRun(pool, wl)
+    dry0 = sum(int((pool.download(s)[0][0] < 0.01).sum()) for s in wl.slots)
+    assert dry0 > 1000
+    for it in range(300):
+        pool.step(1)
+        orun.step()
+        if it in (0, 9, 99):
+            worst, exact, total = _compare(pool, orun, wl.slots)
+            print(f"wet/dry step {it + 1}: max rel {worst:.3e}, exact {exact}/{total}")
+            assert worst <= REL_STEP
+    worst, exact, total = _compare(pool, orun, wl.slots)
+    dry = sum(int((pool.download(s)[0][0] < 0.01).sum()) for s in wl.slots)
+    print(f"wet/dry after 300 steps: max rel {worst:.3e}, exact {exact}/{total}, "
+          f"dry cells {dry0} -> {dry}")
+    assert worst <= REL_LONG
+    pool.close()
+
+
 def test_refinement_flags_bitexact():
     from peanoclaw_amd.scenarios import dam_break_grid
     pool = new_pool()
