@@ -187,10 +187,25 @@ int pc_scenario_get(pc_pool *pool, int scenario, double *time,
                     double *timestep_size, double *estimated_next_dt,
                     long long *steps);
 /* Execution options: tasks per launch (0 = one task per workgroup,
- * > 0 = persistent grid of that many workgroups), and whether the next-step
- * edge speeds are computed inside the update kernel ("fused"). */
+ * > 0 = persistent grid of that many workgroups), and how the CFL edge
+ * speeds of the next step are obtained (grids with side sources only):
+ *   fused = 0: a separate exact speed pass before every dt reduction;
+ *   fused = 1: exact speeds of every interior edge inside the update kernel;
+ *   fused = 2: pruned - the update bounds every edge speed cheaply and
+ *              computes the exact speed only where it may reach the previous
+ *              CFL limit; an exact fallback pass runs whenever the pruned
+ *              minimum is not proven exact.  dt and the scenario estimate are
+ *              identical in all three modes; in mode 2 a patch's
+ *              estimated_next_timestep_size reports the scenario estimate.
+ * patches_per_task_cap limits patches per wavefront task (0 = no cap). */
 int pc_set_launch_config(pc_pool *pool, int persistent_workgroups, int fused,
                          int patches_per_task_cap);
+/* Tuning of fused = 2 (no reference counterpart): exact speeds are computed
+ * for edges whose bound reaches the speed of a CFL step est * slack
+ * (default 1.03125); the pruned minimum is accepted when it is at most
+ * est * slack * (1 - 1/256), else the exact fallback pass runs.  Results do
+ * not depend on slack; slack < 1 makes every step take the fallback. */
+int pc_set_prune_slack(pc_pool *pool, double slack);
 /* Run k global-dt steps of every scenario that has patches. */
 int pc_step_all(pc_pool *pool, int k);
 /* Plain per-patch-launch form of the same step (reference call pattern:

@@ -2,8 +2,8 @@
 
 Knobs (all keep the results bitwise identical, they only change the
 schedule): whether the next step's CFL speeds are computed inside the
-update kernel (one pass over the cells per step) or by a separate speed
-pass, how many same-size patches share one wavefront task (the segments of
+update kernel (exactly for every edge, or pruned by a cheap bound with an
+exact fallback) or by a separate speed pass, how many same-size patches share one wavefront task (the segments of
 the row march), and whether the update kernel runs one task per workgroup
 or as a persistent grid of N workgroups pulling tasks from the device work
 list (large patches first).  Each candidate is timed over a few steps on
@@ -13,7 +13,7 @@ import itertools
 
 
 def candidates(num_cus=256):
-    fused = [1, 0]
+    fused = [2, 1, 0]
     persistent = [0, 8 * num_cus, 16 * num_cus]
     caps = [0, 2]
     return list(itertools.product(fused, persistent, caps))

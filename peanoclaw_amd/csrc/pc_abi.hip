@@ -44,6 +44,7 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kNumFields = 7;  // h0 h1 hu0 hu1 hv0 hv1 b
 constexpr size_t kBytesPerCell = kNumFields * sizeof(double);
+constexpr int kFallbackWgs = 2048;  // grid of the (usually idle) exact fallback pass
 
 struct Slot {
   bool live = false;
@@ -102,7 +103,8 @@ struct pc_pool {
   size_t strips_cap = 0;
   std::vector<GhostStrip> h_strips;
   std::vector<int> strip_begin;  // per slot (for the per-patch form)
-  int* d_counter = nullptr;
+  int* d_counter = nullptr;  // [0] persistent task counter, [1] pruned-mode redo flag
+  int* d_redo = nullptr;     // = d_counter + 1
   int* d_slot1 = nullptr;  // one-entry patch list of pc_solve_timestep
   double* d_result = nullptr;
   double* d_aux = nullptr;
@@ -117,6 +119,7 @@ struct pc_pool {
   int persistent_wgs = 0;
   int fused = 0;
   int task_cap = 0;
+  double prune_slack = kPruneSlackDefault;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   float last_total_ms = 0.f, last_update_ms = 0.f;
   std::vector<hipEvent_t> upd_ev;  // per-step events around the update launch
@@ -453,11 +456,11 @@ int prepare_batched(pc_pool* p) {
     if (p->sides_any)
       hipLaunchKernelGGL(k_speed<true>, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
                          p->ntasks, p->d_plist, p->d_pd, fields_of(p), p->d_scn, p->d_sides,
-                         edges_of(p), p->d_smax_int);
+                         edges_of(p), p->d_smax_int, nullptr);
     else
       hipLaunchKernelGGL(k_speed<false>, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
                          p->ntasks, p->d_plist, p->d_pd, fields_of(p), p->d_scn, nullptr,
-                         edges_of(p), p->d_smax_int);
+                         edges_of(p), p->d_smax_int, nullptr);
     HIPCHK(hipGetLastError());
     p->fused_ready = true;
   }
@@ -471,6 +474,7 @@ int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
   EdgeBufs E = edges_of(p);
   const SideDev* sides = p->sides_any ? p->d_sides : nullptr;
   const bool fused = p->fused && p->sides_any;
+  const bool prune = fused && p->fused == 2;
   if (p->nstrips > 0) {
     hipLaunchKernelGGL(k_gather, dim3((p->nstrips + 3) / 4), dim3(256), 0, p->stream,
                        p->d_strips, p->nstrips, p->d_pd, F, p->d_scn);
@@ -482,36 +486,55 @@ int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
                          sides, E, p->d_smax_int, p->d_smax);
     else if (sides)
       hipLaunchKernelGGL(k_speed<true>, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
-                         p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E, p->d_smax);
+                         p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E, p->d_smax,
+                         nullptr);
     else
       hipLaunchKernelGGL(k_speed<false>, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
-                         p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E, p->d_smax);
+                         p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E, p->d_smax,
+                         nullptr);
   }
   const int nscn = (int)p->scn.size();
   if (nscn > 0) {
     if (p->nentries > 0)
       hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
                          p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
-                         p->d_pd, p->d_smax, p->d_est);
+                         p->d_pd, p->d_smax, p->d_est, prune ? p->d_redo : nullptr, nullptr);
     hipLaunchKernelGGL(k_dt_final, dim3((nscn + 63) / 64), dim3(64), 0, p->stream, p->d_scn,
-                       nscn, p->d_counter);
+                       nscn, p->d_counter, prune ? 1 : 0, p->d_redo, p->prune_slack);
+    if (prune) {
+      // exact fallback, a no-op unless the pruned minimum was not proven exact
+      if (p->ntasks > 0)
+        hipLaunchKernelGGL(k_speed<true>, dim3(std::min(p->ntasks, kFallbackWgs)), dim3(64), 0,
+                           p->stream, p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn,
+                           sides, E, p->d_smax, p->d_redo);
+      if (p->nentries > 0)
+        hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
+                           p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
+                           p->d_pd, p->d_smax, p->d_est, nullptr, p->d_redo);
+      hipLaunchKernelGGL(k_dt_final, dim3((nscn + 63) / 64), dim3(64), 0, p->stream,
+                         p->d_scn, nscn, p->d_counter, 2, p->d_redo, p->prune_slack);
+    }
   }
   if (p->ntasks > 0) {
     if (timed_update) HIPCHK(hipEventRecord(eb ? eb : p->ev2, p->stream));
     const int grid = p->persistent_wgs > 0 ? p->persistent_wgs : p->ntasks;
     const int pers = p->persistent_wgs > 0 ? 1 : 0;
-    if (fused)
-      hipLaunchKernelGGL((k_update<true, true>), dim3(grid), dim3(64), 0, p->stream,
+    if (prune)
+      hipLaunchKernelGGL((k_update<2, true>), dim3(grid), dim3(64), 0, p->stream,
                          p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
-                         p->d_smax_int, p->d_counter, pers);
+                         p->d_smax_int, p->d_counter, pers, p->prune_slack);
+    else if (fused)
+      hipLaunchKernelGGL((k_update<1, true>), dim3(grid), dim3(64), 0, p->stream,
+                         p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
+                         p->d_smax_int, p->d_counter, pers, p->prune_slack);
     else if (sides)
-      hipLaunchKernelGGL((k_update<false, true>), dim3(grid), dim3(64), 0, p->stream,
+      hipLaunchKernelGGL((k_update<0, true>), dim3(grid), dim3(64), 0, p->stream,
                          p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
-                         p->d_smax_int, p->d_counter, pers);
+                         p->d_smax_int, p->d_counter, pers, p->prune_slack);
     else
-      hipLaunchKernelGGL((k_update<false, false>), dim3(grid), dim3(64), 0, p->stream,
+      hipLaunchKernelGGL((k_update<0, false>), dim3(grid), dim3(64), 0, p->stream,
                          p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
-                         p->d_smax_int, p->d_counter, pers);
+                         p->d_smax_int, p->d_counter, pers, p->prune_slack);
     if (timed_update) HIPCHK(hipEventRecord(ee ? ee : p->ev3, p->stream));
   }
   HIPCHK(hipGetLastError());
@@ -542,8 +565,9 @@ int pc_pool_create(int device, size_t hbm_budget_bytes, pc_pool** out) {
     delete p;
     return fail(PC_E_HIP, "hipStreamCreate failed");
   }
-  HIPCHK(hipMalloc(&p->d_counter, sizeof(int)));
-  HIPCHK(hipMemsetAsync(p->d_counter, 0, sizeof(int), p->stream));
+  HIPCHK(hipMalloc(&p->d_counter, 2 * sizeof(int)));
+  HIPCHK(hipMemsetAsync(p->d_counter, 0, 2 * sizeof(int), p->stream));
+  p->d_redo = p->d_counter + 1;
   HIPCHK(hipMalloc(&p->d_result, 8 * sizeof(double)));
   HIPCHK(hipMalloc(&p->d_slot1, sizeof(int)));
   HIPCHK(hipEventCreate(&p->ev0));
@@ -670,7 +694,10 @@ int pc_patch_get_description(pc_pool* p, int slot, pc_cell_description* out) {
     out->time = S.time;
     out->timestep_size = S.dt;
     double e = 0.0;
-    if (p->d_est && !p->topo_dirty) {
+    if (p->fused == 2 && p->sides_any) {
+      // pruned speeds: only the scenario minimum is exact
+      e = S.est;
+    } else if (p->d_est && !p->topo_dirty) {
       HIPCHK(hipMemcpyAsync(&e, p->d_est + slot, sizeof(double), hipMemcpyDeviceToHost,
                             p->stream));
       HIPCHK(hipStreamSynchronize(p->stream));
@@ -1416,11 +1443,19 @@ int pc_scenario_get(pc_pool* p, int scenario, double* time, double* dt, double* 
   return PC_OK;
 }
 
+int pc_set_prune_slack(pc_pool* p, double slack) {
+  if (!p || !(slack > 0.0) || !(slack < 1e3)) return fail(PC_E_INVALID, "slack must be in (0, 1e3)");
+  p->prune_slack = slack;
+  CHK(invalidate_graph(p));
+  return PC_OK;
+}
+
 int pc_set_launch_config(pc_pool* p, int persistent_wgs, int fused, int cap) {
   if (!p || persistent_wgs < 0 || cap < 0) return fail(PC_E_INVALID, "invalid config");
   p->persistent_wgs = persistent_wgs;
+  if (fused > 2) return fail(PC_E_INVALID, "fused must be 0, 1 or 2");
   if ((fused != 0) != (p->fused != 0)) p->fused_ready = false;
-  p->fused = fused ? 1 : 0;
+  p->fused = fused;
   if (cap != p->task_cap) p->topo_dirty = true;
   p->task_cap = cap;
   CHK(invalidate_graph(p));
@@ -1507,28 +1542,28 @@ int pc_step_all_per_patch(pc_pool* p, int k) {
     for (int i = 0; i < n; ++i) {
       if (sides)
         hipLaunchKernelGGL(k_speed<true>, dim3(1), dim3(64), 0, p->stream, d_single + i, 1,
-                           d_sl, p->d_pd, F, p->d_scn, sides, E, p->d_smax);
+                           d_sl, p->d_pd, F, p->d_scn, sides, E, p->d_smax, nullptr);
       else
         hipLaunchKernelGGL(k_speed<false>, dim3(1), dim3(64), 0, p->stream, d_single + i, 1,
-                           d_sl, p->d_pd, F, p->d_scn, sides, E, p->d_smax);
+                           d_sl, p->d_pd, F, p->d_scn, sides, E, p->d_smax, nullptr);
     }
     if (p->nentries > 0)
       hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
                          p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
-                         p->d_pd, p->d_smax, p->d_est);
+                         p->d_pd, p->d_smax, p->d_est, nullptr, nullptr);
     if (nscn > 0)
       hipLaunchKernelGGL(k_dt_final, dim3((nscn + 63) / 64), dim3(64), 0, p->stream,
-                         p->d_scn, nscn, p->d_counter);
+                         p->d_scn, nscn, p->d_counter, 0, p->d_redo, 1.0);
     // per-patch update, one launch per patch
     for (int i = 0; i < n; ++i) {
       if (sides)
-        hipLaunchKernelGGL((k_update<false, true>), dim3(1), dim3(64), 0, p->stream,
+        hipLaunchKernelGGL((k_update<0, true>), dim3(1), dim3(64), 0, p->stream,
                            d_single + i, 1, d_sl, p->d_pd, F, p->d_scn, sides, E,
-                           p->d_smax_int, p->d_counter, 0);
+                           p->d_smax_int, p->d_counter, 0, 1.0);
       else
-        hipLaunchKernelGGL((k_update<false, false>), dim3(1), dim3(64), 0, p->stream,
+        hipLaunchKernelGGL((k_update<0, false>), dim3(1), dim3(64), 0, p->stream,
                            d_single + i, 1, d_sl, p->d_pd, F, p->d_scn, sides, E,
-                           p->d_smax_int, p->d_counter, 0);
+                           p->d_smax_int, p->d_counter, 0, 1.0);
     }
     HIPCHK(hipGetLastError());
   }

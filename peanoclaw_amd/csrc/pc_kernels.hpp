@@ -112,7 +112,20 @@ struct ScnDev {
   int stepc;          // parity counter: state (stepc & 1) is current
   int advance;        // current step advances this scenario
   int pbeg, pend;     // its patches in the scenario patch list
+  int redo;           // pruned CFL minimum not proven exact: fallback pass
+  int pad;
 };
+
+// Pruned CFL pass (launch mode fused = 2): the update computes exact
+// next-state edge speeds only where a cheap bound (bound_terms) can reach
+// kCfl * min(dx, dy) / (est * slack), est = the scenario's last CFL step.
+// Any edge left out is slower than that, so its patch step exceeds
+// est * slack * (1 - 1e-6) (bound margin 2^-10 minus FP32 and rounding
+// error); the next step accepts the pruned minimum only if it is
+// <= est * slack * kPruneCheckFrac, otherwise an exact speed pass runs
+// first.  dt is therefore the exact minimum over all edges for any slack > 0.
+constexpr double kPruneSlackDefault = 1.03125;
+constexpr double kPruneCheckFrac = 1.0 - 1.0 / 256.0;
 
 struct GhostStrip {
   int dst_slot, dst_cell, dst_stride;
@@ -154,10 +167,16 @@ struct MarchArgs {
   int par;
   double dt;
   double* smax;       // kSpeed: full per-patch max; fused: next-state interior max
+  const ScnDev* scn;  // pruned mode: the scenario's last CFL step (threshold)
+  double slack;       // pruned mode: threshold slack
 };
 
-template <int MODE, bool SIDES, bool FUSED>
+// FUSE: 0 = no next-state speeds, 1 = exact speeds of every interior edge of
+// the new state, 2 = pruned (exact speeds only for bound candidates).
+template <int MODE, bool SIDES, int FUSE>
 __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, MarchLds& L) {
+  constexpr bool FUSED = FUSE != 0;
+  constexpr bool PRUNE = FUSE == 2;
   const int lane = threadIdx.x & 63;
   const int nx = t.nx, ny = t.ny;
   const int W = pow2ceil(ny);
@@ -253,6 +272,12 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   const double dxinv = P.dxinv, dyinv = P.dyinv;
   const double dt = A.dt;
   double smaxl = 0.0;
+  // pruned mode: speed threshold of this lane's patch (0 = every edge exact)
+  float thr = 0.0f;
+  if (PRUNE) {
+    const double est = gl(&A.scn[P.scn].est);
+    if (est > 0.0) thr = (float)(kCfl * dmin(P.dx, P.dy) / (est * A.slack));
+  }
   // ghost-row sources (x sides), kept in scalars
   int kx0 = kSideTile, kx1 = kSideTile;
   long long gx0 = 0, gx1 = 0;  // index of the first interior cell of the ghost row source
@@ -308,6 +333,10 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   CellBase nbase = {};  // new row terms awaiting their y exchange
   double nhv = 0.0;
   bool have_new = false;
+  // pruned: bound terms and raw h, hu of the latest new row
+  BoundTerms pbt = {0.0f, 0.0f, 0.0f};
+  double pnh = 1.0, pnhu = 0.0;
+  float2* const nb = reinterpret_cast<float2*>(&L.nc[0][0]);
 
   double h, hu, hv, b;
   load_row(0, h, hu, hv, b);
@@ -354,7 +383,23 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
               gs(Ed + kERows, hv1);
             }
           }
-          if (FUSED) {
+          if (PRUNE) {
+            const BoundTerms bt = bound_terms(h1, hu1, hv1);
+            if (r >= 3) {
+              const bool cand = active && bound_candidate(pbt.ax, pbt.c, bt.ax, bt.c, thr);
+              if (__any(cand ? 1 : 0)) {
+                const CellQ ea = cell_dir(cell_base(pnh, 0.0), pnhu);
+                const CellQ eb = cell_dir(cell_base(h1, 0.0), hu1);
+                const double sp = edge_speed_any(ea, eb);
+                if (active) smaxl = dmax(smaxl, sp);
+              }
+            }
+            pbt = bt;
+            pnh = h1;
+            pnhu = hu1;
+            nhv = hv1;
+            have_new = true;
+          } else if (FUSED) {
             nbase = cell_base(h1, 0.0);
             const CellQ ncx = cell_dir(nbase, hu1);
             if (r >= 3) {
@@ -381,8 +426,13 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
       L.yc[4][lane] = cy.sq;
       L.yc[5][lane] = cy.c;
       L.yc[6][lane] = cy.f;
+      if (PRUNE && have_new) {
+        nb[lane] = make_float2(pbt.ay, pbt.c);
+        L.nc[1][lane] = pnh;
+        L.nc[2][lane] = nhv;
+      }
       CellQ ncy = {};
-      if (FUSED && have_new) {
+      if (FUSED && !PRUNE && have_new) {
         ncy = cell_dir(nbase, nhv);
         L.nc[0][lane] = ncy.h;
         L.nc[1][lane] = ncy.q;
@@ -392,11 +442,12 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
       }
       wave_sync();
       const int slot = seg * nx + r - 1;
-      const int lm = lane > 0 ? lane - 1 : 0;
+      // left neighbour lane; a lane without one (col 1) reads its own terms,
+      // which makes its y-edge a dummy whose results are not used
+      const int lm = col >= 2 ? lane - 1 : lane;
       CellQ cl;
       cl.h = L.yc[0][lm]; cl.q = L.yc[1][lm]; cl.b = L.yc[2][lm];
       cl.u = L.yc[3][lm]; cl.sq = L.yc[4][lm]; cl.c = L.yc[5][lm]; cl.f = L.yc[6][lm];
-      if (col < 2) cl = cy;  // no left neighbour inside the patch: a dummy edge
       if (MODE == kSpeed) {
         cl.b = 0.0;
         const double sp = edge_speed_any(cl, cy);
@@ -419,11 +470,20 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
           up_h = segok ? L.pre[0][slot] : 0.0;
           up_hv = segok ? L.pre[1][slot] : 0.0;
         }
-        if (FUSED && have_new) {
+        if (PRUNE && have_new) {
+          const float2 lb = nb[lm];
+          const bool cand = col >= 2 && active && bound_candidate(lb.x, lb.y, pbt.ay, pbt.c, thr);
+          if (__any(cand ? 1 : 0)) {
+            const CellQ nr = cell_dir(cell_base(pnh, 0.0), nhv);
+            CellQ nl = cell_dir(cell_base(L.nc[1][lm], 0.0), L.nc[2][lm]);
+            const double sp = edge_speed_any(nl, nr);
+            if (col >= 2 && active) smaxl = dmax(smaxl, sp);
+          }
+        }
+        if (FUSED && !PRUNE && have_new) {
           CellQ nl;
           nl.h = L.nc[0][lm]; nl.q = L.nc[1][lm]; nl.b = 0.0;
           nl.u = L.nc[2][lm]; nl.sq = L.nc[3][lm]; nl.c = L.nc[4][lm]; nl.f = 0.0;
-          if (col < 2) nl = ncy;
           const double sp = edge_speed_any(nl, ncy);
           if (col >= 2 && active) smaxl = dmax(smaxl, sp);
         }
@@ -444,7 +504,24 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
     phv = chv;
   }
 
-  if (FUSED && MODE == kUpdate) {
+  if (PRUNE && MODE == kUpdate) {
+    // the y-edges of the last new row (row nx) are still open
+    nb[lane] = make_float2(pbt.ay, pbt.c);
+    L.nc[1][lane] = pnh;
+    L.nc[2][lane] = nhv;
+    wave_sync();
+    const int lm = col >= 2 ? lane - 1 : lane;
+    const float2 lb = nb[lm];
+    const bool cand = col >= 2 && active && bound_candidate(lb.x, lb.y, pbt.ay, pbt.c, thr);
+    if (__any(cand ? 1 : 0)) {
+      const CellQ nr = cell_dir(cell_base(pnh, 0.0), nhv);
+      CellQ nl = cell_dir(cell_base(L.nc[1][lm], 0.0), L.nc[2][lm]);
+      const double sp = edge_speed_any(nl, nr);
+      if (col >= 2 && active) smaxl = dmax(smaxl, sp);
+    }
+    wave_sync();
+  }
+  if (FUSED && !PRUNE && MODE == kUpdate) {
     // the y-edges of the last new row (row nx) are still open
     const CellQ ncy = cell_dir(nbase, nhv);
     L.nc[0][lane] = ncy.h;
@@ -453,11 +530,10 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
     L.nc[3][lane] = ncy.sq;
     L.nc[4][lane] = ncy.c;
     wave_sync();
-    const int lm = lane > 0 ? lane - 1 : 0;
+    const int lm = col >= 2 ? lane - 1 : lane;
     CellQ nl;
     nl.h = L.nc[0][lm]; nl.q = L.nc[1][lm]; nl.b = 0.0;
     nl.u = L.nc[2][lm]; nl.sq = L.nc[3][lm]; nl.c = L.nc[4][lm]; nl.f = 0.0;
-    if (col < 2) nl = ncy;
     const double sp = edge_speed_any(nl, ncy);
     if (col >= 2 && active) smaxl = dmax(smaxl, sp);
     wave_sync();
@@ -471,18 +547,23 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   }
 }
 
+// Exact CFL edge speeds of the current state, per patch.  The grid strides
+// over the tasks; with `only_if` set the launch does nothing unless
+// *only_if != 0 (the exact fallback of the pruned mode).
 template <bool SIDES>
 __global__ void __launch_bounds__(64)
 k_speed(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plist,
         const PatchDev* __restrict__ pd, Fields F, const ScnDev* __restrict__ scn,
-        const SideDev* __restrict__ sides, EdgeBufs E, double* __restrict__ smax) {
+        const SideDev* __restrict__ sides, EdgeBufs E, double* __restrict__ smax,
+        const int* __restrict__ only_if) {
   __shared__ MarchLds L;
-  const int ti = blockIdx.x;
-  if (ti >= ntasks) return;
-  const Task t = gl(tasks + ti);
-  const int par = gl(&scn[gl(&pd[gl(plist + t.first)].scn)].stepc) & 1;
-  MarchArgs A{plist, pd, F, sides, E, par, 0.0, smax};
-  march_task<kSpeed, SIDES, false>(t, A, L);
+  if (only_if && gl(only_if) == 0) return;
+  for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
+    const Task t = gl(tasks + ti);
+    const int par = gl(&scn[gl(&pd[gl(plist + t.first)].scn)].stepc) & 1;
+    MarchArgs A{plist, pd, F, sides, E, par, 0.0, smax, scn, 1.0};
+    march_task<kSpeed, SIDES, 0>(t, A, L);
+  }
 }
 
 // Occupancy target of the update kernel (waves per SIMD): the register
@@ -493,13 +574,13 @@ k_speed(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plis
 
 // One task per workgroup, or a persistent grid pulling tasks from a
 // device-side counter (reset by k_dt_final before every update launch).
-template <bool FUSED, bool SIDES>
+template <int FUSE, bool SIDES>
 __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(PC_UPDATE_WAVES_PER_EU, 8)))
 k_update(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plist,
          const PatchDev* __restrict__ pd, Fields F, const ScnDev* __restrict__ scn,
          const SideDev* __restrict__ sides, EdgeBufs E, double* __restrict__ smax_next,
-         int* __restrict__ counter, int persistent) {
+         int* __restrict__ counter, int persistent, double slack) {
   __shared__ MarchLds L;
   __shared__ int next;
   int ti = blockIdx.x;
@@ -516,8 +597,8 @@ k_update(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ pli
     if (gl(&S->advance)) {
       // k_dt_final already moved the parity counter past this step
       MarchArgs A{plist, pd, F, sides, E, (gl(&S->stepc) - 1) & 1, gl(&S->dt_applied),
-                  smax_next};
-      march_task<kUpdate, SIDES, FUSED && SIDES>(t, A, L);
+                  smax_next, scn, slack};
+      march_task<kUpdate, SIDES, SIDES ? FUSE : 0>(t, A, L);
     }
     if (!persistent) return;
   }
@@ -629,12 +710,17 @@ __global__ void k_edges_init(const int* __restrict__ slots, int nslots,
 // scenario patch list is cut into chunks; a chunk inside one scenario is
 // reduced in LDS and folded with one atomic min on the bit pattern (all
 // values are positive), otherwise each entry folds its own value.
+// `redo_clear` (pruned mode) is reset here, ahead of the check in
+// k_dt_final; with `only_if` the launch does nothing unless *only_if != 0.
 __global__ void __launch_bounds__(256)
 k_dt_partial(ScnDev* __restrict__ scn, const int* __restrict__ plist_scn,
              const int* __restrict__ scn_of_entry, int nentries,
              const PatchDev* __restrict__ pd, const double* __restrict__ smax,
-             double* __restrict__ est_patch) {
+             double* __restrict__ est_patch, int* __restrict__ redo_clear,
+             const int* __restrict__ only_if) {
   __shared__ double red[256];
+  if (only_if && gl(only_if) == 0) return;
+  if (redo_clear && blockIdx.x == 0 && threadIdx.x == 0) *redo_clear = 0;
   const int beg = blockIdx.x * 1024;
   const int end = min(beg + 1024, nentries);
   if (beg >= end) return;
@@ -662,17 +748,35 @@ k_dt_partial(ScnDev* __restrict__ scn, const int* __restrict__ plist_scn,
 
 // Second stage, one thread per scenario: dt = min(CFL, end - (t + dt_prev)),
 // the time bookkeeping of SWEKernel.cpp:83-87 and the parity flip.
-__global__ void k_dt_final(ScnDev* __restrict__ scn, int nscn, int* __restrict__ counter) {
+//   mode 0: commit every scenario with its minimum;
+//   mode 1 (pruned speeds): commit a scenario that advances only if its
+//           minimum is proven exact (<= est * slack * kPruneCheckFrac), else flag it
+//           and set *redo for the exact fallback pass;
+//   mode 2 (after the fallback pass; nothing unless *redo): commit the
+//           flagged scenarios, reset the running minimum of all.
+__global__ void k_dt_final(ScnDev* __restrict__ scn, int nscn, int* __restrict__ counter,
+                           int mode, int* __restrict__ redo, double slack) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mode == 2 && gl(redo) == 0) return;
   if (s == 0 && counter) *counter = 0;
   if (s >= nscn) return;
   ScnDev& S = scn[s];
   const double cfl = __longlong_as_double((long long)S.minbits);
   S.minbits = (unsigned long long)__double_as_longlong(1.7976931348623157e308);
   const double maxDt = S.end_time - (S.time + S.dt);
+  const bool can_advance = S.pend > S.pbeg && maxDt - 0.0 > kLaEps;
+  if (mode == 1 && can_advance && !(cfl <= S.est * (slack * kPruneCheckFrac))) {
+    S.redo = 1;
+    *redo = 1;
+    return;
+  }
+  if (mode == 2) {
+    if (!S.redo) return;
+    S.redo = 0;
+  }
   int adv = 0;
   double dt = 0.0;
-  if (S.pend > S.pbeg && maxDt - 0.0 > kLaEps) {
+  if (can_advance) {
     dt = fmin(cfl, maxDt);
     adv = 1;
     if (dt - 0.0 > kLaEps) {

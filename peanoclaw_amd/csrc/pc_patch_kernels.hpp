@@ -27,8 +27,8 @@ k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restric
   Task t;
   t.first = 0; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;
   EdgeBufs none{{nullptr, nullptr}, nullptr};
-  MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax};
-  march_task<kSpeed, false, false>(t, A, L);
+  MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
+  march_task<kSpeed, false, 0>(t, A, L);
   __syncthreads();
   const double s = smax[slot];
   const double cfl = max_timestep(s, P.dx, P.dy);
@@ -36,7 +36,7 @@ k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restric
   A.par = cur;
   A.dt = dt;
   A.smax = nullptr;
-  march_task<kUpdateSwap, false, false>(t, A, L);
+  march_task<kUpdateSwap, false, 0>(t, A, L);
   if (threadIdx.x == 0) {
     out[0] = dt;
     out[1] = cfl;

@@ -32,6 +32,44 @@ struct CellQ {
   double u, sq, c, f;
 };
 
+// FP64 square root and quotient without the exponent-scaling steps of the
+// general IEEE sequences.  sqrt_nr: hardware 1/sqrt estimate refined by one
+// Goldschmidt step and two Newton corrections on the remainder x - g*g;
+// div_nr: hardware reciprocal refined by two Newton steps, quotient
+// corrected once with the remainder a - b*q (Markstein's correction),
+// special operands (zeros, infinities, NaN) resolved by v_div_fixup.  Both
+// return the correctly rounded result (bit-identical to sqrt() and a / b)
+// whenever the scaling would be the identity: x in [2^-767, DBL_MAX] for
+// sqrt_nr; for div_nr b and a / b normal and well inside the exponent range,
+// a zero or normal.  Every use below stays in that range for a wet cell
+// (h >= 0.01; momenta, speeds and Roe averages of moderate size); the terms
+// of dry cells fall outside it (h = 0) and are replaced by the dry
+// substitution before anything reads them.
+__device__ __forceinline__ double sqrt_nr(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double hh = y * 0.5;
+  const double r = fma(-hh, g, 0.5);
+  g = fma(g, r, g);
+  hh = fma(hh, r, hh);
+  double d = fma(-g, g, x);
+  g = fma(d, hh, g);
+  d = fma(-g, g, x);
+  g = fma(d, hh, g);
+  return g;
+}
+
+__device__ __forceinline__ double div_nr(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  const double q = a * r;
+  const double rem = fma(-b, q, a);
+  return __builtin_amdgcn_div_fixup(fma(rem, r, q), b, a);
+}
+
 __device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
 __device__ __forceinline__ double dmax(double a, double b) { return a < b ? b : a; }
 
@@ -44,8 +82,8 @@ __device__ __forceinline__ CellBase cell_base(double h, double b) {
   CellBase r;
   r.h = h;
   r.b = b;
-  r.sq = sqrt(h);
-  r.c = sqrt(kGravity * h);
+  r.sq = sqrt_nr(h);
+  r.c = sqrt_nr(kGravity * h);
   r.p = (kHalfG * h) * h;
   return r;
 }
@@ -55,7 +93,7 @@ __device__ __forceinline__ CellQ cell_dir(const CellBase& cb, double q) {
   r.h = cb.h;
   r.q = q;
   r.b = cb.b;
-  r.u = q / cb.h;
+  r.u = div_nr(q, cb.h);
   r.sq = cb.sq;
   r.c = cb.c;
   r.f = q * r.u + cb.p;
@@ -81,8 +119,8 @@ __device__ __forceinline__ void apply_dry(CellQ& L, CellQ& R, bool& leftDry,
 __device__ __forceinline__ void wave_speeds(const CellQ& L, const CellQ& R,
                                             double& s1, double& s2) {
   const double hRoe = 0.5 * (L.h + R.h);
-  const double uRoe = (L.u * L.sq + R.u * R.sq) / (L.sq + R.sq);
-  const double cRoe = sqrt(kGravity * hRoe);
+  const double uRoe = div_nr(L.u * L.sq + R.u * R.sq, L.sq + R.sq);
+  const double cRoe = sqrt_nr(kGravity * hRoe);
   s1 = dmin(L.u - L.c, uRoe - cRoe);
   s2 = dmax(R.u + R.c, uRoe + cRoe);
 }
@@ -107,7 +145,7 @@ __device__ __forceinline__ void edge_solve(CellQ L, CellQ R, double& dhL,
   wave_speeds(L, R, s1, s2);
   const double d0 = R.q - L.q;
   const double d1 = (R.f - L.f) + (kHalfG * (L.h + R.h)) * (R.b - L.b);
-  const double inv = 1.0 / (s2 - s1);
+  const double inv = div_nr(1.0, s2 - s1);
   const double beta1 = (s2 * d0 - d1) * inv;
   const double beta2 = (d1 - s1 * d0) * inv;
   const double z1 = beta1 * s1, z2 = beta2 * s2;
@@ -153,7 +191,7 @@ __device__ __forceinline__ void edge_solve_wet(const CellQ& L, const CellQ& R, d
   wave_speeds(L, R, s1, s2);
   const double d0 = R.q - L.q;
   const double d1 = (R.f - L.f) + (kHalfG * (L.h + R.h)) * (R.b - L.b);
-  const double inv = 1.0 / (s2 - s1);
+  const double inv = div_nr(1.0, s2 - s1);
   const double beta1 = (s2 * d0 - d1) * inv;
   const double beta2 = (d1 - s1 * d0) * inv;
   const double z1 = beta1 * s1, z2 = beta2 * s2;
@@ -194,6 +232,39 @@ __device__ __forceinline__ void edge_solve_any(const CellQ& L, const CellQ& R, d
 __device__ __forceinline__ double edge_speed_any(const CellQ& L, const CellQ& R) {
   if (wave_all(L.h >= kDryTol && R.h >= kDryTol)) return edge_speed_wet(L, R);
   return edge_speed(L, R);
+}
+
+// ---- cheap upper bounds of edge speeds (pruned CFL pass) ----
+//
+// For the Einfeldt speeds above, |s1|, |s2| <= max(|uL|,|uR|) + max(cL,cR):
+// the Roe velocity is a convex combination of uL and uR, and
+// sqrt(g (hL+hR)/2) <= max(cL, cR).  A dry side is replaced by the mirror
+// of the wet side (same |u| and c), and two dry sides give speed 0, so a dry
+// cell may contribute (0, 0).  The terms are evaluated in FP32 with the
+// hardware reciprocal and square root (relative error below 1e-6); callers
+// compare with a margin of 2^-10, far above that error.
+struct BoundTerms {
+  float ax, ay, c;  // |hu/h|, |hv/h|, sqrt(g h); all 0 for a dry cell
+};
+
+__device__ __forceinline__ BoundTerms bound_terms(double h, double hu, double hv) {
+  BoundTerms t;
+  const float hf = (float)h;
+  const float inv = __builtin_amdgcn_rcpf(hf);
+  const bool wet = h >= kDryTol;
+  t.ax = wet ? fabsf((float)hu) * inv : 0.0f;
+  t.ay = wet ? fabsf((float)hv) * inv : 0.0f;
+  t.c = wet ? __builtin_amdgcn_sqrtf(9.81f * hf) : 0.0f;
+  return t;
+}
+
+constexpr float kBoundMargin = 1.0f + 1.0f / 1024.0f;
+
+// True unless the bound proves the edge slower than `thr` (NaN -> true).
+__device__ __forceinline__ bool bound_candidate(float aL, float cL, float aR, float cR,
+                                                float thr) {
+  const float b = (fmaxf(aL, aR) + fmaxf(cL, cR)) * kBoundMargin;
+  return !(b < thr);
 }
 
 // SWE_WaveAccumulationBlock::computeNumericalFluxes (:225-239): CFL step of

@@ -153,6 +153,10 @@ class PatchPool:
         check(self.lib.pc_set_launch_config(self.h, persistent_workgroups, fused,
                                             patches_per_task_cap))
 
+    def prune_slack(self, slack):
+        """Threshold slack of the pruned speed mode (fused=2); see the C header."""
+        check(self.lib.pc_set_prune_slack(self.h, float(slack)))
+
     def step(self, k=1):
         check(self.lib.pc_step_all(self.h, k))
 
