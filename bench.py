@@ -9,7 +9,9 @@ of every patch with one global dt.  One "step" advances all 68M cells once.
 Prints one JSON line: cell-updates/s of the batched path, its HBM roofline
 fraction (update kernel, HIP events around every launch of the timed
 steps), the plain per-patch-launch path on the same pool, the hipGraph
-replay rate, and the CPU oracle timed on the host cores on a sample.
+replay rate, p50/p99 time-to-solution of a served stream of independent
+dam-break requests (Poisson arrivals, continuous batching under an HBM
+budget), and the CPU oracle timed on the host cores on a sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -126,6 +128,39 @@ def cpu_baseline(num_patches, seconds, threads):
                       f"FP64 oracle in C (gcc -O2 -ffp-contract=off, OpenMP)"}
 
 
+def serve_stream(args, tuned):
+    """Served stream: independent dam-break requests (grid log-uniform in
+    64^2..2048^2 cells, 32^2-cell patches, t_end = 1) arriving as a Poisson
+    stream, continuously batched on one pool with an HBM budget."""
+    import numpy as np
+
+    from peanoclaw_amd.pool import PatchPool
+    from peanoclaw_amd.serving import Server, run_stream, served_requests
+
+    reqs = served_requests(args.serve_requests, seed=5)
+    with PatchPool(0, hbm_budget_bytes=args.serve_budget_mib << 20) as pool:
+        if tuned:
+            pool.launch_config(persistent_workgroups=tuned["persistent_workgroups"],
+                               fused=tuned["fused"],
+                               patches_per_task_cap=tuned["patches_per_task_cap"])
+        with Server(pool, max_active=64) as srv:
+            res, lat, wall, cells = run_stream(srv, reqs, rate=args.serve_rate, seed=5,
+                                               steps_per_round=args.serve_steps_per_round)
+    sizes = [r.cells_per_side for r in reqs]
+    out = {"requests": len(reqs), "arrival_rate_per_s": args.serve_rate,
+           "grid_cells_per_side_min": min(sizes), "grid_cells_per_side_max": max(sizes),
+           "hbm_budget_mib": args.serve_budget_mib, "max_active": 64,
+           "steps_per_round": args.serve_steps_per_round,
+           "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+           "mean_s": float(lat.mean()), "max_s": float(lat.max()), "wall_s": wall,
+           "cell_updates": int(cells), "throughput_cell_updates_per_s": cells / wall,
+           "total_steps": int(sum(r.steps for r in res.values()))}
+    log(f"served stream: {len(reqs)} requests, p50 {out['p50_s']:.3f} s, "
+        f"p99 {out['p99_s']:.3f} s, {out['throughput_cell_updates_per_s'] / 1e9:.2f} G/s "
+        f"over {wall:.1f} s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +177,11 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=1.0,
                     help="GiB copied by the plain copy-kernel probe (HBM rate, counter scale)")
     ap.add_argument("--copy-reps", type=int, default=5)
+    ap.add_argument("--serve-requests", type=int, default=32,
+                    help="requests of the served stream (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-budget-mib", type=int, default=512)
+    ap.add_argument("--serve-steps-per-round", type=int, default=8)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -230,6 +270,10 @@ def main():
                 "copy_kernel_gbs": round(copy_gbs, 1),
                 "frac_of_copy_kernel": round(achieved / copy_gbs, 4)}
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve_stream(args, tuned)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -262,6 +306,9 @@ def main():
         "autotune": None if tuned is None else {k: v for k, v in tuned.items()
                                                 if k != "candidates"},
         "mass_rel_drift": abs(mass1 - mass0) / mass0,
+        "p50_solve_latency_s": serving["p50_s"] if serving else None,
+        "p99_solve_latency_s": serving["p99_s"] if serving else None,
+        "serving": serving,
         "sim_time": t, "steps_done": nsteps,
     }
     print(json.dumps(out), flush=True)

@@ -239,6 +239,63 @@ int pc_edge_batch(pc_pool *pool, long long n, const double *in, double *out);
  * the byte scale of hardware traffic counters. */
 int pc_copy_probe(pc_pool *pool, long long bytes, int reps, double *best_gbs);
 
+/* ---- serving: a stream of independent dam-break scenarios ----
+ *
+ * No reference counterpart: PeanoClaw runs one simulation per process
+ * (native/sweMain.cpp, peanoclaw/runners).  A server keeps every admitted
+ * scenario's patches and state resident in the pool and advances all of
+ * them in shared batched steps (pc_step_all), each with its own CFL dt and
+ * end time.  Calls on a server are thread-compatible; destroy the server
+ * before its pool. */
+#define PC_MAX_HILLS 8
+typedef struct {
+  int32_t cells_per_side;   /* N: N x N cells on [0,1]^2                    */
+  int32_t patch_cells;      /* n: patches of n x n cells, N % n == 0, n<=64 */
+  double end_time;          /* simulated time to reach                      */
+  double column_x, column_y, column_radius, column_height;
+  double depth;             /* water depth outside the column               */
+  int32_t num_hills;        /* Gaussian hills of the bathymetry (<= 8)      */
+  int32_t reserved;
+  double hill_x[PC_MAX_HILLS], hill_y[PC_MAX_HILLS];
+  double hill_sigma[PC_MAX_HILLS], hill_height[PC_MAX_HILLS];
+} pc_request;
+/* Initial state: h = column_height where |x - (column_x, column_y)| <=
+ * column_radius, depth elsewhere; hu = hv = 0; b = sum of
+ * hill_height * exp(-|x - hill|^2 / hill_sigma^2) at every cell centre
+ * (ghost ring included); reflective walls at the box. */
+
+typedef struct {
+  int32_t id;
+  int32_t scenario;          /* pool scenario id it ran under              */
+  long long steps;           /* time steps taken                           */
+  long long cell_updates;    /* steps * N^2                                */
+  double time;               /* simulated time reached                     */
+  double mass;               /* final water volume sum(h) dx dy            */
+  uint64_t state_hash;       /* order-independent digest of the bits of   */
+                             /* h, hu, hv of every interior cell           */
+  long long admitted_round, completed_round;
+} pc_request_result;
+
+typedef struct pc_server pc_server;
+/* keep_completed != 0: a finished request keeps its patches (readable via
+ * pc_server_request_slots / pc_patch_download) until pc_server_release. */
+int pc_server_create(pc_pool *pool, int max_active, int keep_completed,
+                     pc_server **out);
+int pc_server_destroy(pc_server *srv);
+/* Queue a request (arrival order); PC_E_OOM if it can never fit. */
+int pc_server_submit(pc_server *srv, const pc_request *req, int *out_id);
+/* One scheduling round: admit waiting requests that fit the pool's HBM
+ * budget and the active limit (first fit in arrival order), run `steps`
+ * batched steps of all active requests, retire those that reached their end
+ * time.  Reports the active and queued counts after the round. */
+int pc_server_round(pc_server *srv, int steps, int *active, int *queued);
+/* Results of retired requests, in completion order (each reported once). */
+int pc_server_poll(pc_server *srv, pc_request_result *out, int cap, int *n);
+/* Slots of an active or kept request in grid order (k = i * (N/n) + j);
+ * with slots == NULL only *n is set. */
+int pc_server_request_slots(pc_server *srv, int id, int *slots, int cap, int *n);
+int pc_server_release(pc_server *srv, int id);
+
 /* Raw device pointers of the pool (for zero-copy interop); state = 0/1. */
 int pc_pool_device_fields(pc_pool *pool, int state, double **h, double **hu,
                           double **hv, double **b);

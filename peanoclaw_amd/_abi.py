@@ -56,6 +56,44 @@ GHOST_STRIP_DTYPE = np.dtype([(n, np.int32) for n in (
     "src_stride", "length", "negate")])
 
 
+MAX_HILLS = 8
+
+
+class Request(ctypes.Structure):
+    """pc_request: one dam-break scenario of the served stream."""
+    _fields_ = [
+        ("cells_per_side", ctypes.c_int32),
+        ("patch_cells", ctypes.c_int32),
+        ("end_time", ctypes.c_double),
+        ("column_x", ctypes.c_double),
+        ("column_y", ctypes.c_double),
+        ("column_radius", ctypes.c_double),
+        ("column_height", ctypes.c_double),
+        ("depth", ctypes.c_double),
+        ("num_hills", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("hill_x", ctypes.c_double * MAX_HILLS),
+        ("hill_y", ctypes.c_double * MAX_HILLS),
+        ("hill_sigma", ctypes.c_double * MAX_HILLS),
+        ("hill_height", ctypes.c_double * MAX_HILLS),
+    ]
+
+
+class RequestResult(ctypes.Structure):
+    """pc_request_result."""
+    _fields_ = [
+        ("id", ctypes.c_int32),
+        ("scenario", ctypes.c_int32),
+        ("steps", ctypes.c_longlong),
+        ("cell_updates", ctypes.c_longlong),
+        ("time", ctypes.c_double),
+        ("mass", ctypes.c_double),
+        ("state_hash", ctypes.c_uint64),
+        ("admitted_round", ctypes.c_longlong),
+        ("completed_round", ctypes.c_longlong),
+    ]
+
+
 class PeanoClawAmdError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"peanoclaw_amd error {code}: {msg}")
@@ -130,6 +168,16 @@ _SIGS = {
     "pc_last_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "pc_edge_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp]),
     "pc_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, _dp]),
+    "pc_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "pc_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "pc_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Request), _ip]),
+    "pc_server_round": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip]),
+    "pc_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RequestResult),
+                                      ctypes.c_int, _ip]),
+    "pc_server_request_slots": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip,
+                                               ctypes.c_int, _ip]),
+    "pc_server_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pc_pool_device_fields": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] +
                               [ctypes.POINTER(_dp)] * 4),
 }

@@ -298,6 +298,7 @@ int rebuild_topology(pc_pool* p) {
   for (int i = 0; i < ns; ++i)
     if (p->slots[i].live) pd[i] = make_patchdev(p->slots[i]);
   const size_t old_cap = p->pd_cap;
+  bool realloc = false;
   CHK(ensure_dev(&p->d_pd, &p->pd_cap, pd.size()));
   HIPCHK(hipMemcpyAsync(p->d_pd, pd.data(), pd.size() * sizeof(PatchDev),
                         hipMemcpyHostToDevice, p->stream));
@@ -317,6 +318,7 @@ int rebuild_topology(pc_pool* p) {
       HIPCHK(hipMemsetAsync(*arrs[k], 0, sizes[k], p->stream));
     }
     p->edges_dirty = true;
+    realloc = true;
   }
   p->h_sides.resize(ns);
   bool any = false;
@@ -416,7 +418,10 @@ int rebuild_topology(pc_pool* p) {
   CHK(upload_scn(p));
   HIPCHK(hipStreamSynchronize(p->stream));
   p->topo_dirty = false;
-  p->fused_ready = false;
+  // The interior speed maxima of unchanged patches stay valid; the pruned
+  // mode re-derives those of scenarios whose patch set changed through its
+  // exact fallback (their estimate is reset on register / release).
+  if (p->fused != 2 || realloc) p->fused_ready = false;
   CHK(invalidate_graph(p));
   return PC_OK;
 }
@@ -670,6 +675,7 @@ int pc_patch_register(pc_pool* p, const pc_cell_description* d, int* out_slot) {
     p->scn[d->scenario].time = d->time;
     p->scn[d->scenario].dt = d->timestep_size;
   }
+  p->scn[d->scenario].est = 0.0;  // patch set changed: no pruning until exact again
   p->scn_dev_dirty = true;
   p->topo_dirty = true;
   *out_slot = slot;
@@ -678,6 +684,9 @@ int pc_patch_register(pc_pool* p, const pc_cell_description* d, int* out_slot) {
 
 int pc_patch_release(pc_pool* p, int slot) {
   if (!valid_slot(p, slot)) return fail(PC_E_INVALID, "invalid slot");
+  CHK(sync_scn_host(p));
+  p->scn[p->slots[slot].desc.scenario].est = 0.0;
+  p->scn_dev_dirty = true;
   free_cells(p, p->slots[slot].off, p->slots[slot].cells);
   p->slots[slot].live = false;
   if ((size_t)slot < p->h_sides.size()) p->h_sides[slot] = SideDev{};
@@ -717,7 +726,12 @@ int pc_patch_set_description(pc_pool* p, int slot, const pc_cell_description* d)
   bool geom = d->size[0] != s.desc.size[0] || d->size[1] != s.desc.size[1];
   s.desc = *d;
   s.clock_from_scenario = false;
-  if (geom) p->topo_dirty = true;
+  if (geom) {
+    p->topo_dirty = true;
+    CHK(sync_scn_host(p));
+    p->scn[d->scenario].est = 0.0;
+    p->scn_dev_dirty = true;
+  }
   return PC_OK;
 }
 
@@ -1729,3 +1743,5 @@ int pc_pool_device_fields(pc_pool* p, int state, double** h, double** hu, double
 }
 
 }  // extern "C"
+
+#include "pc_server.hpp"

@@ -549,7 +549,8 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
 
 // Exact CFL edge speeds of the current state, per patch.  The grid strides
 // over the tasks; with `only_if` set the launch does nothing unless
-// *only_if != 0 (the exact fallback of the pruned mode).
+// *only_if != 0, and then covers only the tasks of scenarios flagged for
+// the exact fallback of the pruned mode (a task holds one scenario).
 template <bool SIDES>
 __global__ void __launch_bounds__(64)
 k_speed(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plist,
@@ -560,7 +561,10 @@ k_speed(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plis
   if (only_if && gl(only_if) == 0) return;
   for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
     const Task t = gl(tasks + ti);
-    const int par = gl(&scn[gl(&pd[gl(plist + t.first)].scn)].stepc) & 1;
+    const ScnDev* S = scn + gl(&pd[gl(plist + t.first)].scn);
+    // fallback: only the scenarios whose pruned minimum was rejected
+    if (only_if && gl(&S->redo) == 0) continue;
+    const int par = gl(&S->stepc) & 1;
     MarchArgs A{plist, pd, F, sides, E, par, 0.0, smax, scn, 1.0};
     march_task<kSpeed, SIDES, 0>(t, A, L);
   }

@@ -332,6 +332,86 @@ __global__ void k_edge_batch(long long n, const double* __restrict__ in,
   }
 }
 
+// ---- served requests ----
+
+constexpr int kMaxHills = 8;
+
+// Initial state of one dam-break request on [0,1]^2: h = column height
+// inside the disc around (cx, cy), `depth` elsewhere, hu = hv = 0, and
+// b = sum of Gaussian hills a_k exp(-|x - c_k|^2 / s2_k), evaluated at
+// every cell centre of the tile including the ghost ring.
+struct RequestInit {
+  double dx;
+  double cx, cy, r2, col_h, depth;
+  int nh, P, n, cur;
+  double hx[kMaxHills], hy[kMaxHills], hs2[kMaxHills], ha[kMaxHills];
+};
+
+// One workgroup per patch; slots[k] holds grid patch (k / P, k % P).
+__global__ void __launch_bounds__(256)
+k_init_request(const int* __restrict__ slots, int nslots, const PatchDev* __restrict__ pd,
+               Fields F, RequestInit a) {
+  const int k = blockIdx.x;
+  if (k >= nslots) return;
+  const PatchDev P = pd[slots[k]];
+  const int pi = k / a.P, pj = k - (k / a.P) * a.P;
+  const int s = a.n + 2;
+  for (int c = threadIdx.x; c < s * s; c += blockDim.x) {
+    const int x = c / s, y = c - (c / s) * s;
+    const double X = ((double)(pi * a.n + x - 1) + 0.5) * a.dx;
+    const double Y = ((double)(pj * a.n + y - 1) + 0.5) * a.dx;
+    double b = 0.0;
+    for (int m = 0; m < a.nh; ++m) {
+      const double ex = X - a.hx[m], ey = Y - a.hy[m];
+      b += a.ha[m] * exp(-(ex * ex + ey * ey) / a.hs2[m]);
+    }
+    const double ex = X - a.cx, ey = Y - a.cy;
+    const long long i = P.off + c;
+    F.b[i] = b;
+    F.H(a.cur)[i] = (ex * ex + ey * ey <= a.r2) ? a.col_h : a.depth;
+    F.HU(a.cur)[i] = 0.0;
+    F.HV(a.cur)[i] = 0.0;
+  }
+}
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Order-independent 64-bit digest of the interior bits of h, hu, hv of a
+// grid of patches (slots in grid order): sum over cells of a mixed hash of
+// (cell index, bits of h, hu, hv).  Equal states give equal digests; any
+// changed bit changes the digest except with probability ~2^-64.
+__global__ void __launch_bounds__(256)
+k_hash_request(const int* __restrict__ slots, int nslots, const PatchDev* __restrict__ pd,
+               Fields F, int cur, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[256];
+  const int k = blockIdx.x;
+  if (k >= nslots) return;
+  const PatchDev P = pd[slots[k]];
+  const int nx = P.nx, ny = P.ny;
+  unsigned long long acc = 0;
+  for (int c = threadIdx.x; c < nx * ny; c += blockDim.x) {
+    const int x = c / ny, y = c - (c / ny) * ny;
+    const long long i = P.off + (long long)(x + 1) * (ny + 2) + (y + 1);
+    unsigned long long v = mix64((unsigned long long)k * (unsigned long long)(nx * ny) + c);
+    v = mix64(v ^ (unsigned long long)__double_as_longlong(F.H(cur)[i]));
+    v = mix64(v ^ (unsigned long long)__double_as_longlong(F.HU(cur)[i]));
+    v = mix64(v ^ (unsigned long long)__double_as_longlong(F.HV(cur)[i]));
+    acc += v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int m = 128; m > 0; m >>= 1) {
+    if ((int)threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
 // Plain device copy (16-byte lanes, grid-stride): the bandwidth probe whose
 // byte count is known exactly, n2 = number of double2 elements.
 __global__ void __launch_bounds__(256) k_copy_probe(long long n2,

@@ -95,6 +95,9 @@ class PatchPool:
         check(self.lib.pc_patch_upload(self.h, slot, dptr(unew), dptr(uold), dptr(b)))
 
     def download(self, slot):
+        if slot not in self._dims:  # registered inside the library (e.g. by a server)
+            d = self.description(slot)
+            self._dims[slot] = (d.subdivision_factor[0], d.subdivision_factor[1])
         nx, ny = self._dims[slot]
         unew = np.empty((3, nx, ny))
         uold = np.empty((3, nx + 2, ny + 2))
