@@ -119,7 +119,7 @@ def cpu_baseline(num_patches, seconds, threads):
         step()
         nsteps += 1
         el = time.perf_counter() - t0
-        if el >= seconds or nsteps >= 200:
+        if el >= seconds or nsteps >= 5000:
             break
     return {"value": cells * nsteps / el, "unit": "cell-updates/s", "cores": threads,
             "kind": "port",

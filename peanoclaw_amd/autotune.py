@@ -10,6 +10,7 @@ list (large patches first).  Each candidate is timed over a few steps on
 the real pool; the fastest is applied and returned.
 """
 import itertools
+import sys
 
 
 def candidates(num_cus=256):
@@ -35,7 +36,8 @@ def tune(pool, steps=3, num_cus=256, verbose=False):
                         "update_ms": upd_ms})
         if verbose:
             print(f"autotune: fused={fused} persistent={pw:5d} cap={cap} -> "
-                  f"{total_ms / steps:.3f} ms/step (update {upd_ms:.3f} ms)", flush=True)
+                  f"{total_ms / steps:.3f} ms/step (update {upd_ms:.3f} ms)", file=sys.stderr,
+                  flush=True)
         if best is None or total_ms < best[0]:
             best = (total_ms, fused, pw, cap)
     pool.launch_config(persistent_workgroups=best[2], fused=best[1], patches_per_task_cap=best[3])
