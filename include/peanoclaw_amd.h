@@ -105,6 +105,11 @@ int pc_pool_copy_fields(pc_pool *pool, int to_device, int state,
 int pc_solve_timestep(pc_pool *pool, int slot, double maximum_timestep_size,
                       const int boundary_flags[4], double *out_dt,
                       double *out_estimated_next_dt);
+/* The CFL step solveTimestep would take on this patch with an unbounded
+ * maximumTimestepSize (SWEKernel.cpp:60-68, computeNumericalFluxes' rule
+ * SWE_WaveAccumulationBlock.cpp:225-239), without updating anything: the
+ * minimum over patches gives a global time step. */
+int pc_patch_cfl(pc_pool *pool, int slot, double *out_dt);
 /* Numerics::fillBoundaryLayer (Numerics.h:131-135; SWEKernel.cpp:176-237) */
 int pc_fill_boundary_layer(pc_pool *pool, int slot, int dimension,
                            int set_upper);

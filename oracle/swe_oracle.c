@@ -304,7 +304,7 @@ void or_copy_unew_to_uold(or_patch *p) {
 }
 
 /* SWEKernel::solveTimestep -> advanceBlockInTime (SWEKernel.cpp:50-88) */
-void or_solve_timestep(or_patch *p, double maximumTimestepSize) {
+double or_solve_timestep(or_patch *p, double maximumTimestepSize) {
   double *h, *hu, *hv, *b;
   block_arrays(p, &h, &hu, &hv, &b);
   int area = (p->nx + 2) * (p->ny + 2);
@@ -323,6 +323,22 @@ void or_solve_timestep(or_patch *p, double maximumTimestepSize) {
     p->dt = dt;
   }
   p->est_next = estimatedNext;
+  return dt;
+}
+
+/* The CFL step solveTimestep would take with an unbounded maximum
+ * (SWEKernel.cpp:60-68 without the update): max edge speed over uOld with
+ * its ghost layer -> computeNumericalFluxes' maxTimestep rule. */
+double or_patch_cfl(or_patch *p) {
+  double *h, *hu, *hv, *b;
+  block_arrays(p, &h, &hu, &hv, &b);
+  int area = (p->nx + 2) * (p->ny + 2);
+  double *net = (double *)calloc(3 * (size_t)area, sizeof(double));
+  double dx = p->subcell[0], dy = p->subcell[1];
+  double smax = or_block_fluxes(p->nx, p->ny, dx, dy, h, hu, hv, b, net,
+                                net + area, net + 2 * area);
+  free(net);
+  return or_block_max_timestep(smax, dx, dy);
 }
 
 /* SWEKernel::fillBoundaryLayer (SWEKernel.cpp:176-237): reflective wall */

@@ -84,7 +84,8 @@ void or_block_update(int nx, int ny, double dt, double *h, double *hu,
                      double *hv, double *netH, double *netHu, double *netHv);
 
 /* ---- Numerics (SWEKernel) on or_patch ---- */
-void or_solve_timestep(or_patch *p, double maximumTimestepSize);
+double or_solve_timestep(or_patch *p, double maximumTimestepSize);
+double or_patch_cfl(or_patch *p);
 void or_fill_boundary_layer(or_patch *p, int dimension, int setUpper);
 void or_copy_unew_to_uold(or_patch *p);
 void or_swap_unew_uold(or_patch *p);

@@ -126,6 +126,7 @@ _SIGS = {
                                            _dp, _dp, _dp, _dp]),
     "pc_solve_timestep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                          _ip, _dp, _dp]),
+    "pc_patch_cfl": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
     "pc_fill_boundary_layer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_int]),
     "pc_transfer_ghostlayer": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip, _ip,

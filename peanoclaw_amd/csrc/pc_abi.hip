@@ -867,6 +867,20 @@ int pc_solve_timestep(pc_pool* p, int slot, double maxDt, const int flags[4],
   return PC_OK;
 }
 
+int pc_patch_cfl(pc_pool* p, int slot, double* out) {
+  if (!valid_slot(p, slot) || !out) return fail(PC_E_INVALID, "invalid slot or output");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  HIPCHK(hipMemcpyAsync(p->d_slot1, &slot, sizeof(int), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  hipLaunchKernelGGL(k_patch_cfl, dim3(1), dim3(64), 0, p->stream, slot, p->d_slot1, p->d_pd,
+                     fields_of(p), cur_of(p, slot), p->d_smax, p->d_result);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, p->d_result, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return PC_OK;
+}
+
 int pc_fill_boundary_layer(pc_pool* p, int slot, int dim, int upper) {
   if (!valid_slot(p, slot) || (dim != 0 && dim != 1))
     return fail(PC_E_INVALID, "invalid slot or dimension");
@@ -920,9 +934,17 @@ static int transform(pc_pool* p, int slot, const int off[2], const int size[2], 
   const int n = size[0] * size[1];
   if (n <= 0) return PC_OK;
   const int cur = cur_of(p, slot);
+  const auto& d = p->slots[slot].desc;
+  const int nx = d.subdivision_factor[0], ny = d.subdivision_factor[1];
+  for (int k = 0; k < 2; ++k)
+    if (off[k] < -1 || off[k] + size[k] > (k ? ny : nx) + 1)
+      return fail(PC_E_INVALID, "transform region out of bounds");
+  if (!uold && (off[0] < 0 || off[1] < 0 ||
+                (long long)(off[0] + size[0] - 1) * ny + (off[1] + size[1] - 1) >= 3LL * nx * ny))
+    return fail(PC_E_INVALID, "uNew transform region outside the unknowns");
   hipLaunchKernelGGL(k_transform, dim3((n + 255) / 256), dim3(256), 0, p->stream, p->d_pd,
                      slot, fields_of(p), uold ? 1 - cur : cur, off[0], off[1], size[0],
-                     size[1], to_above_sea_floor ? -1.0 : 1.0);
+                     size[1], to_above_sea_floor ? -1.0 : 1.0, uold ? 0 : 1);
   HIPCHK(hipGetLastError());
   return PC_OK;
 }
@@ -973,7 +995,8 @@ int pc_interpolate_solution(pc_pool* p, const int dsize[2], const int doff[2], i
     } else {
       ss[d] = std::min(sn[d], ss[d] + 1);
     }
-    if (so[d] < 0 || so[d] + ss[d] > sn[d])
+    // the grown region may reach index n (the reference transforms it too)
+    if (so[d] < 0 || so[d] + ss[d] > sn[d] + 1)
       return fail(PC_E_INVALID, "destination region is not covered by the source patch");
   }
   // DefaultInterpolation.cpph:45-67 time factor

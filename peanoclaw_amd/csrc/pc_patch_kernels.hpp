@@ -43,6 +43,22 @@ k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restric
   }
 }
 
+// The CFL step of one patch from uOld (state 1 - cur) with its ghost layer,
+// without the update: out[0] = max_timestep(max edge speed).
+__global__ void __launch_bounds__(64)
+k_patch_cfl(int slot, const int* __restrict__ plist, const PatchDev* __restrict__ pd,
+            Fields F, int cur, double* __restrict__ smax, double* __restrict__ out) {
+  __shared__ MarchLds L;
+  const PatchDev P = pd[slot];
+  Task t;
+  t.first = 0; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;
+  EdgeBufs none{{nullptr, nullptr}, nullptr};
+  MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
+  march_task<kSpeed, false, 0>(t, A, L);
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = max_timestep(smax[slot], P.dx, P.dy);
+}
+
 // SWEKernel::fillBoundaryLayer (SWEKernel.cpp:176-237) on uOld (state st).
 __global__ void k_fill_boundary(const PatchDev* __restrict__ pd, int slot, Fields F,
                                 int st, int dim, int upper) {
@@ -104,14 +120,30 @@ __global__ void k_copy_swap(const PatchDev* __restrict__ pd, int slot, Fields F,
 }
 
 // SWEKernel::transformWaterHeight (SWEKernel.cpp:21-48) over a region.
+// uNew has no ghost layer in the reference's layout (q, x, y fastest, n^2
+// per unknown), so a region cell (x, y) with x = nx or y = ny - reached by
+// the grown source region of interpolateSolution (SWEKernel.cpp:252-262) -
+// addresses element x*ny + y of that array: unknown q = idx / (nx ny) at
+// (x', y') = ((idx mod nx ny) / ny, idx mod ny).  With unew_layout set the
+// kernel writes that element, as the reference does; b is read at (x, y).
 __global__ void k_transform(const PatchDev* __restrict__ pd, int slot, Fields F,
-                            int st, int ox, int oy, int sx, int sy, double sign) {
+                            int st, int ox, int oy, int sx, int sy, double sign,
+                            int unew_layout) {
   const PatchDev P = pd[slot];
   for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < sx * sy;
        k += blockDim.x * gridDim.x) {
-    const int x = k / sy, y = k - (k / sy) * sy;
-    const long long i = tile_index(P, ox + x, oy + y);
-    F.H(st)[i] = F.H(st)[i] + sign * F.b[i];
+    const int x = ox + k / sy, y = oy + (k - (k / sy) * sy);
+    const long long ib = tile_index(P, x, y);
+    double* f = F.H(st);
+    long long i = ib;
+    if (unew_layout && (x >= P.nx || y >= P.ny)) {
+      const int lin = x * P.ny + y;
+      const int area = P.nx * P.ny;
+      const int q = lin / area, r = lin - q * area;
+      f = q == 0 ? F.H(st) : (q == 1 ? F.HU(st) : F.HV(st));
+      i = tile_index(P, r / P.ny, r - (r / P.ny) * P.ny);
+    }
+    f[i] = f[i] + sign * F.b[ib];
   }
 }
 

@@ -1,0 +1,61 @@
+"""Oracle backend of the two-level AMR driver (test infrastructure): the same
+Numerics calls as peanoclaw_amd.amr.DeviceBackend, run by the C oracle on
+patches in the reference's layout."""
+import numpy as np
+
+from oracle_lib import Patch, iarr, lib
+
+
+class OracleBackend:
+    def __init__(self):
+        self.p = []
+
+    def new_patch(self, pos, size, n, level, virtual):
+        P = Patch(3, 1, n, 1, pos, size, level, 0.0, 0.0)
+        P.s.is_virtual = 1 if virtual else 0
+        P.s.is_leaf = 0 if virtual else 1
+        self.p.append(P)
+        return len(self.p) - 1
+
+    def set_state(self, s, h, b):
+        P = self.p[s]
+        P.unew[:] = 0.0
+        P.unew[0] = h
+        P.uold[:] = 0.0
+        P.pwg[0] = b
+
+    def clear(self, s):
+        self.p[s].unew[:] = 0.0
+        self.p[s].uold[:] = 0.0
+
+    def restrict(self, src, dst):
+        lib().or_restrict_swe(self.p[src].ref, self.p[dst].ref, 0)
+
+    def post_restrict(self, dst):
+        lib().or_post_process_restriction_swe(self.p[dst].ref, 0)
+
+    def copy_unew_to_uold(self, s):
+        lib().or_copy_unew_to_uold(self.p[s].ref)
+
+    def transfer(self, size, soff, doff, src, dst):
+        lib().or_transfer_ghostlayer(iarr(*size), iarr(*soff), iarr(*doff), self.p[src].ref,
+                                     self.p[dst].ref)
+
+    def interpolate(self, size, doff, src, dst, to_uold, to_current, use_time):
+        lib().or_interpolate_swe(iarr(*size), iarr(*doff), self.p[src].ref, self.p[dst].ref,
+                                 to_uold, to_current, use_time)
+
+    def fill_boundary(self, s, dim, upper):
+        lib().or_fill_boundary_layer(self.p[s].ref, dim, upper)
+
+    def cfl(self, s):
+        return lib().or_patch_cfl(self.p[s].ref)
+
+    def solve(self, s, max_dt):
+        return lib().or_solve_timestep(self.p[s].ref, float(max_dt))
+
+    def state(self, s):
+        return np.array(self.p[s].unew)
+
+    def uold(self, s):
+        return np.array(self.p[s].uold)

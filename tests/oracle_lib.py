@@ -89,7 +89,8 @@ def lib():
                               ctypes.c_double, ctypes.c_double, ctypes.c_double,
                               ctypes.c_int, ctypes.c_double, ctypes.c_double,
                               ctypes.c_double, _dp, _dp, _dp]),
-            "or_solve_timestep": (None, [P, ctypes.c_double]),
+            "or_solve_timestep": (ctypes.c_double, [P, ctypes.c_double]),
+            "or_patch_cfl": (ctypes.c_double, [P]),
             "or_fill_boundary_layer": (None, [P, ctypes.c_int, ctypes.c_int]),
             "or_copy_unew_to_uold": (None, [P]),
             "or_swap_unew_uold": (None, [P]),
