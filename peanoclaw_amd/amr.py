@@ -231,6 +231,36 @@ class DeviceBackend:
         return self.pool.download(s)[0]
 
 
+def refined_set_from_flags(pool, px, n, fn, threshold=0.05):
+    """Coarse patches the BreakingDam criterion refines on the initial data
+    (SolveTimestep::leaveCell with the demanded mesh width of
+    BreakingDam.cpp:85-139; minimal width one level finer, maximal the
+    coarse width): evaluated on the device for a px x px grid of n x n
+    cells in a scratch scenario of `pool`, which is released afterwards."""
+    size = 1.0 / px
+    slots = {}
+    for i in range(px):
+        for j in range(px):
+            s = pool.register(make_description((i * size, j * size), size, n, scenario=0))
+            slots[(i, j)] = s
+    ii = np.arange(-1, n + 1)
+    for (i, j), s in slots.items():
+        sub = size / n
+        X = i * size + (ii[:, None] + 0.5) * sub
+        Y = j * size + (ii[None, :] + 0.5) * sub
+        X, Y = np.broadcast_arrays(X, Y)
+        h, b = fn(X, Y)
+        unew = np.zeros((3, n, n))
+        unew[0] = h[1:-1, 1:-1]
+        pool.upload(s, unew=unew, b=np.ascontiguousarray(b, dtype=np.float64))
+    dx = size / n
+    flags = pool.refinement_flags(threshold, dx / 3.0, dx, max(slots.values()) + 1)
+    refined = {ij for ij, s in slots.items() if flags[s] == 1}
+    for s in slots.values():
+        pool.release(s)
+    return refined
+
+
 def dam_break_fields(centre=(0.5, 0.5), radius=0.25, h_in=2.0, h_out=1.0, hills=()):
     """Initial data of the 2-level dam break: radial column, optional
     Gaussian hills (x, y, sigma, height) as bathymetry."""
@@ -244,4 +274,4 @@ def dam_break_fields(centre=(0.5, 0.5), radius=0.25, h_in=2.0, h_out=1.0, hills=
     return fn
 
 
-__all__ = ["TwoLevelGrid", "DeviceBackend", "dam_break_fields"]
+__all__ = ["TwoLevelGrid", "DeviceBackend", "dam_break_fields", "refined_set_from_flags"]

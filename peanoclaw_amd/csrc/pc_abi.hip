@@ -484,12 +484,8 @@ int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
     hipLaunchKernelGGL(k_gather, dim3((p->nstrips + 3) / 4), dim3(256), 0, p->stream,
                        p->d_strips, p->nstrips, p->d_pd, F, p->d_scn);
   }
-  if (p->ntasks > 0) {
-    if (fused)
-      hipLaunchKernelGGL(k_boundary_speed, dim3((unsigned)p->h_live.size()), dim3(64), 0,
-                         p->stream, p->d_live, (int)p->h_live.size(), p->d_pd, F, p->d_scn,
-                         sides, E, p->d_smax_int, p->d_smax);
-    else if (sides)
+  if (p->ntasks > 0 && !fused) {
+    if (sides)
       hipLaunchKernelGGL(k_speed<true>, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
                          p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E, p->d_smax,
                          nullptr);
@@ -500,10 +496,23 @@ int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
   }
   const int nscn = (int)p->scn.size();
   if (nscn > 0) {
-    if (p->nentries > 0)
+    if (p->nentries > 0) {
+      if (fused) {
+        const int nl = (int)p->h_live.size();
+        const dim3 g((nl + 3) / 4);
+        if (prune)
+          hipLaunchKernelGGL(k_boundary_speed<true>, g, dim3(256), 0, p->stream, p->d_scn,
+                             p->d_live, nl, p->d_pd, F, sides, E, p->d_smax_int, p->d_smax,
+                             p->prune_slack);
+        else
+          hipLaunchKernelGGL(k_boundary_speed<false>, g, dim3(256), 0, p->stream, p->d_scn,
+                             p->d_live, nl, p->d_pd, F, sides, E, p->d_smax_int, p->d_smax,
+                             1.0);
+      }
       hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
                          p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
                          p->d_pd, p->d_smax, p->d_est, prune ? p->d_redo : nullptr, nullptr);
+    }
     hipLaunchKernelGGL(k_dt_final, dim3((nscn + 63) / 64), dim3(64), 0, p->stream, p->d_scn,
                        nscn, p->d_counter, prune ? 1 : 0, p->d_redo, p->prune_slack);
     if (prune) {

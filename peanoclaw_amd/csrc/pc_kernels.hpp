@@ -377,6 +377,16 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
             gs(Hdst + o, h1);
             gs(HUdst + o, hu1);
             gs(HVdst + o, hv1);
+            if (MODE == kUpdate && (col == 1 || col == ny)) {
+              // Also store into the row's ghost column (0 or ny+1) of the new
+              // state, which no one reads before it is refilled: the row's
+              // cache lines are then written completely (no masked partial
+              // writebacks at the row boundaries).
+              const long long og = o + (col == 1 ? -1 : 1);
+              gs(Hdst + og, h1);
+              gs(HUdst + og, hu1);
+              gs(HVdst + og, hv1);
+            }
             if (SIDES && (col == 1 || col == ny)) {
               double* Ed = Edst + (long long)((p * 2 + (col == 1 ? 0 : 1)) * 2) * kERows + r - 2;
               gs(Ed, h1);
@@ -608,79 +618,107 @@ k_update(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ pli
   }
 }
 
-// Edge speeds of the four boundary lines of every patch (ghost values from
-// the side sources), folded with the interior maximum of the same state
-// left by the fused update: smax[p] = max(smax_int[p], boundary max).
-__global__ void __launch_bounds__(64)
-k_boundary_speed(const int* __restrict__ slots, int nslots, const PatchDev* __restrict__ pd,
-                 Fields F, const ScnDev* __restrict__ scn, const SideDev* __restrict__ sides,
-                 EdgeBufs E, const double* __restrict__ smax_int, double* __restrict__ smax) {
-  const int k = blockIdx.x;
+// Ghost-edge speeds of every patch folded with the interior maximum of the
+// same state (left by the fused update): smax[p] = max(smax_int[p],
+// ghost-edge maximum).  One wavefront per patch (4 per workgroup) walks the
+// 2 (nx + ny) ghost edges of its patch as one list (x sides: ghost row 0|1
+// and nx|nx+1 along the columns; y sides: columns 0|1 and ny|ny+1 along the
+// rows, the own column from the edge buffer).  PRUNE: exact speeds only
+// where the bound may reach the scenario's last CFL limit, as in the update.
+template <bool PRUNE>
+__global__ void __launch_bounds__(256)
+k_boundary_speed(const ScnDev* __restrict__ scn, const int* __restrict__ slots, int nslots,
+                 const PatchDev* __restrict__ pd, Fields F, const SideDev* __restrict__ sides,
+                 EdgeBufs E, const double* __restrict__ smax_int, double* __restrict__ smax,
+                 double slack) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (k >= nslots) return;
-  const int p = gl(slots + k);
-  const PatchDev P = gl(pd + p);
-  const int par = gl(&scn[P.scn].stepc) & 1;
-  const SideDev sd = gl(sides + p);
-  const int lane = threadIdx.x;
-  const int stride = P.ny + 2;
-  const double* H = F.H(par);
-  const double* HU = F.HU(par);
-  const double* HV = F.HV(par);
-  double m = 0.0;
-  // x sides: edges (0|1) and (nx|nx+1) along the ny columns
-  for (int side = 0; side < 2; ++side) {
-    const int kk = side ? sd.kind[1] : sd.kind[0];
-    const int srcp = side ? sd.src[1] : sd.src[0];
-    for (int c = lane + 1; c <= P.ny; c += 64) {
-      const int orow = side ? P.nx : 1;
-      const long long io = P.off + (long long)orow * stride + c;
-      const double ho = gl(H + io), huo = gl(HU + io);
-      double hg, hug;
-      if (kk == kSideNeighbour) {
-        const PatchDev Q = gl(pd + srcp);
-        const long long j = Q.off + (long long)(side ? 1 : Q.nx) * stride + c;
-        hg = gl(H + j);
-        hug = gl(HU + j);
-      } else if (kk == kSideWall) {
-        hg = ho;
-        hug = -huo;
-      } else {
-        const long long j = P.off + (long long)(side ? P.nx + 1 : 0) * stride + c;
-        hg = gl(H + j);
-        hug = gl(HU + j);
-      }
-      const CellQ co = cell_dir(cell_base(ho, 0.0), huo);
-      const CellQ cg = cell_dir(cell_base(hg, 0.0), hug);
-      m = dmax(m, side ? edge_speed(co, cg) : edge_speed(cg, co));
+  {
+    const int p = gl(slots + k);
+    const PatchDev P = gl(pd + p);
+    const int par = gl(&scn[P.scn].stepc) & 1;
+    const SideDev sd = gl(sides + p);
+    float thr = 0.0f;
+    if (PRUNE) {
+      const double est = gl(&scn[P.scn].est);
+      if (est > 0.0) thr = (float)(kCfl * dmin(P.dx, P.dy) / (est * slack));
     }
-  }
-  // y sides: edges (r,0|1) and (r,ny|ny+1) along the nx rows
-  for (int side = 0; side < 2; ++side) {
-    const int kk = side ? sd.kind[3] : sd.kind[2];
-    const int srcp = side ? sd.src[3] : sd.src[2];
-    for (int r = lane + 1; r <= P.nx; r += 64) {
-      const double* Eo = (par ? E.e[1] : E.e[0]) + (long long)((p * 2 + side) * 2) * kERows + r - 1;
-      const double ho = gl(Eo), hvo = gl(Eo + kERows);
-      double hg, hvg;
-      if (kk == kSideNeighbour) {
-        const double* Eq = (par ? E.e[1] : E.e[0]) + (long long)((srcp * 2 + (1 - side)) * 2) * kERows + r - 1;
-        hg = gl(Eq);
-        hvg = gl(Eq + kERows);
-      } else if (kk == kSideWall) {
-        hg = ho;
-        hvg = -hvo;
-      } else {
-        const long long j = P.off + (long long)r * stride + (side ? P.ny + 1 : 0);
-        hg = gl(H + j);
-        hvg = gl(HV + j);
+    const int nx = P.nx, ny = P.ny, stride = ny + 2;
+    const int ne = 2 * (nx + ny);
+    const double* H = F.H(par);
+    const double* HU = F.HU(par);
+    const double* HV = F.HV(par);
+    const double* Ecur = par ? E.e[1] : E.e[0];
+    double sm = 0.0;
+    for (int e0 = 0; e0 < ne; e0 += 64) {
+      const int e = e0 + lane;
+      double ho = 1.0, qo = 0.0, hg = 1.0, qg = 0.0;
+      bool upper = false;
+      if (e < 2 * ny) {
+        // x side: edge (row 0 | 1) or (nx | nx+1) at column c
+        upper = e >= ny;
+        const int c = (upper ? e - ny : e) + 1;
+        const int kk = upper ? sd.kind[1] : sd.kind[0];
+        const long long io = P.off + (long long)(upper ? nx : 1) * stride + c;
+        ho = gl(H + io);
+        qo = gl(HU + io);
+        if (kk == kSideNeighbour) {
+          const PatchDev Q = gl(pd + (upper ? sd.src[1] : sd.src[0]));
+          const long long j = Q.off + (long long)(upper ? 1 : Q.nx) * stride + c;
+          hg = gl(H + j);
+          qg = gl(HU + j);
+        } else if (kk == kSideWall) {
+          hg = ho;
+          qg = -qo;
+        } else {
+          const long long j = P.off + (long long)(upper ? nx + 1 : 0) * stride + c;
+          hg = gl(H + j);
+          qg = gl(HU + j);
+        }
+      } else if (e < ne) {
+        // y side: edge (col 0 | 1) or (ny | ny+1) at row r
+        const int e2 = e - 2 * ny;
+        upper = e2 >= nx;
+        const int side = upper ? 1 : 0;
+        const int r = (upper ? e2 - nx : e2) + 1;
+        const int kk = sd.kind[2 + side];
+        const double* Eo = Ecur + (long long)((p * 2 + side) * 2) * kERows + r - 1;
+        ho = gl(Eo);
+        qo = gl(Eo + kERows);
+        if (kk == kSideNeighbour) {
+          const double* Eq =
+              Ecur + (long long)((sd.src[2 + side] * 2 + (1 - side)) * 2) * kERows + r - 1;
+          hg = gl(Eq);
+          qg = gl(Eq + kERows);
+        } else if (kk == kSideWall) {
+          hg = ho;
+          qg = -qo;
+        } else {
+          const long long j = P.off + (long long)r * stride + (upper ? ny + 1 : 0);
+          hg = gl(H + j);
+          qg = gl(HV + j);
+        }
       }
-      const CellQ co = cell_dir(cell_base(ho, 0.0), hvo);
-      const CellQ cg = cell_dir(cell_base(hg, 0.0), hvg);
-      m = dmax(m, side ? edge_speed(co, cg) : edge_speed(cg, co));
+      const bool valid = e < ne;
+      // orientation: the lower side's ghost is the left cell of its edge
+      const double hL = upper ? ho : hg, qL = upper ? qo : qg;
+      const double hR = upper ? hg : ho, qR = upper ? qg : qo;
+      bool need = valid;
+      if (PRUNE) {
+        const BoundTerms bl = bound_terms(hL, qL, 0.0), br = bound_terms(hR, qR, 0.0);
+        need = valid && bound_candidate(bl.ax, bl.c, br.ax, br.c, thr);
+      }
+      if (__any(need ? 1 : 0)) {
+        const CellQ cl = cell_dir(cell_base(hL, 0.0), qL);
+        const CellQ cr = cell_dir(cell_base(hR, 0.0), qR);
+        const double sp = edge_speed_any(cl, cr);
+        if (valid) sm = dmax(sm, sp);
+      }
     }
+    for (int o = 32; o >= 1; o >>= 1) sm = dmax(sm, __shfl_xor(sm, o));
+    if (lane == 0) smax[p] = dmax(gl(smax_int + p), sm);
   }
-  for (int s = 32; s >= 1; s >>= 1) m = dmax(m, __shfl_xor(m, s));
-  if (lane == 0) smax[p] = dmax(smax_int[p], m);
 }
 
 // Edge buffers of the current state from the tiles (after uploads or

@@ -44,3 +44,36 @@ def test_two_level_amr_bitwise(hills):
         assert dev.time == ora.time
     finally:
         pool.close()
+
+
+def test_two_level_amr_breaking_dam_27x27():
+    """The 2-level dam break at its full size: 27 x 27 coarse patches of
+    16^2 cells, jittered centre (seed 1), patches refined by factor 3 where
+    the BreakingDam criterion (|grad h| > 0.05) asks for it on the initial
+    data; a few steps bitwise against the oracle."""
+    import time
+
+    from peanoclaw_amd.amr import (DeviceBackend, TwoLevelGrid, dam_break_fields,
+                                   refined_set_from_flags)
+    from peanoclaw_amd.pool import PatchPool
+    rng = np.random.default_rng(1)
+    centre = tuple(0.5 + rng.uniform(-0.05, 0.05, 2))
+    fn = dam_break_fields(centre=centre, radius=0.25)
+    pool = PatchPool(0)
+    try:
+        refined = refined_set_from_flags(pool, 27, 16, fn)
+        frac = len(refined) / 27 ** 2
+        dev = TwoLevelGrid(DeviceBackend(pool), 27, 27, 16, refined, fn)
+        ora = TwoLevelGrid(OracleBackend(), 27, 27, 16, refined, fn)
+        t0 = time.perf_counter()
+        for it in range(4):
+            assert dev.step() == ora.step()
+        el = time.perf_counter() - t0
+        same, total = _bitwise_equal(dev, ora)
+        print(f"2-level dam break 27x27x16^2: refined fraction {frac:.3f} of coarse patches, "
+              f"{len(dev.leaves)} leaves, {dev.cells} leaf cells; bitwise {same}/{total}; "
+              f"{el / 4:.3f} s per step for both drivers")
+        assert same == total
+        assert 0.0 < frac < 1.0
+    finally:
+        pool.close()
