@@ -35,6 +35,7 @@ namespace pcamd {
 
 constexpr int kERows = 64;  // edge-buffer rows per side (max nx)
 
+
 // Global-memory accessors: loads and stores through address space 1, so the
 // compiler emits global_* instead of flat_* instructions (flat accesses
 // also count against the LDS wait counter, which would make every LDS

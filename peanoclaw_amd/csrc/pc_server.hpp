@@ -7,7 +7,8 @@
 // (first fit in arrival order), registers its N/n x N/n patches in one
 // contiguous tile range, writes its initial state on the device, then
 // advances all active requests by `steps` shared batched steps (each
-// scenario with its own CFL dt), and retires the requests that reached
+// scenario with its own CFL dt; replayed from a captured graph while the
+// set of requests is unchanged), and retires the requests that reached
 // their end time: digest of the final state, mass, step count, patches
 // released.  The reference has no counterpart (it runs one simulation per
 // process); the per-step numerics are those of the batched pool step.
@@ -333,7 +334,24 @@ int pc_server_round(pc_server* s, int steps, int* active_out, int* queued_out) {
     used += q.tile_cells;
     it = s->queue.erase(it);
   }
-  if (!s->active.empty() && steps > 0) CHK(pc_step_all(p, steps));
+  if (!s->active.empty() && steps > 0) {
+    // Between admissions and completions the step is the same launch
+    // sequence: replay it as a captured graph (state read on the device);
+    // after a change, one plain step applies it and the graph is captured
+    // again.
+    const bool replay = !p->topo_dirty && p->graph_valid && !p->scn_dev_dirty &&
+                        !(p->sides_any && p->edges_dirty) &&
+                        !(p->fused && p->sides_any && !p->fused_ready);
+    if (replay) {
+      CHK(pc_graph_replay(p, steps));
+    } else {
+      CHK(pc_step_all(p, 1));
+      if (steps > 1) {
+        CHK(pc_graph_capture(p));
+        CHK(pc_graph_replay(p, steps - 1));
+      }
+    }
+  }
   // retire the requests that reached their end time
   CHK(sync_scn_host(p));
   std::vector<int> still;
