@@ -226,6 +226,8 @@ double or_block_max_timestep(double maxWaveSpeed, double dx, double dy) {
   return DBL_MAX;
 }
 
+long long or_clamp_events = 0;
+
 void or_block_update(int nx, int ny, double dt, double *h, double *hu,
                      double *hv, double *netH, double *netHu, double *netHv) {
   const int sy = ny + 2;
@@ -237,7 +239,10 @@ void or_block_update(int nx, int ny, double dt, double *h, double *hu,
       hv[k] -= dt * netHv[k];
       netH[k] = 0.0; netHu[k] = 0.0; netHv[k] = 0.0;
       if (h[k] < 0.1) { hu[k] = 0.0; hv[k] = 0.0; }
-      if (h[k] < 0.0) h[k] = 0.0;
+      if (h[k] < 0.0) {
+        h[k] = 0.0;
+        __atomic_fetch_add(&or_clamp_events, 1, __ATOMIC_RELAXED);
+      }
     }
   }
 }

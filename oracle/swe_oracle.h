@@ -80,6 +80,9 @@ double or_block_fluxes(int nx, int ny, double dx, double dy,
                        const double *b, double *netH, double *netHu,
                        double *netHv);
 double or_block_max_timestep(double maxWaveSpeed, double dx, double dy);
+/* number of cells whose depth the update clamped from below 0 to 0 (the
+   only place mass changes on a single level); test diagnostics */
+extern long long or_clamp_events;
 void or_block_update(int nx, int ny, double dt, double *h, double *hu,
                      double *hv, double *netH, double *netHu, double *netHv);
 

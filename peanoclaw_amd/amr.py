@@ -1,20 +1,25 @@
-"""Two-level adaptive patch grid (refinement factor 3) with global time steps.
+"""Adaptive patch grids (refinement factor 3) with global time steps.
 
 The grid is px x py coarse patches of n x n cells on [0,1]^2; a refined
-coarse patch becomes virtual and holds 3 x 3 fine patches of n x n cells.
-One step is the sequence of Numerics calls PeanoClaw's traversal makes for
-a grid whose leaves all share one clock (maximumTimestepSize = the global
-CFL minimum, SolveTimestep.cpp:591-751):
+patch becomes virtual and holds 3 x 3 patches of n x n cells one level
+finer, to any depth.  One step is the sequence of Numerics calls
+PeanoClaw's traversal makes for a grid whose leaves all share one clock
+(maximumTimestepSize = the global CFL minimum, SolveTimestep.cpp:591-751):
 
-1. every virtual patch is cleared and receives the restriction of its nine
-   children, then the SWE post-processing (GridLevelTransfer ->
-   Numerics::restrictSolution / postProcessRestriction, SWEKernel.cpp:288-316);
+1. every virtual patch is cleared; every leaf, in depth-first tree order,
+   is restricted into each virtual patch that contains it, nearest first
+   (GridLevelTransfer::restrictToOverlappingVirtualSubgrids,
+   GridLevelTransfer.cpp:140-198: only leaves restrict, so a leaf two levels
+   down feeds its grandparent directly and restricted data, which lives on
+   the [0,1] time basis, is never restricted again); then the SWE
+   post-processing of every virtual patch (Numerics::restrictSolution /
+   postProcessRestriction, SWEKernel.cpp:288-316);
 2. every leaf copies uNew to uOld (SolveTimestep.cpp:638);
 3. every leaf face gets its ghost layer from the patch across it: a
    same-level leaf or virtual patch by transferGhostlayer
-   (GhostlayerCompositorFunctors.cpp:59), a coarse leaf next to a fine patch
-   by interpolateSolution(toUOld, not toCurrentTime, useTimeUNewOrUOld)
-   (GhostlayerCompositorFunctors.cpp:61-69);
+   (GhostlayerCompositorFunctors.cpp:59), otherwise the coarser leaf covering
+   that region by interpolateSolution(toUOld, not toCurrentTime,
+   useTimeUNewOrUOld) (GhostlayerCompositorFunctors.cpp:61-69);
 4. domain walls by fillBoundaryLayer in the order left, top, right, bottom
    (SolveTimestep.cpp:87-142);
 5. the global step = min over leaves of their CFL step (bounded by the end
@@ -49,42 +54,65 @@ class AmrPatch:
         self.children = []
 
 
-class TwoLevelGrid:
-    def __init__(self, backend, px, py, n, refined, fn):
-        """fn(X, Y) -> (h, b) at cell centres (arrays of equal shape)."""
+class AmrGrid:
+    """Patch tree with refinement factor 3: level 1 is a px x py grid of
+    patches of n x n cells on [0,1]^2; a patch at level l that `refine(l,
+    (i, j))` selects becomes virtual and holds 3 x 3 patches at level l+1
+    (indices (3i+a, 3j+c)), down to `max_level`.  The refinement is fixed at
+    construction."""
+
+    def __init__(self, backend, px, py, n, refine, fn, max_level=2):
         self.b = backend
         self.px, self.py, self.n = px, py, n
-        self.refined = set(refined)
-        size_c = 1.0 / px
-        self.coarse = {}
-        self.fine = {}
-        for i in range(px):
-            for j in range(py):
-                pos = (i * size_c, j * size_c)
-                virt = (i, j) in self.refined
-                cp = AmrPatch(backend.new_patch(pos, size_c, n, 1, virt), 1, pos, size_c, n,
-                              virt, (i, j))
-                self.coarse[(i, j)] = cp
+        self.max_level = max_level
+        self.levels = {}
+        todo = [(1, (i, j), (i / px, j / py), 1.0 / px) for i in range(px) for j in range(py)]
+        while todo:
+            nxt = []
+            for lev, ij, pos, size in todo:
+                virt = lev < max_level and bool(refine(lev, ij))
+                pt = AmrPatch(backend.new_patch(pos, size, n, lev, virt), lev, pos, size, n, virt,
+                              ij)
+                self.levels.setdefault(lev, {})[ij] = pt
                 if virt:
-                    sf = size_c / 3.0
+                    sf = size / 3.0
                     for a in range(3):
                         for c in range(3):
-                            fpos = (pos[0] + a * sf, pos[1] + c * sf)
-                            fp = AmrPatch(backend.new_patch(fpos, sf, n, 2, False), 2, fpos, sf,
-                                          n, False, (3 * i + a, 3 * j + c))
-                            self.fine[fp.ij] = fp
-                            cp.children.append(fp)
-        self.virtuals = [p for p in self.coarse.values() if p.virtual]
-        self.leaves = [p for p in self.coarse.values() if not p.virtual] + list(self.fine.values())
-        for p in list(self.coarse.values()) + list(self.fine.values()):
-            ii = np.arange(-1, n + 1)
-            sub = p.size / n
-            X = p.pos[0] + (ii[:, None] + 0.5) * sub
-            Y = p.pos[1] + (ii[None, :] + 0.5) * sub
-            X, Y = np.broadcast_arrays(X, Y)
-            h, bb = fn(X, Y)
-            backend.set_state(p.handle, np.ascontiguousarray(h[1:-1, 1:-1], dtype=np.float64),
-                              np.ascontiguousarray(bb, dtype=np.float64))
+                            nxt.append((lev + 1, (3 * ij[0] + a, 3 * ij[1] + c),
+                                        (pos[0] + a * sf, pos[1] + c * sf), sf))
+            todo = nxt
+        for lev in sorted(self.levels):
+            for ij, pt in self.levels[lev].items():
+                if pt.virtual:
+                    pt.children = [self.levels[lev + 1][(3 * ij[0] + a, 3 * ij[1] + c)]
+                                   for a in range(3) for c in range(3)]
+        self.virtuals = [pt for lev in sorted(self.levels)
+                         for pt in self.levels[lev].values() if pt.virtual]
+        self.leaves = [pt for lev in sorted(self.levels)
+                       for pt in self.levels[lev].values() if not pt.virtual]
+        # (leaf, virtual ancestors nearest first) in depth-first tree order
+        self._restrictions = []
+
+        def walk(pt, ancestors):
+            if pt.virtual:
+                for c in pt.children:
+                    walk(c, [pt] + ancestors)
+            elif ancestors:
+                self._restrictions.append((pt, ancestors))
+        for i in range(px):
+            for j in range(py):
+                walk(self.levels[1][(i, j)], [])
+        for lev in sorted(self.levels):
+            for pt in self.levels[lev].values():
+                ii = np.arange(-1, n + 1)
+                sub = pt.size / n
+                X = pt.pos[0] + (ii[:, None] + 0.5) * sub
+                Y = pt.pos[1] + (ii[None, :] + 0.5) * sub
+                X, Y = np.broadcast_arrays(X, Y)
+                h, bb = fn(X, Y)
+                backend.set_state(pt.handle,
+                                  np.ascontiguousarray(h[1:-1, 1:-1], dtype=np.float64),
+                                  np.ascontiguousarray(bb, dtype=np.float64))
         self.time, self.dt = 0.0, 0.0
         self.steps = 0
         self._plan = self._ghost_plan()
@@ -94,20 +122,23 @@ class TwoLevelGrid:
         return len(self.leaves) * self.n * self.n
 
     def _face(self, p, dim, upper):
-        """('transfer'|'interp'|'wall', source patch) for a face of leaf p."""
-        if p.level == 1:
-            i, j = p.ij
-            q = (i + (1 if upper else -1), j) if dim == 0 else (i, j + (1 if upper else -1))
-            if not (0 <= q[0] < self.px and 0 <= q[1] < self.py):
-                return ("wall", None)
-            return ("transfer", self.coarse[q])
-        I, J = p.ij
-        q = (I + (1 if upper else -1), J) if dim == 0 else (I, J + (1 if upper else -1))
-        if not (0 <= q[0] < 3 * self.px and 0 <= q[1] < 3 * self.py):
+        """('transfer'|'interp'|'wall', source patch) for a face of leaf p: the
+        same-level patch across the face, else the leaf covering that region
+        on the nearest coarser level."""
+        lev = p.level
+        i, j = p.ij
+        q = (i + (1 if upper else -1), j) if dim == 0 else (i, j + (1 if upper else -1))
+        k = 3 ** (lev - 1)
+        if not (0 <= q[0] < self.px * k and 0 <= q[1] < self.py * k):
             return ("wall", None)
-        if q in self.fine:
-            return ("transfer", self.fine[q])
-        return ("interp", self.coarse[(q[0] // 3, q[1] // 3)])
+        if q in self.levels[lev]:
+            return ("transfer", self.levels[lev][q])
+        while lev > 1:
+            lev -= 1
+            q = (q[0] // 3, q[1] // 3)
+            if q in self.levels[lev]:
+                return ("interp", self.levels[lev][q])
+        raise RuntimeError("face region not covered")
 
     def _ghost_plan(self):
         n = self.n
@@ -151,13 +182,15 @@ class TwoLevelGrid:
         return dt
 
     def fill_ghosts(self):
-        """Restriction into the virtual patches, uNew -> uOld, and the ghost
-        layers of every leaf (steps 1-4 above)."""
+        """Restriction of the leaves into the virtual patches, uNew -> uOld,
+        and the ghost layers of every leaf (steps 1-4 above)."""
         B = self.b
         for v in self.virtuals:
             B.clear(v.handle)
-            for c in v.children:
-                B.restrict(c.handle, v.handle)
+        for leaf, ancestors in self._restrictions:
+            for v in ancestors:
+                B.restrict(leaf.handle, v.handle)
+        for v in self.virtuals:
             B.post_restrict(v.handle)
         for p in self.leaves:
             B.copy_unew_to_uold(p.handle)
@@ -173,6 +206,17 @@ class TwoLevelGrid:
     def leaf_states(self):
         """uNew of every leaf, keyed by (level, i, j)."""
         return {(p.level,) + p.ij: self.b.state(p.handle) for p in self.leaves}
+
+
+class TwoLevelGrid(AmrGrid):
+    """Two levels: the coarse patches in `refined` hold 3 x 3 fine patches."""
+
+    def __init__(self, backend, px, py, n, refined, fn):
+        refined = set(refined)
+        super().__init__(backend, px, py, n, lambda lev, ij: lev == 1 and ij in refined, fn,
+                         max_level=2)
+        self.coarse = self.levels[1]
+        self.fine = self.levels.get(2, {})
 
 
 class DeviceBackend:
@@ -261,6 +305,46 @@ def refined_set_from_flags(pool, px, n, fn, threshold=0.05):
     return refined
 
 
+def wet_dry_fields(seed=4, column=(0.05, 0.35, 0.3, 0.6333333333333333), column_h=1.5):
+    """Wet/dry basin: b a slope from -1 (x = 0) to +0.5 (x = 1) plus six
+    seeded bumps (|height| <= 0.15), still water at surface level 0 where
+    b < 0, dry land (h = 0) elsewhere, a column of depth 1.5 over
+    x0 <= x <= x1, y0 <= y <= y1 (10% of the domain)."""
+    rng = np.random.default_rng(seed)
+    bumps = [(rng.uniform(0.2, 0.9), rng.uniform(0.1, 0.9), rng.uniform(0.03, 0.1),
+              rng.uniform(-0.15, 0.15)) for _ in range(6)]
+    x0, x1, y0, y1 = column
+
+    def fn(X, Y):
+        b = -1.0 + 1.5 * X
+        for cx, cy, s, a in bumps:
+            b = b + a * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * s * s))
+        h = np.maximum(0.0, 0.0 - b)
+        col = (X >= x0) & (X <= x1) & (Y >= y0) & (Y <= y1)
+        return np.where(col, column_h, h), b
+    return fn
+
+
+def front_refinement(px, py, n, fn, max_level, jump=0.5):
+    """refine(level, ij) for the wet/dry basin: a patch refines when its
+    cells (initial data) hold both dry (h = 0) and wet cells, or when the
+    depth jumps by more than `jump` between neighbouring cells (the column
+    edge).  Evaluated once on the host at the cell centres."""
+    def refine(lev, ij):
+        if lev >= max_level:
+            return False
+        size = 1.0 / (px * 3 ** (lev - 1))
+        c = (np.arange(n) + 0.5) * (size / n)
+        X, Y = np.meshgrid(ij[0] * size + c, ij[1] * size + c, indexing="ij")
+        h, _ = fn(X, Y)
+        dry = h == 0.0
+        if dry.any() and not dry.all():
+            return True
+        return bool(max(np.abs(np.diff(h, axis=0)).max(), np.abs(np.diff(h, axis=1)).max())
+                    > jump)
+    return refine
+
+
 def dam_break_fields(centre=(0.5, 0.5), radius=0.25, h_in=2.0, h_out=1.0, hills=()):
     """Initial data of the 2-level dam break: radial column, optional
     Gaussian hills (x, y, sigma, height) as bathymetry."""
@@ -274,4 +358,5 @@ def dam_break_fields(centre=(0.5, 0.5), radius=0.25, h_in=2.0, h_out=1.0, hills=
     return fn
 
 
-__all__ = ["TwoLevelGrid", "DeviceBackend", "dam_break_fields", "refined_set_from_flags"]
+__all__ = ["AmrGrid", "TwoLevelGrid", "DeviceBackend", "dam_break_fields",
+           "front_refinement", "refined_set_from_flags", "wet_dry_fields"]

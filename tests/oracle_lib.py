@@ -140,6 +140,11 @@ def lib():
     return _lib
 
 
+def clamp_events():
+    """Cells whose depth the oracle's update has clamped from below zero."""
+    return ctypes.c_longlong.in_dll(lib(), "or_clamp_events").value
+
+
 def dptr(a):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_dp)

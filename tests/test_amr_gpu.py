@@ -77,3 +77,45 @@ def test_two_level_amr_breaking_dam_27x27():
         assert 0.0 < frac < 1.0
     finally:
         pool.close()
+
+
+def test_three_level_wet_dry_basin_full_size():
+    """The wet/dry stress case at its full size: 9 x 9 coarse patches of 27^2
+    cells, three levels (finest dx = 1/2187), sloping basin with dry land
+    and a depth-1.5 column; patches refine where dry and wet cells meet or
+    the depth jumps (column edge).  Ghosts cross up to two levels.  Steps
+    bitwise against the oracle; mass drift and depth clamps reported."""
+    import time
+
+    from oracle_lib import clamp_events
+    from peanoclaw_amd.amr import AmrGrid, DeviceBackend, front_refinement, wet_dry_fields
+    from peanoclaw_amd.pool import PatchPool
+    fn = wet_dry_fields()
+    px, n = 9, 27
+    refine = front_refinement(px, px, n, fn, 3)
+    pool = PatchPool(0)
+    try:
+        dev = AmrGrid(DeviceBackend(pool), px, px, n, refine, fn, max_level=3)
+        ora = AmrGrid(OracleBackend(), px, px, n, refine, fn, max_level=3)
+        levels = {lev: sum(p.level == lev for p in dev.leaves) for lev in (1, 2, 3)}
+        assert all(v > 0 for v in levels.values())
+
+        def mass(grid):
+            return sum(float(v[0].sum()) * (1.0 / (px * n * 3 ** (k[0] - 1))) ** 2
+                       for k, v in grid.leaf_states().items())
+        m0 = mass(dev)
+        c0 = clamp_events()
+        t0 = time.perf_counter()
+        steps = 12
+        for it in range(steps):
+            assert dev.step() == ora.step(), f"dt differs at step {it}"
+        el = time.perf_counter() - t0
+        same, total = _bitwise_equal(dev, ora)
+        m1 = mass(dev)
+        print(f"3-level wet/dry 9x9x27^2: leaves per level {levels}, {dev.cells} leaf cells "
+              f"({dev.cells / 2187 ** 2:.3f} of 2187^2); {steps} steps to t={dev.time + dev.dt:.3e}; "
+              f"bitwise {same}/{total}; mass drift {(m1 - m0) / m0:.2e}; "
+              f"depth clamps {clamp_events() - c0}; {el / steps:.3f} s per step for both drivers")
+        assert same == total
+    finally:
+        pool.close()

@@ -11,24 +11,26 @@ import numpy as np
 
 from amr_oracle import OracleBackend
 from oracle_lib import OrPool, dptr, lib
-from peanoclaw_amd.amr import TwoLevelGrid, dam_break_fields
+import pytest
+
+from peanoclaw_amd.amr import AmrGrid, TwoLevelGrid, dam_break_fields, wet_dry_fields
 
 
-def test_fully_refined_grid_equals_uniform_fine_grid():
-    px = 2
-    n = 8
+@pytest.mark.parametrize("levels,px,n", [(2, 2, 8), (3, 1, 4)])
+def test_fully_refined_grid_equals_uniform_fine_grid(levels, px, n):
     fn = dam_break_fields(centre=(0.47, 0.52), radius=0.25)
-    grid = TwoLevelGrid(OracleBackend(), px, px, n, {(i, j) for i in range(px) for j in range(px)},
-                        fn)
+    grid = AmrGrid(OracleBackend(), px, px, n, lambda lev, ij: True, fn, max_level=levels)
+    finest = grid.levels[levels]
+    assert len(grid.leaves) == len(finest) == (px * 3 ** (levels - 1)) ** 2
     # the same fine grid as one pool, ghost fill by or_grid_fill
-    P = 3 * px
+    P = px * 3 ** (levels - 1)
     s = n + 2
     H = [np.zeros(P * P * s * s), np.zeros(P * P * s * s)]
     HU = [np.zeros_like(H[0]), np.zeros_like(H[0])]
     HV = [np.zeros_like(H[0]), np.zeros_like(H[0])]
     B = np.zeros_like(H[0])
     slot_of = np.arange(P * P, dtype=np.int32)
-    for (I, J), fp in grid.fine.items():
+    for (I, J), fp in finest.items():
         k = I * P + J
         u = grid.b.state(fp.handle)
         H[0][k * s * s:(k + 1) * s * s].reshape(s, s)[1:-1, 1:-1] = u[0]
@@ -53,7 +55,7 @@ def test_fully_refined_grid_equals_uniform_fine_grid():
         d_amr = grid.step()
         assert d_amr == d_pool
     c = pl.cur
-    for (I, J), fp in grid.fine.items():
+    for (I, J), fp in finest.items():
         k = I * P + J
         u = grid.b.state(fp.handle)
         for q, arr in enumerate((H[c], HU[c], HV[c])):
@@ -77,6 +79,32 @@ def test_ghost_fill_reproduces_a_linear_field():
 
     be = OracleBackend()
     grid = TwoLevelGrid(be, 4, 4, 6, {(1, 1), (2, 1), (1, 2)}, fn)
+    _check_linear_ghosts(grid, be, ax, ay)
+
+
+def test_three_level_ghost_fill_reproduces_a_linear_field():
+    """As above with three levels, including level-3 patches whose face
+    neighbour is a level-1 leaf (ghosts interpolated across two levels) and
+    level-2 virtual patches restricted from level-3 children."""
+    ax, ay = -0.3, 0.55
+    hills = [(0.5, 0.4, 0.2, 0.25)]
+
+    def fn(X, Y):
+        b = np.zeros(X.shape)
+        for x, y, s, a in hills:
+            b = b + a * np.exp(-((X - x) ** 2 + (Y - y) ** 2) / (s * s))
+        return 2.0 + ax * X + ay * Y - b, b
+
+    refine = {(1, (1, 1)), (1, (2, 1)), (2, (3, 3)), (2, (5, 4)), (2, (6, 3))}
+    be = OracleBackend()
+    grid = AmrGrid(be, 4, 4, 6, lambda lev, ij: (lev, ij) in refine, fn, max_level=3)
+    kinds = {(p.level, grid._face(p, d, u)[1].level if grid._face(p, d, u)[1] else 0)
+             for p in grid.leaves for d, u in ((0, 0), (0, 1), (1, 0), (1, 1))}
+    assert (3, 1) in kinds and (3, 2) in kinds and (2, 1) in kinds
+    _check_linear_ghosts(grid, be, ax, ay)
+
+
+def _check_linear_ghosts(grid, be, ax, ay):
     grid.fill_ghosts()
     checked = 0
     for p in grid.leaves:
@@ -97,6 +125,22 @@ def test_ghost_fill_reproduces_a_linear_field():
                 assert u[1, x, y] == 0.0 and u[2, x, y] == 0.0
                 checked += 1
     assert checked > 100
+
+
+def test_three_level_wet_dry_grid_runs_and_stays_finite():
+    """Wet/dry basin on three levels: finite, non-negative depths and
+    a positive step while the column collapses onto dry land."""
+    fn = wet_dry_fields()
+    refine = {(1, (0, 1)), (1, (1, 1)), (1, (2, 2)), (2, (2, 4)), (2, (3, 4)), (2, (7, 7))}
+    grid = AmrGrid(OracleBackend(), 3, 3, 6, lambda lev, ij: (lev, ij) in refine, fn,
+                   max_level=3)
+    assert {p.level for p in grid.leaves} == {1, 2, 3}
+    for _ in range(8):
+        assert grid.step() > 0
+    st = grid.leaf_states()
+    assert all(np.isfinite(v).all() for v in st.values())
+    assert all((v[0] >= 0.0).all() for v in st.values())
+    assert any((v[0] == 0.0).any() for v in st.values())
 
 
 def test_two_level_grid_runs_and_stays_finite():
