@@ -205,6 +205,14 @@ int pc_scenario_get(pc_pool *pool, int scenario, double *time,
  * patches_per_task_cap limits patches per wavefront task (0 = no cap). */
 int pc_set_launch_config(pc_pool *pool, int persistent_workgroups, int fused,
                          int patches_per_task_cap);
+/* Edge solver of every update of the pool, the build-time choice of the
+ * reference (SConstruct:291-303, WAVE_PROPAGATION_SOLVER): 1 = f-wave,
+ * 2 = augmented Riemann (George 2008), 4 = f-wave (the reference's
+ * vectorised variant; same arithmetic here).  Applies to the batched step,
+ * the per-patch entry points and pc_edge_batch; with the augmented solver
+ * the batched step runs a separate exact speed pass (fused settings are
+ * kept but not used).  Default 1. */
+int pc_set_wave_propagation_solver(pc_pool *pool, int solver);
 /* Tuning of fused = 2 (no reference counterpart): exact speeds are computed
  * for edges whose bound reaches the speed of a CFL step est * slack
  * (default 1.03125); the pruned minimum is accepted when it is at most
@@ -231,7 +239,7 @@ int pc_scenario_mass(pc_pool *pool, int scenario, double *out);
  * event pair recorded around every update launch. */
 int pc_last_timing(pc_pool *pool, double *total_ms, double *update_ms);
 
-/* Diagnostic: run the device f-wave edge solver on n host edges
+/* Diagnostic: run the pool's device edge solver on n host edges
  * in[6n] = (hL, hR, huL, huR, bL, bR) -> out[5n] = (dhL, dhR, dqL, dqR,
  * speed); the per-edge computeNetUpdates call of
  * tools/SWE_WaveAccumulationBlock.cpp:154-159. */

@@ -177,6 +177,277 @@ double or_fwave_speed(double hL, double hR, double huL, double huR) {
 }
 
 /* ------------------------------------------------------------------ */
+/* Augmented Riemann solver                                             */
+/* ------------------------------------------------------------------ */
+/* SWE's solverType 2 (SConstruct:295-296, WAVE_PROPAGATION_SOLVER=2) is
+ * the augmented Riemann solver of D. L. George, "Augmented Riemann solvers
+ * for the shallow water equations over variable topography with steady
+ * states and inundation", J. Comput. Phys. 227 (2008) 3089-3113.  Its
+ * source is in the absent SWE library; this restates the published method
+ * in the form of its published implementation (GeoClaw's riemann_aug_JCP,
+ * riemanntype and the wet/dry handling of rpn2), reduced to the normal
+ * direction: three waves with speeds
+ *   l1 = min(Einfeldt s1, middle-state 2-speed), l3 = max(Einfeldt s2,
+ *   middle-state 1-speed), l2 = (l1 + l3)/2,
+ * eigenvectors (1, l1, l1^2), (0, 0, 1), (1, l3, l3^2), and the jump
+ * (dh - ddh, dhu, dphi - ddphi) with the steady-state wave (ddh, ddphi)
+ * that removes the bathymetry source, solved by Cramer's rule.  One pass
+ * of the steady-state estimate and of the middle-state iteration (the
+ * published default), no rarefaction entropy corrector (off there).  A cell
+ * is wet when h > dryTol (0.01); a dry side either becomes a wall (mirrored
+ * state; the waves into it are removed) when the wet side cannot reach its
+ * bathymetry, or is inundated.  Waves are split by speed as in the f-wave
+ * (< -zeroTol left, > zeroTol right, else half each); the edge speed is
+ * max |l| over the waves kept. */
+
+/* Middle state of the Riemann problem without bathymetry: two
+ * rarefactions in closed form; two shocks by `maxiter` Newton steps on
+ * sqrt(h) from max(hL, hR); one shock and one rarefaction by `maxiter`
+ * regula-falsi steps on [min(hL, hR), max(hL, hR)].  out = (hm, s1m, s2m,
+ * rare1, rare2). */
+void or_aug_middle_state(double hL, double hR, double uL, double uR, int maxiter,
+                         double out[5]) {
+  const double g = OR_GRAVITY;
+  const double hmin = dmin(hR, hL), hmax = dmax(hR, hL);
+  const double du = uR - uL;
+  double hm, s1m, s2m;
+  int rare1, rare2;
+  if (hmin <= OR_DRY_TOL) {
+    hm = 0.0;
+    s1m = ((uR + uL) - 2.0 * sqrt(g * hR)) + 2.0 * sqrt(g * hL);
+    s2m = s1m;
+    rare1 = hL <= 0.0 ? 0 : 1;
+    rare2 = 1 - rare1;
+  } else {
+    const double cmax = sqrt(g * hmax);
+    const double fmin = du + 2.0 * (sqrt(g * hmin) - cmax);
+    const double fmax = du + (hmax - hmin) * sqrt(((0.5 * g) * (hmax + hmin)) / (hmax * hmin));
+    if (fmin > 0.0) {
+      const double cL = sqrt(g * hL), cR = sqrt(g * hR);
+      const double w = dmax(0.0, 2.0 * (cL + cR) - du);
+      hm = (1.0 / (16.0 * g)) * (w * w);
+      const double cm = sqrt(g * hm);
+      s1m = (uL + 2.0 * cL) - 3.0 * cm;
+      s2m = (uR - 2.0 * cR) + 3.0 * cm;
+      rare1 = 1;
+      rare2 = 1;
+    } else if (fmax <= 0.0) {
+      double h0 = hmax;
+      for (int it = 0; it < maxiter; it++) {
+        const double gL = sqrt((0.5 * g) * (1.0 / h0 + 1.0 / hL));
+        const double gR = sqrt((0.5 * g) * (1.0 / h0 + 1.0 / hR));
+        const double f0 = (du + (h0 - hL) * gL) + (h0 - hR) * gR;
+        const double h02 = 4.0 * (h0 * h0);
+        const double dfdh = (gL - (g * (h0 - hL)) / (h02 * gL)) + (gR - (g * (h0 - hR)) / (h02 * gR));
+        const double slope = (2.0 * sqrt(h0)) * dfdh;
+        const double r = sqrt(h0) - f0 / slope;
+        h0 = r * r;
+      }
+      hm = h0;
+      const double u1m = uL - (hm - hL) * sqrt((0.5 * g) * (1.0 / hm + 1.0 / hL));
+      const double u2m = uR + (hm - hR) * sqrt((0.5 * g) * (1.0 / hm + 1.0 / hR));
+      const double cm = sqrt(g * hm);
+      s1m = u1m - cm;
+      s2m = u2m + cm;
+      rare1 = 0;
+      rare2 = 0;
+    } else {
+      /* regula falsi on [hmin, hmax] (F(hmin) <= 0 < F(hmax)); its first
+         step is the chord through both ends */
+      double lo = hmin, hi = hmax, fhi = fmax, h0 = hmin;
+      double flo = (du + 2.0 * (sqrt(g * lo) - cmax)) +
+                   (lo - hmin) * sqrt((0.5 * g) * (1.0 / lo + 1.0 / hmin));
+      for (int it = 0; it < maxiter; it++) {
+        const double slope = (fhi - flo) / (hi - lo);
+        h0 = lo - flo / slope;
+        if (it + 1 == maxiter) break;
+        const double f0 = (du + 2.0 * (sqrt(g * h0) - cmax)) +
+                          (h0 - hmin) * sqrt((0.5 * g) * (1.0 / h0 + 1.0 / hmin));
+        if (f0 <= 0.0) {
+          lo = h0;
+          flo = f0;
+        } else {
+          hi = h0;
+          fhi = f0;
+        }
+      }
+      hm = h0;
+      const double cm = sqrt(g * hm);
+      if (hL > hR) {
+        const double w = uL + 2.0 * sqrt(g * hL);
+        s1m = w - 3.0 * cm;
+        s2m = w - cm;
+        rare1 = 1;
+        rare2 = 0;
+      } else {
+        const double w = uR - 2.0 * sqrt(g * hR);
+        s2m = w + 3.0 * cm;
+        s1m = w + cm;
+        rare1 = 0;
+        rare2 = 1;
+      }
+    }
+  }
+  out[0] = hm;
+  out[1] = s1m;
+  out[2] = s2m;
+  out[3] = rare1;
+  out[4] = rare2;
+}
+
+static double det3(double a00, double a01, double a02, double a10, double a11, double a12,
+                   double a20, double a21, double a22) {
+  return (a00 * (a11 * a22 - a12 * a21) - a01 * (a10 * a22 - a12 * a20)) +
+         a02 * (a10 * a21 - a11 * a20);
+}
+
+/* Waves of one edge: speeds sw[3] (0 for the waves a wall removes) and
+ * f-waves fh[3] (mass flux jump), fq[3] (momentum flux jump).  Returns 0
+ * when both sides are dry (no waves). */
+int or_aug_waves(double hL, double hR, double huL, double huR, double bL, double bR,
+                 double sw[3], double fh[3], double fq[3]) {
+  const double g = OR_GRAVITY;
+  const double crit = 1e-6;
+  if (!(hL > OR_DRY_TOL) && !(hR > OR_DRY_TOL)) {
+    for (int k = 0; k < 3; k++) sw[k] = fh[k] = fq[k] = 0.0;
+    return 0;
+  }
+  double uL, uR, phiL, phiR;
+  if (hR > OR_DRY_TOL) {
+    uR = huR / hR;
+    phiR = (0.5 * g) * (hR * hR) + (huR * huR) / hR;
+  } else {
+    hR = 0.0; huR = 0.0; uR = 0.0; phiR = 0.0;
+  }
+  if (hL > OR_DRY_TOL) {
+    uL = huL / hL;
+    phiL = (0.5 * g) * (hL * hL) + (huL * huL) / hL;
+  } else {
+    hL = 0.0; huL = 0.0; uL = 0.0; phiL = 0.0;
+  }
+  double wall[3] = {1.0, 1.0, 1.0};
+  double m[5];
+  if (hR <= OR_DRY_TOL) {
+    or_aug_middle_state(hL, hL, uL, -uL, 1, m);
+    if (dmax(hL, m[0]) + bL < bR) {
+      wall[1] = 0.0; wall[2] = 0.0;
+      hR = hL; huR = -huL; bR = bL; phiR = phiL; uR = -uL;
+    } else if (hL + bL < bR) {
+      bR = hL + bL;
+    }
+  } else if (hL <= OR_DRY_TOL) {
+    or_aug_middle_state(hR, hR, -uR, uR, 1, m);
+    if (dmax(hR, m[0]) + bR < bL) {
+      wall[0] = 0.0; wall[1] = 0.0;
+      hL = hR; huL = -huR; bL = bR; phiL = phiR; uL = -uR;
+    } else if (hR + bR < bL) {
+      bL = hR + bR;
+    }
+  }
+  /* Einfeldt speeds */
+  const double cL = sqrt(g * hL), cR = sqrt(g * hR);
+  const double uhat = (cL * uL + cR * uR) / (cR + cL);
+  const double chat = sqrt(g * (0.5 * (hR + hL)));
+  double sE1 = dmin(uL - cL, uhat - chat);
+  double sE2 = dmax(uR + cR, uhat + chat);
+
+  const double delh = hR - hL, delhu = huR - huL, delphi = phiR - phiL, delb = bR - bL;
+  or_aug_middle_state(hL, hR, uL, uR, 1, m);
+  const double s1m = m[1], s2m = m[2];
+  const double l1 = dmin(sE1, s2m), l3 = dmax(sE2, s1m);
+  sE1 = l1;
+  sE2 = l3;
+  const double l2 = 0.5 * (l1 + l3);
+  const double hstarHLL = dmax((((huL - huR) + sE2 * hR) - sE1 * hL) / (sE2 - sE1), 0.0);
+
+  /* steady-state wave */
+  const double hbar = dmax(0.5 * (hL + hR), 0.0);
+  const double s1s2bar = 0.25 * ((uL + uR) * (uL + uR)) - g * hbar;
+  const double s1s2tilde = dmax(0.0, uL * uR) - g * hbar;
+  const int sonic = fabs(s1s2bar) <= crit || s1s2bar * s1s2tilde <= crit ||
+                    (s1s2bar * sE1) * sE2 <= crit || dmin(fabs(sE1), fabs(sE2)) < crit ||
+                    (sE1 < 0.0 && s1m > 0.0) || (sE2 > 0.0 && s2m < 0.0) ||
+                    (uL + cL) * (uR + cR) < 0.0 || (uL - cL) * (uR - cR) < 0.0;
+  double ddh = sonic ? -delb : ((delb * g) * hbar) / s1s2bar;
+  if (sE1 < -crit && sE2 > crit) {
+    ddh = dmin(ddh, (hstarHLL * (sE2 - sE1)) / sE2);
+    ddh = dmax(ddh, (hstarHLL * (sE2 - sE1)) / sE1);
+  } else if (sE1 >= crit) {
+    ddh = dmin(ddh, (hstarHLL * (sE2 - sE1)) / sE1);
+    ddh = dmax(ddh, -hL);
+  } else if (sE2 <= -crit) {
+    ddh = dmin(ddh, hR);
+    ddh = dmax(ddh, (hstarHLL * (sE2 - sE1)) / sE2);
+  }
+  double ddphi = sonic ? ((-g) * hbar) * delb : ((((-delb) * g) * hbar) * s1s2tilde) / s1s2bar;
+  ddphi = dmin(ddphi, g * dmax((-hL) * delb, (-hR) * delb));
+  ddphi = dmax(ddphi, g * dmin((-hL) * delb, (-hR) * delb));
+  const double d0 = delh - ddh, d1 = delhu, d2 = delphi - ddphi;
+
+  /* Cramer's rule on the eigenvector matrix (columns = waves) */
+  const double r20 = l1 * l1, r22 = l3 * l3;
+  const double det = det3(1.0, 0.0, 1.0, l1, 0.0, l3, r20, 1.0, r22);
+  const double beta0 = det3(d0, 0.0, 1.0, d1, 0.0, l3, d2, 1.0, r22) / det;
+  const double beta1 = det3(1.0, d0, 1.0, l1, d1, l3, r20, d2, r22) / det;
+  const double beta2 = det3(1.0, 0.0, d0, l1, 0.0, d1, r20, 1.0, d2) / det;
+  sw[0] = l1 * wall[0];
+  sw[1] = l2 * wall[1];
+  sw[2] = l3 * wall[2];
+  fh[0] = (beta0 * l1) * wall[0];
+  fh[1] = (beta1 * 0.0) * wall[1];
+  fh[2] = (beta2 * l3) * wall[2];
+  fq[0] = (beta0 * r20) * wall[0];
+  fq[1] = (beta1 * 1.0) * wall[1];
+  fq[2] = (beta2 * r22) * wall[2];
+  return 1;
+}
+
+void or_augrie(double hL, double hR, double huL, double huR, double bL, double bR,
+               double out[5]) {
+  double sw[3], fh[3], fq[3];
+  double dhL = 0.0, dhR = 0.0, dqL = 0.0, dqR = 0.0, smax = 0.0;
+  if (or_aug_waves(hL, hR, huL, huR, bL, bR, sw, fh, fq)) {
+    for (int k = 0; k < 3; k++) {
+      if (sw[k] < -OR_ZERO_TOL) {
+        dhL += fh[k]; dqL += fq[k];
+      } else if (sw[k] > OR_ZERO_TOL) {
+        dhR += fh[k]; dqR += fq[k];
+      } else {
+        dhL += 0.5 * fh[k]; dqL += 0.5 * fq[k];
+        dhR += 0.5 * fh[k]; dqR += 0.5 * fq[k];
+      }
+      smax = dmax(smax, fabs(sw[k]));
+    }
+  }
+  out[0] = dhL; out[1] = dhR; out[2] = dqL; out[3] = dqR; out[4] = smax;
+}
+
+/* Edge solver of the SWE block: 1 or 4 = f-wave (the vectorised variant
+ * performs the same arithmetic), 2 = augmented Riemann (SConstruct:291-303). */
+static int or_solver = 1;
+
+int or_set_wave_propagation_solver(int solver) {
+  if (solver != 1 && solver != 2 && solver != 4) return -1;
+  or_solver = solver;
+  return 0;
+}
+
+void or_edge(double hL, double hR, double huL, double huR, double bL, double bR,
+             double out[5]) {
+  if (or_solver == 2)
+    or_augrie(hL, hR, huL, huR, bL, bR, out);
+  else
+    or_fwave(hL, hR, huL, huR, bL, bR, out);
+}
+
+double or_edge_speed(double hL, double hR, double huL, double huR, double bL, double bR) {
+  if (or_solver != 2) return or_fwave_speed(hL, hR, huL, huR);
+  double out[5];
+  or_augrie(hL, hR, huL, huR, bL, bR, out);
+  return out[4];
+}
+
+/* ------------------------------------------------------------------ */
 /* SWE block (tools/SWE_WaveAccumulationBlock.cpp:84-296)               */
 /* arrays are (nx+2)*(ny+2), index i*(ny+2)+j, interior i,j in [1,n]    */
 /* ------------------------------------------------------------------ */
@@ -193,7 +464,7 @@ double or_block_fluxes(int nx, int ny, double dx, double dy,
   for (int i = 1; i < nx + 2; i++) {
     for (int j = 1; j < ny + 1; j++) {
       int l = (i - 1) * sy + j, r = i * sy + j;
-      or_fwave(h[l], h[r], hu[l], hu[r], b[l], b[r], o);
+      or_edge(h[l], h[r], hu[l], hu[r], b[l], b[r], o);
       netH[l] += dx_inv * o[0];
       netHu[l] += dx_inv * o[2];
       netH[r] += dx_inv * o[1];
@@ -205,7 +476,7 @@ double or_block_fluxes(int nx, int ny, double dx, double dy,
   for (int i = 1; i < nx + 1; i++) {
     for (int j = 1; j < ny + 2; j++) {
       int d = i * sy + (j - 1), u = i * sy + j;
-      or_fwave(h[d], h[u], hv[d], hv[u], b[d], b[u], o);
+      or_edge(h[d], h[u], hv[d], hv[u], b[d], b[u], o);
       netH[d] += dy_inv * o[0];
       netHv[d] += dy_inv * o[2];
       netH[u] += dy_inv * o[1];
@@ -865,20 +1136,20 @@ int or_refinement_decision(const double subcell[2], const double demanded[2]) {
 /* Global-dt drivers                                                    */
 /* ------------------------------------------------------------------ */
 /* Edge speeds only (used for the global dt): identical function of the
- * edge's states as the speed output of or_fwave. */
+ * edge's states as the speed output of the edge solver. */
 static double tile_max_speed(int nx, int ny, const double *h, const double *hu,
-                             const double *hv) {
+                             const double *hv, const double *b) {
   const int sy = ny + 2;
   double m = 0.0;
   for (int i = 1; i < nx + 2; i++)
     for (int j = 1; j < ny + 1; j++) {
       int l = (i - 1) * sy + j, r = i * sy + j;
-      m = dmax(m, or_fwave_speed(h[l], h[r], hu[l], hu[r]));
+      m = dmax(m, or_edge_speed(h[l], h[r], hu[l], hu[r], b[l], b[r]));
     }
   for (int i = 1; i < nx + 1; i++)
     for (int j = 1; j < ny + 2; j++) {
       int d = i * sy + (j - 1), u = i * sy + j;
-      m = dmax(m, or_fwave_speed(h[d], h[u], hv[d], hv[u]));
+      m = dmax(m, or_edge_speed(h[d], h[u], hv[d], hv[u], b[d], b[u]));
     }
   return m;
 }
@@ -899,7 +1170,7 @@ void or_pool_max_speeds(const or_pool *pl, int state, double *smax) {
   for (int p = 0; p < pl->np; p++) {
     long long o = pl->off[p];
     smax[p] = tile_max_speed(pl->nx[p], pl->ny[p], pl->h[state] + o,
-                             pl->hu[state] + o, pl->hv[state] + o);
+                             pl->hu[state] + o, pl->hv[state] + o, pl->b + o);
   }
 }
 
@@ -918,7 +1189,7 @@ double or_pool_step(or_pool *pl, double maxDt, int nthreads) {
   for (int p = 0; p < pl->np; p++) {
     long long o = pl->off[p];
     smax[p] = tile_max_speed(pl->nx[p], pl->ny[p], pl->h[c] + o, pl->hu[c] + o,
-                             pl->hv[c] + o);
+                             pl->hv[c] + o, pl->b + o);
   }
   for (int p = 0; p < pl->np; p++)
     dtmin = dmin(dtmin, or_block_max_timestep(smax[p], pl->dx[p], pl->dy[p]));
@@ -1032,7 +1303,7 @@ double or_grid_max_speed(const or_grid *gr) {
   for (int P = 0; P < np; P++) {
     long long o = (long long)P * s * s;
     m = dmax(m, tile_max_speed(gr->n, gr->n, gr->h[c] + o, gr->hu[c] + o,
-                               gr->hv[c] + o));
+                               gr->hv[c] + o, gr->b + o));
   }
   return m;
 }
@@ -1082,4 +1353,11 @@ void or_fwave_batch(long long n, const double *in, double *out) {
   for (long long e = 0; e < n; e++) or_fwave(in[6 * e], in[6 * e + 1], in[6 * e + 2],
                                              in[6 * e + 3], in[6 * e + 4], in[6 * e + 5],
                                              out + 5 * e);
+}
+
+/* The selected edge solver over n edges (same layout). */
+void or_edge_batch(long long n, const double *in, double *out) {
+  for (long long e = 0; e < n; e++) or_edge(in[6 * e], in[6 * e + 1], in[6 * e + 2],
+                                            in[6 * e + 3], in[6 * e + 4], in[6 * e + 5],
+                                            out + 5 * e);
 }

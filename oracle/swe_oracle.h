@@ -73,6 +73,19 @@ void or_fwave(double hL, double hR, double huL, double huR, double bL, double bR
               double out[5]); /* dhL, dhR, dqL, dqR, speed */
 double or_fwave_speed(double hL, double hR, double huL, double huR);
 void or_fwave_batch(long long n, const double *in, double *out);
+/* augmented Riemann solver (George 2008); see swe_oracle.c */
+void or_aug_middle_state(double hL, double hR, double uL, double uR, int maxiter,
+                         double out[5]);
+int or_aug_waves(double hL, double hR, double huL, double huR, double bL, double bR,
+                 double sw[3], double fh[3], double fq[3]);
+void or_augrie(double hL, double hR, double huL, double huR, double bL, double bR,
+               double out[5]);
+/* edge solver of the block: 1, 4 = f-wave, 2 = augmented Riemann */
+int or_set_wave_propagation_solver(int solver);
+void or_edge(double hL, double hR, double huL, double huR, double bL, double bR,
+             double out[5]);
+double or_edge_speed(double hL, double hR, double huL, double huR, double bL, double bR);
+void or_edge_batch(long long n, const double *in, double *out);
 
 /* ---- SWE block on one (nx+2)x(ny+2) tile, ghost width 1 ---- */
 double or_block_fluxes(int nx, int ny, double dx, double dy,

@@ -119,6 +119,7 @@ struct pc_pool {
   int persistent_wgs = 0;
   int fused = 0;
   int task_cap = 0;
+  int solver = 0;  // edge solver: 0 f-wave, 1 augmented Riemann
   double prune_slack = kPruneSlackDefault;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   float last_total_ms = 0.f, last_update_ms = 0.f;
@@ -127,6 +128,56 @@ struct pc_pool {
 };
 
 namespace {
+
+// Next-step speeds are fused into the update for the f-wave only; the
+// augmented solver runs the separate exact speed pass.
+inline int fused_mode(const pc_pool* p) { return p->solver ? 0 : p->fused; }
+
+// Exact CFL speed pass over `ntasks` tasks (grid `g`), picking the kernel
+// instance for the ghost-source mode and the edge solver.
+void launch_speed(pc_pool* p, dim3 g, const Task* tasks, int ntasks, const int* plist,
+                  Fields F, const SideDev* sides, EdgeBufs E, double* smax,
+                  const int* only_if) {
+  if (sides) {
+    if (p->solver)
+      hipLaunchKernelGGL((k_speed<true, 1>), g, dim3(64), 0, p->stream, tasks, ntasks, plist,
+                         p->d_pd, F, p->d_scn, sides, E, smax, only_if);
+    else
+      hipLaunchKernelGGL((k_speed<true, 0>), g, dim3(64), 0, p->stream, tasks, ntasks, plist,
+                         p->d_pd, F, p->d_scn, sides, E, smax, only_if);
+  } else {
+    if (p->solver)
+      hipLaunchKernelGGL((k_speed<false, 1>), g, dim3(64), 0, p->stream, tasks, ntasks, plist,
+                         p->d_pd, F, p->d_scn, sides, E, smax, only_if);
+    else
+      hipLaunchKernelGGL((k_speed<false, 0>), g, dim3(64), 0, p->stream, tasks, ntasks, plist,
+                         p->d_pd, F, p->d_scn, sides, E, smax, only_if);
+  }
+}
+
+// Update without fused next-step speeds.
+void launch_update_plain(pc_pool* p, dim3 g, const Task* tasks, int ntasks, const int* plist,
+                         Fields F, const SideDev* sides, EdgeBufs E, int pers) {
+  if (sides) {
+    if (p->solver)
+      hipLaunchKernelGGL((k_update<0, true, 1>), g, dim3(64), 0, p->stream, tasks, ntasks,
+                         plist, p->d_pd, F, p->d_scn, sides, E, p->d_smax_int, p->d_counter,
+                         pers, 1.0);
+    else
+      hipLaunchKernelGGL((k_update<0, true, 0>), g, dim3(64), 0, p->stream, tasks, ntasks,
+                         plist, p->d_pd, F, p->d_scn, sides, E, p->d_smax_int, p->d_counter,
+                         pers, 1.0);
+  } else {
+    if (p->solver)
+      hipLaunchKernelGGL((k_update<0, false, 1>), g, dim3(64), 0, p->stream, tasks, ntasks,
+                         plist, p->d_pd, F, p->d_scn, sides, E, p->d_smax_int, p->d_counter,
+                         pers, 1.0);
+    else
+      hipLaunchKernelGGL((k_update<0, false, 0>), g, dim3(64), 0, p->stream, tasks, ntasks,
+                         plist, p->d_pd, F, p->d_scn, sides, E, p->d_smax_int, p->d_counter,
+                         pers, 1.0);
+  }
+}
 
 Fields fields_of(pc_pool* p) {
   Fields F;
@@ -421,7 +472,7 @@ int rebuild_topology(pc_pool* p) {
   // The interior speed maxima of unchanged patches stay valid; the pruned
   // mode re-derives those of scenarios whose patch set changed through its
   // exact fallback (their estimate is reset on register / release).
-  if (p->fused != 2 || realloc) p->fused_ready = false;
+  if (fused_mode(p) != 2 || realloc) p->fused_ready = false;
   CHK(invalidate_graph(p));
   return PC_OK;
 }
@@ -454,18 +505,12 @@ int prepare_batched(pc_pool* p) {
     p->edges_dirty = false;
     p->fused_ready = false;
   }
-  if (p->fused && !p->fused_ready && p->ntasks > 0) {
+  if (fused_mode(p) && !p->fused_ready && p->ntasks > 0) {
     if (p->nstrips > 0)
       hipLaunchKernelGGL(k_gather, dim3((p->nstrips + 3) / 4), dim3(256), 0, p->stream,
                          p->d_strips, p->nstrips, p->d_pd, fields_of(p), p->d_scn);
-    if (p->sides_any)
-      hipLaunchKernelGGL(k_speed<true>, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
-                         p->ntasks, p->d_plist, p->d_pd, fields_of(p), p->d_scn, p->d_sides,
-                         edges_of(p), p->d_smax_int, nullptr);
-    else
-      hipLaunchKernelGGL(k_speed<false>, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
-                         p->ntasks, p->d_plist, p->d_pd, fields_of(p), p->d_scn, nullptr,
-                         edges_of(p), p->d_smax_int, nullptr);
+    launch_speed(p, dim3(p->ntasks), p->d_tasks, p->ntasks, p->d_plist, fields_of(p),
+                 p->sides_any ? p->d_sides : nullptr, edges_of(p), p->d_smax_int, nullptr);
     HIPCHK(hipGetLastError());
     p->fused_ready = true;
   }
@@ -478,22 +523,15 @@ int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
   Fields F = fields_of(p);
   EdgeBufs E = edges_of(p);
   const SideDev* sides = p->sides_any ? p->d_sides : nullptr;
-  const bool fused = p->fused && p->sides_any;
-  const bool prune = fused && p->fused == 2;
+  const bool fused = fused_mode(p) && p->sides_any;
+  const bool prune = fused && fused_mode(p) == 2;
   if (p->nstrips > 0) {
     hipLaunchKernelGGL(k_gather, dim3((p->nstrips + 3) / 4), dim3(256), 0, p->stream,
                        p->d_strips, p->nstrips, p->d_pd, F, p->d_scn);
   }
-  if (p->ntasks > 0 && !fused) {
-    if (sides)
-      hipLaunchKernelGGL(k_speed<true>, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
-                         p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E, p->d_smax,
-                         nullptr);
-    else
-      hipLaunchKernelGGL(k_speed<false>, dim3(p->ntasks), dim3(64), 0, p->stream, p->d_tasks,
-                         p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E, p->d_smax,
-                         nullptr);
-  }
+  if (p->ntasks > 0 && !fused)
+    launch_speed(p, dim3(p->ntasks), p->d_tasks, p->ntasks, p->d_plist, F, sides, E,
+                 p->d_smax, nullptr);
   const int nscn = (int)p->scn.size();
   if (nscn > 0) {
     if (p->nentries > 0) {
@@ -518,9 +556,9 @@ int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
     if (prune) {
       // exact fallback, a no-op unless the pruned minimum was not proven exact
       if (p->ntasks > 0)
-        hipLaunchKernelGGL(k_speed<true>, dim3(std::min(p->ntasks, kFallbackWgs)), dim3(64), 0,
-                           p->stream, p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn,
-                           sides, E, p->d_smax, p->d_redo);
+        hipLaunchKernelGGL((k_speed<true, 0>), dim3(std::min(p->ntasks, kFallbackWgs)),
+                           dim3(64), 0, p->stream, p->d_tasks, p->ntasks, p->d_plist, p->d_pd,
+                           F, p->d_scn, sides, E, p->d_smax, p->d_redo);
       if (p->nentries > 0)
         hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
                            p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
@@ -534,21 +572,15 @@ int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
     const int grid = p->persistent_wgs > 0 ? p->persistent_wgs : p->ntasks;
     const int pers = p->persistent_wgs > 0 ? 1 : 0;
     if (prune)
-      hipLaunchKernelGGL((k_update<2, true>), dim3(grid), dim3(64), 0, p->stream,
+      hipLaunchKernelGGL((k_update<2, true, 0>), dim3(grid), dim3(64), 0, p->stream,
                          p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
                          p->d_smax_int, p->d_counter, pers, p->prune_slack);
     else if (fused)
-      hipLaunchKernelGGL((k_update<1, true>), dim3(grid), dim3(64), 0, p->stream,
-                         p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
-                         p->d_smax_int, p->d_counter, pers, p->prune_slack);
-    else if (sides)
-      hipLaunchKernelGGL((k_update<0, true>), dim3(grid), dim3(64), 0, p->stream,
+      hipLaunchKernelGGL((k_update<1, true, 0>), dim3(grid), dim3(64), 0, p->stream,
                          p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
                          p->d_smax_int, p->d_counter, pers, p->prune_slack);
     else
-      hipLaunchKernelGGL((k_update<0, false>), dim3(grid), dim3(64), 0, p->stream,
-                         p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
-                         p->d_smax_int, p->d_counter, pers, p->prune_slack);
+      launch_update_plain(p, dim3(grid), p->d_tasks, p->ntasks, p->d_plist, F, sides, E, pers);
     if (timed_update) HIPCHK(hipEventRecord(ee ? ee : p->ev3, p->stream));
   }
   HIPCHK(hipGetLastError());
@@ -712,7 +744,7 @@ int pc_patch_get_description(pc_pool* p, int slot, pc_cell_description* out) {
     out->time = S.time;
     out->timestep_size = S.dt;
     double e = 0.0;
-    if (p->fused == 2 && p->sides_any) {
+    if (fused_mode(p) == 2 && p->sides_any) {
       // pruned speeds: only the scenario minimum is exact
       e = S.est;
     } else if (p->d_est && !p->topo_dirty) {
@@ -858,8 +890,12 @@ int pc_solve_timestep(pc_pool* p, int slot, double maxDt, const int flags[4],
   }
   HIPCHK(hipMemcpyAsync(p->d_slot1, &slot, sizeof(int), hipMemcpyHostToDevice, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
-  hipLaunchKernelGGL(k_patch_solve, dim3(1), dim3(64), 0, p->stream, slot, p->d_slot1,
-                     p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_smax, p->d_result);
+  if (p->solver)
+    hipLaunchKernelGGL(k_patch_solve<1>, dim3(1), dim3(64), 0, p->stream, slot, p->d_slot1,
+                       p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_smax, p->d_result);
+  else
+    hipLaunchKernelGGL(k_patch_solve<0>, dim3(1), dim3(64), 0, p->stream, slot, p->d_slot1,
+                       p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_smax, p->d_result);
   HIPCHK(hipGetLastError());
   double res[2];
   HIPCHK(hipMemcpyAsync(res, p->d_result, 2 * sizeof(double), hipMemcpyDeviceToHost,
@@ -882,8 +918,12 @@ int pc_patch_cfl(pc_pool* p, int slot, double* out) {
   CHK(prepare(p));
   HIPCHK(hipMemcpyAsync(p->d_slot1, &slot, sizeof(int), hipMemcpyHostToDevice, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
-  hipLaunchKernelGGL(k_patch_cfl, dim3(1), dim3(64), 0, p->stream, slot, p->d_slot1, p->d_pd,
-                     fields_of(p), cur_of(p, slot), p->d_smax, p->d_result);
+  if (p->solver)
+    hipLaunchKernelGGL(k_patch_cfl<1>, dim3(1), dim3(64), 0, p->stream, slot, p->d_slot1,
+                       p->d_pd, fields_of(p), cur_of(p, slot), p->d_smax, p->d_result);
+  else
+    hipLaunchKernelGGL(k_patch_cfl<0>, dim3(1), dim3(64), 0, p->stream, slot, p->d_slot1,
+                       p->d_pd, fields_of(p), cur_of(p, slot), p->d_smax, p->d_result);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, p->d_result, sizeof(double), hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
@@ -1496,6 +1536,23 @@ int pc_set_prune_slack(pc_pool* p, double slack) {
   return PC_OK;
 }
 
+int pc_set_wave_propagation_solver(pc_pool* p, int solver) {
+  if (!p) return fail(PC_E_INVALID, "pool is NULL");
+  if (solver != 1 && solver != 2 && solver != 4)
+    return fail(PC_E_INVALID, "solver must be 1 (f-wave), 2 (augmented Riemann) or 4 (f-wave)");
+  const int s = solver == 2 ? 1 : 0;
+  if (s != p->solver) {
+    hipSetDevice(p->device);
+    CHK(sync_scn_host(p));
+    p->solver = s;
+    p->fused_ready = false;
+    for (auto& S : p->scn) S.est = 0.0;  // pruning thresholds came from the other solver
+    p->scn_dev_dirty = true;
+    CHK(invalidate_graph(p));
+  }
+  return PC_OK;
+}
+
 int pc_set_launch_config(pc_pool* p, int persistent_wgs, int fused, int cap) {
   if (!p || persistent_wgs < 0 || cap < 0) return fail(PC_E_INVALID, "invalid config");
   p->persistent_wgs = persistent_wgs;
@@ -1585,14 +1642,8 @@ int pc_step_all_per_patch(pc_pool* p, int k) {
                            p->d_strips + b, e - b, p->d_pd, F, p->d_scn);
     }
     // per-patch CFL estimate, one launch per patch
-    for (int i = 0; i < n; ++i) {
-      if (sides)
-        hipLaunchKernelGGL(k_speed<true>, dim3(1), dim3(64), 0, p->stream, d_single + i, 1,
-                           d_sl, p->d_pd, F, p->d_scn, sides, E, p->d_smax, nullptr);
-      else
-        hipLaunchKernelGGL(k_speed<false>, dim3(1), dim3(64), 0, p->stream, d_single + i, 1,
-                           d_sl, p->d_pd, F, p->d_scn, sides, E, p->d_smax, nullptr);
-    }
+    for (int i = 0; i < n; ++i)
+      launch_speed(p, dim3(1), d_single + i, 1, d_sl, F, sides, E, p->d_smax, nullptr);
     if (p->nentries > 0)
       hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
                          p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
@@ -1601,16 +1652,8 @@ int pc_step_all_per_patch(pc_pool* p, int k) {
       hipLaunchKernelGGL(k_dt_final, dim3((nscn + 63) / 64), dim3(64), 0, p->stream,
                          p->d_scn, nscn, p->d_counter, 0, p->d_redo, 1.0);
     // per-patch update, one launch per patch
-    for (int i = 0; i < n; ++i) {
-      if (sides)
-        hipLaunchKernelGGL((k_update<0, true>), dim3(1), dim3(64), 0, p->stream,
-                           d_single + i, 1, d_sl, p->d_pd, F, p->d_scn, sides, E,
-                           p->d_smax_int, p->d_counter, 0, 1.0);
-      else
-        hipLaunchKernelGGL((k_update<0, false>), dim3(1), dim3(64), 0, p->stream,
-                           d_single + i, 1, d_sl, p->d_pd, F, p->d_scn, sides, E,
-                           p->d_smax_int, p->d_counter, 0, 1.0);
-    }
+    for (int i = 0; i < n; ++i)
+      launch_update_plain(p, dim3(1), d_single + i, 1, d_sl, F, sides, E, 0);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(p->ev1, p->stream));
@@ -1690,7 +1733,7 @@ int pc_graph_replay(pc_pool* p, int k) {
   if (!p || k < 0) return fail(PC_E_INVALID, "invalid argument");
   hipSetDevice(p->device);
   if (p->topo_dirty || !p->graph_valid || p->scn_dev_dirty ||
-      (p->sides_any && p->edges_dirty) || (p->fused && p->sides_any && !p->fused_ready))
+      (p->sides_any && p->edges_dirty) || (fused_mode(p) && p->sides_any && !p->fused_ready))
     return fail(PC_E_INVALID, "no valid captured graph (grid, scenarios or state changed)");
   CHK(sync_scn_host(p));
   HIPCHK(hipEventRecord(p->ev0, p->stream));
@@ -1720,8 +1763,12 @@ int pc_edge_batch(pc_pool* p, long long n, const double* in, double* out) {
   HIPCHK(hipMemcpyAsync(din, in, (size_t)n * 6 * sizeof(double), hipMemcpyHostToDevice,
                         p->stream));
   const long long blocks = std::min<long long>((n + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_edge_batch, dim3((unsigned)blocks), dim3(256), 0, p->stream, n, din,
-                     dout);
+  if (p->solver)
+    hipLaunchKernelGGL(k_edge_batch<1>, dim3((unsigned)blocks), dim3(256), 0, p->stream, n,
+                       din, dout);
+  else
+    hipLaunchKernelGGL(k_edge_batch<0>, dim3((unsigned)blocks), dim3(256), 0, p->stream, n,
+                       din, dout);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, dout, (size_t)n * 5 * sizeof(double), hipMemcpyDeviceToHost,
                         p->stream));

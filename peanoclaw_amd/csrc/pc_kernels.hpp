@@ -174,10 +174,15 @@ struct MarchArgs {
 
 // FUSE: 0 = no next-state speeds, 1 = exact speeds of every interior edge of
 // the new state, 2 = pruned (exact speeds only for bound candidates).
-template <int MODE, bool SIDES, int FUSE>
+template <int MODE, bool SIDES, int FUSE, int SOLVER>
 __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, MarchLds& L) {
   constexpr bool FUSED = FUSE != 0;
   constexpr bool PRUNE = FUSE == 2;
+  // the f-wave's speeds do not depend on b; the augmented solver's do (the
+  // wall decision at a dry side), so its speed passes read b as well
+  constexpr bool SPEED_B = SOLVER != 0;
+  static_assert(SOLVER == 0 || FUSE == 0, "next-step speeds are fused for the f-wave only");
+  using ES = EdgeSolver<SOLVER>;
   const int lane = threadIdx.x & 63;
   const int nx = t.nx, ny = t.ny;
   const int W = pow2ceil(ny);
@@ -252,15 +257,15 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
       hL = gl(Hs + iL); hvL = gl(HVs + iL); bL = gl(Bs + iL);
       hR = gl(Hs + iL + 1); hvR = gl(HVs + iL + 1); bR = gl(Bs + iL + 1);
     }
-    if (MODE == kSpeed) { bL = 0.0; bR = 0.0; }
+    if (MODE == kSpeed && !SPEED_B) { bL = 0.0; bR = 0.0; }
     const CellQ cl = cell_dir(cell_base(hL, bL), hvL);
     const CellQ cr = cell_dir(cell_base(hR, bR), hvR);
     const int slot = s * nx + row - 1;
     if (MODE == kSpeed) {
-      L.pre[side][slot] = edge_speed_any(cl, cr);
+      L.pre[side][slot] = ES::speed(cl, cr);
     } else {
       double dhL, dhR, dqL, dqR;
-      edge_solve_any(cl, cr, dhL, dhR, dqL, dqR);
+      ES::solve(cl, cr, dhL, dhR, dqL, dqR);
       // side 0: the right cell (row,1) takes the "upwards" part;
       // side 1: the left cell (row,ny) takes the "downwards" part.
       L.pre[side * 2 + 0][slot] = Q.dyinv * (side ? dhL : dhR);
@@ -311,7 +316,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
     h = 1.0; hu = 0.0; hv = 0.0; b = 0.0;
     if (!active) return;
     const long long i = base + (long long)r * stride;
-    if (MODE != kSpeed) b = gl(Bs + i);
+    if (MODE != kSpeed || SPEED_B) b = gl(Bs + i);
     if (SIDES && (r == 0 || r == nx + 1)) {
       const long long j = r == 0 ? gx0 : gx1;
       h = gl(Hs + j);
@@ -350,10 +355,10 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
     if (r >= 1) {
       // x-edge (r-1 | r)
       if (MODE == kSpeed) {
-        smaxl = dmax(smaxl, edge_speed_any(px, cx));
+        smaxl = dmax(smaxl, ES::speed(px, cx));
       } else {
         double dhL, dhR, dqL, dqR;
-        edge_solve_any(px, cx, dhL, dhR, dqL, dqR);
+        ES::solve(px, cx, dhL, dhR, dqL, dqR);
         if (r >= 2) {
           // finalise row r-1: x-left (already in acc), x-right, y-up, y-down
           double nH = accH + dxinv * dhL;
@@ -401,7 +406,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
               if (__any(cand ? 1 : 0)) {
                 const CellQ ea = cell_dir(cell_base(pnh, 0.0), pnhu);
                 const CellQ eb = cell_dir(cell_base(h1, 0.0), hu1);
-                const double sp = edge_speed_any(ea, eb);
+                const double sp = ES::speed(ea, eb);
                 if (active) smaxl = dmax(smaxl, sp);
               }
             }
@@ -414,7 +419,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
             nbase = cell_base(h1, 0.0);
             const CellQ ncx = cell_dir(nbase, hu1);
             if (r >= 3) {
-              const double sp = edge_speed_any(pnx, ncx);
+              const double sp = ES::speed(pnx, ncx);
               if (active) smaxl = dmax(smaxl, sp);
             }
             pnx = ncx;
@@ -460,15 +465,15 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
       cl.h = L.yc[0][lm]; cl.q = L.yc[1][lm]; cl.b = L.yc[2][lm];
       cl.u = L.yc[3][lm]; cl.sq = L.yc[4][lm]; cl.c = L.yc[5][lm]; cl.f = L.yc[6][lm];
       if (MODE == kSpeed) {
-        cl.b = 0.0;
-        const double sp = edge_speed_any(cl, cy);
+        if (!SPEED_B) cl.b = 0.0;
+        const double sp = ES::speed(cl, cy);
         if (col >= 2 && active) smaxl = dmax(smaxl, sp);
         if (active && col == 1) smaxl = dmax(smaxl, L.pre[0][slot]);
         if (active && col == ny) smaxl = dmax(smaxl, L.pre[1][slot]);
         wave_sync();
       } else {
         double dhL, dhR, dqL, dqR;
-        edge_solve_any(cl, cy, dhL, dhR, dqL, dqR);
+        ES::solve(cl, cy, dhL, dhR, dqL, dqR);
         double up_h, up_hv;
         if (col >= 2) {
           up_h = dyinv * dhR;
@@ -487,7 +492,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
           if (__any(cand ? 1 : 0)) {
             const CellQ nr = cell_dir(cell_base(pnh, 0.0), nhv);
             CellQ nl = cell_dir(cell_base(L.nc[1][lm], 0.0), L.nc[2][lm]);
-            const double sp = edge_speed_any(nl, nr);
+            const double sp = ES::speed(nl, nr);
             if (col >= 2 && active) smaxl = dmax(smaxl, sp);
           }
         }
@@ -495,7 +500,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
           CellQ nl;
           nl.h = L.nc[0][lm]; nl.q = L.nc[1][lm]; nl.b = 0.0;
           nl.u = L.nc[2][lm]; nl.sq = L.nc[3][lm]; nl.c = L.nc[4][lm]; nl.f = 0.0;
-          const double sp = edge_speed_any(nl, ncy);
+          const double sp = ES::speed(nl, ncy);
           if (col >= 2 && active) smaxl = dmax(smaxl, sp);
         }
         wave_sync();
@@ -527,7 +532,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
     if (__any(cand ? 1 : 0)) {
       const CellQ nr = cell_dir(cell_base(pnh, 0.0), nhv);
       CellQ nl = cell_dir(cell_base(L.nc[1][lm], 0.0), L.nc[2][lm]);
-      const double sp = edge_speed_any(nl, nr);
+      const double sp = ES::speed(nl, nr);
       if (col >= 2 && active) smaxl = dmax(smaxl, sp);
     }
     wave_sync();
@@ -545,7 +550,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
     CellQ nl;
     nl.h = L.nc[0][lm]; nl.q = L.nc[1][lm]; nl.b = 0.0;
     nl.u = L.nc[2][lm]; nl.sq = L.nc[3][lm]; nl.c = L.nc[4][lm]; nl.f = 0.0;
-    const double sp = edge_speed_any(nl, ncy);
+    const double sp = ES::speed(nl, ncy);
     if (col >= 2 && active) smaxl = dmax(smaxl, sp);
     wave_sync();
   }
@@ -562,7 +567,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
 // over the tasks; with `only_if` set the launch does nothing unless
 // *only_if != 0, and then covers only the tasks of scenarios flagged for
 // the exact fallback of the pruned mode (a task holds one scenario).
-template <bool SIDES>
+template <bool SIDES, int SOLVER>
 __global__ void __launch_bounds__(64)
 k_speed(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plist,
         const PatchDev* __restrict__ pd, Fields F, const ScnDev* __restrict__ scn,
@@ -577,7 +582,7 @@ k_speed(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plis
     if (only_if && gl(&S->redo) == 0) continue;
     const int par = gl(&S->stepc) & 1;
     MarchArgs A{plist, pd, F, sides, E, par, 0.0, smax, scn, 1.0};
-    march_task<kSpeed, SIDES, 0>(t, A, L);
+    march_task<kSpeed, SIDES, 0, SOLVER>(t, A, L);
   }
 }
 
@@ -589,7 +594,7 @@ k_speed(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plis
 
 // One task per workgroup, or a persistent grid pulling tasks from a
 // device-side counter (reset by k_dt_final before every update launch).
-template <int FUSE, bool SIDES>
+template <int FUSE, bool SIDES, int SOLVER>
 __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(PC_UPDATE_WAVES_PER_EU, 8)))
 k_update(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plist,
@@ -613,7 +618,7 @@ k_update(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ pli
       // k_dt_final already moved the parity counter past this step
       MarchArgs A{plist, pd, F, sides, E, (gl(&S->stepc) - 1) & 1, gl(&S->dt_applied),
                   smax_next, scn, slack};
-      march_task<kUpdate, SIDES, SIDES ? FUSE : 0>(t, A, L);
+      march_task<kUpdate, SIDES, SIDES ? FUSE : 0, SOLVER>(t, A, L);
     }
     if (!persistent) return;
   }

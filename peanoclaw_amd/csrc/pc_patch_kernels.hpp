@@ -18,6 +18,7 @@ __device__ __forceinline__ long long tile_index(const PatchDev& P, int x, int y)
 // every edge of uOld, dt = fmin(CFL, maxDt), update, swap.
 // `plist` is a one-entry patch list in global memory holding `slot` (the
 // march reads its patch list with global loads).
+template <int SOLVER>
 __global__ void __launch_bounds__(64)
 k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restrict__ pd,
               Fields F, int cur, double maxDt, double* __restrict__ smax,
@@ -28,7 +29,7 @@ k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restric
   t.first = 0; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;
   EdgeBufs none{{nullptr, nullptr}, nullptr};
   MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
-  march_task<kSpeed, false, 0>(t, A, L);
+  march_task<kSpeed, false, 0, SOLVER>(t, A, L);
   __syncthreads();
   const double s = smax[slot];
   const double cfl = max_timestep(s, P.dx, P.dy);
@@ -36,7 +37,7 @@ k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restric
   A.par = cur;
   A.dt = dt;
   A.smax = nullptr;
-  march_task<kUpdateSwap, false, 0>(t, A, L);
+  march_task<kUpdateSwap, false, 0, SOLVER>(t, A, L);
   if (threadIdx.x == 0) {
     out[0] = dt;
     out[1] = cfl;
@@ -45,6 +46,7 @@ k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restric
 
 // The CFL step of one patch from uOld (state 1 - cur) with its ghost layer,
 // without the update: out[0] = max_timestep(max edge speed).
+template <int SOLVER>
 __global__ void __launch_bounds__(64)
 k_patch_cfl(int slot, const int* __restrict__ plist, const PatchDev* __restrict__ pd,
             Fields F, int cur, double* __restrict__ smax, double* __restrict__ out) {
@@ -54,7 +56,7 @@ k_patch_cfl(int slot, const int* __restrict__ plist, const PatchDev* __restrict_
   t.first = 0; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;
   EdgeBufs none{{nullptr, nullptr}, nullptr};
   MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
-  march_task<kSpeed, false, 0>(t, A, L);
+  march_task<kSpeed, false, 0, SOLVER>(t, A, L);
   __syncthreads();
   if (threadIdx.x == 0) out[0] = max_timestep(smax[slot], P.dx, P.dy);
 }
@@ -349,18 +351,25 @@ k_patch_sums(const int* __restrict__ slots, int nslots, const PatchDev* __restri
 __global__ void k_reset_counter(int* c) { *c = 0; }
 
 // The edge solver on a batch of independent edges (diagnostics/tests).
+template <int SOLVER>
 __global__ void k_edge_batch(long long n, const double* __restrict__ in,
                              double* __restrict__ out) {
   for (long long e = threadIdx.x + (long long)blockIdx.x * blockDim.x; e < n;
        e += (long long)blockDim.x * gridDim.x) {
     const double* v = in + 6 * e;
-    const CellQ L = cell_dir(cell_base(v[0], v[4]), v[2]);
-    const CellQ R = cell_dir(cell_base(v[1], v[5]), v[3]);
-    double dhL, dhR, dqL, dqR;
-    edge_solve(L, R, dhL, dhR, dqL, dqR);
+    double dhL, dhR, dqL, dqR, sp;
+    if (SOLVER == 0) {
+      const CellQ L = cell_dir(cell_base(v[0], v[4]), v[2]);
+      const CellQ R = cell_dir(cell_base(v[1], v[5]), v[3]);
+      edge_solve(L, R, dhL, dhR, dqL, dqR);
+      sp = edge_speed(L, R);
+    } else {
+      aug_solve(v[0], v[1], v[2], v[3], v[4], v[5], dhL, dhR, dqL, dqR);
+      sp = aug_speed(v[0], v[1], v[2], v[3], v[4], v[5]);
+    }
     double* o = out + 5 * e;
     o[0] = dhL; o[1] = dhR; o[2] = dqL; o[3] = dqR;
-    o[4] = edge_speed(L, R);
+    o[4] = sp;
   }
 }
 

@@ -224,7 +224,7 @@ int server_admit(pc_server* s, int id) {
                        fields_of(p), p->d_scn, edges_of(p), p->d_bcol);
     HIPCHK(hipGetLastError());
   }
-  if (p->fused != 2) p->fused_ready = false;  // exact modes recompute all speeds
+  if (fused_mode(p) != 2) p->fused_ready = false;  // exact modes recompute all speeds
   HIPCHK(hipStreamSynchronize(p->stream));
   q.status = 1;
   q.admitted_round = s->round;
@@ -341,7 +341,7 @@ int pc_server_round(pc_server* s, int steps, int* active_out, int* queued_out) {
     // again.
     const bool replay = !p->topo_dirty && p->graph_valid && !p->scn_dev_dirty &&
                         !(p->sides_any && p->edges_dirty) &&
-                        !(p->fused && p->sides_any && !p->fused_ready);
+                        !(fused_mode(p) && p->sides_any && !p->fused_ready);
     if (replay) {
       CHK(pc_graph_replay(p, steps));
     } else {

@@ -234,6 +234,247 @@ __device__ __forceinline__ double edge_speed_any(const CellQ& L, const CellQ& R)
   return edge_speed(L, R);
 }
 
+// ---- augmented Riemann solver ----
+//
+// SWE's solverType 2 (SConstruct:295-296): the augmented Riemann solver of
+// D. L. George, J. Comput. Phys. 227 (2008) 3089-3113, in the form of its
+// published implementation (GeoClaw riemann_aug_JCP / riemanntype and the
+// rpn2 wet/dry handling), normal direction only, one pass of the
+// steady-state wave and of the middle-state estimate, no rarefaction
+// entropy corrector.  Three waves with speeds l1 = min(Einfeldt s1, middle
+// 2-speed), l2 = (l1 + l3) / 2, l3 = max(Einfeldt s2, middle 1-speed) and
+// eigenvectors (1, l1, l1^2), (0, 0, 1), (1, l3, l3^2); the jump
+// (dh - ddh, dhu, dphi - ddphi) minus the steady-state wave is solved by
+// Cramer's rule.  A side is wet when h > dryTol; a dry side becomes a wall
+// (mirrored state, the waves into it removed) when the wet side cannot
+// reach its bathymetry, otherwise it is inundated.  Plain sqrt() and '/'
+// (correctly rounded) throughout: dry sides produce zeros and the middle
+// state may be 0, outside the range of sqrt_nr / div_nr.
+constexpr double kAugCritTol = 1e-6;
+
+// Middle-state speeds of the Riemann problem without bathymetry: two
+// rarefactions in closed form; two shocks one Newton step on sqrt(h) from
+// max(hL, hR); shock + rarefaction one chord step on [min, max].
+__device__ __forceinline__ void aug_middle(double hL, double hR, double uL, double uR,
+                                           double& hm, double& s1m, double& s2m) {
+  const double g = kGravity;
+  const double hmin = dmin(hR, hL), hmax = dmax(hR, hL);
+  const double du = uR - uL;
+  if (hmin <= kDryTol) {
+    hm = 0.0;
+    s1m = ((uR + uL) - 2.0 * sqrt(g * hR)) + 2.0 * sqrt(g * hL);
+    s2m = s1m;
+    return;
+  }
+  const double cmax = sqrt(g * hmax);
+  const double fmin = du + 2.0 * (sqrt(g * hmin) - cmax);
+  const double fmax = du + (hmax - hmin) * sqrt(((0.5 * g) * (hmax + hmin)) / (hmax * hmin));
+  if (fmin > 0.0) {
+    const double cL = sqrt(g * hL), cR = sqrt(g * hR);
+    const double w = dmax(0.0, 2.0 * (cL + cR) - du);
+    hm = (1.0 / (16.0 * g)) * (w * w);
+    const double cm = sqrt(g * hm);
+    s1m = (uL + 2.0 * cL) - 3.0 * cm;
+    s2m = (uR - 2.0 * cR) + 3.0 * cm;
+  } else if (fmax <= 0.0) {
+    const double h0 = hmax;
+    const double gL = sqrt((0.5 * g) * (1.0 / h0 + 1.0 / hL));
+    const double gR = sqrt((0.5 * g) * (1.0 / h0 + 1.0 / hR));
+    const double f0 = (du + (h0 - hL) * gL) + (h0 - hR) * gR;
+    const double h02 = 4.0 * (h0 * h0);
+    const double dfdh = (gL - (g * (h0 - hL)) / (h02 * gL)) + (gR - (g * (h0 - hR)) / (h02 * gR));
+    const double slope = (2.0 * sqrt(h0)) * dfdh;
+    const double r = sqrt(h0) - f0 / slope;
+    hm = r * r;
+    const double u1m = uL - (hm - hL) * sqrt((0.5 * g) * (1.0 / hm + 1.0 / hL));
+    const double u2m = uR + (hm - hR) * sqrt((0.5 * g) * (1.0 / hm + 1.0 / hR));
+    const double cm = sqrt(g * hm);
+    s1m = u1m - cm;
+    s2m = u2m + cm;
+  } else {
+    const double lo = hmin;
+    const double flo = (du + 2.0 * (sqrt(g * lo) - cmax)) +
+                       (lo - hmin) * sqrt((0.5 * g) * (1.0 / lo + 1.0 / hmin));
+    const double slope = (fmax - flo) / (hmax - lo);
+    hm = lo - flo / slope;
+    const double cm = sqrt(g * hm);
+    if (hL > hR) {
+      const double w = uL + 2.0 * sqrt(g * hL);
+      s1m = w - 3.0 * cm;
+      s2m = w - cm;
+    } else {
+      const double w = uR - 2.0 * sqrt(g * hR);
+      s2m = w + 3.0 * cm;
+      s1m = w + cm;
+    }
+  }
+}
+
+__device__ __forceinline__ double det3(double a00, double a01, double a02, double a10,
+                                       double a11, double a12, double a20, double a21,
+                                       double a22) {
+  return (a00 * (a11 * a22 - a12 * a21) - a01 * (a10 * a22 - a12 * a20)) +
+         a02 * (a10 * a21 - a11 * a20);
+}
+
+// Wet/dry handling and wave speeds; returns false for two dry sides.  On
+// return the states are the modified ones (zeroed dry side, mirrored wall
+// state, lowered bathymetry of an inundated dry side).
+struct AugEdge {
+  double hL, hR, huL, huR, bL, bR, uL, uR, phiL, phiR, cL, cR;
+  double l1, l2, l3, s1m, s2m;
+  double w0, w1, w2;
+};
+
+__device__ __forceinline__ bool aug_setup(double hL, double hR, double huL, double huR,
+                                          double bL, double bR, AugEdge& e) {
+  const double g = kGravity;
+  if (!(hL > kDryTol) && !(hR > kDryTol)) return false;
+  double uL, uR, phiL, phiR;
+  if (hR > kDryTol) {
+    uR = huR / hR;
+    phiR = (0.5 * g) * (hR * hR) + (huR * huR) / hR;
+  } else {
+    hR = 0.0; huR = 0.0; uR = 0.0; phiR = 0.0;
+  }
+  if (hL > kDryTol) {
+    uL = huL / hL;
+    phiL = (0.5 * g) * (hL * hL) + (huL * huL) / hL;
+  } else {
+    hL = 0.0; huL = 0.0; uL = 0.0; phiL = 0.0;
+  }
+  double w0 = 1.0, w1 = 1.0, w2 = 1.0;
+  double hs, t1, t2;
+  if (hR <= kDryTol) {
+    aug_middle(hL, hL, uL, -uL, hs, t1, t2);
+    if (dmax(hL, hs) + bL < bR) {
+      w1 = 0.0; w2 = 0.0;
+      hR = hL; huR = -huL; bR = bL; phiR = phiL; uR = -uL;
+    } else if (hL + bL < bR) {
+      bR = hL + bL;
+    }
+  } else if (hL <= kDryTol) {
+    aug_middle(hR, hR, -uR, uR, hs, t1, t2);
+    if (dmax(hR, hs) + bR < bL) {
+      w0 = 0.0; w1 = 0.0;
+      hL = hR; huL = -huR; bL = bR; phiL = phiR; uL = -uR;
+    } else if (hR + bR < bL) {
+      bL = hR + bR;
+    }
+  }
+  const double cL = sqrt(g * hL), cR = sqrt(g * hR);
+  const double uhat = (cL * uL + cR * uR) / (cR + cL);
+  const double chat = sqrt(g * (0.5 * (hR + hL)));
+  const double sE1 = dmin(uL - cL, uhat - chat);
+  const double sE2 = dmax(uR + cR, uhat + chat);
+  double hm, s1m, s2m;
+  aug_middle(hL, hR, uL, uR, hm, s1m, s2m);
+  e.l1 = dmin(sE1, s2m);
+  e.l3 = dmax(sE2, s1m);
+  e.l2 = 0.5 * (e.l1 + e.l3);
+  e.hL = hL; e.hR = hR; e.huL = huL; e.huR = huR; e.bL = bL; e.bR = bR;
+  e.uL = uL; e.uR = uR; e.phiL = phiL; e.phiR = phiR; e.cL = cL; e.cR = cR;
+  e.s1m = s1m; e.s2m = s2m;
+  e.w0 = w0; e.w1 = w1; e.w2 = w2;
+  return true;
+}
+
+__device__ __forceinline__ double aug_speed(double hL, double hR, double huL, double huR,
+                                            double bL, double bR) {
+  AugEdge e;
+  if (!aug_setup(hL, hR, huL, huR, bL, bR, e)) return 0.0;
+  double s = 0.0;
+  s = dmax(s, fabs(e.l1 * e.w0));
+  s = dmax(s, fabs(e.l2 * e.w1));
+  s = dmax(s, fabs(e.l3 * e.w2));
+  return s;
+}
+
+__device__ __forceinline__ void aug_split(double sw, double fh, double fq, double& a, double& b,
+                                          double& c, double& d) {
+  if (sw < -kZeroTol) {
+    a += fh; c += fq;
+  } else if (sw > kZeroTol) {
+    b += fh; d += fq;
+  } else {
+    a += 0.5 * fh; c += 0.5 * fq;
+    b += 0.5 * fh; d += 0.5 * fq;
+  }
+}
+
+__device__ __forceinline__ void aug_solve(double hL0, double hR0, double huL0, double huR0,
+                                          double bL0, double bR0, double& dhL, double& dhR,
+                                          double& dqL, double& dqR) {
+  const double g = kGravity;
+  const double crit = kAugCritTol;
+  AugEdge e;
+  double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+  if (aug_setup(hL0, hR0, huL0, huR0, bL0, bR0, e)) {
+    const double hL = e.hL, hR = e.hR, uL = e.uL, uR = e.uR, cL = e.cL, cR = e.cR;
+    const double sE1 = e.l1, sE2 = e.l3, l1 = e.l1, l3 = e.l3;
+    const double delh = hR - hL, delhu = e.huR - e.huL, delphi = e.phiR - e.phiL;
+    const double delb = e.bR - e.bL;
+    const double hstarHLL =
+        dmax((((e.huL - e.huR) + sE2 * hR) - sE1 * hL) / (sE2 - sE1), 0.0);
+    const double hbar = dmax(0.5 * (hL + hR), 0.0);
+    const double s1s2bar = 0.25 * ((uL + uR) * (uL + uR)) - g * hbar;
+    const double s1s2tilde = dmax(0.0, uL * uR) - g * hbar;
+    const bool sonic = fabs(s1s2bar) <= crit || s1s2bar * s1s2tilde <= crit ||
+                       (s1s2bar * sE1) * sE2 <= crit || dmin(fabs(sE1), fabs(sE2)) < crit ||
+                       (sE1 < 0.0 && e.s1m > 0.0) || (sE2 > 0.0 && e.s2m < 0.0) ||
+                       (uL + cL) * (uR + cR) < 0.0 || (uL - cL) * (uR - cR) < 0.0;
+    double ddh = sonic ? -delb : ((delb * g) * hbar) / s1s2bar;
+    if (sE1 < -crit && sE2 > crit) {
+      ddh = dmin(ddh, (hstarHLL * (sE2 - sE1)) / sE2);
+      ddh = dmax(ddh, (hstarHLL * (sE2 - sE1)) / sE1);
+    } else if (sE1 >= crit) {
+      ddh = dmin(ddh, (hstarHLL * (sE2 - sE1)) / sE1);
+      ddh = dmax(ddh, -hL);
+    } else if (sE2 <= -crit) {
+      ddh = dmin(ddh, hR);
+      ddh = dmax(ddh, (hstarHLL * (sE2 - sE1)) / sE2);
+    }
+    double ddphi =
+        sonic ? ((-g) * hbar) * delb : ((((-delb) * g) * hbar) * s1s2tilde) / s1s2bar;
+    ddphi = dmin(ddphi, g * dmax((-hL) * delb, (-hR) * delb));
+    ddphi = dmax(ddphi, g * dmin((-hL) * delb, (-hR) * delb));
+    const double d0 = delh - ddh, d1 = delhu, d2 = delphi - ddphi;
+    const double r20 = l1 * l1, r22 = l3 * l3;
+    const double det = det3(1.0, 0.0, 1.0, l1, 0.0, l3, r20, 1.0, r22);
+    const double beta0 = det3(d0, 0.0, 1.0, d1, 0.0, l3, d2, 1.0, r22) / det;
+    const double beta1 = det3(1.0, d0, 1.0, l1, d1, l3, r20, d2, r22) / det;
+    const double beta2 = det3(1.0, 0.0, d0, l1, 0.0, d1, r20, 1.0, d2) / det;
+    aug_split(l1 * e.w0, (beta0 * l1) * e.w0, (beta0 * r20) * e.w0, a, b, c, d);
+    aug_split(e.l2 * e.w1, (beta1 * 0.0) * e.w1, (beta1 * 1.0) * e.w1, a, b, c, d);
+    aug_split(l3 * e.w2, (beta2 * l3) * e.w2, (beta2 * r22) * e.w2, a, b, c, d);
+  }
+  dhL = a; dhR = b; dqL = c; dqR = d;
+}
+
+// Edge solver selection for the patch kernels: 0 = f-wave (with its
+// wave-uniform fast paths), 1 = augmented Riemann.
+template <int SOLVER>
+struct EdgeSolver {
+  __device__ static __forceinline__ double speed(const CellQ& L, const CellQ& R) {
+    return edge_speed_any(L, R);
+  }
+  __device__ static __forceinline__ void solve(const CellQ& L, const CellQ& R, double& dhL,
+                                               double& dhR, double& dqL, double& dqR) {
+    edge_solve_any(L, R, dhL, dhR, dqL, dqR);
+  }
+};
+
+template <>
+struct EdgeSolver<1> {
+  __device__ static __forceinline__ double speed(const CellQ& L, const CellQ& R) {
+    return aug_speed(L.h, R.h, L.q, R.q, L.b, R.b);
+  }
+  __device__ static __forceinline__ void solve(const CellQ& L, const CellQ& R, double& dhL,
+                                               double& dhR, double& dqL, double& dqR) {
+    aug_solve(L.h, R.h, L.q, R.q, L.b, R.b, dhL, dhR, dqL, dqR);
+  }
+};
+
 // ---- cheap upper bounds of edge speeds (pruned CFL pass) ----
 //
 // For the Einfeldt speeds above, |s1|, |s2| <= max(|uL|,|uR|) + max(cL,cR):

@@ -156,6 +156,11 @@ class PatchPool:
         check(self.lib.pc_set_launch_config(self.h, persistent_workgroups, fused,
                                             patches_per_task_cap))
 
+    def wave_propagation_solver(self, solver):
+        """Edge solver of the pool's updates: 1 or 4 f-wave, 2 augmented
+        Riemann (the reference's solverType build option)."""
+        check(self.lib.pc_set_wave_propagation_solver(self.h, int(solver)))
+
     def prune_slack(self, slack):
         """Threshold slack of the pruned speed mode (fused=2); see the C header."""
         check(self.lib.pc_set_prune_slack(self.h, float(slack)))

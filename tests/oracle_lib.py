@@ -79,6 +79,13 @@ def lib():
             "or_fwave": (None, [ctypes.c_double] * 6 + [_dp]),
             "or_fwave_speed": (ctypes.c_double, [ctypes.c_double] * 4),
             "or_fwave_batch": (None, [ctypes.c_longlong, _dp, _dp]),
+            "or_aug_middle_state": (None, [ctypes.c_double] * 4 + [ctypes.c_int, _dp]),
+            "or_aug_waves": (ctypes.c_int, [ctypes.c_double] * 6 + [_dp] * 3),
+            "or_augrie": (None, [ctypes.c_double] * 6 + [_dp]),
+            "or_set_wave_propagation_solver": (ctypes.c_int, [ctypes.c_int]),
+            "or_edge": (None, [ctypes.c_double] * 6 + [_dp]),
+            "or_edge_speed": (ctypes.c_double, [ctypes.c_double] * 6),
+            "or_edge_batch": (None, [ctypes.c_longlong, _dp, _dp]),
             "or_block_fluxes": (ctypes.c_double, [ctypes.c_int, ctypes.c_int,
                                 ctypes.c_double, ctypes.c_double] + [_dp] * 7),
             "or_block_max_timestep": (ctypes.c_double, [ctypes.c_double] * 3),
@@ -138,6 +145,40 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def augrie(hL, hR, huL, huR, bL, bR):
+    """(dhL, dhR, dqL, dqR, max speed) of the oracle's augmented solver."""
+    o = (ctypes.c_double * 5)()
+    lib().or_augrie(hL, hR, huL, huR, bL, bR, o)
+    return tuple(o)
+
+
+def aug_waves(hL, hR, huL, huR, bL, bR):
+    sw, fh, fq = (ctypes.c_double * 3)(), (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
+    lib().or_aug_waves(hL, hR, huL, huR, bL, bR, sw, fh, fq)
+    return tuple(sw), tuple(fh), tuple(fq)
+
+
+def aug_middle_state(hL, hR, uL, uR, maxiter=1):
+    o = (ctypes.c_double * 5)()
+    lib().or_aug_middle_state(hL, hR, uL, uR, maxiter, o)
+    return tuple(o)
+
+
+class wave_propagation_solver:
+    """Context manager selecting the oracle's edge solver (1, 4 f-wave;
+    2 augmented Riemann), restoring the f-wave on exit."""
+
+    def __init__(self, solver):
+        self.solver = solver
+
+    def __enter__(self):
+        assert lib().or_set_wave_propagation_solver(self.solver) == 0
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_wave_propagation_solver(1)
 
 
 def clamp_events():
