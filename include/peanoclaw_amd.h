@@ -309,6 +309,48 @@ int pc_server_poll(pc_server *srv, pc_request_result *out, int cap, int *n);
 int pc_server_request_slots(pc_server *srv, int id, int *slots, int cap, int *n);
 int pc_server_release(pc_server *srv, int id);
 
+/* ---- batched step of an adaptive grid ----
+ * The per-call Numerics sequence of one global AMR step (restriction of
+ * every leaf into each virtual patch containing it, copyUNewToUOld, the
+ * ghost layer of every leaf face from the same-level patch or by
+ * interpolation from the coarser leaf, domain walls, global CFL minimum,
+ * solveTimestep of every leaf; GridLevelTransfer.cpp:140-198,
+ * GhostlayerCompositorFunctors.cpp:59-69, SolveTimestep.cpp:87-142 and
+ * 591-751) handed over once as a plan and run as nine batched launches per
+ * step.  Level transfers read h + b instead of transforming the source in
+ * place and back (the reference's round trip can move a depth by an ulp;
+ * results agree with the per-call form to ~1e-14 relative).  No reference
+ * counterpart as one entry point: it replaces the traversal's calls for a
+ * grid whose leaves share one clock.  The plan stays valid while its
+ * patches stay registered; the patches must belong to one scenario, which
+ * pc_step_all must not advance meanwhile. */
+typedef struct pc_amr pc_amr;
+/* kind 0: transferGhostlayer of size at src_offset of src to dst_offset of
+ * dst (src a leaf or virtual patch of the same level); kind 1:
+ * interpolateSolution(toUOld, not toCurrentTime, useTimeUNewOrUOld) from
+ * the coarser leaf src into the region (size, dst_offset) of dst.  The
+ * destination region must be a strip of dst's ghost layer. */
+typedef struct {
+  int32_t kind, src, dst;
+  int32_t size[2], src_offset[2], dst_offset[2];
+} pc_amr_ghost;
+/* fillBoundaryLayer(dimension, upper) of a leaf; entries of one leaf are
+ * consecutive, in call order. */
+typedef struct {
+  int32_t slot, dimension, upper;
+} pc_amr_wall;
+/* restrictions: n pairs (leaf, virtual) in call order. */
+int pc_amr_create(pc_pool *pool, const int32_t *leaves, int nleaves,
+                  const int32_t *virtuals, int nvirtuals, const int32_t *restrictions,
+                  int nrestrictions, const pc_amr_ghost *ghosts, int nghosts,
+                  const pc_amr_wall *walls, int nwalls, pc_amr **out);
+/* k global steps, each bounded by end_time - (time + timestep_size) and
+ * skipped once that is <= 1e-8; *last_dt = the last step's dt (0 if
+ * skipped).  The leaves' descriptors carry the clock afterwards. */
+int pc_amr_step(pc_amr *amr, int k, double end_time, double *last_dt);
+int pc_amr_clock(pc_amr *amr, double *time, double *timestep_size, long long *steps);
+int pc_amr_destroy(pc_amr *amr);
+
 /* Raw device pointers of the pool (for zero-copy interop); state = 0/1. */
 int pc_pool_device_fields(pc_pool *pool, int state, double **h, double **hu,
                           double **hv, double **b);

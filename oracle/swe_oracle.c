@@ -692,9 +692,11 @@ static void subcell_center(const or_patch *p, int x, int y, double c[2]) {
 
 /* DefaultInterpolationTemplate::interpolateSolution
  * (DefaultInterpolation.cpph:29-270) */
-void or_interpolate_default(const int dstSize[2], const int dstOff[2],
-                            or_patch *src, or_patch *dst, int toUOld,
-                            int toCurrentTime, int useTimeUNewOrUOld) {
+/* add_b: the source depth is read as h + b (the SWE transform applied to
+ * the values read instead of to the source in place) */
+static void interpolate_values(const int dstSize[2], const int dstOff[2], or_patch *src,
+                               or_patch *dst, int toUOld, int toCurrentTime,
+                               int useTimeUNewOrUOld, int add_b) {
   double timeFactor;
   if (la_equals(src->dt, 0.0)) {
     timeFactor = 1.0;
@@ -747,6 +749,11 @@ void or_interpolate_default(const int dstSize[2], const int dstOff[2],
         for (int q = 0; q < src->nq; q++) {
           double so = src->uold[or_uold_index(src, q, nb[0], nb[1])];
           double sn_ = src->unew[or_unew_index(src, q, nb[0], nb[1])];
+          if (add_b && q == 0) {
+            const double bv = src->pwg[or_uold_index(src, 0, nb[0], nb[1])];
+            so = so + 1.0 * bv;
+            sn_ = sn_ + 1.0 * bv;
+          }
           if (toUOld) {
             int t = or_uold_index(dst, q, dx_, dy_);
             dst->uold[t] = dst->uold[t] +
@@ -782,6 +789,24 @@ void or_region_map_to_patch(const int off[2], const int size[2],
 }
 
 /* SWEKernel::interpolateSolution (SWEKernel.cpp:239-286) */
+void or_interpolate_default(const int dstSize[2], const int dstOff[2],
+                            or_patch *src, or_patch *dst, int toUOld,
+                            int toCurrentTime, int useTimeUNewOrUOld) {
+  interpolate_values(dstSize, dstOff, src, dst, toUOld, toCurrentTime, useTimeUNewOrUOld, 0);
+}
+
+/* SWE interpolation without modifying the source: the values read are
+ * h + b as after SWEKernel's in-place transform of the source region, and
+ * the source is not transformed back (the reference's round trip
+ * (h + b) - b can move h by an ulp; here the source keeps its bits).  The
+ * destination region is transformed back to h with its own b. */
+void or_interpolate_swe_pure(const int dstSize[2], const int dstOff[2], or_patch *src,
+                             or_patch *dst, int toUOld, int toCurrentTime,
+                             int useTimeUNewOrUOld) {
+  interpolate_values(dstSize, dstOff, src, dst, toUOld, toCurrentTime, useTimeUNewOrUOld, 1);
+  or_transform_water_height(dst, dstOff, dstSize, toUOld, 1);
+}
+
 void or_interpolate_swe(const int dstSize[2], const int dstOff[2],
                         or_patch *src, or_patch *dst, int toUOld,
                         int toCurrentTime, int useTimeUNewOrUOld) {
@@ -888,7 +913,7 @@ int or_regions_overlapped_by_ghostlayers(const double lower[2],
 
 /* DefaultRestrictionTemplate::restrictRegion (DefaultRestriction.cpph:16-106) */
 static void restrict_region(or_patch *src, or_patch *dst, const int rOff[2],
-                            const int rSize[2]) {
+                            const int rSize[2], int add_b) {
   const double epsilon = 1e-12;
   const double dstTimeUOld = 0.0, dstTimeUNew = 1.0;
   double srcSub[2], dstSub[2];
@@ -923,6 +948,11 @@ static void restrict_region(or_patch *src, or_patch *dst, const int rOff[2],
           for (int q = 0; q < src->nq; q++) {
             double vo = src->uold[or_uold_index(src, q, si[0], si[1])];
             double vn = src->unew[or_unew_index(src, q, si[0], si[1])];
+            if (add_b && q == 0) {
+              const double bv = src->pwg[or_uold_index(src, 0, si[0], si[1])];
+              vo = vo + 1.0 * bv;
+              vn = vn + 1.0 * bv;
+            }
             double dOld = (vo * (1.0 - tfOld) + vn * tfOld) * overlap / dstRegion;
             double dNew = (vo * (1.0 - tfNew) + vn * tfNew) * overlap / dstRegion;
             int a = or_uold_index(dst, q, di[0], di[1]);
@@ -943,12 +973,19 @@ void or_restrict_default(or_patch *src, or_patch *dst, int onlyOverlapped) {
         src->subcell, sn, off, size);
     for (int r = 0; r < nr; r++) {
       if (size[2 * r] > 0 && size[2 * r + 1] > 0)
-        restrict_region(src, dst, off + 2 * r, size + 2 * r);
+        restrict_region(src, dst, off + 2 * r, size + 2 * r, 0);
     }
   } else {
     int z[2] = {0, 0};
-    restrict_region(src, dst, z, sn);
+    restrict_region(src, dst, z, sn, 0);
   }
+}
+
+/* SWE restriction of the whole source without modifying it: values read
+ * as h + b (see or_interpolate_swe_pure). */
+void or_restrict_swe_pure(or_patch *src, or_patch *dst) {
+  int z[2] = {0, 0}, sn[2] = {src->nx, src->ny};
+  restrict_region(src, dst, z, sn, 1);
 }
 
 /* SWEKernel::restrictSolution / postProcessRestriction (SWEKernel.cpp:288-316) */

@@ -115,6 +115,12 @@ void or_interpolate_swe(const int dstSize[2], const int dstOff[2],
                         int toCurrentTime, int useTimeUNewOrUOld);
 void or_restrict_default(or_patch *src, or_patch *dst, int onlyOverlapped);
 void or_restrict_swe(or_patch *src, or_patch *dst, int onlyOverlapped);
+/* the SWE transfers between levels with the h <-> h+b transform applied to
+   the values read, the source left unchanged (batched AMR semantics) */
+void or_restrict_swe_pure(or_patch *src, or_patch *dst);
+void or_interpolate_swe_pure(const int dstSize[2], const int dstOff[2], or_patch *src,
+                             or_patch *dst, int toUOld, int toCurrentTime,
+                             int useTimeUNewOrUOld);
 void or_post_process_restriction_swe(or_patch *dst, int onlyOverlapped);
 void or_transform_water_height(or_patch *p, const int off[2], const int size[2],
                                int modifyUOld, int absoluteToAboveSeaFloor);

@@ -94,6 +94,18 @@ class RequestResult(ctypes.Structure):
     ]
 
 
+class AmrGhost(ctypes.Structure):
+    """pc_amr_ghost: kind 0 transfer, 1 interpolation."""
+    _fields_ = [("kind", ctypes.c_int32), ("src", ctypes.c_int32), ("dst", ctypes.c_int32),
+                ("size", ctypes.c_int32 * 2), ("src_offset", ctypes.c_int32 * 2),
+                ("dst_offset", ctypes.c_int32 * 2)]
+
+
+class AmrWall(ctypes.Structure):
+    _fields_ = [("slot", ctypes.c_int32), ("dimension", ctypes.c_int32),
+                ("upper", ctypes.c_int32)]
+
+
 class PeanoClawAmdError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"peanoclaw_amd error {code}: {msg}")
@@ -182,6 +194,14 @@ _SIGS = {
     "pc_server_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pc_pool_device_fields": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] +
                               [ctypes.POINTER(_dp)] * 4),
+    "pc_amr_create": (ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.c_int, _ip, ctypes.c_int,
+                                     _ip, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_void_p)]),
+    "pc_amr_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp]),
+    "pc_amr_clock": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp,
+                                    ctypes.POINTER(ctypes.c_longlong)]),
+    "pc_amr_destroy": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGS)

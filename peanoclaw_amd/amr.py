@@ -61,8 +61,11 @@ class AmrGrid:
     (indices (3i+a, 3j+c)), down to `max_level`.  The refinement is fixed at
     construction."""
 
-    def __init__(self, backend, px, py, n, refine, fn, max_level=2):
+    def __init__(self, backend, px, py, n, refine, fn, max_level=2, pure=False):
         self.b = backend
+        # pure: transfers between levels read h + b without transforming the
+        # source in place (the semantics of the batched device step)
+        self.pure = pure
         self.px, self.py, self.n = px, py, n
         self.max_level = max_level
         self.levels = {}
@@ -187,19 +190,21 @@ class AmrGrid:
         B = self.b
         for v in self.virtuals:
             B.clear(v.handle)
+        restrict = B.restrict_pure if self.pure else B.restrict
         for leaf, ancestors in self._restrictions:
             for v in ancestors:
-                B.restrict(leaf.handle, v.handle)
+                restrict(leaf.handle, v.handle)
         for v in self.virtuals:
             B.post_restrict(v.handle)
         for p in self.leaves:
             B.copy_unew_to_uold(p.handle)
         plan, walls = self._plan
+        interpolate = B.interpolate_pure if self.pure else B.interpolate
         for kind, src, dst, size, soff, doff in plan:
             if kind == "transfer":
                 B.transfer(size, soff, doff, src.handle, dst.handle)
             else:
-                B.interpolate(size, doff, src.handle, dst.handle, 1, 0, 1)
+                interpolate(size, doff, src.handle, dst.handle, 1, 0, 1)
         for p, dim, upper in walls:
             B.fill_boundary(p.handle, dim, upper)
 
@@ -325,6 +330,62 @@ def wet_dry_fields(seed=4, column=(0.05, 0.35, 0.3, 0.6333333333333333), column_
     return fn
 
 
+class BatchedAmr:
+    """The grid's global steps as one batched device plan (pc_amr_*): the
+    same restriction, ghost and wall sequence as AmrGrid.step with
+    pure=True, nine launches per step.  `grid` must run on a DeviceBackend;
+    afterwards its leaves hold the state and clock of the batched steps."""
+
+    def __init__(self, grid):
+        from ._abi import AmrGhost, AmrWall
+        self.grid = grid
+        self.pool = grid.b.pool
+        self.lib = load()
+        leaves = iarr_list([p.handle for p in grid.leaves])
+        virtuals = iarr_list([v.handle for v in grid.virtuals])
+        pairs = [x for leaf, anc in grid._restrictions for v in anc for x in (leaf.handle,
+                                                                                v.handle)]
+        rest = iarr_list(pairs)
+        plan, walls = grid._plan
+        gh = (AmrGhost * max(len(plan), 1))()
+        for k, (kind, src, dst, size, soff, doff) in enumerate(plan):
+            g = gh[k]
+            g.kind = 0 if kind == "transfer" else 1
+            g.src, g.dst = src.handle, dst.handle
+            g.size[:] = size
+            g.src_offset[:] = soff
+            g.dst_offset[:] = doff
+        wl = (AmrWall * max(len(walls), 1))()
+        for k, (p, dim, upper) in enumerate(walls):
+            wl[k].slot, wl[k].dimension, wl[k].upper = p.handle, dim, upper
+        h = ctypes.c_void_p()
+        check(self.lib.pc_amr_create(self.pool.h, leaves, len(grid.leaves), virtuals,
+                                     len(grid.virtuals), rest, len(pairs) // 2, gh, len(plan),
+                                     wl, len(walls), ctypes.byref(h)))
+        self.h = h
+
+    def step(self, k=1, end_time=np.inf):
+        """k global steps; returns the last step's dt (0 once the end time
+        is reached)."""
+        dt = ctypes.c_double()
+        check(self.lib.pc_amr_step(self.h, int(k), float(end_time), ctypes.byref(dt)))
+        return dt.value
+
+    def clock(self):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        check(self.lib.pc_amr_clock(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)))
+        return t.value, dt.value, n.value
+
+    def close(self):
+        if self.h:
+            check(self.lib.pc_amr_destroy(self.h))
+            self.h = None
+
+
+def iarr_list(v):
+    return (ctypes.c_int * max(len(v), 1))(*v)
+
+
 def front_refinement(px, py, n, fn, max_level, jump=0.5):
     """refine(level, ij) for the wet/dry basin: a patch refines when its
     cells (initial data) hold both dry (h = 0) and wet cells, or when the
@@ -358,5 +419,5 @@ def dam_break_fields(centre=(0.5, 0.5), radius=0.25, h_in=2.0, h_out=1.0, hills=
     return fn
 
 
-__all__ = ["AmrGrid", "TwoLevelGrid", "DeviceBackend", "dam_break_fields",
+__all__ = ["AmrGrid", "BatchedAmr", "TwoLevelGrid", "DeviceBackend", "dam_break_fields",
            "front_refinement", "refined_set_from_flags", "wet_dry_fields"]

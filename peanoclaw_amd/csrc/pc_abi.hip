@@ -1824,3 +1824,4 @@ int pc_pool_device_fields(pc_pool* p, int state, double** h, double** hu, double
 }  // extern "C"
 
 #include "pc_server.hpp"
+#include "pc_amr.hpp"

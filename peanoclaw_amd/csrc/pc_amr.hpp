@@ -1,0 +1,736 @@
+// Batched step of an adaptive patch grid (included at the end of
+// pc_abi.hip; uses the pool internals).
+//
+// The per-call form of a global AMR step is a long sequence of Numerics
+// calls: restriction of every leaf into each virtual patch containing it
+// (GridLevelTransfer::restrictToOverlappingVirtualSubgrids,
+// GridLevelTransfer.cpp:140-198, with SWEKernel::restrictSolution and
+// postProcessRestriction, SWEKernel.cpp:288-316), copyUNewToUOld of every
+// leaf, a ghost-layer fill of every leaf face by transferGhostlayer from
+// the same-level patch or interpolateSolution from the coarser leaf
+// (GhostlayerCompositorFunctors.cpp:59-69; SWEKernel.cpp:239-286), walls
+// (SolveTimestep.cpp:87-142), the global CFL minimum and solveTimestep of
+// every leaf.  Here the host hands over that plan once (a device-side
+// table rebuilt only when the grid changes) and every step is nine
+// launches, each phase over all patches at once:
+//
+//   begin    step bound from the end time; steps past it do nothing
+//   restrict one workgroup per virtual patch: every cell sums the
+//            contributions of its leaves in plan order, then h+b -> h
+//   copy     uNew -> uOld of every leaf
+//   ghosts   one wavefront per face: copy with time factor, or d-linear
+//            interpolation from the coarser leaf (h+b, back to h with the
+//            destination's b)
+//   walls    one workgroup per leaf with domain walls, in call order
+//   speed    exact edge speeds of every leaf (row march over uOld)
+//   dt       global CFL minimum, bounded by the end time
+//   solve    update of every leaf with that step (row march, uOld -> uNew)
+//   end      the leaves' clock
+//
+// The level transfers read h + b instead of transforming the source in
+// place and back: the reference's round trip (h + b) - b can move a depth
+// by one ulp, which the per-call entry points reproduce and this form does
+// not (results agree to ~1e-14 relative; both are checked against the
+// oracle's matching drivers).  Every other operation is the per-call
+// arithmetic in the same order.
+#pragma once
+
+namespace {
+
+struct AmrClock {
+  double time, dt;  // the leaves' clock (time, timestep_size)
+  double maxdt;     // end_time - (time + dt) of the step being run
+  double bound;     // dt of the step being run
+  int advance;      // the step runs (maxdt > tolerance)
+  int pad;
+  long long steps;
+};
+
+struct AmrVirtual {
+  int slot;
+  int pbeg, pend;  // restriction pairs [pbeg, pend)
+  int pad;
+};
+
+// Restriction of the whole leaf `src` into one virtual patch
+// (DefaultRestriction.cpph:16-106 over region (0, n)).
+struct AmrRestrict {
+  int src;
+  int rSize[2];
+  int dOff[2], dSize[2];
+  int pad;
+  double srcSub[2], dstSub[2], dstRegion;
+  double spos[2], dpos[2];
+};
+
+enum AmrGhostKind { kAmrTransferLeaf = 0, kAmrTransferVirtual = 1, kAmrInterpolate = 2 };
+
+struct AmrGhost {
+  int kind, src, dst;
+  int sx, sy, sox, soy, dox, doy;
+  int sn[2];
+  int pad;
+  double spos[2], ssub[2], sinv[2], dpos[2], dsub[2];
+};
+
+struct AmrWalls {
+  int slot, first, count, pad;  // walls[first .. first+count) as (dim, upper) pairs
+};
+
+__global__ void k_amr_begin(AmrClock* c, double end_time) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double maxdt = end_time - (c->time + c->dt);
+  c->maxdt = maxdt;
+  c->advance = (maxdt - 0.0 > kLaEps) ? 1 : 0;
+}
+
+// Restriction time factors of a leaf source on the [0, 1] basis
+// (DefaultRestriction.cpph:46-52).
+__device__ __forceinline__ void amr_restrict_factors(const AmrClock& c, double& tfOld,
+                                                     double& tfNew) {
+  tfOld = 1.0;
+  tfNew = 1.0;
+  if (c.dt - 0.0 > kLaEps) {
+    const double sOld = c.time, sNew = c.time + c.dt;
+    tfOld = (0.0 - sOld) / (sNew - sOld);
+    tfNew = (1.0 - sOld) / (sNew - sOld);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_amr_restrict(const AmrVirtual* __restrict__ virt, const AmrRestrict* __restrict__ pairs,
+               const PatchDev* __restrict__ pd, Fields F, int cur, const AmrClock* clock) {
+  const AmrClock c = *clock;
+  if (!c.advance) return;
+  const AmrVirtual V = virt[blockIdx.x];
+  const PatchDev D = pd[V.slot];
+  double tfOld, tfNew;
+  amr_restrict_factors(c, tfOld, tfNew);
+  const double eps = 1e-12;
+  const int oldSt = 1 - cur;
+  for (int k = threadIdx.x; k < D.nx * D.ny; k += blockDim.x) {
+    const int di0 = k / D.ny, di1 = k - (k / D.ny) * D.ny;
+    double accO[3] = {0.0, 0.0, 0.0};
+    double accN[3] = {0.0, 0.0, 0.0};
+    for (int pi = V.pbeg; pi < V.pend; ++pi) {
+      const AmrRestrict a = pairs[pi];
+      if (di0 < a.dOff[0] || di0 >= a.dOff[0] + a.dSize[0] || di1 < a.dOff[1] ||
+          di1 >= a.dOff[1] + a.dSize[1])
+        continue;
+      const PatchDev S = pd[a.src];
+      const double dp[2] = {a.dpos[0] + (double)di0 * a.dstSub[0],
+                            a.dpos[1] + (double)di1 * a.dstSub[1]};
+      int cOff[2], cSize[2];
+      for (int d = 0; d < 2; ++d) {
+        double ot = (dp[d] - a.spos[d] + eps);
+        ot /= a.srcSub[d];
+        const int o = (int)ot;
+        double st = dp[d] + a.dstSub[d] - a.spos[d] - eps;
+        st /= a.srcSub[d];
+        st -= ot;
+        st += 1.0;
+        const int s = (int)st;
+        const int upper = o + s, rUpper = a.rSize[d];
+        cOff[d] = o > 0 ? o : 0;
+        cSize[d] = (upper < rUpper ? upper : rUpper) - cOff[d];
+      }
+      for (int sx = 0; sx < cSize[0]; ++sx)
+        for (int sy = 0; sy < cSize[1]; ++sy) {
+          const int s0 = cOff[0] + sx, s1 = cOff[1] + sy;
+          const double sp[2] = {a.spos[0] + (double)s0 * a.srcSub[0],
+                                a.spos[1] + (double)s1 * a.srcSub[1]};
+          double region = 1.0;
+          for (int d = 0; d < 2; ++d) {
+            const double hi = dmin(dp[d] + a.dstSub[d], sp[d] + a.srcSub[d]);
+            const double lo = dmax(dp[d], sp[d]);
+            region *= hi - lo;
+            region = dmax(region, 0.0);
+          }
+          const long long si = tile_index(S, s0, s1);
+          const double bv = F.b[si];
+          const double vo[3] = {F.H(oldSt)[si] + 1.0 * bv, F.HU(oldSt)[si], F.HV(oldSt)[si]};
+          const double vn[3] = {F.H(cur)[si] + 1.0 * bv, F.HU(cur)[si], F.HV(cur)[si]};
+          for (int q = 0; q < 3; ++q) {
+            const double dO = (vo[q] * (1.0 - tfOld) + vn[q] * tfOld) * region / a.dstRegion;
+            const double dN = (vo[q] * (1.0 - tfNew) + vn[q] * tfNew) * region / a.dstRegion;
+            accO[q] = accO[q] + dO;
+            accN[q] = accN[q] + dN;
+          }
+        }
+    }
+    // SWEKernel::postProcessRestriction: h + b -> h with the virtual's b
+    const long long di = tile_index(D, di0, di1);
+    const double bd = F.b[di];
+    F.H(oldSt)[di] = accO[0] + (-1.0) * bd;
+    F.HU(oldSt)[di] = accO[1];
+    F.HV(oldSt)[di] = accO[2];
+    F.H(cur)[di] = accN[0] + (-1.0) * bd;
+    F.HU(cur)[di] = accN[1];
+    F.HV(cur)[di] = accN[2];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_amr_copy(const int* __restrict__ leaves, const PatchDev* __restrict__ pd, Fields F, int cur,
+           const AmrClock* clock) {
+  if (!clock->advance) return;
+  const PatchDev P = pd[leaves[blockIdx.x]];
+  for (int k = threadIdx.x; k < P.nx * P.ny; k += blockDim.x) {
+    const long long i = tile_index(P, k / P.ny, k - (k / P.ny) * P.ny);
+    F.H(1 - cur)[i] = F.H(cur)[i];
+    F.HU(1 - cur)[i] = F.HU(cur)[i];
+    F.HV(1 - cur)[i] = F.HV(cur)[i];
+  }
+}
+
+// One wavefront per face entry (4 per workgroup).
+__global__ void __launch_bounds__(256)
+k_amr_ghosts(const AmrGhost* __restrict__ ghosts, int nghosts, const PatchDev* __restrict__ pd,
+             Fields F, int cur, const AmrClock* clock) {
+  const AmrClock c = *clock;
+  if (!c.advance) return;
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (e >= nghosts) return;
+  const int lane = threadIdx.x & 63;
+  const AmrGhost g = ghosts[e];
+  const PatchDev S = pd[g.src], D = pd[g.dst];
+  const int oldSt = 1 - cur;
+  // every leaf runs on the clock (time, dt): time_unew = time + dt
+  const double dstTimeUNew = c.time + c.dt;
+  if (g.kind != kAmrInterpolate) {
+    // DefaultTransfer.cpph:27-51
+    double tf;
+    if (g.kind == kAmrTransferVirtual) {
+      tf = (dstTimeUNew - 0.0) / 1.0;
+    } else {
+      const double sNew = c.time + c.dt, sOld = c.time;
+      tf = (sNew - sOld - 0.0 > kLaEps) ? (dstTimeUNew - sOld) / (sNew - sOld) : 1.0;
+    }
+    const double inv = 1.0 - tf;
+    for (int k = lane; k < g.sx * g.sy; k += 64) {
+      const int x = k / g.sy, y = k - (k / g.sy) * g.sy;
+      const long long si = tile_index(S, g.sox + x, g.soy + y);
+      const long long di = tile_index(D, g.dox + x, g.doy + y);
+      F.H(oldSt)[di] = F.H(cur)[si] * tf + F.H(oldSt)[si] * inv;
+      F.HU(oldSt)[di] = F.HU(cur)[si] * tf + F.HU(oldSt)[si] * inv;
+      F.HV(oldSt)[di] = F.HV(cur)[si] * tf + F.HV(oldSt)[si] * inv;
+    }
+    return;
+  }
+  // DefaultInterpolation.cpph:45-67 (toCurrentTime = 0, useTimeUNewOrUOld = 1)
+  double tf;
+  if (fabs(c.dt - 0.0) <= kLaEps) {
+    tf = 1.0;
+  } else {
+    const double sOld = c.time, sNew = c.time + c.dt;
+    tf = (dstTimeUNew - sOld) / (sNew - sOld);
+  }
+  tf = dmax(0.0, dmin(1.0, tf));
+  for (int k = lane; k < g.sx * g.sy; k += 64) {
+    const int x = g.dox + k / g.sy, y = g.doy + (k - (k / g.sy) * g.sy);
+    const long long di = tile_index(D, x, y);
+    double acc[3] = {0.0, 0.0, 0.0};
+    double cc[2];
+    cc[0] = g.dpos[0] + g.dsub[0] * (double)x + g.dsub[0] * 0.5;
+    cc[1] = g.dpos[1] + g.dsub[1] * (double)y + g.dsub[1] * 0.5;
+    int si[2];
+    for (int d = 0; d < 2; ++d) {
+      const double base = g.spos[d] + g.ssub[d] * 0.5;
+      const double loc = cc[d] - base;
+      int v = (int)floor(loc * g.sinv[d]);
+      v = v < g.sn[d] - 2 ? v : g.sn[d] - 2;
+      v = v > 0 ? v : 0;
+      si[d] = v;
+    }
+    for (int o = 0; o < 4; ++o) {
+      const int off[2] = {o & 1, (o >> 1) & 1};
+      const int nb[2] = {si[0] + off[0], si[1] + off[1]};
+      double np[2];
+      np[0] = g.spos[0] + g.ssub[0] * (double)nb[0] + g.ssub[0] * 0.5;
+      np[1] = g.spos[1] + g.ssub[1] * (double)nb[1] + g.ssub[1] * 0.5;
+      double sf = 1.0;
+      for (int d = 0; d < 2; ++d) {
+        const double sign = (double)(1 - 2 * off[d]);
+        sf *= 1.0 - (sign * (cc[d] - np[d])) * g.sinv[d];
+      }
+      const long long sidx = tile_index(S, nb[0], nb[1]);
+      const double bv = F.b[sidx];
+      const double vo[3] = {F.H(oldSt)[sidx] + 1.0 * bv, F.HU(oldSt)[sidx], F.HV(oldSt)[sidx]};
+      const double vn[3] = {F.H(cur)[sidx] + 1.0 * bv, F.HU(cur)[sidx], F.HV(cur)[sidx]};
+      for (int q = 0; q < 3; ++q) acc[q] = acc[q] + sf * (vo[q] * (1.0 - tf) + vn[q] * tf);
+    }
+    F.H(oldSt)[di] = acc[0] + (-1.0) * F.b[di];
+    F.HU(oldSt)[di] = acc[1];
+    F.HV(oldSt)[di] = acc[2];
+  }
+}
+
+// SWEKernel::fillBoundaryLayer (SWEKernel.cpp:176-237) of one leaf, its
+// walls in call order (corner cells read what the previous wall wrote).
+__global__ void __launch_bounds__(64)
+k_amr_walls(const AmrWalls* __restrict__ walls, const int* __restrict__ dims,
+            const PatchDev* __restrict__ pd, Fields F, int cur, const AmrClock* clock) {
+  if (!clock->advance) return;
+  const AmrWalls W = walls[blockIdx.x];
+  const PatchDev P = pd[W.slot];
+  const int st = 1 - cur;
+  for (int w = 0; w < W.count; ++w) {
+    const int dim = dims[2 * (W.first + w)], upper = dims[2 * (W.first + w) + 1];
+    const int n_other = dim == 0 ? P.ny : P.nx;
+    for (int k = -1 + (int)threadIdx.x; k < n_other + 1; k += blockDim.x) {
+      int dx, dy, sx, sy;
+      if (dim == 0) {
+        dx = upper ? P.nx : -1;
+        sx = dx + (upper ? -1 : 1);
+        dy = sy = k;
+      } else {
+        dy = upper ? P.ny : -1;
+        sy = dy + (upper ? -1 : 1);
+        dx = sx = k;
+      }
+      const long long d = tile_index(P, dx, dy), s = tile_index(P, sx, sy);
+      const double h = F.H(st)[s], hu = F.HU(st)[s], hv = F.HV(st)[s];
+      F.H(st)[d] = h;
+      F.HU(st)[d] = dim == 0 ? -hu : hu;
+      F.HV(st)[d] = dim == 1 ? -hv : hv;
+    }
+    __syncthreads();
+  }
+}
+
+template <int SOLVER>
+__global__ void __launch_bounds__(64)
+k_amr_speed(const Task* __restrict__ tasks, const int* __restrict__ plist,
+            const PatchDev* __restrict__ pd, Fields F, int cur, double* __restrict__ smax,
+            const AmrClock* clock) {
+  __shared__ MarchLds L;
+  if (!clock->advance) return;
+  const Task t = tasks[blockIdx.x];
+  EdgeBufs none{{nullptr, nullptr}, nullptr};
+  MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
+  march_task<kSpeed, false, 0, SOLVER>(t, A, L);
+}
+
+// Global CFL minimum over the leaves (one workgroup), bounded by the step
+// left to the end time (AMR driver: min(min over leaves, maxdt)).
+__global__ void __launch_bounds__(256)
+k_amr_dt(const int* __restrict__ leaves, int nleaves, const PatchDev* __restrict__ pd,
+         const double* __restrict__ smax, AmrClock* clock) {
+  __shared__ double red[256];
+  if (!clock->advance) return;
+  double m = 1.7976931348623157e308;
+  for (int k = threadIdx.x; k < nleaves; k += blockDim.x) {
+    const int s = leaves[k];
+    const PatchDev P = pd[s];
+    m = dmin(m, max_timestep(smax[s], P.dx, P.dy));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = dmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double cfl = red[0];
+    const double md = clock->maxdt;
+    clock->bound = md < cfl ? md : cfl;
+  }
+}
+
+template <int SOLVER>
+__global__ void __launch_bounds__(64)
+k_amr_solve(const Task* __restrict__ tasks, const int* __restrict__ plist,
+            const PatchDev* __restrict__ pd, Fields F, int cur, const AmrClock* clock) {
+  __shared__ MarchLds L;
+  if (!clock->advance) return;
+  const Task t = tasks[blockIdx.x];
+  EdgeBufs none{{nullptr, nullptr}, nullptr};
+  // every leaf's step fmin(own CFL step, bound) equals the bound, which is
+  // at most the minimum of the CFL steps
+  MarchArgs A{plist, pd, F, nullptr, none, cur, clock->bound, nullptr, nullptr, 1.0};
+  march_task<kUpdateSwap, false, 0, SOLVER>(t, A, L);
+}
+
+// SWEKernel.cpp:83-87 for every leaf.
+__global__ void k_amr_end(AmrClock* c) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || !c->advance) return;
+  const double dt = c->bound;
+  if (dt - 0.0 > kLaEps) {
+    c->time = c->time + c->dt;
+    c->dt = dt;
+  }
+  c->steps += 1;
+}
+
+}  // namespace
+
+struct pc_amr {
+  pc_pool* pool = nullptr;
+  std::vector<int> leaves, virtuals, plan_slots;
+  std::vector<int> leaf_nx, leaf_ny;
+  int ntasks = 0, nvirt = 0, nghosts = 0, nwall_leaves = 0;
+  Task* d_tasks = nullptr;
+  int* d_plist = nullptr;
+  int* d_leaves = nullptr;
+  AmrVirtual* d_virt = nullptr;
+  AmrRestrict* d_pairs = nullptr;
+  AmrGhost* d_ghosts = nullptr;
+  AmrWalls* d_walls = nullptr;
+  int* d_wall_dims = nullptr;
+  double* d_smax = nullptr;
+  AmrClock* d_clock = nullptr;
+  double last_dt = 0.0;
+};
+
+namespace {
+
+template <class T>
+int amr_upload(T** d, const std::vector<T>& h) {
+  HIPCHK(hipMalloc(d, std::max<size_t>(h.size(), 1) * sizeof(T)));
+  if (!h.empty()) HIPCHK(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return PC_OK;
+}
+
+void amr_free(pc_amr* a) {
+  hipFree(a->d_tasks);
+  hipFree(a->d_plist);
+  hipFree(a->d_leaves);
+  hipFree(a->d_virt);
+  hipFree(a->d_pairs);
+  hipFree(a->d_ghosts);
+  hipFree(a->d_walls);
+  hipFree(a->d_wall_dims);
+  hipFree(a->d_smax);
+  hipFree(a->d_clock);
+}
+
+// The plan's slots must still be live with the same role and size.
+int amr_check_slots(pc_amr* a) {
+  pc_pool* p = a->pool;
+  for (size_t i = 0; i < a->leaves.size(); ++i) {
+    const int s = a->leaves[i];
+    if (!valid_slot(p, s) || p->slots[s].desc.is_virtual ||
+        p->slots[s].desc.subdivision_factor[0] != a->leaf_nx[i] ||
+        p->slots[s].desc.subdivision_factor[1] != a->leaf_ny[i])
+      return fail(PC_E_INVALID, "a leaf of the AMR plan changed or was released");
+  }
+  for (int s : a->virtuals)
+    if (!valid_slot(p, s) || !p->slots[s].desc.is_virtual)
+      return fail(PC_E_INVALID, "a virtual patch of the AMR plan changed or was released");
+  return PC_OK;
+}
+
+int amr_cur(pc_amr* a, int* cur) {
+  pc_pool* p = a->pool;
+  const int c = cur_of(p, a->plan_slots[0]);
+  for (int s : a->plan_slots)
+    if (cur_of(p, s) != c)
+      return fail(PC_E_INVALID, "AMR patches must share one state parity (one scenario)");
+  *cur = c;
+  return PC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t* virtuals,
+                  int nvirtuals, const int32_t* restrictions, int nrestrictions,
+                  const pc_amr_ghost* ghosts, int nghosts, const pc_amr_wall* walls,
+                  int nwalls, pc_amr** out) {
+  if (!p || !out || nleaves < 1 || nvirtuals < 0 || nrestrictions < 0 || nghosts < 0 ||
+      nwalls < 0 || !leaves || (nvirtuals && !virtuals) || (nrestrictions && !restrictions) ||
+      (nghosts && !ghosts) || (nwalls && !walls))
+    return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  std::vector<char> role(p->slots.size(), 0);  // 1 leaf, 2 virtual
+  for (int i = 0; i < nleaves; ++i) {
+    const int s = leaves[i];
+    if (!valid_slot(p, s) || p->slots[s].desc.is_virtual || role[s])
+      return fail(PC_E_INVALID, "leaves must be distinct live non-virtual patches");
+    role[s] = 1;
+  }
+  for (int i = 0; i < nvirtuals; ++i) {
+    const int s = virtuals[i];
+    if (!valid_slot(p, s) || !p->slots[s].desc.is_virtual || role[s])
+      return fail(PC_E_INVALID, "virtual patches must be distinct live virtual patches");
+    role[s] = 2;
+  }
+  pc_amr* a = new pc_amr;
+  a->pool = p;
+  a->leaves.assign(leaves, leaves + nleaves);
+  a->virtuals.assign(virtuals, virtuals + nvirtuals);
+  a->plan_slots = a->leaves;
+  a->plan_slots.insert(a->plan_slots.end(), a->virtuals.begin(), a->virtuals.end());
+  // leaf clock: every leaf on one clock
+  double t0, dt0;
+  slot_clock(p, leaves[0], &t0, &dt0);
+  for (int s : a->leaves) {
+    double t, dt;
+    slot_clock(p, s, &t, &dt);
+    if (t != t0 || dt != dt0) {
+      delete a;
+      return fail(PC_E_INVALID, "AMR leaves must share one clock");
+    }
+  }
+  auto bail = [&](const char* msg) {
+    amr_free(a);
+    delete a;
+    return fail(PC_E_INVALID, msg);
+  };
+  // leaf tasks: runs of equal-size leaves packed 64 / pow2(ny) per wavefront
+  std::vector<Task> tasks;
+  std::vector<int> plist;
+  for (int s : a->leaves) {
+    const auto& d = p->slots[s].desc;
+    a->leaf_nx.push_back(d.subdivision_factor[0]);
+    a->leaf_ny.push_back(d.subdivision_factor[1]);
+    if (d.subdivision_factor[1] > 64) return bail("leaf patches must have ny <= 64");
+  }
+  for (size_t i = 0; i < a->leaves.size();) {
+    const int nx = a->leaf_nx[i], ny = a->leaf_ny[i];
+    int w = 1;
+    while (w < ny) w <<= 1;
+    const int per = 64 / w;
+    Task t{(int)plist.size(), 0, nx, ny};
+    while (i < a->leaves.size() && t.nseg < per && a->leaf_nx[i] == nx && a->leaf_ny[i] == ny) {
+      plist.push_back(a->leaves[i]);
+      ++t.nseg;
+      ++i;
+    }
+    tasks.push_back(t);
+  }
+  a->ntasks = (int)tasks.size();
+  // restriction pairs grouped by virtual patch, in the given order
+  std::vector<AmrVirtual> vir;
+  std::vector<AmrRestrict> pairs;
+  for (int vi = 0; vi < nvirtuals; ++vi) {
+    const int v = virtuals[vi];
+    AmrVirtual V{v, (int)pairs.size(), 0, 0};
+    const auto& dd = p->slots[v].desc;
+    const PatchDev D = make_patchdev(p->slots[v]);
+    for (int r = 0; r < nrestrictions; ++r) {
+      if (restrictions[2 * r + 1] != v) continue;
+      const int s = restrictions[2 * r];
+      if (s < 0 || s >= (int)role.size() || role[s] != 1)
+        return bail("a restriction source is not a leaf of the plan");
+      const auto& sd = p->slots[s].desc;
+      const PatchDev S = make_patchdev(p->slots[s]);
+      AmrRestrict R{};
+      R.src = s;
+      const int z[2] = {0, 0};
+      const int sn[2] = {sd.subdivision_factor[0], sd.subdivision_factor[1]};
+      const int dn[2] = {dd.subdivision_factor[0], dd.subdivision_factor[1]};
+      map_to_patch(z, sn, S, sd, D, dd, 1e-12, R.dOff, R.dSize);
+      for (int d = 0; d < 2; ++d) {
+        if (R.dOff[d] < 0 || R.dOff[d] + R.dSize[d] > dn[d])
+          return bail("restriction source is not inside the destination");
+        R.rSize[d] = sn[d];
+        R.srcSub[d] = sd.size[d] * (1.0 / (double)sn[d]);
+        R.dstSub[d] = dd.size[d] * (1.0 / (double)dn[d]);
+        R.spos[d] = sd.position[d];
+        R.dpos[d] = dd.position[d];
+      }
+      R.dstRegion = R.dstSub[0] * R.dstSub[1];
+      pairs.push_back(R);
+    }
+    V.pend = (int)pairs.size();
+    vir.push_back(V);
+  }
+  for (int r = 0; r < nrestrictions; ++r) {
+    const int v = restrictions[2 * r + 1];
+    if (v < 0 || v >= (int)role.size() || role[v] != 2)
+      return bail("a restriction destination is not a virtual patch of the plan");
+  }
+  a->nvirt = (int)vir.size();
+  // ghost faces
+  std::vector<AmrGhost> gh;
+  for (int i = 0; i < nghosts; ++i) {
+    const pc_amr_ghost& G = ghosts[i];
+    const int s = G.src, d = G.dst;
+    if (s < 0 || d < 0 || s >= (int)role.size() || d >= (int)role.size() || role[d] != 1 ||
+        !role[s])
+      return bail("a ghost entry needs a leaf destination and a plan source");
+    const auto& sd = p->slots[s].desc;
+    const auto& dd = p->slots[d].desc;
+    AmrGhost g{};
+    g.src = s;
+    g.dst = d;
+    g.sx = G.size[0];
+    g.sy = G.size[1];
+    g.dox = G.dst_offset[0];
+    g.doy = G.dst_offset[1];
+    for (int k = 0; k < 2; ++k)
+      if (G.size[k] < 0 || G.dst_offset[k] < -1 ||
+          G.dst_offset[k] + G.size[k] > dd.subdivision_factor[k] + 1)
+        return bail("ghost destination region out of bounds");
+    // destinations are strips of the ghost ring (entries run concurrently
+    // and read interior cells only)
+    bool ring = false;
+    for (int k = 0; k < 2; ++k)
+      ring = ring || (G.size[k] == 1 && (G.dst_offset[k] == -1 ||
+                                         G.dst_offset[k] == dd.subdivision_factor[k]));
+    if (!ring) return bail("ghost destination must be a strip of the ghost layer");
+    if (G.kind == 0) {
+      g.kind = sd.is_virtual ? kAmrTransferVirtual : kAmrTransferLeaf;
+      g.sox = G.src_offset[0];
+      g.soy = G.src_offset[1];
+      for (int k = 0; k < 2; ++k)
+        if (G.src_offset[k] < 0 || G.src_offset[k] + G.size[k] > sd.subdivision_factor[k])
+          return bail("transfer source region out of bounds");
+    } else if (G.kind == 1) {
+      if (role[s] != 1) return bail("interpolation sources must be leaves");
+      g.kind = kAmrInterpolate;
+      const PatchDev S = make_patchdev(p->slots[s]), D = make_patchdev(p->slots[d]);
+      // the cells read (the grown source region, SWEKernel.cpp:252-263)
+      // must lie in the source
+      int so[2], ss[2];
+      map_to_patch(G.dst_offset, G.size, D, dd, S, sd, 1e-12, so, ss);
+      for (int k = 0; k < 2; ++k) {
+        const int n = sd.subdivision_factor[k];
+        if (so[k] > 0) {
+          so[k] = so[k] - 1;
+          ss[k] = std::min(n, ss[k] + 2);
+        } else {
+          ss[k] = std::min(n, ss[k] + 1);
+        }
+        if (so[k] < 0 || so[k] + ss[k] > n + 1 || n < 2)
+          return bail("interpolation region is not covered by the source patch");
+        g.sn[k] = n;
+      }
+      g.spos[0] = sd.position[0];
+      g.spos[1] = sd.position[1];
+      g.ssub[0] = S.dx;
+      g.ssub[1] = S.dy;
+      g.sinv[0] = 1.0 / S.dx;
+      g.sinv[1] = 1.0 / S.dy;
+      g.dpos[0] = dd.position[0];
+      g.dpos[1] = dd.position[1];
+      g.dsub[0] = D.dx;
+      g.dsub[1] = D.dy;
+    } else {
+      return bail("ghost kind must be 0 (transfer) or 1 (interpolation)");
+    }
+    gh.push_back(g);
+  }
+  a->nghosts = (int)gh.size();
+  // walls grouped per leaf in call order
+  std::vector<AmrWalls> wl;
+  std::vector<int> wdims;
+  for (int i = 0; i < nwalls;) {
+    const int s = walls[i].slot;
+    if (s < 0 || s >= (int)role.size() || role[s] != 1) return bail("walls belong to leaves");
+    AmrWalls W{s, (int)(wdims.size() / 2), 0, 0};
+    while (i < nwalls && walls[i].slot == s) {
+      if ((walls[i].dimension != 0 && walls[i].dimension != 1) ||
+          (walls[i].upper != 0 && walls[i].upper != 1))
+        return bail("wall dimension and side must be 0 or 1");
+      wdims.push_back(walls[i].dimension);
+      wdims.push_back(walls[i].upper);
+      ++W.count;
+      ++i;
+    }
+    wl.push_back(W);
+  }
+  a->nwall_leaves = (int)wl.size();
+  AmrClock c{};
+  c.time = t0;
+  c.dt = dt0;
+  int rc = PC_OK;
+  if ((rc = amr_upload(&a->d_tasks, tasks)) != PC_OK ||
+      (rc = amr_upload(&a->d_plist, plist)) != PC_OK ||
+      (rc = amr_upload(&a->d_leaves, a->leaves)) != PC_OK ||
+      (rc = amr_upload(&a->d_virt, vir)) != PC_OK ||
+      (rc = amr_upload(&a->d_pairs, pairs)) != PC_OK ||
+      (rc = amr_upload(&a->d_ghosts, gh)) != PC_OK ||
+      (rc = amr_upload(&a->d_walls, wl)) != PC_OK ||
+      (rc = amr_upload(&a->d_wall_dims, wdims)) != PC_OK ||
+      (rc = amr_upload(&a->d_smax, std::vector<double>(p->slots.size(), 0.0))) != PC_OK ||
+      (rc = amr_upload(&a->d_clock, std::vector<AmrClock>{c})) != PC_OK) {
+    amr_free(a);
+    delete a;
+    return rc;
+  }
+  *out = a;
+  return PC_OK;
+}
+
+int pc_amr_step(pc_amr* a, int k, double end_time, double* last_dt) {
+  if (!a || k < 0) return fail(PC_E_INVALID, "invalid argument");
+  pc_pool* p = a->pool;
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  CHK(amr_check_slots(a));
+  int cur;
+  CHK(amr_cur(a, &cur));
+  mark_state_changed(p);
+  Fields F = fields_of(p);
+  const int nl = (int)a->leaves.size();
+  for (int it = 0; it < k; ++it) {
+    hipLaunchKernelGGL(k_amr_begin, dim3(1), dim3(64), 0, p->stream, a->d_clock, end_time);
+    if (a->nvirt > 0)
+      hipLaunchKernelGGL(k_amr_restrict, dim3(a->nvirt), dim3(256), 0, p->stream, a->d_virt,
+                         a->d_pairs, p->d_pd, F, cur, a->d_clock);
+    hipLaunchKernelGGL(k_amr_copy, dim3(nl), dim3(256), 0, p->stream, a->d_leaves, p->d_pd, F,
+                       cur, a->d_clock);
+    if (a->nghosts > 0)
+      hipLaunchKernelGGL(k_amr_ghosts, dim3((a->nghosts + 3) / 4), dim3(256), 0, p->stream,
+                         a->d_ghosts, a->nghosts, p->d_pd, F, cur, a->d_clock);
+    if (a->nwall_leaves > 0)
+      hipLaunchKernelGGL(k_amr_walls, dim3(a->nwall_leaves), dim3(64), 0, p->stream,
+                         a->d_walls, a->d_wall_dims, p->d_pd, F, cur, a->d_clock);
+    if (p->solver)
+      hipLaunchKernelGGL(k_amr_speed<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                         a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock);
+    else
+      hipLaunchKernelGGL(k_amr_speed<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                         a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock);
+    hipLaunchKernelGGL(k_amr_dt, dim3(1), dim3(256), 0, p->stream, a->d_leaves, nl, p->d_pd,
+                       a->d_smax, a->d_clock);
+    if (p->solver)
+      hipLaunchKernelGGL(k_amr_solve<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                         a->d_plist, p->d_pd, F, cur, a->d_clock);
+    else
+      hipLaunchKernelGGL(k_amr_solve<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                         a->d_plist, p->d_pd, F, cur, a->d_clock);
+    hipLaunchKernelGGL(k_amr_end, dim3(1), dim3(64), 0, p->stream, a->d_clock);
+    HIPCHK(hipGetLastError());
+  }
+  AmrClock c;
+  HIPCHK(hipMemcpyAsync(&c, a->d_clock, sizeof(c), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  // the leaves' descriptors follow the device clock (as pc_solve_timestep
+  // would have left them)
+  for (int s : a->leaves) {
+    p->slots[s].desc.time = c.time;
+    p->slots[s].desc.timestep_size = c.dt;
+    p->slots[s].clock_from_scenario = false;
+  }
+  a->last_dt = c.advance ? c.bound : 0.0;
+  if (last_dt) *last_dt = a->last_dt;
+  return PC_OK;
+}
+
+int pc_amr_clock(pc_amr* a, double* time, double* timestep_size, long long* steps) {
+  if (!a) return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(a->pool->device);
+  AmrClock c;
+  HIPCHK(hipMemcpyAsync(&c, a->d_clock, sizeof(c), hipMemcpyDeviceToHost, a->pool->stream));
+  HIPCHK(hipStreamSynchronize(a->pool->stream));
+  if (time) *time = c.time;
+  if (timestep_size) *timestep_size = c.dt;
+  if (steps) *steps = c.steps;
+  return PC_OK;
+}
+
+int pc_amr_destroy(pc_amr* a) {
+  if (!a) return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(a->pool->device);
+  hipStreamSynchronize(a->pool->stream);
+  amr_free(a);
+  delete a;
+  return PC_OK;
+}
+
+}  // extern "C"

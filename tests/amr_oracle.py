@@ -31,6 +31,13 @@ class OracleBackend:
     def restrict(self, src, dst):
         lib().or_restrict_swe(self.p[src].ref, self.p[dst].ref, 0)
 
+    def restrict_pure(self, src, dst):
+        lib().or_restrict_swe_pure(self.p[src].ref, self.p[dst].ref)
+
+    def interpolate_pure(self, size, doff, src, dst, to_uold, to_current, use_time):
+        lib().or_interpolate_swe_pure(iarr(*size), iarr(*doff), self.p[src].ref, self.p[dst].ref,
+                                      to_uold, to_current, use_time)
+
     def post_restrict(self, dst):
         lib().or_post_process_restriction_swe(self.p[dst].ref, 0)
 

@@ -108,6 +108,9 @@ def lib():
                                    ctypes.c_int, ctypes.c_int]),
             "or_restrict_default": (None, [P, P, ctypes.c_int]),
             "or_restrict_swe": (None, [P, P, ctypes.c_int]),
+            "or_restrict_swe_pure": (None, [P, P]),
+            "or_interpolate_swe_pure": (None, [_ip, _ip, P, P, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int]),
             "or_post_process_restriction_swe": (None, [P, ctypes.c_int]),
             "or_overlapping_region": (ctypes.c_double, [_dp] * 4),
             "or_region_map_to_patch": (None, [_ip, _ip, P, P, ctypes.c_double,

@@ -1,0 +1,161 @@
+"""Batched AMR step (pc_amr_*: nine launches per global step) against the
+oracle's AMR driver with the same level-transfer semantics (pure=True):
+bitwise per step on the 2-level dam break and the 3-level wet/dry basin
+at full size, with the f-wave and the augmented solver; long runs of both
+workloads at their full step counts on the device."""
+import time
+
+import numpy as np
+import pytest
+
+from amr_oracle import OracleBackend
+from oracle_lib import wave_propagation_solver
+
+pytestmark = pytest.mark.gpu
+
+
+def _equal(batched_grid, ora):
+    a, b = batched_grid.leaf_states(), ora.leaf_states()
+    assert a.keys() == b.keys()
+    same = sum(int((a[k].view(np.uint64) == b[k].view(np.uint64)).sum()) for k in a)
+    return same, sum(v.size for v in a.values())
+
+
+def _c1(pool, pure=True):
+    from peanoclaw_amd.amr import (DeviceBackend, TwoLevelGrid, dam_break_fields,
+                                   refined_set_from_flags)
+    rng = np.random.default_rng(1)
+    centre = tuple(0.5 + rng.uniform(-0.05, 0.05, 2))
+    fn = dam_break_fields(centre=centre, radius=0.25)
+    refined = refined_set_from_flags(pool, 27, 16, fn)
+    dev = TwoLevelGrid(DeviceBackend(pool), 27, 27, 16, refined, fn)
+    ora = TwoLevelGrid(OracleBackend(), 27, 27, 16, refined, fn)
+    ora.pure = pure
+    return dev, ora
+
+
+def _c3(pool):
+    from peanoclaw_amd.amr import AmrGrid, DeviceBackend, front_refinement, wet_dry_fields
+    fn = wet_dry_fields()
+    refine = front_refinement(9, 9, 27, fn, 3)
+    dev = AmrGrid(DeviceBackend(pool), 9, 9, 27, refine, fn, max_level=3)
+    ora = AmrGrid(OracleBackend(), 9, 9, 27, refine, fn, max_level=3, pure=True)
+    return dev, ora
+
+
+def test_batched_amr_small_two_level_with_hills():
+    from peanoclaw_amd.amr import (BatchedAmr, DeviceBackend, TwoLevelGrid,
+                                   dam_break_fields)
+    from peanoclaw_amd.pool import PatchPool
+    fn = dam_break_fields(centre=(0.47, 0.53), radius=0.25,
+                          hills=[(0.3, 0.6, 0.1, 0.25), (0.7, 0.35, 0.08, 0.2)])
+    refined = {(1, 1), (1, 2), (2, 1), (4, 4), (3, 2)}
+    pool = PatchPool(0)
+    try:
+        dev = TwoLevelGrid(DeviceBackend(pool), 6, 6, 8, refined, fn)
+        ora = TwoLevelGrid(OracleBackend(), 6, 6, 8, refined, fn)
+        ora.pure = True
+        amr = BatchedAmr(dev)
+        for it in range(40):
+            d1 = amr.step(1)
+            d0 = ora.step()
+            assert d1 == d0, f"dt differs at step {it}"
+        same, total = _equal(dev, ora)
+        print(f"batched 2-level 6x6x8^2 with hills, 40 steps: bitwise {same}/{total}")
+        assert same == total
+        t, dt, n = amr.clock()
+        assert (t, dt, n) == (ora.time, ora.dt, 40)
+        # several steps per call continue identically; a step past the end
+        # time does nothing
+        amr.step(5)
+        for _ in range(5):
+            ora.step()
+        assert _equal(dev, ora)[0] == total
+        end = ora.time + ora.dt + 0.5 * ora.dt
+        assert amr.step(3, end_time=end) == 0.0
+        for _ in range(3):
+            ora.step(end)
+        assert amr.clock()[:2] == (ora.time, ora.dt)
+        assert _equal(dev, ora)[0] == total
+        amr.close()
+    finally:
+        pool.close()
+
+
+def test_batched_amr_breaking_dam_27x27_bitwise():
+    from peanoclaw_amd.amr import BatchedAmr
+    from peanoclaw_amd.pool import PatchPool
+    pool = PatchPool(0)
+    try:
+        dev, ora = _c1(pool)
+        amr = BatchedAmr(dev)
+        for it in range(25):
+            assert amr.step(1) == ora.step(), f"dt differs at step {it}"
+        same, total = _equal(dev, ora)
+        print(f"batched 2-level dam break 27x27x16^2 ({len(dev.leaves)} leaves), 25 steps: "
+              f"bitwise {same}/{total}")
+        assert same == total
+        amr.close()
+    finally:
+        pool.close()
+
+
+@pytest.mark.parametrize("solver", [1, 2])
+def test_batched_amr_three_level_wet_dry_bitwise(solver):
+    from peanoclaw_amd.amr import BatchedAmr
+    from peanoclaw_amd.pool import PatchPool
+    pool = PatchPool(0)
+    try:
+        pool.wave_propagation_solver(solver)
+        dev, ora = _c3(pool)
+        amr = BatchedAmr(dev)
+        with wave_propagation_solver(solver):
+            for it in range(15):
+                assert amr.step(1) == ora.step(), f"dt differs at step {it}"
+        same, total = _equal(dev, ora)
+        print(f"batched 3-level wet/dry 9x9x27^2, solver {solver}, 15 steps: "
+              f"bitwise {same}/{total}")
+        assert same == total
+        amr.close()
+    finally:
+        pool.close()
+
+
+def _mass(grid, px, n):
+    return sum(float(v[0].sum()) * (1.0 / (px * n * 3 ** (k[0] - 1))) ** 2
+               for k, v in grid.leaf_states().items())
+
+
+@pytest.mark.parametrize("case,steps", [("c1", 500), ("c3", 1000)])
+def test_batched_amr_full_runs(case, steps):
+    """The 2-level dam break for 500 steps and the 3-level wet/dry basin
+    for 1000 steps, batched on the device: finite, depths non-negative,
+    mass drift reported (no flux correction between levels), throughput
+    in leaf cell updates per second."""
+    from peanoclaw_amd.amr import BatchedAmr
+    from peanoclaw_amd.pool import PatchPool
+    pool = PatchPool(0)
+    try:
+        dev, _ = _c1(pool) if case == "c1" else _c3(pool)
+        px, n = (27, 16) if case == "c1" else (9, 27)
+        m0 = _mass(dev, px, n)
+        amr = BatchedAmr(dev)
+        amr.step(2)
+        pool.synchronize()
+        t0 = time.perf_counter()
+        amr.step(steps - 2)
+        pool.synchronize()
+        el = time.perf_counter() - t0
+        st = dev.leaf_states()
+        assert all(np.isfinite(v).all() for v in st.values())
+        assert all((v[0] >= 0.0).all() for v in st.values())
+        m1 = _mass(dev, px, n)
+        t, dt, k = amr.clock()
+        assert k == steps
+        rate = dev.cells * (steps - 2) / el
+        print(f"batched AMR {case}: {len(dev.leaves)} leaves, {dev.cells} leaf cells, {steps} "
+              f"steps to t={t + dt:.4e}; {el / (steps - 2) * 1e3:.3f} ms/step, "
+              f"{rate / 1e9:.3f} G leaf cell-updates/s; mass drift {(m1 - m0) / m0:.2e}")
+        amr.close()
+    finally:
+        pool.close()
