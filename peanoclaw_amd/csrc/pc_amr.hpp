@@ -11,12 +11,13 @@
 // (GhostlayerCompositorFunctors.cpp:59-69; SWEKernel.cpp:239-286), walls
 // (SolveTimestep.cpp:87-142), the global CFL minimum and solveTimestep of
 // every leaf.  Here the host hands over that plan once (a device-side
-// table rebuilt only when the grid changes) and every step is nine
-// launches, each phase over all patches at once:
+// table rebuilt only when the grid changes) and every step is seven
+// launches, each phase over all patches at once, replayed from a captured
+// graph:
 //
-//   begin    step bound from the end time; steps past it do nothing
-//   restrict one workgroup per virtual patch: every cell sums the
-//            contributions of its leaves in plan order, then h+b -> h
+//   restrict one thread per virtual-patch cell: the contributions of the
+//            leaves covering it (a per-cell list, in plan order), then
+//            h+b -> h
 //   copy     uNew -> uOld of every leaf
 //   ghosts   one wavefront per face: copy with time factor, or d-linear
 //            interpolation from the coarser leaf (h+b, back to h with the
@@ -24,8 +25,11 @@
 //   walls    one workgroup per leaf with domain walls, in call order
 //   speed    exact edge speeds of every leaf (row march over uOld)
 //   dt       global CFL minimum, bounded by the end time
-//   solve    update of every leaf with that step (row march, uOld -> uNew)
-//   end      the leaves' clock
+//   solve    update of every leaf with that step (row march, uOld -> uNew),
+//            then the leaves' clock
+//
+// Every phase first checks the step bound end_time - (time + dt); a step
+// past the end time does nothing.
 //
 // The level transfers read h + b instead of transforming the source in
 // place and back: the reference's round trip (h + b) - b can move a depth
@@ -39,16 +43,24 @@ namespace {
 
 struct AmrClock {
   double time, dt;  // the leaves' clock (time, timestep_size)
-  double maxdt;     // end_time - (time + dt) of the step being run
+  double end_time;
   double bound;     // dt of the step being run
-  int advance;      // the step runs (maxdt > tolerance)
+  int advance;      // the step runs (set by the dt phase for the solve)
   int pad;
   long long steps;
 };
 
-struct AmrVirtual {
-  int slot;
-  int pbeg, pend;  // restriction pairs [pbeg, pend)
+// end_time - (time + dt) > tolerance (AMR driver's step condition)
+__device__ __forceinline__ bool amr_advance(const AmrClock& c) {
+  const double maxdt = c.end_time - (c.time + c.dt);
+  return maxdt - 0.0 > kLaEps;
+}
+
+// 64 consecutive cells of one virtual patch
+struct AmrVBlock {
+  int slot;   // the virtual patch
+  int cell0;  // first cell (x * ny + y)
+  int base;   // index of cell 0 of this patch in the per-cell lists
   int pad;
 };
 
@@ -77,13 +89,6 @@ struct AmrWalls {
   int slot, first, count, pad;  // walls[first .. first+count) as (dim, upper) pairs
 };
 
-__global__ void k_amr_begin(AmrClock* c, double end_time) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double maxdt = end_time - (c->time + c->dt);
-  c->maxdt = maxdt;
-  c->advance = (maxdt - 0.0 > kLaEps) ? 1 : 0;
-}
-
 // Restriction time factors of a leaf source on the [0, 1] basis
 // (DefaultRestriction.cpph:46-52).
 __device__ __forceinline__ void amr_restrict_factors(const AmrClock& c, double& tfOld,
@@ -97,26 +102,29 @@ __device__ __forceinline__ void amr_restrict_factors(const AmrClock& c, double& 
   }
 }
 
-__global__ void __launch_bounds__(256)
-k_amr_restrict(const AmrVirtual* __restrict__ virt, const AmrRestrict* __restrict__ pairs,
+// One thread per virtual-patch cell; `cptr`/`cpairs` list, per cell, the
+// restriction pairs whose destination region holds it, in plan order.
+__global__ void __launch_bounds__(64)
+k_amr_restrict(const AmrVBlock* __restrict__ blocks, const int* __restrict__ cptr,
+               const int* __restrict__ cpairs, const AmrRestrict* __restrict__ pairs,
                const PatchDev* __restrict__ pd, Fields F, int cur, const AmrClock* clock) {
   const AmrClock c = *clock;
-  if (!c.advance) return;
-  const AmrVirtual V = virt[blockIdx.x];
-  const PatchDev D = pd[V.slot];
+  if (!amr_advance(c)) return;
+  const AmrVBlock B = blocks[blockIdx.x];
+  const PatchDev D = pd[B.slot];
+  const int k = B.cell0 + (int)threadIdx.x;
+  if (k >= D.nx * D.ny) return;
   double tfOld, tfNew;
   amr_restrict_factors(c, tfOld, tfNew);
   const double eps = 1e-12;
   const int oldSt = 1 - cur;
-  for (int k = threadIdx.x; k < D.nx * D.ny; k += blockDim.x) {
+  {
     const int di0 = k / D.ny, di1 = k - (k / D.ny) * D.ny;
     double accO[3] = {0.0, 0.0, 0.0};
     double accN[3] = {0.0, 0.0, 0.0};
-    for (int pi = V.pbeg; pi < V.pend; ++pi) {
-      const AmrRestrict a = pairs[pi];
-      if (di0 < a.dOff[0] || di0 >= a.dOff[0] + a.dSize[0] || di1 < a.dOff[1] ||
-          di1 >= a.dOff[1] + a.dSize[1])
-        continue;
+    const int pe = cptr[B.base + k + 1];
+    for (int j = cptr[B.base + k]; j < pe; ++j) {
+      const AmrRestrict a = pairs[cpairs[j]];
       const PatchDev S = pd[a.src];
       const double dp[2] = {a.dpos[0] + (double)di0 * a.dstSub[0],
                             a.dpos[1] + (double)di1 * a.dstSub[1]};
@@ -173,7 +181,7 @@ k_amr_restrict(const AmrVirtual* __restrict__ virt, const AmrRestrict* __restric
 __global__ void __launch_bounds__(256)
 k_amr_copy(const int* __restrict__ leaves, const PatchDev* __restrict__ pd, Fields F, int cur,
            const AmrClock* clock) {
-  if (!clock->advance) return;
+  if (!amr_advance(*clock)) return;
   const PatchDev P = pd[leaves[blockIdx.x]];
   for (int k = threadIdx.x; k < P.nx * P.ny; k += blockDim.x) {
     const long long i = tile_index(P, k / P.ny, k - (k / P.ny) * P.ny);
@@ -188,7 +196,7 @@ __global__ void __launch_bounds__(256)
 k_amr_ghosts(const AmrGhost* __restrict__ ghosts, int nghosts, const PatchDev* __restrict__ pd,
              Fields F, int cur, const AmrClock* clock) {
   const AmrClock c = *clock;
-  if (!c.advance) return;
+  if (!amr_advance(c)) return;
   const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (e >= nghosts) return;
   const int lane = threadIdx.x & 63;
@@ -270,7 +278,7 @@ k_amr_ghosts(const AmrGhost* __restrict__ ghosts, int nghosts, const PatchDev* _
 __global__ void __launch_bounds__(64)
 k_amr_walls(const AmrWalls* __restrict__ walls, const int* __restrict__ dims,
             const PatchDev* __restrict__ pd, Fields F, int cur, const AmrClock* clock) {
-  if (!clock->advance) return;
+  if (!amr_advance(*clock)) return;
   const AmrWalls W = walls[blockIdx.x];
   const PatchDev P = pd[W.slot];
   const int st = 1 - cur;
@@ -304,7 +312,7 @@ k_amr_speed(const Task* __restrict__ tasks, const int* __restrict__ plist,
             const PatchDev* __restrict__ pd, Fields F, int cur, double* __restrict__ smax,
             const AmrClock* clock) {
   __shared__ MarchLds L;
-  if (!clock->advance) return;
+  if (!amr_advance(*clock)) return;
   const Task t = tasks[blockIdx.x];
   EdgeBufs none{{nullptr, nullptr}, nullptr};
   MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
@@ -317,7 +325,11 @@ __global__ void __launch_bounds__(256)
 k_amr_dt(const int* __restrict__ leaves, int nleaves, const PatchDev* __restrict__ pd,
          const double* __restrict__ smax, AmrClock* clock) {
   __shared__ double red[256];
-  if (!clock->advance) return;
+  const bool adv = amr_advance(*clock);
+  if (!adv) {
+    if (threadIdx.x == 0) clock->advance = 0;
+    return;
+  }
   double m = 1.7976931348623157e308;
   for (int k = threadIdx.x; k < nleaves; k += blockDim.x) {
     const int s = leaves[k];
@@ -332,34 +344,34 @@ k_amr_dt(const int* __restrict__ leaves, int nleaves, const PatchDev* __restrict
   }
   if (threadIdx.x == 0) {
     const double cfl = red[0];
-    const double md = clock->maxdt;
+    const double md = clock->end_time - (clock->time + clock->dt);
     clock->bound = md < cfl ? md : cfl;
+    clock->advance = 1;
   }
 }
 
 template <int SOLVER>
 __global__ void __launch_bounds__(64)
 k_amr_solve(const Task* __restrict__ tasks, const int* __restrict__ plist,
-            const PatchDev* __restrict__ pd, Fields F, int cur, const AmrClock* clock) {
+            const PatchDev* __restrict__ pd, Fields F, int cur, AmrClock* clock) {
   __shared__ MarchLds L;
+  // the dt phase decided; this launch's blocks read only advance and bound
   if (!clock->advance) return;
   const Task t = tasks[blockIdx.x];
+  const double dt = clock->bound;
   EdgeBufs none{{nullptr, nullptr}, nullptr};
   // every leaf's step fmin(own CFL step, bound) equals the bound, which is
   // at most the minimum of the CFL steps
-  MarchArgs A{plist, pd, F, nullptr, none, cur, clock->bound, nullptr, nullptr, 1.0};
+  MarchArgs A{plist, pd, F, nullptr, none, cur, dt, nullptr, nullptr, 1.0};
   march_task<kUpdateSwap, false, 0, SOLVER>(t, A, L);
-}
-
-// SWEKernel.cpp:83-87 for every leaf.
-__global__ void k_amr_end(AmrClock* c) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || !c->advance) return;
-  const double dt = c->bound;
-  if (dt - 0.0 > kLaEps) {
-    c->time = c->time + c->dt;
-    c->dt = dt;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // SWEKernel.cpp:83-87 for every leaf
+    if (dt - 0.0 > kLaEps) {
+      clock->time = clock->time + clock->dt;
+      clock->dt = dt;
+    }
+    clock->steps += 1;
   }
-  c->steps += 1;
 }
 
 }  // namespace
@@ -368,11 +380,13 @@ struct pc_amr {
   pc_pool* pool = nullptr;
   std::vector<int> leaves, virtuals, plan_slots;
   std::vector<int> leaf_nx, leaf_ny;
-  int ntasks = 0, nvirt = 0, nghosts = 0, nwall_leaves = 0;
+  int ntasks = 0, nvblocks = 0, nghosts = 0, nwall_leaves = 0;
   Task* d_tasks = nullptr;
   int* d_plist = nullptr;
   int* d_leaves = nullptr;
-  AmrVirtual* d_virt = nullptr;
+  AmrVBlock* d_vblocks = nullptr;
+  int* d_cptr = nullptr;
+  int* d_cpairs = nullptr;
   AmrRestrict* d_pairs = nullptr;
   AmrGhost* d_ghosts = nullptr;
   AmrWalls* d_walls = nullptr;
@@ -380,6 +394,12 @@ struct pc_amr {
   double* d_smax = nullptr;
   AmrClock* d_clock = nullptr;
   double last_dt = 0.0;
+  // captured step and what it was captured with
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const PatchDev* g_pd = nullptr;
+  Fields g_F{};
+  int g_solver = -1, g_cur = -1;
 };
 
 namespace {
@@ -395,13 +415,17 @@ void amr_free(pc_amr* a) {
   hipFree(a->d_tasks);
   hipFree(a->d_plist);
   hipFree(a->d_leaves);
-  hipFree(a->d_virt);
+  hipFree(a->d_vblocks);
+  hipFree(a->d_cptr);
+  hipFree(a->d_cpairs);
   hipFree(a->d_pairs);
   hipFree(a->d_ghosts);
   hipFree(a->d_walls);
   hipFree(a->d_wall_dims);
   hipFree(a->d_smax);
   hipFree(a->d_clock);
+  if (a->gexec) hipGraphExecDestroy(a->gexec);
+  if (a->graph) hipGraphDestroy(a->graph);
 }
 
 // The plan's slots must still be live with the same role and size.
@@ -502,12 +526,14 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
     tasks.push_back(t);
   }
   a->ntasks = (int)tasks.size();
-  // restriction pairs grouped by virtual patch, in the given order
-  std::vector<AmrVirtual> vir;
+  // restriction pairs grouped by virtual patch, in the given order; per
+  // virtual cell the list of pairs covering it
+  std::vector<AmrVBlock> vblocks;
+  std::vector<int> cptr{0}, cpairs;
   std::vector<AmrRestrict> pairs;
   for (int vi = 0; vi < nvirtuals; ++vi) {
     const int v = virtuals[vi];
-    AmrVirtual V{v, (int)pairs.size(), 0, 0};
+    const int pbeg = (int)pairs.size();
     const auto& dd = p->slots[v].desc;
     const PatchDev D = make_patchdev(p->slots[v]);
     for (int r = 0; r < nrestrictions; ++r) {
@@ -535,15 +561,26 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
       R.dstRegion = R.dstSub[0] * R.dstSub[1];
       pairs.push_back(R);
     }
-    V.pend = (int)pairs.size();
-    vir.push_back(V);
+    const int nx = dd.subdivision_factor[0], ny = dd.subdivision_factor[1];
+    const int base = (int)cptr.size() - 1;
+    for (int c0 = 0; c0 < nx * ny; c0 += 64) vblocks.push_back(AmrVBlock{v, c0, base, 0});
+    for (int c = 0; c < nx * ny; ++c) {
+      const int x = c / ny, y = c % ny;
+      for (int pi = pbeg; pi < (int)pairs.size(); ++pi) {
+        const AmrRestrict& R = pairs[pi];
+        if (x >= R.dOff[0] && x < R.dOff[0] + R.dSize[0] && y >= R.dOff[1] &&
+            y < R.dOff[1] + R.dSize[1])
+          cpairs.push_back(pi);
+      }
+      cptr.push_back((int)cpairs.size());
+    }
   }
   for (int r = 0; r < nrestrictions; ++r) {
     const int v = restrictions[2 * r + 1];
     if (v < 0 || v >= (int)role.size() || role[v] != 2)
       return bail("a restriction destination is not a virtual patch of the plan");
   }
-  a->nvirt = (int)vir.size();
+  a->nvblocks = (int)vblocks.size();
   // ghost faces
   std::vector<AmrGhost> gh;
   for (int i = 0; i < nghosts; ++i) {
@@ -641,7 +678,9 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
   if ((rc = amr_upload(&a->d_tasks, tasks)) != PC_OK ||
       (rc = amr_upload(&a->d_plist, plist)) != PC_OK ||
       (rc = amr_upload(&a->d_leaves, a->leaves)) != PC_OK ||
-      (rc = amr_upload(&a->d_virt, vir)) != PC_OK ||
+      (rc = amr_upload(&a->d_vblocks, vblocks)) != PC_OK ||
+      (rc = amr_upload(&a->d_cptr, cptr)) != PC_OK ||
+      (rc = amr_upload(&a->d_cpairs, cpairs)) != PC_OK ||
       (rc = amr_upload(&a->d_pairs, pairs)) != PC_OK ||
       (rc = amr_upload(&a->d_ghosts, gh)) != PC_OK ||
       (rc = amr_upload(&a->d_walls, wl)) != PC_OK ||
@@ -656,6 +695,75 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
   return PC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+void amr_launch_step(pc_amr* a, Fields F, int cur) {
+  pc_pool* p = a->pool;
+  const int nl = (int)a->leaves.size();
+  if (a->nvblocks > 0)
+    hipLaunchKernelGGL(k_amr_restrict, dim3(a->nvblocks), dim3(64), 0, p->stream, a->d_vblocks,
+                       a->d_cptr, a->d_cpairs, a->d_pairs, p->d_pd, F, cur, a->d_clock);
+  hipLaunchKernelGGL(k_amr_copy, dim3(nl), dim3(256), 0, p->stream, a->d_leaves, p->d_pd, F,
+                     cur, a->d_clock);
+  if (a->nghosts > 0)
+    hipLaunchKernelGGL(k_amr_ghosts, dim3((a->nghosts + 3) / 4), dim3(256), 0, p->stream,
+                       a->d_ghosts, a->nghosts, p->d_pd, F, cur, a->d_clock);
+  if (a->nwall_leaves > 0)
+    hipLaunchKernelGGL(k_amr_walls, dim3(a->nwall_leaves), dim3(64), 0, p->stream, a->d_walls,
+                       a->d_wall_dims, p->d_pd, F, cur, a->d_clock);
+  if (p->solver)
+    hipLaunchKernelGGL(k_amr_speed<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock);
+  else
+    hipLaunchKernelGGL(k_amr_speed<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock);
+  hipLaunchKernelGGL(k_amr_dt, dim3(1), dim3(256), 0, p->stream, a->d_leaves, nl, p->d_pd,
+                     a->d_smax, a->d_clock);
+  if (p->solver)
+    hipLaunchKernelGGL(k_amr_solve<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_clock);
+  else
+    hipLaunchKernelGGL(k_amr_solve<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_clock);
+}
+
+bool same_fields(const Fields& x, const Fields& y) {
+  return x.h[0] == y.h[0] && x.h[1] == y.h[1] && x.hu[0] == y.hu[0] && x.hu[1] == y.hu[1] &&
+         x.hv[0] == y.hv[0] && x.hv[1] == y.hv[1] && x.b == y.b;
+}
+
+// (Re)capture the step graph when the pool's arrays, the solver or the
+// state parity differ from those it was captured with.
+int amr_graph(pc_amr* a, Fields F, int cur) {
+  pc_pool* p = a->pool;
+  if (a->gexec && a->g_pd == p->d_pd && same_fields(a->g_F, F) && a->g_solver == p->solver &&
+      a->g_cur == cur)
+    return PC_OK;
+  if (a->gexec) HIPCHK(hipGraphExecDestroy(a->gexec));
+  if (a->graph) HIPCHK(hipGraphDestroy(a->graph));
+  a->gexec = nullptr;
+  a->graph = nullptr;
+  HIPCHK(hipStreamSynchronize(p->stream));
+  HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+  amr_launch_step(a, F, cur);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(p->stream, &g);
+  if (e != hipSuccess) return fail(PC_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+  a->graph = g;
+  HIPCHK(hipGraphInstantiate(&a->gexec, g, nullptr, nullptr, 0));
+  a->g_pd = p->d_pd;
+  a->g_F = F;
+  a->g_solver = p->solver;
+  a->g_cur = cur;
+  return PC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int pc_amr_step(pc_amr* a, int k, double end_time, double* last_dt) {
   if (!a || k < 0) return fail(PC_E_INVALID, "invalid argument");
   pc_pool* p = a->pool;
@@ -666,37 +774,10 @@ int pc_amr_step(pc_amr* a, int k, double end_time, double* last_dt) {
   CHK(amr_cur(a, &cur));
   mark_state_changed(p);
   Fields F = fields_of(p);
-  const int nl = (int)a->leaves.size();
-  for (int it = 0; it < k; ++it) {
-    hipLaunchKernelGGL(k_amr_begin, dim3(1), dim3(64), 0, p->stream, a->d_clock, end_time);
-    if (a->nvirt > 0)
-      hipLaunchKernelGGL(k_amr_restrict, dim3(a->nvirt), dim3(256), 0, p->stream, a->d_virt,
-                         a->d_pairs, p->d_pd, F, cur, a->d_clock);
-    hipLaunchKernelGGL(k_amr_copy, dim3(nl), dim3(256), 0, p->stream, a->d_leaves, p->d_pd, F,
-                       cur, a->d_clock);
-    if (a->nghosts > 0)
-      hipLaunchKernelGGL(k_amr_ghosts, dim3((a->nghosts + 3) / 4), dim3(256), 0, p->stream,
-                         a->d_ghosts, a->nghosts, p->d_pd, F, cur, a->d_clock);
-    if (a->nwall_leaves > 0)
-      hipLaunchKernelGGL(k_amr_walls, dim3(a->nwall_leaves), dim3(64), 0, p->stream,
-                         a->d_walls, a->d_wall_dims, p->d_pd, F, cur, a->d_clock);
-    if (p->solver)
-      hipLaunchKernelGGL(k_amr_speed<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
-                         a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock);
-    else
-      hipLaunchKernelGGL(k_amr_speed<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
-                         a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock);
-    hipLaunchKernelGGL(k_amr_dt, dim3(1), dim3(256), 0, p->stream, a->d_leaves, nl, p->d_pd,
-                       a->d_smax, a->d_clock);
-    if (p->solver)
-      hipLaunchKernelGGL(k_amr_solve<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
-                         a->d_plist, p->d_pd, F, cur, a->d_clock);
-    else
-      hipLaunchKernelGGL(k_amr_solve<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
-                         a->d_plist, p->d_pd, F, cur, a->d_clock);
-    hipLaunchKernelGGL(k_amr_end, dim3(1), dim3(64), 0, p->stream, a->d_clock);
-    HIPCHK(hipGetLastError());
-  }
+  CHK(amr_graph(a, F, cur));
+  HIPCHK(hipMemcpyAsync(&a->d_clock->end_time, &end_time, sizeof(double),
+                        hipMemcpyHostToDevice, p->stream));
+  for (int it = 0; it < k; ++it) HIPCHK(hipGraphLaunch(a->gexec, p->stream));
   AmrClock c;
   HIPCHK(hipMemcpyAsync(&c, a->d_clock, sizeof(c), hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
@@ -707,7 +788,7 @@ int pc_amr_step(pc_amr* a, int k, double end_time, double* last_dt) {
     p->slots[s].desc.timestep_size = c.dt;
     p->slots[s].clock_from_scenario = false;
   }
-  a->last_dt = c.advance ? c.bound : 0.0;
+  a->last_dt = (k > 0 && c.advance) ? c.bound : 0.0;
   if (last_dt) *last_dt = a->last_dt;
   return PC_OK;
 }
