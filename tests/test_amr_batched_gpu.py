@@ -159,3 +159,33 @@ def test_batched_amr_full_runs(case, steps):
         amr.close()
     finally:
         pool.close()
+
+
+def test_batched_amr_three_level_1000_steps_bitwise():
+    """1000 steps of a small three-level wet/dry grid (3 x 3 coarse
+    patches of 9^2, both refined levels present, augmented solver): the
+    batched device step against the oracle's driver, bitwise at the end."""
+    from peanoclaw_amd.amr import AmrGrid, BatchedAmr, DeviceBackend, wet_dry_fields
+    from peanoclaw_amd.pool import PatchPool
+    fn = wet_dry_fields()
+    refine = {(1, (0, 1)), (1, (1, 1)), (1, (2, 2)), (2, (2, 4)), (2, (3, 4)), (2, (7, 7))}
+    sel = lambda lev, ij: (lev, ij) in refine  # noqa: E731
+    pool = PatchPool(0)
+    try:
+        pool.wave_propagation_solver(2)
+        dev = AmrGrid(DeviceBackend(pool), 3, 3, 9, sel, fn, max_level=3)
+        amr = BatchedAmr(dev)
+        amr.step(1000)
+        with wave_propagation_solver(2):
+            ora = AmrGrid(OracleBackend(), 3, 3, 9, sel, fn, max_level=3, pure=True)
+            for _ in range(1000):
+                ora.step()
+        same, total = _equal(dev, ora)
+        t, dt, k = amr.clock()
+        print(f"batched 3-level 3x3x9^2 augmented, 1000 steps to t={t + dt:.4e}: "
+              f"bitwise {same}/{total}")
+        assert k == 1000 and (t, dt) == (ora.time, ora.dt)
+        assert same == total
+        amr.close()
+    finally:
+        pool.close()
