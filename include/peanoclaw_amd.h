@@ -228,9 +228,14 @@ int pc_step_all_per_patch(pc_pool *pool, int k);
 int pc_graph_capture(pc_pool *pool);
 int pc_graph_replay(pc_pool *pool, int k);
 /* Refinement flags of all patches (1 refine, 0 coarsening allowed, 2 keep),
- * SolveTimestep::leaveCell (SolveTimestep.cpp:753-797). */
+ * SolveTimestep::leaveCell (SolveTimestep.cpp:753-797), indexed by slot id
+ * (-1 for unused ids); flags_out holds cap >= pc_pool_slot_capacity
+ * entries, else PC_E_INVALID. */
 int pc_refinement_flags(pc_pool *pool, double threshold, double minimal_width,
-                        double maximal_width, int32_t *flags_out);
+                        double maximal_width, int32_t *flags_out, int cap);
+/* Number of slot ids in use so far (live or released): the size of
+ * per-slot outputs. */
+int pc_pool_slot_capacity(pc_pool *pool, int *n);
 /* Total water volume sum(h)*dx*dy of one scenario (compensated sum). */
 int pc_scenario_mass(pc_pool *pool, int scenario, double *out);
 /* Device time (ms) of the last pc_step_all / pc_graph_replay call measured

@@ -176,8 +176,9 @@ _SIGS = {
     "pc_step_all_per_patch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pc_graph_capture": (ctypes.c_int, [ctypes.c_void_p]),
     "pc_graph_replay": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
-    "pc_refinement_flags": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
-                                           ctypes.c_double, ctypes.c_double, _ip]),
+    "pc_refinement_flags": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, _ip, ctypes.c_int]),
+    "pc_pool_slot_capacity": (ctypes.c_int, [ctypes.c_void_p, _ip]),
     "pc_scenario_mass": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
     "pc_last_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "pc_edge_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp]),

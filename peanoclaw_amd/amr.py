@@ -58,8 +58,8 @@ class AmrGrid:
     """Patch tree with refinement factor 3: level 1 is a px x py grid of
     patches of n x n cells on [0,1]^2; a patch at level l that `refine(l,
     (i, j))` selects becomes virtual and holds 3 x 3 patches at level l+1
-    (indices (3i+a, 3j+c)), down to `max_level`.  The refinement is fixed at
-    construction."""
+    (indices (3i+a, 3j+c)), down to `max_level`.  `regrid` refines and
+    coarsens during a run."""
 
     def __init__(self, backend, px, py, n, refine, fn, max_level=2, pure=False):
         self.b = backend
@@ -68,6 +68,7 @@ class AmrGrid:
         self.pure = pure
         self.px, self.py, self.n = px, py, n
         self.max_level = max_level
+        self.fn = fn
         self.levels = {}
         todo = [(1, (i, j), (i / px, j / py), 1.0 / px) for i in range(px) for j in range(py)]
         while todo:
@@ -85,10 +86,33 @@ class AmrGrid:
                                         (pos[0] + a * sf, pos[1] + c * sf), sf))
             todo = nxt
         for lev in sorted(self.levels):
+            for pt in self.levels[lev].values():
+                h, bb = self._fields(pt)
+                backend.set_state(pt.handle, np.ascontiguousarray(h[1:-1, 1:-1]), bb)
+        self.time, self.dt = 0.0, 0.0
+        self.steps = 0
+        self._rebuild()
+
+    def _fields(self, pt):
+        """(h, b) of the initial data at the cell centres of pt, ghost ring
+        included."""
+        n = self.n
+        ii = np.arange(-1, n + 1)
+        sub = pt.size / n
+        X = pt.pos[0] + (ii[:, None] + 0.5) * sub
+        Y = pt.pos[1] + (ii[None, :] + 0.5) * sub
+        X, Y = np.broadcast_arrays(X, Y)
+        h, bb = self.fn(X, Y)
+        return (np.ascontiguousarray(h, dtype=np.float64),
+                np.ascontiguousarray(bb, dtype=np.float64))
+
+    def _rebuild(self):
+        """Children, leaves, virtual patches, restriction and ghost plans
+        from the current tree."""
+        for lev in sorted(self.levels):
             for ij, pt in self.levels[lev].items():
-                if pt.virtual:
-                    pt.children = [self.levels[lev + 1][(3 * ij[0] + a, 3 * ij[1] + c)]
-                                   for a in range(3) for c in range(3)]
+                pt.children = ([self.levels[lev + 1][(3 * ij[0] + a, 3 * ij[1] + c)]
+                                for a in range(3) for c in range(3)] if pt.virtual else [])
         self.virtuals = [pt for lev in sorted(self.levels)
                          for pt in self.levels[lev].values() if pt.virtual]
         self.leaves = [pt for lev in sorted(self.levels)
@@ -102,27 +126,87 @@ class AmrGrid:
                     walk(c, [pt] + ancestors)
             elif ancestors:
                 self._restrictions.append((pt, ancestors))
-        for i in range(px):
-            for j in range(py):
+        for i in range(self.px):
+            for j in range(self.py):
                 walk(self.levels[1][(i, j)], [])
-        for lev in sorted(self.levels):
-            for pt in self.levels[lev].values():
-                ii = np.arange(-1, n + 1)
-                sub = pt.size / n
-                X = pt.pos[0] + (ii[:, None] + 0.5) * sub
-                Y = pt.pos[1] + (ii[None, :] + 0.5) * sub
-                X, Y = np.broadcast_arrays(X, Y)
-                h, bb = fn(X, Y)
-                backend.set_state(pt.handle,
-                                  np.ascontiguousarray(h[1:-1, 1:-1], dtype=np.float64),
-                                  np.ascontiguousarray(bb, dtype=np.float64))
-        self.time, self.dt = 0.0, 0.0
-        self.steps = 0
         self._plan = self._ghost_plan()
+
+    def regrid(self, threshold=0.05, minimal_width=None, maximal_width=None):
+        """One remeshing pass on the current state with the BreakingDam
+        criterion (demanded mesh width minimal_width where the maximal
+        forward-difference |grad h| of a leaf exceeds `threshold`, else
+        maximal_width; SolveTimestep::leaveCell, SolveTimestep.cpp:753-797),
+        defaults: the finest and the coarsest subcell width.
+        - A leaf flagged for refinement below max_level becomes virtual and
+          gets 3 x 3 new leaves on the leaves' clock, b from the initial
+          data, uNew and uOld interpolated from it (Remesh::createCell,
+          Remesh.cpp:331-417: interpolateSolution to uNew at the new time,
+          then to uOld at the current time, without neighbour times).
+        - A virtual patch whose nine children are leaves that all allow
+          coarsening receives the restriction of all of them and becomes a
+          leaf: its [0, 1]-basis values are evaluated at the clock's time
+          and time + dt (Patch::switchToLeaf, Patch.cpp:793-830); the
+          children are released.
+        Returns (refined, coarsened) patch counts."""
+        B = self.b
+        n = self.n
+        finest = 1.0 / (self.px * 3 ** (self.max_level - 1) * n)
+        coarsest = 1.0 / (self.px * n)
+        minw = finest if minimal_width is None else minimal_width
+        maxw = coarsest if maximal_width is None else maximal_width
+        flags = B.flags([p.handle for p in self.leaves], threshold, minw, maxw)
+        flag = {p.handle: f for p, f in zip(self.leaves, flags)}
+        refine = [p for p in self.leaves if flag[p.handle] == 1 and p.level < self.max_level]
+        coarsen = [v for v in self.virtuals
+                   if all(not c.virtual and flag.get(c.handle) == 0 for c in v.children)]
+        for v in coarsen:
+            B.clear(v.handle)
+            for c in v.children:
+                B.restrict(c.handle, v.handle)
+            B.post_restrict(v.handle)
+            B.switch_to_leaf(v.handle, self.time, self.dt)
+            for c in v.children:
+                del self.levels[c.level][c.ij]
+                B.release(c.handle)
+            v.virtual = False
+        for p in refine:
+            sf = p.size / 3.0
+            lev = p.level + 1
+            for a in range(3):
+                for c in range(3):
+                    ij = (3 * p.ij[0] + a, 3 * p.ij[1] + c)
+                    pos = (p.pos[0] + a * sf, p.pos[1] + c * sf)
+                    ch = AmrPatch(B.new_patch(pos, sf, n, lev, False, self.time, self.dt), lev,
+                                  pos, sf, n, False, ij)
+                    _, bb = self._fields(ch)
+                    B.set_b(ch.handle, bb)
+                    B.interpolate((n, n), (0, 0), p.handle, ch.handle, 0, 0, 0)
+                    B.interpolate((n, n), (0, 0), p.handle, ch.handle, 1, 1, 0)
+                    self.levels.setdefault(lev, {})[ij] = ch
+            B.set_virtual(p.handle, True)
+            p.virtual = True
+        if refine or coarsen:
+            self._rebuild()
+        return len(refine), len(coarsen)
 
     @property
     def cells(self):
         return len(self.leaves) * self.n * self.n
+
+    def refined_fraction(self):
+        """Fraction of the domain covered by leaves finer than level 1."""
+        return sum(p.size * p.size for p in self.leaves if p.level > 1)
+
+    def run(self, steps, regrid_every=0, end_time=np.inf, **criterion):
+        """`steps` global steps with a regrid after every `regrid_every`
+        (0: never); returns the refined fraction after each regrid."""
+        fractions = []
+        for k in range(1, steps + 1):
+            self.step(end_time)
+            if regrid_every > 0 and k % regrid_every == 0:
+                self.regrid(**criterion)
+                fractions.append(self.refined_fraction())
+        return fractions
 
     def _face(self, p, dim, upper):
         """('transfer'|'interp'|'wall', source patch) for a face of leaf p: the
@@ -231,9 +315,36 @@ class DeviceBackend:
         self.pool = pool
         self.lib = load()
 
-    def new_patch(self, pos, size, n, level, virtual):
+    def new_patch(self, pos, size, n, level, virtual, time=0.0, dt=0.0):
         return self.pool.register(make_description(pos, size, n, level=level,
-                                                   is_virtual=virtual))
+                                                   is_virtual=virtual, time=time,
+                                                   timestep_size=dt))
+
+    def set_b(self, s, b):
+        self.pool.upload(s, b=b)
+
+    def set_virtual(self, s, virtual):
+        d = self.pool.description(s)
+        d.is_virtual = 1 if virtual else 0
+        self.pool.set_description(s, d)
+
+    def release(self, s):
+        self.pool.release(s)
+
+    def flags(self, slots, threshold, minimal_width, maximal_width):
+        f = self.pool.refinement_flags(threshold, minimal_width, maximal_width,
+                                       max(slots) + 1)
+        return [int(f[s]) for s in slots]
+
+    def switch_to_leaf(self, s, time, dt):
+        unew, uold, _ = self.pool.download(s)
+        a, b = switch_to_leaf_values(uold[:, 1:-1, 1:-1], unew, time, dt)
+        uold[:, 1:-1, 1:-1] = a
+        self.pool.upload(s, unew=b, uold=uold)
+        d = self.pool.description(s)
+        d.is_virtual = 0
+        d.time, d.timestep_size = time, dt
+        self.pool.set_description(s, d)
 
     def set_state(self, s, h, b):
         n = h.shape[0]
@@ -278,6 +389,14 @@ class DeviceBackend:
 
     def state(self, s):
         return self.pool.download(s)[0]
+
+
+def switch_to_leaf_values(uold, unew, time, dt):
+    """Patch::switchToLeaf (Patch.cpp:793-830): values restricted on the
+    [0, 1] time basis evaluated at time (uOld) and time + dt (uNew)."""
+    f_old = (time - 0.0) / 1.0
+    f_new = (time + dt - 0.0) / 1.0
+    return uold * (1.0 - f_old) + unew * f_old, uold * (1.0 - f_new) + unew * f_new
 
 
 def refined_set_from_flags(pool, px, n, fn, threshold=0.05):
@@ -337,10 +456,16 @@ class BatchedAmr:
     afterwards its leaves hold the state and clock of the batched steps."""
 
     def __init__(self, grid):
-        from ._abi import AmrGhost, AmrWall
         self.grid = grid
         self.pool = grid.b.pool
         self.lib = load()
+        self.h = None
+        self.steps_before = 0  # steps of earlier plans (before regrids)
+        self._create()
+
+    def _create(self):
+        from ._abi import AmrGhost, AmrWall
+        grid = self.grid
         leaves = iarr_list([p.handle for p in grid.leaves])
         virtuals = iarr_list([v.handle for v in grid.virtuals])
         pairs = [x for leaf, anc in grid._restrictions for v in anc for x in (leaf.handle,
@@ -372,9 +497,37 @@ class BatchedAmr:
         return dt.value
 
     def clock(self):
+        """(time, timestep_size, steps) of the leaves."""
         t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
         check(self.lib.pc_amr_clock(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)))
-        return t.value, dt.value, n.value
+        return t.value, dt.value, self.steps_before + n.value
+
+    def regrid(self, **criterion):
+        """AmrGrid.regrid on the current state, then a new device plan."""
+        t, dt, n = self.clock()
+        self.grid.time, self.grid.dt = t, dt
+        counts = self.grid.regrid(**criterion)
+        if counts != (0, 0):
+            self.steps_before = n
+            check(self.lib.pc_amr_destroy(self.h))
+            self.h = None
+            self._create()
+        return counts
+
+    def run(self, steps, regrid_every=0, end_time=np.inf, **criterion):
+        """`steps` global steps with a regrid after every `regrid_every`
+        (0: never); returns the refined leaf-cell fraction after each
+        regrid."""
+        fractions = []
+        done = 0
+        while done < steps:
+            k = steps - done if regrid_every <= 0 else min(regrid_every, steps - done)
+            self.step(k, end_time)
+            done += k
+            if regrid_every > 0 and done % regrid_every == 0:
+                self.regrid(**criterion)
+                fractions.append(self.grid.refined_fraction())
+        return fractions
 
     def close(self):
         if self.h:
@@ -419,5 +572,5 @@ def dam_break_fields(centre=(0.5, 0.5), radius=0.25, h_in=2.0, h_out=1.0, hills=
     return fn
 
 
-__all__ = ["AmrGrid", "BatchedAmr", "TwoLevelGrid", "DeviceBackend", "dam_break_fields",
+__all__ = ["AmrGrid", "BatchedAmr", "switch_to_leaf_values", "TwoLevelGrid", "DeviceBackend", "dam_break_fields",
            "front_refinement", "refined_set_from_flags", "wet_dry_fields"]

@@ -1277,14 +1277,23 @@ int pc_demanded_mesh_width(pc_pool* p, int slot, double thr, double minW, double
   return PC_OK;
 }
 
-int pc_refinement_flags(pc_pool* p, double thr, double minW, double maxW, int32_t* flags) {
+int pc_pool_slot_capacity(pc_pool* p, int* n) {
+  if (!p || !n) return fail(PC_E_INVALID, "NULL argument");
+  *n = (int)p->slots.size();
+  return PC_OK;
+}
+
+int pc_refinement_flags(pc_pool* p, double thr, double minW, double maxW, int32_t* flags,
+                        int cap) {
   if (!p || !flags) return fail(PC_E_INVALID, "NULL argument");
+  if (cap < (int)p->slots.size())
+    return fail(PC_E_INVALID, "flags capacity below pc_pool_slot_capacity");
   hipSetDevice(p->device);
   CHK(prepare(p));
   // flags are returned per slot index (dead slots get -1)
   std::vector<int32_t> f(p->h_live.size());
   CHK(refine_flags_slots(p, p->h_live, thr, minW, maxW, f.data(), nullptr));
-  for (size_t i = 0; i < p->slots.size(); ++i) flags[i] = -1;
+  for (int i = 0; i < cap; ++i) flags[i] = -1;
   for (size_t k = 0; k < p->h_live.size(); ++k) flags[p->h_live[k]] = f[k];
   return PC_OK;
 }

@@ -187,10 +187,17 @@ class PatchPool:
         check(self.lib.pc_scenario_mass(self.h, scenario, ctypes.byref(m)))
         return m.value
 
-    def refinement_flags(self, threshold, minimal_width, maximal_width, nslots):
-        f = np.empty(nslots, dtype=np.int32)
+    def slot_capacity(self):
+        n = ctypes.c_int()
+        check(self.lib.pc_pool_slot_capacity(self.h, ctypes.byref(n)))
+        return n.value
+
+    def refinement_flags(self, threshold, minimal_width, maximal_width, nslots=0):
+        """Flags per slot id (-1 unused), at least `nslots` entries."""
+        cap = max(int(nslots), self.slot_capacity())
+        f = np.empty(cap, dtype=np.int32)
         check(self.lib.pc_refinement_flags(self.h, threshold, minimal_width, maximal_width,
-                                           f.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+                                           f.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), cap))
         return f
 
     def reserve(self, cells):

@@ -1,6 +1,8 @@
 """Oracle backend of the two-level AMR driver (test infrastructure): the same
 Numerics calls as peanoclaw_amd.amr.DeviceBackend, run by the C oracle on
 patches in the reference's layout."""
+import ctypes
+
 import numpy as np
 
 from oracle_lib import Patch, iarr, lib
@@ -10,8 +12,8 @@ class OracleBackend:
     def __init__(self):
         self.p = []
 
-    def new_patch(self, pos, size, n, level, virtual):
-        P = Patch(3, 1, n, 1, pos, size, level, 0.0, 0.0)
+    def new_patch(self, pos, size, n, level, virtual, time=0.0, dt=0.0):
+        P = Patch(3, 1, n, 1, pos, size, level, time, dt)
         P.s.is_virtual = 1 if virtual else 0
         P.s.is_leaf = 0 if virtual else 1
         self.p.append(P)
@@ -23,6 +25,37 @@ class OracleBackend:
         P.unew[0] = h
         P.uold[:] = 0.0
         P.pwg[0] = b
+
+    def set_b(self, s, b):
+        self.p[s].pwg[0] = b
+
+    def set_virtual(self, s, virtual):
+        self.p[s].s.is_virtual = 1 if virtual else 0
+        self.p[s].s.is_leaf = 0 if virtual else 1
+
+    def release(self, s):
+        self.p[s] = None
+
+    def flags(self, slots, threshold, minimal_width, maximal_width):
+        out = []
+        for s in slots:
+            P = self.p[s]
+            dw = (ctypes.c_double * 2)()
+            lib().or_demanded_mesh_width(P.ref, threshold, minimal_width, maximal_width, dw)
+            sub = (ctypes.c_double * 2)(*P.s.subcell)
+            out.append(lib().or_refinement_decision(sub, dw))
+        return out
+
+    def switch_to_leaf(self, s, time, dt):
+        from peanoclaw_amd.amr import switch_to_leaf_values
+        P = self.p[s]
+        uold = np.array(P.uold)
+        a, b = switch_to_leaf_values(uold[:, 1:-1, 1:-1], np.array(P.unew), time, dt)
+        uold[:, 1:-1, 1:-1] = a
+        P.uold[:] = uold
+        P.unew[:] = b
+        self.set_virtual(s, False)
+        P.s.time, P.s.dt = time, dt
 
     def clear(self, s):
         self.p[s].unew[:] = 0.0

@@ -189,3 +189,77 @@ def test_batched_amr_three_level_1000_steps_bitwise():
         amr.close()
     finally:
         pool.close()
+
+
+def test_regrid_breaking_dam_27x27_bitwise():
+    """The 2-level dam break with a regrid every 10 steps (BreakingDam
+    criterion on the device, new patches interpolated from their parent,
+    coarsening through full restriction): batched steps plus device regrid
+    against the oracle's driver, 60 steps, bitwise, same refined fractions."""
+    from peanoclaw_amd.amr import BatchedAmr
+    from peanoclaw_amd.pool import PatchPool
+    pool = PatchPool(0)
+    try:
+        dev, ora = _c1(pool)
+        amr = BatchedAmr(dev)
+        fr_dev = amr.run(60, regrid_every=10)
+        fr_ora = ora.run(60, regrid_every=10)
+        same, total = _equal(dev, ora)
+        print(f"regridded 2-level dam break, 60 steps: refined fraction "
+              f"{[round(f, 4) for f in fr_dev]}, {len(dev.leaves)} leaves; "
+              f"bitwise {same}/{total}")
+        assert fr_dev == fr_ora
+        assert amr.clock()[:2] == (ora.time, ora.dt)
+        assert same == total
+        amr.close()
+    finally:
+        pool.close()
+
+
+def test_regrid_per_call_small_grid_bitwise():
+    """Regridding driven through the per-call entry points (reference
+    semantics, in-place transforms) against the oracle, 60 steps."""
+    from peanoclaw_amd.amr import AmrGrid, DeviceBackend, dam_break_fields
+    from peanoclaw_amd.pool import PatchPool
+    fn = dam_break_fields(centre=(0.47, 0.53), radius=0.25, hills=[(0.3, 0.6, 0.1, 0.2)])
+    sel = lambda lev, ij: False  # noqa: E731
+    pool = PatchPool(0)
+    try:
+        dev = AmrGrid(DeviceBackend(pool), 6, 6, 9, sel, fn, max_level=2)
+        ora = AmrGrid(OracleBackend(), 6, 6, 9, sel, fn, max_level=2)
+        assert dev.regrid() == ora.regrid()
+        fr_dev = dev.run(60, regrid_every=10)
+        fr_ora = ora.run(60, regrid_every=10)
+        same, total = _equal(dev, ora)
+        print(f"per-call regrid 6x6x9^2: refined fraction {[round(f, 3) for f in fr_dev]}; "
+              f"bitwise {same}/{total}")
+        assert fr_dev == fr_ora and same == total
+    finally:
+        pool.close()
+
+
+def test_regrid_breaking_dam_500_steps():
+    """C1 at its full step count with a regrid every 10 steps, batched on
+    the device: refined fraction over time, throughput, mass drift."""
+    from peanoclaw_amd.amr import BatchedAmr
+    from peanoclaw_amd.pool import PatchPool
+    pool = PatchPool(0)
+    try:
+        dev, _ = _c1(pool)
+        m0 = _mass(dev, 27, 16)
+        amr = BatchedAmr(dev)
+        t0 = time.perf_counter()
+        fr = amr.run(500, regrid_every=10)
+        pool.synchronize()
+        el = time.perf_counter() - t0
+        st = dev.leaf_states()
+        assert all(np.isfinite(v).all() for v in st.values())
+        m1 = _mass(dev, 27, 16)
+        t, dt, k = amr.clock()
+        assert k == 500
+        print(f"regridded C1 500 steps to t={t + dt:.4e}: {el:.2f} s incl. 50 regrids; "
+              f"refined fraction every 50 steps {[round(f, 3) for f in fr[4::5]]}; "
+              f"{len(dev.leaves)} leaves at the end; mass drift {(m1 - m0) / m0:.2e}")
+        amr.close()
+    finally:
+        pool.close()

@@ -183,3 +183,71 @@ def test_pure_level_transfers_track_the_reference_call_sequence(levels):
             worst = max(worst, rel)
             assert rel <= (1e-12 if it < 5 else 1e-10), (it, rel)
     print(f"{levels} levels: pure vs reference call sequence, max rel {worst:.2e}")
+
+
+def _front_grid(pure=False):
+    fn = dam_break_fields(centre=(0.47, 0.53), radius=0.25, hills=[(0.3, 0.6, 0.1, 0.2)])
+    return AmrGrid(OracleBackend(), 6, 6, 9, lambda lev, ij: False, fn, max_level=2, pure=pure)
+
+
+def test_regrid_follows_the_front_and_conserves_mass_roughly():
+    grid = _front_grid()
+    refined, coarsened = grid.regrid()
+    assert refined > 0 and coarsened == 0
+    first = grid.refined_fraction()
+    assert 0.0 < first < 1.0
+    m0 = sum(float(v[0].sum()) * (1.0 / (54 * 3 ** (k[0] - 1))) ** 2
+             for k, v in grid.leaf_states().items())
+    fr = grid.run(60, regrid_every=10)
+    st = grid.leaf_states()
+    assert all(np.isfinite(v).all() for v in st.values())
+    m1 = sum(float(v[0].sum()) * (1.0 / (54 * 3 ** (k[0] - 1))) ** 2 for k, v in st.items())
+    assert abs(m1 - m0) / m0 < 5e-3
+    assert len(fr) == 6 and all(0.0 < f < 1.0 for f in fr)
+    print("refined fractions:", [round(f, 3) for f in [first] + fr])
+
+
+def test_refined_patches_reproduce_a_linear_field():
+    """New patches are interpolated from their parent: a linear surface
+    (Gaussian b) is reproduced at every new cell centre."""
+    ax, ay = 0.6, -0.3
+
+    def fn(X, Y):
+        b = 0.3 * np.exp(-((X - 0.4) ** 2 + (Y - 0.6) ** 2) / 0.05)
+        return 2.0 + ax * X + ay * Y - b, b
+    be = OracleBackend()
+    grid = AmrGrid(be, 3, 3, 6, lambda lev, ij: False, fn, max_level=2)
+    # every leaf flagged for refinement: demanded width below the coarse width
+    assert grid.regrid(threshold=-1.0) == (9, 0)
+    assert len(grid.leaves) == 81
+    for p in grid.leaves:
+        u = be.state(p.handle)
+        uo = be.uold(p.handle)[:, 1:-1, 1:-1]
+        b = be.p[p.handle].pwg[0][1:-1, 1:-1]
+        c = (np.arange(6) + 0.5) * (p.size / 6)
+        X, Y = np.meshgrid(p.pos[0] + c, p.pos[1] + c, indexing="ij")
+        for arr in (u, uo):
+            np.testing.assert_allclose(arr[0] + b, 2.0 + ax * X + ay * Y, rtol=0, atol=1e-12)
+            assert np.abs(arr[1:]).max() == 0.0
+
+
+def test_coarsening_a_lake_at_rest_keeps_it():
+    """A fully refined lake at rest (flat surface over a low bump, depth
+    gradient below the threshold): every fine leaf allows coarsening; after
+    coarsening the coarse leaves hold the same still surface and it stays
+    at rest."""
+    def fn(X, Y):
+        # negligible at the walls, whose ghost b is not reflected
+        b = -1.0 + 0.004 * np.exp(-((X - 0.5) ** 2 + (Y - 0.5) ** 2) / 0.01)
+        return 0.0 - b, b
+    be = OracleBackend()
+    grid = AmrGrid(be, 3, 3, 6, lambda lev, ij: True, fn, max_level=2)
+    grid.run(3)
+    assert grid.regrid() == (0, 9)
+    assert len(grid.leaves) == 9 and all(p.level == 1 for p in grid.leaves)
+    grid.run(5)
+    for p in grid.leaves:
+        u = be.state(p.handle)
+        b = be.p[p.handle].pwg[0][1:-1, 1:-1]
+        assert np.abs(u[0] + b).max() <= 1e-10
+        assert np.abs(u[1:]).max() <= 1e-10
