@@ -321,7 +321,7 @@ int pc_server_release(pc_server *srv, int id);
  * interpolation from the coarser leaf, domain walls, global CFL minimum,
  * solveTimestep of every leaf; GridLevelTransfer.cpp:140-198,
  * GhostlayerCompositorFunctors.cpp:59-69, SolveTimestep.cpp:87-142 and
- * 591-751) handed over once as a plan and run as nine batched launches per
+ * 591-751) handed over once as a plan and run as seven batched launches per
  * step.  Level transfers read h + b instead of transforming the source in
  * place and back (the reference's round trip can move a depth by an ulp;
  * results agree with the per-call form to ~1e-14 relative).  No reference

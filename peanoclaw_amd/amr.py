@@ -452,7 +452,7 @@ def wet_dry_fields(seed=4, column=(0.05, 0.35, 0.3, 0.6333333333333333), column_
 class BatchedAmr:
     """The grid's global steps as one batched device plan (pc_amr_*): the
     same restriction, ghost and wall sequence as AmrGrid.step with
-    pure=True, nine launches per step.  `grid` must run on a DeviceBackend;
+    pure=True, seven launches per step.  `grid` must run on a DeviceBackend;
     afterwards its leaves hold the state and clock of the batched steps."""
 
     def __init__(self, grid):

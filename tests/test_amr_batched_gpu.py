@@ -1,4 +1,4 @@
-"""Batched AMR step (pc_amr_*: nine launches per global step) against the
+"""Batched AMR step (pc_amr_*: seven launches per global step) against the
 oracle's AMR driver with the same level-transfer semantics (pure=True):
 bitwise per step on the 2-level dam break and the 3-level wet/dry basin
 at full size, with the f-wave and the augmented solver; long runs of both
