@@ -52,6 +52,7 @@ class AmrPatch:
         self.virtual = virtual
         self.ij = ij  # index on its level
         self.children = []
+        self.age = 0  # global steps since creation
 
 
 class AmrGrid:
@@ -143,10 +144,14 @@ class AmrGrid:
           Remesh.cpp:331-417: interpolateSolution to uNew at the new time,
           then to uOld at the current time, without neighbour times).
         - A virtual patch whose nine children are leaves that all allow
-          coarsening receives the restriction of all of them and becomes a
-          leaf: its [0, 1]-basis values are evaluated at the clock's time
-          and time + dt (Patch::switchToLeaf, Patch.cpp:793-830); the
-          children are released.
+          coarsening and have taken at least 5 steps (the age veto,
+          SolveTimestep.cpp:791-797) receives the restriction of all of
+          them and becomes a leaf: its [0, 1]-basis values are evaluated at
+          the clock's time and time + dt (Patch::switchToLeaf,
+          Patch.cpp:793-830); the children are released.
+        Decisions are per patch: Peano's vertex-based refinement (refining
+        a cell's vertices refines the cells around them; erase vetoes
+        shared across vertices) is not reproduced.
         Returns (refined, coarsened) patch counts."""
         B = self.b
         n = self.n
@@ -158,7 +163,8 @@ class AmrGrid:
         flag = {p.handle: f for p, f in zip(self.leaves, flags)}
         refine = [p for p in self.leaves if flag[p.handle] == 1 and p.level < self.max_level]
         coarsen = [v for v in self.virtuals
-                   if all(not c.virtual and flag.get(c.handle) == 0 for c in v.children)]
+                   if all(not c.virtual and flag.get(c.handle) == 0 and c.age >= 5
+                          for c in v.children)]
         for v in coarsen:
             B.clear(v.handle)
             for c in v.children:
@@ -169,6 +175,7 @@ class AmrGrid:
                 del self.levels[c.level][c.ij]
                 B.release(c.handle)
             v.virtual = False
+            v.age = 0
         for p in refine:
             sf = p.size / 3.0
             lev = p.level + 1
@@ -263,6 +270,7 @@ class AmrGrid:
         dt = 0.0
         for p in self.leaves:
             dt = B.solve(p.handle, bound)
+            p.age += 1
         if dt - 0.0 > LA_EPS:
             self.time, self.dt = self.time + self.dt, dt
         self.steps += 1
@@ -493,7 +501,11 @@ class BatchedAmr:
         """k global steps; returns the last step's dt (0 once the end time
         is reached)."""
         dt = ctypes.c_double()
+        before = self.clock()[2]
         check(self.lib.pc_amr_step(self.h, int(k), float(end_time), ctypes.byref(dt)))
+        taken = self.clock()[2] - before
+        for p in self.grid.leaves:
+            p.age += taken
         return dt.value
 
     def clock(self):

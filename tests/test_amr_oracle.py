@@ -242,7 +242,10 @@ def test_coarsening_a_lake_at_rest_keeps_it():
         return 0.0 - b, b
     be = OracleBackend()
     grid = AmrGrid(be, 3, 3, 6, lambda lev, ij: True, fn, max_level=2)
-    grid.run(3)
+    grid.run(4)
+    # fine leaves younger than 5 steps veto the coarsening of their parent
+    assert grid.regrid() == (0, 0)
+    grid.run(1)
     assert grid.regrid() == (0, 9)
     assert len(grid.leaves) == 9 and all(p.level == 1 for p in grid.leaves)
     grid.run(5)
