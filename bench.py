@@ -255,6 +255,12 @@ def main():
     copy_gbs = pool.copy_probe(copy_bytes, reps=args.copy_reps)
     log(f"copy probe: {copy_bytes} bytes read + {copy_bytes} written per launch, "
         f"best {copy_gbs:.0f} GB/s")
+    # the same traffic mix as the update (4 doubles read, 3 written per
+    # cell) as one flat stream without neighbours: the achievable rate of
+    # the update's bytes on this GPU
+    stream_gbs = pool.stream_probe(cells, reps=args.copy_reps)
+    log(f"4-read/3-write stream probe over {cells} cells: best {stream_gbs:.0f} GB/s "
+        f"(56 B per cell)")
 
     mass1 = pool.mass()
     t, dt, est, nsteps = pool.scenario_get(0)
@@ -268,7 +274,9 @@ def main():
                 "algorithmic_bytes_per_launch": cells * BYTES_PER_CELL_UPDATE,
                 "avg_launch_ms": round(upd_ms, 4),
                 "copy_kernel_gbs": round(copy_gbs, 1),
-                "frac_of_copy_kernel": round(achieved / copy_gbs, 4)}
+                "frac_of_copy_kernel": round(achieved / copy_gbs, 4),
+                "stream_4r3w_gbs": round(stream_gbs, 1),
+                "frac_of_stream_4r3w": round(achieved / stream_gbs, 4)}
 
     serving = None
     if args.serve_requests > 0:

@@ -257,6 +257,14 @@ int pc_edge_batch(pc_pool *pool, long long n, const double *in, double *out);
  * the byte scale of hardware traffic counters. */
 int pc_copy_probe(pc_pool *pool, long long bytes, int reps, double *best_gbs);
 
+/* Diagnostic (no reference counterpart): the streaming ceiling of the
+ * update's traffic mix.  Seven scratch arrays of `cells` doubles; one flat
+ * grid-stride kernel (k_stream_probe) reads four of them and writes three per
+ * element, as the update reads h, hu, hv, b and writes h, hu, hv of a cell,
+ * with no neighbour access.  `reps` timed launches after a warm-up;
+ * *best_gbs = 56 * cells / fastest launch. */
+int pc_stream_probe(pc_pool *pool, long long cells, int reps, double *best_gbs);
+
 /* ---- serving: a stream of independent dam-break scenarios ----
  *
  * No reference counterpart: PeanoClaw runs one simulation per process

@@ -1820,6 +1820,41 @@ int pc_copy_probe(pc_pool* p, long long bytes, int reps, double* best_gbs) {
   return PC_OK;
 }
 
+int pc_stream_probe(pc_pool* p, long long cells, int reps, double* best_gbs) {
+  if (!p || cells < 1 || reps < 1 || !best_gbs) return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(p->device);
+  double* a[7] = {};
+  for (int k = 0; k < 7; ++k) {
+    if (hipMalloc(&a[k], (size_t)cells * sizeof(double)) != hipSuccess) {
+      for (int j = 0; j < k; ++j) hipFree(a[j]);
+      return fail(PC_E_OOM, "stream probe: hipMalloc failed");
+    }
+  }
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 7 && e == hipSuccess; ++k)
+    e = hipMemsetAsync(a[k], 0, (size_t)cells * sizeof(double), p->stream);
+  int ncu = 256;
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device);
+  const long long blocks = std::min<long long>((cells + 255) / 256, (long long)ncu * 32);
+  float best = 1e30f;
+  for (int r = 0; r <= reps && e == hipSuccess; ++r) {  // r = 0 is an untimed warm-up
+    if ((e = hipEventRecord(p->ev0, p->stream)) != hipSuccess) break;
+    hipLaunchKernelGGL(k_stream_probe, dim3((unsigned)blocks), dim3(256), 0, p->stream, cells,
+                       a[0], a[1], a[2], a[3], a[4], a[5], a[6]);
+    if ((e = hipGetLastError()) != hipSuccess) break;
+    if ((e = hipEventRecord(p->ev1, p->stream)) != hipSuccess) break;
+    if ((e = hipEventSynchronize(p->ev1)) != hipSuccess) break;
+    float ms = 0.f;
+    if ((e = hipEventElapsedTime(&ms, p->ev0, p->ev1)) != hipSuccess) break;
+    if (r > 0) best = std::min(best, ms);
+  }
+  hipStreamSynchronize(p->stream);
+  for (int k = 0; k < 7; ++k) hipFree(a[k]);
+  if (e != hipSuccess) return fail(PC_E_HIP, std::string("stream probe: ") + hipGetErrorString(e));
+  *best_gbs = 56.0 * (double)cells / (best * 1e-3) / 1e9;
+  return PC_OK;
+}
+
 int pc_pool_device_fields(pc_pool* p, int state, double** h, double** hu, double** hv,
                           double** b) {
   if (!p || (state != 0 && state != 1)) return fail(PC_E_INVALID, "invalid argument");

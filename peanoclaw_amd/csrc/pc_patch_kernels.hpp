@@ -463,4 +463,22 @@ __global__ void __launch_bounds__(256) k_copy_probe(long long n2,
     dst[i] = src[i];
 }
 
+// Flat stream with the update's traffic mix per element: 4 reads (h, hu, hv,
+// b), 3 writes (h, hu, hv).
+__global__ void __launch_bounds__(256) k_stream_probe(long long n, const double* __restrict__ h,
+                                                      const double* __restrict__ hu,
+                                                      const double* __restrict__ hv,
+                                                      const double* __restrict__ b,
+                                                      double* __restrict__ oh,
+                                                      double* __restrict__ ohu,
+                                                      double* __restrict__ ohv) {
+  for (long long i = threadIdx.x + (long long)blockIdx.x * blockDim.x; i < n;
+       i += (long long)blockDim.x * gridDim.x) {
+    const double x = h[i] + 0.5 * b[i];
+    oh[i] = x;
+    ohu[i] = hu[i] * x;
+    ohv[i] = hv[i] * x;
+  }
+}
+
 }  // namespace pcamd

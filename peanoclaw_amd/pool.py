@@ -224,5 +224,12 @@ class PatchPool:
         check(self.lib.pc_copy_probe(self.h, int(nbytes), int(reps), ctypes.byref(v)))
         return v.value
 
+    def stream_probe(self, cells, reps=5):
+        """Best rate in GB/s of a flat stream with the update's traffic mix
+        (4 doubles read, 3 written per element; 56 B x `cells`)."""
+        v = ctypes.c_double()
+        check(self.lib.pc_stream_probe(self.h, int(cells), int(reps), ctypes.byref(v)))
+        return v.value
+
 
 __all__ = ["PatchPool", "make_description", "iarr"]
