@@ -97,6 +97,8 @@ struct pc_pool {
   std::vector<int> h_plist;
   std::vector<int> h_live;  // live slots (for flags / sums)
   int* d_live = nullptr;
+  BoundaryDesc* d_bdesc = nullptr;  // ghost-edge sources of the live slots
+  size_t bdesc_cap = 0;
   size_t live_cap = 0;
   GhostStrip* d_strips = nullptr;
   int nstrips = 0;
@@ -462,6 +464,62 @@ int rebuild_topology(pc_pool* p) {
   if (!p->h_tasks.empty())
     HIPCHK(hipMemcpyAsync(p->d_tasks, p->h_tasks.data(), p->h_tasks.size() * sizeof(Task),
                           hipMemcpyHostToDevice, p->stream));
+  {
+    // ghost-edge sources of every live patch for the boundary speed pass
+    std::vector<BoundaryDesc> bdv(live.size());
+    for (size_t k = 0; k < live.size(); ++k) {
+      const int i = live[k];
+      const PatchDev& P = pd[i];
+      const SideDev& sd = p->h_sides[i];
+      const long long stride = P.ny + 2;
+      BoundaryDesc& D = bdv[k];
+      D.off = P.off;
+      D.nx = P.nx;
+      D.ny = P.ny;
+      D.scn = P.scn;
+      D.slot = i;
+      D.dx = P.dx;
+      D.dy = P.dy;
+      for (int side = 0; side < 2; ++side) {
+        const int kx = sd.kind[side];
+        D.xwall[side] = kx == kSideWall;
+        if (kx == kSideNeighbour) {
+          const PatchDev& Q = pd[sd.src[side]];
+          D.xrow[side] = Q.off + (side ? 1 : Q.nx) * stride;
+        } else if (kx == kSideWall) {
+          D.xrow[side] = P.off + (side ? P.nx : 1) * stride;
+        } else {
+          D.xrow[side] = P.off + (side ? P.nx + 1 : 0) * stride;
+        }
+        const int ky = sd.kind[2 + side];
+        D.ebo[side] = (long long)((i * 2 + side) * 2) * kERows;
+        if (ky == kSideNeighbour)
+          D.ebg[side] = (long long)((sd.src[2 + side] * 2 + (1 - side)) * 2) * kERows;
+        else
+          D.ebg[side] = ky == kSideWall ? -1 : -2;
+      }
+      for (int d = 0; d < 2; ++d) {
+        // lower side d shared with a same-size neighbour that sees this patch
+        // across its upper side
+        D.shared[d] = 0;
+        if (sd.kind[2 * d] != kSideNeighbour) continue;
+        const int q = sd.src[2 * d];
+        const SideDev& sq = p->h_sides[q];
+        const PatchDev& Q = pd[q];
+        D.shared[d] = sq.kind[2 * d + 1] == kSideNeighbour && sq.src[2 * d + 1] == i &&
+                      Q.dx == P.dx && Q.dy == P.dy && Q.nx == P.nx && Q.ny == P.ny &&
+                      Q.scn == P.scn && p->slots[q].live;
+      }
+    }
+    if (bdv.size() > p->bdesc_cap) {
+      if (p->d_bdesc) HIPCHK(hipFree(p->d_bdesc));
+      p->bdesc_cap = bdv.size() * 2;
+      HIPCHK(hipMalloc(&p->d_bdesc, p->bdesc_cap * sizeof(BoundaryDesc)));
+    }
+    if (!bdv.empty())
+      HIPCHK(hipMemcpyAsync(p->d_bdesc, bdv.data(), bdv.size() * sizeof(BoundaryDesc),
+                            hipMemcpyHostToDevice, p->stream));
+  }
   CHK(ensure_dev(&p->d_live, &p->live_cap, std::max<size_t>(live.size(), 1)));
   if (!live.empty())
     HIPCHK(hipMemcpyAsync(p->d_live, live.data(), live.size() * sizeof(int),
@@ -540,12 +598,10 @@ int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
         const dim3 g((nl + 3) / 4);
         if (prune)
           hipLaunchKernelGGL(k_boundary_speed<true>, g, dim3(256), 0, p->stream, p->d_scn,
-                             p->d_live, nl, p->d_pd, F, sides, E, p->d_smax_int, p->d_smax,
-                             p->prune_slack);
+                             p->d_bdesc, nl, F, E, p->d_smax_int, p->d_smax, p->prune_slack);
         else
           hipLaunchKernelGGL(k_boundary_speed<false>, g, dim3(256), 0, p->stream, p->d_scn,
-                             p->d_live, nl, p->d_pd, F, sides, E, p->d_smax_int, p->d_smax,
-                             1.0);
+                             p->d_bdesc, nl, F, E, p->d_smax_int, p->d_smax, 1.0);
       }
       hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
                          p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
@@ -632,7 +688,7 @@ int pc_pool_destroy(pc_pool* p) {
   for (int k = 0; k < kNumFields; ++k)
     if (p->f[k]) hipFree(p->f[k]);
   void* ptrs[] = {p->d_pd, p->d_smax, p->d_smax_int, p->d_est, p->d_scn, p->d_tasks,
-                  p->d_plist, p->d_plist_scn, p->d_scn_of, p->d_live, p->d_strips,
+                  p->d_plist, p->d_plist_scn, p->d_scn_of, p->d_live, p->d_bdesc, p->d_strips,
                   p->d_counter, p->d_slot1, p->d_result, p->d_aux, p->d_flags, p->d_ebuf[0],
                   p->d_ebuf[1], p->d_bcol, p->d_sides};
   for (void* q : ptrs)

@@ -87,6 +87,30 @@ struct SideDev {
   int src[4];
 };
 
+// Ghost-edge sources of one live patch for the boundary speed pass, resolved
+// on the host when the grid changes (one load chain instead of patch ->
+// sides -> neighbour patch -> data):
+//   xrow[s]: index of column 0 of the row across x side s (the neighbour's
+//            boundary row, the own boundary row for a wall (hu negated), or
+//            the own ghost row);
+//   ebo[s]:  offset of the own edge-buffer column of y side s;
+//   ebg[s]:  offset of the neighbour's edge-buffer column across y side s,
+//            -1 for a wall (own column, hv negated), -2 for the own ghost
+//            column of the tile;
+//   shared[d]: the lower side in dimension d is a neighbour of the same
+//            subcell size whose upper side points back here: the edges
+//            between the two belong to that neighbour's upper side.
+struct BoundaryDesc {
+  long long off;
+  long long xrow[2];
+  long long ebo[2];
+  long long ebg[2];
+  int nx, ny, scn, slot;
+  int xwall[2];
+  int shared[2];
+  double dx, dy;
+};
+
 // Edge buffers: h and hv of the first (side 0) and last (side 1) interior
 // column of every patch, per state: e[s][((p*2 + side)*2 + q)*kERows + row-1]
 // (q 0 = h, 1 = hv); bcol[(p*4 + k)*kERows + row-1] = b at columns
@@ -633,98 +657,108 @@ k_update(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ pli
 // where the bound may reach the scenario's last CFL limit, as in the update.
 template <bool PRUNE>
 __global__ void __launch_bounds__(256)
-k_boundary_speed(const ScnDev* __restrict__ scn, const int* __restrict__ slots, int nslots,
-                 const PatchDev* __restrict__ pd, Fields F, const SideDev* __restrict__ sides,
-                 EdgeBufs E, const double* __restrict__ smax_int, double* __restrict__ smax,
-                 double slack) {
+k_boundary_speed(const ScnDev* __restrict__ scn, const BoundaryDesc* __restrict__ bd, int nslots,
+                 Fields F, EdgeBufs E, const double* __restrict__ smax_int,
+                 double* __restrict__ smax, double slack) {
+  constexpr int kIt = 4;  // 2 (nx + ny) <= 4 * 64 edges (nx, ny <= kERows)
   const int lane = threadIdx.x & 63;
   const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (k >= nslots) return;
-  {
-    const int p = gl(slots + k);
-    const PatchDev P = gl(pd + p);
-    const int par = gl(&scn[P.scn].stepc) & 1;
-    const SideDev sd = gl(sides + p);
-    float thr = 0.0f;
-    if (PRUNE) {
-      const double est = gl(&scn[P.scn].est);
-      if (est > 0.0) thr = (float)(kCfl * dmin(P.dx, P.dy) / (est * slack));
-    }
-    const int nx = P.nx, ny = P.ny, stride = ny + 2;
-    const int ne = 2 * (nx + ny);
-    const double* H = F.H(par);
-    const double* HU = F.HU(par);
-    const double* HV = F.HV(par);
-    const double* Ecur = par ? E.e[1] : E.e[0];
-    double sm = 0.0;
-    for (int e0 = 0; e0 < ne; e0 += 64) {
-      const int e = e0 + lane;
-      double ho = 1.0, qo = 0.0, hg = 1.0, qg = 0.0;
-      bool upper = false;
-      if (e < 2 * ny) {
-        // x side: edge (row 0 | 1) or (nx | nx+1) at column c
-        upper = e >= ny;
-        const int c = (upper ? e - ny : e) + 1;
-        const int kk = upper ? sd.kind[1] : sd.kind[0];
-        const long long io = P.off + (long long)(upper ? nx : 1) * stride + c;
-        ho = gl(H + io);
-        qo = gl(HU + io);
-        if (kk == kSideNeighbour) {
-          const PatchDev Q = gl(pd + (upper ? sd.src[1] : sd.src[0]));
-          const long long j = Q.off + (long long)(upper ? 1 : Q.nx) * stride + c;
-          hg = gl(H + j);
-          qg = gl(HU + j);
-        } else if (kk == kSideWall) {
-          hg = ho;
-          qg = -qo;
-        } else {
-          const long long j = P.off + (long long)(upper ? nx + 1 : 0) * stride + c;
-          hg = gl(H + j);
-          qg = gl(HU + j);
-        }
-      } else if (e < ne) {
-        // y side: edge (col 0 | 1) or (ny | ny+1) at row r
-        const int e2 = e - 2 * ny;
-        upper = e2 >= nx;
-        const int side = upper ? 1 : 0;
-        const int r = (upper ? e2 - nx : e2) + 1;
-        const int kk = sd.kind[2 + side];
-        const double* Eo = Ecur + (long long)((p * 2 + side) * 2) * kERows + r - 1;
-        ho = gl(Eo);
-        qo = gl(Eo + kERows);
-        if (kk == kSideNeighbour) {
-          const double* Eq =
-              Ecur + (long long)((sd.src[2 + side] * 2 + (1 - side)) * 2) * kERows + r - 1;
-          hg = gl(Eq);
-          qg = gl(Eq + kERows);
-        } else if (kk == kSideWall) {
-          hg = ho;
-          qg = -qo;
-        } else {
-          const long long j = P.off + (long long)r * stride + (upper ? ny + 1 : 0);
-          hg = gl(H + j);
-          qg = gl(HV + j);
-        }
-      }
-      const bool valid = e < ne;
-      // orientation: the lower side's ghost is the left cell of its edge
-      const double hL = upper ? ho : hg, qL = upper ? qo : qg;
-      const double hR = upper ? hg : ho, qR = upper ? qg : qo;
-      bool need = valid;
-      if (PRUNE) {
-        const BoundTerms bl = bound_terms(hL, qL, 0.0), br = bound_terms(hR, qR, 0.0);
-        need = valid && bound_candidate(bl.ax, bl.c, br.ax, br.c, thr);
-      }
-      if (__any(need ? 1 : 0)) {
-        const CellQ cl = cell_dir(cell_base(hL, 0.0), qL);
-        const CellQ cr = cell_dir(cell_base(hR, 0.0), qR);
-        const double sp = edge_speed_any(cl, cr);
-        if (valid) sm = dmax(sm, sp);
-      }
-    }
-    for (int o = 32; o >= 1; o >>= 1) sm = dmax(sm, __shfl_xor(sm, o));
-    if (lane == 0) smax[p] = dmax(gl(smax_int + p), sm);
+  const BoundaryDesc D = gl(bd + k);
+  const int par = gl(&scn[D.scn].stepc) & 1;
+  float thr = 0.0f;
+  if (PRUNE) {
+    const double est = gl(&scn[D.scn].est);
+    if (est > 0.0) thr = (float)(kCfl * dmin(D.dx, D.dy) / (est * slack));
   }
+  const int nx = D.nx, ny = D.ny, stride = ny + 2;
+  // edge list: x-lower (ny), x-upper (ny), y-lower (nx), y-upper (nx).  In
+  // pruned mode only the scenario minimum of the CFL steps is used, so an
+  // edge shared with a same-size neighbour is evaluated once, by the patch
+  // below it; each patch keeps its upper sides, walls and tile ghosts.
+  const int l0 = (PRUNE && D.shared[0]) ? 0 : ny;
+  const int l2 = (PRUNE && D.shared[1]) ? 0 : nx;
+  const int b1 = l0, b2 = l0 + ny, b3 = b2 + l2;
+  const int ne = b3 + nx;
+  const double* H = F.H(par);
+  const double* HU = F.HU(par);
+  const double* HV = F.HV(par);
+  const double* Ecur = par ? E.e[1] : E.e[0];
+  // all loads first: own boundary cell (ho, qo) and the cell across the
+  // edge (hg, qg) of every edge this lane owns
+  double ho[kIt], qo[kIt], hg[kIt], qg[kIt];
+  bool up[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int e = it * 64 + lane;
+    ho[it] = 1.0; qo[it] = 0.0; hg[it] = 1.0; qg[it] = 0.0; up[it] = false;
+    if (e < b2) {
+      // x side: edge (row 0 | 1) or (nx | nx+1) at column c
+      const bool upper = e >= b1;
+      const int c = (upper ? e - b1 : e) + 1;
+      const long long io = D.off + (long long)(upper ? nx : 1) * stride + c;
+      const long long ig = (upper ? D.xrow[1] : D.xrow[0]) + c;
+      ho[it] = gl(H + io);
+      qo[it] = gl(HU + io);
+      hg[it] = gl(H + ig);
+      qg[it] = gl(HU + ig);
+      up[it] = upper;
+    } else if (e < ne) {
+      // y side: edge (col 0 | 1) or (ny | ny+1) at row r
+      const bool upper = e >= b3;
+      const int r = (upper ? e - b3 : e - b2) + 1;
+      const long long eo = (upper ? D.ebo[1] : D.ebo[0]) + r - 1;
+      const long long eg = upper ? D.ebg[1] : D.ebg[0];
+      ho[it] = gl(Ecur + eo);
+      qo[it] = gl(Ecur + eo + kERows);
+      if (eg >= 0) {
+        hg[it] = gl(Ecur + eg + r - 1);
+        qg[it] = gl(Ecur + eg + r - 1 + kERows);
+      } else if (eg == -2) {
+        const long long j = D.off + (long long)r * stride + (upper ? ny + 1 : 0);
+        hg[it] = gl(H + j);
+        qg[it] = gl(HV + j);
+      }
+      up[it] = upper;
+    }
+  }
+  double sm = 0.0;
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    if (it * 64 >= ne) break;  // wave-uniform
+    const int e = it * 64 + lane;
+    const bool valid = e < ne;
+    const bool upper = up[it];
+    // walls mirror the own boundary cell with the normal momentum negated
+    bool wall;
+    if (e < b2)
+      wall = upper ? D.xwall[1] != 0 : D.xwall[0] != 0;
+    else
+      wall = (upper ? D.ebg[1] : D.ebg[0]) == -1;
+    double hgv = hg[it], qgv = qg[it];
+    if (e < b2) {
+      if (wall) qgv = -qgv;  // the own boundary row, read as the ghost source
+    } else if (wall) {
+      hgv = ho[it];
+      qgv = -qo[it];
+    }
+    // orientation: the lower side's ghost is the left cell of its edge
+    const double hL = upper ? ho[it] : hgv, qL = upper ? qo[it] : qgv;
+    const double hR = upper ? hgv : ho[it], qR = upper ? qgv : qo[it];
+    bool need = valid;
+    if (PRUNE) {
+      const BoundTerms bl = bound_terms(hL, qL, 0.0), br = bound_terms(hR, qR, 0.0);
+      need = valid && bound_candidate(bl.ax, bl.c, br.ax, br.c, thr);
+    }
+    if (__any(need ? 1 : 0)) {
+      const CellQ cl = cell_dir(cell_base(hL, 0.0), qL);
+      const CellQ cr = cell_dir(cell_base(hR, 0.0), qR);
+      const double sp = edge_speed_any(cl, cr);
+      if (valid) sm = dmax(sm, sp);
+    }
+  }
+  for (int o = 32; o >= 1; o >>= 1) sm = dmax(sm, __shfl_xor(sm, o));
+  if (lane == 0) smax[D.slot] = dmax(gl(smax_int + D.slot), sm);
 }
 
 // Edge buffers of the current state from the tiles (after uploads or
