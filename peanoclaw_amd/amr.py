@@ -300,6 +300,138 @@ class AmrGrid:
         for p, dim, upper in walls:
             B.fill_boundary(p.handle, dim, upper)
 
+    # ---- local time stepping ----
+
+    def _covering(self, lev, q):
+        """The patch at `lev` with index q, else the patch of the nearest
+        coarser level covering that region (None outside the domain)."""
+        k = 3 ** (lev - 1)
+        if not (0 <= q[0] < self.px * k and 0 <= q[1] < self.py * k):
+            return None
+        while lev >= 1:
+            if q in self.levels[lev]:
+                return self.levels[lev][q]
+            lev -= 1
+            q = (q[0] // 3, q[1] // 3)
+        return None
+
+    def _lts_tables(self):
+        """Per leaf: the patches sharing a face or a corner with it (same
+        level if present, else the coarser patch covering the region: the
+        four patches around each of its vertices, GhostLayerCompositor.h:52-56),
+        its ghost-face entries and its walls; per virtual patch the leaves
+        below it in depth-first order."""
+        plan, walls = self._plan
+        faces = {p.handle: [] for p in self.leaves}
+        for entry in plan:
+            faces[entry[2].handle].append(entry)
+        wl = {p.handle: [] for p in self.leaves}
+        for p, dim, upper in walls:
+            wl[p.handle].append((dim, upper))
+        nbrs = {}
+        for p in self.leaves:
+            seen = []
+            for di in (-1, 0, 1):
+                for dj in (-1, 0, 1):
+                    if di == 0 and dj == 0:
+                        continue
+                    q = self._covering(p.level, (p.ij[0] + di, p.ij[1] + dj))
+                    if q is not None and q is not p and all(q is not x for x in seen):
+                        seen.append(q)
+            nbrs[p.handle] = seen
+        below = {v.handle: [] for v in self.virtuals}
+        for leaf, ancestors in self._restrictions:
+            for v in ancestors:
+                below[v.handle].append(leaf)
+        return faces, wl, nbrs, below
+
+    def clock(self, pt):
+        """(time, timestep size) of a leaf, or for a virtual patch the end of
+        the minimal time interval of the leaves below it as (end, 0)
+        (TimeIntervals::updateMinimalFineGridTimeInterval,
+        TimeIntervals.cpp:190-214: the intersection of the fine intervals
+        ends at their minimal end)."""
+        if not pt.virtual:
+            return self.b.clock(pt.handle)
+        return (min(sum(self.b.clock(x.handle)) for x in self._lts[3][pt.handle]), 0.0)
+
+    def _restrict_one(self, v):
+        """Fresh content of virtual patch v from the current states of the
+        leaves below it (all of them, depth-first, then the SWE
+        post-processing)."""
+        B = self.b
+        restrict = B.restrict_pure if self.pure else B.restrict
+        B.clear(v.handle)
+        for leaf in self._lts[3][v.handle]:
+            restrict(leaf.handle, v.handle)
+        B.post_restrict(v.handle)
+
+    def lts_iteration(self, end_time):
+        """One grid iteration with local time stepping: every leaf, coarse
+        levels first, advances one step of its own CFL size if it may
+        (SolveTimestep::enterCell, SolveTimestep.cpp:591-751, with
+        TimeIntervals::isAllowedToAdvanceInTime, TimeIntervals.cpp:24-78):
+        its step must not pass the global end time, and its current end
+        time must not exceed the time constraint (end of the current
+        interval) of any patch sharing a face or a corner with it
+        (isBlockedByNeighbors, TimeIntervals.cpp:80-84; a virtual patch
+        constrains by the minimal end of its leaves).  A leaf that advances
+        takes its ghost layers from its neighbours' current intervals (the
+        time factors of DefaultTransfer / DefaultInterpolation; a virtual
+        source is restricted afresh from its leaves), copies uNew to uOld,
+        fills its walls and solves with maximumTimestepSize = end time -
+        (time + timestep size).  Leaves are visited one after the other, so
+        a neighbour that advanced earlier in the iteration is seen with its
+        new interval.  Returns the number of leaves that advanced."""
+        B = self.b
+        if getattr(self, "_lts", None) is None or self._lts_key is not self._plan:
+            self._lts = self._lts_tables()
+            self._lts_key = self._plan
+        faces, walls, nbrs, below = self._lts
+        interpolate = B.interpolate_pure if self.pure else B.interpolate
+        dirty = {v.handle for v in self.virtuals}
+        ancestors = {leaf.handle: anc for leaf, anc in self._restrictions}
+        advanced = 0
+        for p in self.leaves:
+            t, dt = B.clock(p.handle)
+            max_dt = end_time - (t + dt)
+            if not (max_dt - 0.0 > LA_EPS):
+                continue
+            constraint = min((sum(self.clock(q)) for q in nbrs[p.handle]), default=np.inf)
+            if (t + dt) - constraint > LA_EPS:
+                continue
+            for kind, src, dst, size, soff, doff in faces[p.handle]:
+                if src.virtual and src.handle in dirty:
+                    self._restrict_one(src)
+                    dirty.discard(src.handle)
+                if kind == "transfer":
+                    B.transfer(size, soff, doff, src.handle, dst.handle)
+                else:
+                    interpolate(size, doff, src.handle, dst.handle, 1, 0, 1)
+            B.copy_unew_to_uold(p.handle)
+            for dim, upper in walls[p.handle]:
+                B.fill_boundary(p.handle, dim, upper)
+            B.solve(p.handle, max_dt)
+            p.age += 1
+            advanced += 1
+            for v in ancestors.get(p.handle, ()):
+                dirty.add(v.handle)
+        self.steps += 1
+        return advanced
+
+    def run_lts(self, end_time, max_iterations=100000):
+        """Grid iterations with local time stepping until every leaf has
+        reached `end_time` (or `max_iterations`); returns the number of
+        iterations and the number of leaf steps."""
+        its = leaf_steps = 0
+        while its < max_iterations:
+            k = self.lts_iteration(end_time)
+            its += 1
+            leaf_steps += k
+            if k == 0:
+                break
+        return its, leaf_steps
+
     def leaf_states(self):
         """uNew of every leaf, keyed by (level, i, j)."""
         return {(p.level,) + p.ij: self.b.state(p.handle) for p in self.leaves}
@@ -394,6 +526,10 @@ class DeviceBackend:
         check(self.lib.pc_solve_timestep(self.pool.h, s, float(max_dt), iarr(0, 0, 0, 0),
                                          ctypes.byref(dt), ctypes.byref(est)))
         return dt.value
+
+    def clock(self, s):
+        d = self.pool.description(s)
+        return d.time, d.timestep_size
 
     def state(self, s):
         return self.pool.download(s)[0]

@@ -94,6 +94,9 @@ class OracleBackend:
     def solve(self, s, max_dt):
         return lib().or_solve_timestep(self.p[s].ref, float(max_dt))
 
+    def clock(self, s):
+        return self.p[s].s.time, self.p[s].s.dt
+
     def state(self, s):
         return np.array(self.p[s].unew)
 

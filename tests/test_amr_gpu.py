@@ -119,3 +119,41 @@ def test_three_level_wet_dry_basin_full_size():
         assert same == total
     finally:
         pool.close()
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_local_time_stepping_bitwise(levels):
+    """Local time stepping (every leaf its own CFL step, blocked by its
+    neighbours' time constraints, ghosts interpolated in time): the per-call
+    device sequence against the oracle's, bitwise, with the clocks of every
+    leaf equal after each grid iteration."""
+    from peanoclaw_amd.amr import AmrGrid, DeviceBackend, dam_break_fields
+    from peanoclaw_amd.pool import PatchPool
+    fn = dam_break_fields(centre=(0.5, 0.5), radius=0.2, hills=[(0.3, 0.6, 0.1, 0.15)])
+
+    def refine(lev, ij):
+        return (lev == 1 and abs(ij[0] - 2) <= 1 and abs(ij[1] - 2) <= 1) or (
+            lev == 2 and ij == (7, 7))
+    pool = PatchPool(0)
+    try:
+        dev = AmrGrid(DeviceBackend(pool), 5, 5, 8, refine, fn, max_level=levels)
+        ora = AmrGrid(OracleBackend(), 5, 5, 8, refine, fn, max_level=levels)
+        end = 0.006
+        its = steps = 0
+        while True:
+            k1 = dev.lts_iteration(end)
+            k0 = ora.lts_iteration(end)
+            assert k1 == k0, f"leaves advanced differ in iteration {its}"
+            for p, q in zip(dev.leaves, ora.leaves):
+                assert dev.b.clock(p.handle) == ora.b.clock(q.handle)
+            its += 1
+            steps += k1
+            if k1 == 0:
+                break
+        same, total = _bitwise_equal(dev, ora)
+        levels_seen = sorted({p.level for p in dev.leaves})
+        print(f"LTS {levels}-level ({levels_seen}), {len(dev.leaves)} leaves: {its} iterations, "
+              f"{steps} leaf steps to t={end}; bitwise {same}/{total}")
+        assert same == total
+    finally:
+        pool.close()
