@@ -261,6 +261,11 @@ def main():
     stream_gbs = pool.stream_probe(cells, reps=args.copy_reps)
     log(f"4-read/3-write stream probe over {cells} cells: best {stream_gbs:.0f} GB/s "
         f"(56 B per cell)")
+    # the update's own traversal of the pool (tasks, row march, prefetch)
+    # without the edge arithmetic: the memory-pattern ceiling of the layout
+    march_gbs = pool.march_probe(reps=args.copy_reps)
+    log(f"row-march probe (update traversal, no arithmetic): best {march_gbs:.0f} GB/s "
+        f"(56 B per cell)")
 
     mass1 = pool.mass()
     t, dt, est, nsteps = pool.scenario_get(0)
@@ -276,7 +281,9 @@ def main():
                 "copy_kernel_gbs": round(copy_gbs, 1),
                 "frac_of_copy_kernel": round(achieved / copy_gbs, 4),
                 "stream_4r3w_gbs": round(stream_gbs, 1),
-                "frac_of_stream_4r3w": round(achieved / stream_gbs, 4)}
+                "frac_of_stream_4r3w": round(achieved / stream_gbs, 4),
+                "march_probe_gbs": round(march_gbs, 1),
+                "frac_of_march_probe": round(achieved / march_gbs, 4)}
 
     serving = None
     if args.serve_requests > 0:

@@ -265,6 +265,16 @@ int pc_copy_probe(pc_pool *pool, long long bytes, int reps, double *best_gbs);
  * *best_gbs = 56 * cells / fastest launch. */
 int pc_stream_probe(pc_pool *pool, long long cells, int reps, double *best_gbs);
 
+/* Diagnostic (no reference counterpart): the memory-pattern ceiling of the
+ * batched update in the pool's own layout.  One launch of k_march_probe
+ * walks the batched step's task list exactly as the update does (persistent
+ * grid, a wavefront per task, rows marched in order, four rows prefetched), reading
+ * h, hu, hv, b of every tile row of the current state and writing h, hu, hv
+ * of the interior rows to scratch arrays, without the edge arithmetic; the
+ * pool's state is not changed.  `reps` timed launches after a warm-up;
+ * *best_gbs = 56 * interior cells / fastest launch. */
+int pc_march_probe(pc_pool *pool, int reps, double *best_gbs);
+
 /* ---- serving: a stream of independent dam-break scenarios ----
  *
  * No reference counterpart: PeanoClaw runs one simulation per process

@@ -184,6 +184,7 @@ _SIGS = {
     "pc_edge_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp]),
     "pc_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, _dp]),
     "pc_stream_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, _dp]),
+    "pc_march_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
     "pc_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_void_p)]),
     "pc_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),

@@ -1911,6 +1911,54 @@ int pc_stream_probe(pc_pool* p, long long cells, int reps, double* best_gbs) {
   return PC_OK;
 }
 
+int pc_march_probe(pc_pool* p, int reps, double* best_gbs) {
+  if (!p || reps < 1 || !best_gbs) return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(p->device);
+  CHK(prepare_batched(p));
+  if (p->ntasks == 0) return fail(PC_E_INVALID, "march probe: no live patches");
+  const int nscn = (int)p->scn.size();
+  const int par = nscn > 0 ? (p->scn[0].stepc & 1) : 0;
+  const size_t n = (size_t)p->cap_cells;
+  double* o[3] = {};
+  int* ctr = nullptr;
+  for (int k = 0; k < 3; ++k) {
+    if (hipMalloc(&o[k], n * sizeof(double)) != hipSuccess) {
+      for (int j = 0; j < k; ++j) hipFree(o[j]);
+      return fail(PC_E_OOM, "march probe: hipMalloc failed");
+    }
+  }
+  if (hipMalloc(&ctr, sizeof(int)) != hipSuccess) {
+    for (int k = 0; k < 3; ++k) hipFree(o[k]);
+    return fail(PC_E_OOM, "march probe: hipMalloc failed");
+  }
+  const Fields F = fields_of(p);
+  const int grid = p->persistent_wgs > 0 ? p->persistent_wgs : 4096;
+  long long cells = 0;
+  for (int s : p->h_live)
+    cells += (long long)p->slots[s].desc.subdivision_factor[0] * p->slots[s].desc.subdivision_factor[1];
+  float best = 1e30f;
+  hipError_t e = hipSuccess;
+  for (int r = 0; r <= reps && e == hipSuccess; ++r) {  // r = 0 is an untimed warm-up
+    if ((e = hipMemsetAsync(ctr, 0, sizeof(int), p->stream)) != hipSuccess) break;
+    if ((e = hipEventRecord(p->ev0, p->stream)) != hipSuccess) break;
+    hipLaunchKernelGGL(k_march_probe, dim3(grid), dim3(64), 0, p->stream, p->d_tasks, p->ntasks,
+                       p->d_plist, p->d_pd, F.h[par], F.hu[par], F.hv[par], F.b, o[0], o[1], o[2],
+                       ctr);
+    if ((e = hipGetLastError()) != hipSuccess) break;
+    if ((e = hipEventRecord(p->ev1, p->stream)) != hipSuccess) break;
+    if ((e = hipEventSynchronize(p->ev1)) != hipSuccess) break;
+    float ms = 0.f;
+    if ((e = hipEventElapsedTime(&ms, p->ev0, p->ev1)) != hipSuccess) break;
+    if (r > 0) best = std::min(best, ms);
+  }
+  hipStreamSynchronize(p->stream);
+  for (int k = 0; k < 3; ++k) hipFree(o[k]);
+  hipFree(ctr);
+  if (e != hipSuccess) return fail(PC_E_HIP, std::string("march probe: ") + hipGetErrorString(e));
+  *best_gbs = 56.0 * (double)cells / (best * 1e-3) / 1e9;
+  return PC_OK;
+}
+
 int pc_pool_device_fields(pc_pool* p, int state, double** h, double** hu, double** hv,
                           double** b) {
   if (!p || (state != 0 && state != 1)) return fail(PC_E_INVALID, "invalid argument");

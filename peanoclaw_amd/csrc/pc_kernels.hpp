@@ -761,6 +761,71 @@ k_boundary_speed(const ScnDev* __restrict__ scn, const BoundaryDesc* __restrict_
   if (lane == 0) smax[D.slot] = dmax(gl(smax_int + D.slot), sm);
 }
 
+// Memory-pattern probe of the row march: the update kernel's traversal
+// (same tasks, persistent grid, one wavefront per task, lanes on columns,
+// rows marched in order) reading h, hu, hv, b of every row of the tiles and
+// writing h, hu, hv of the interior rows to scratch arrays, with no edge
+// arithmetic.  Without arithmetic to hide behind, the loads run four rows
+// ahead (rows in blocks of four).  The rate it reaches is the memory rate of
+// this traversal of the layout.
+__global__ void __launch_bounds__(64)
+k_march_probe(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plist,
+              const PatchDev* __restrict__ pd, const double* __restrict__ h,
+              const double* __restrict__ hu, const double* __restrict__ hv,
+              const double* __restrict__ b, double* __restrict__ oh, double* __restrict__ ohu,
+              double* __restrict__ ohv, int* __restrict__ counter) {
+  constexpr int K = 4;
+  __shared__ int next;
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+    __syncthreads();
+    const int ti = next;
+    __syncthreads();
+    if (ti >= ntasks) return;
+    const Task t = gl(tasks + ti);
+    const int W = pow2ceil(t.ny);
+    const int seg = lane / W;
+    const int col = lane - seg * W + 1;
+    const bool segok = seg < t.nseg;
+    const bool active = segok && col <= t.ny;
+    const PatchDev P = gl(pd + gl(plist + t.first + (segok ? seg : 0)));
+    const long long stride = t.ny + 2;
+    const long long base = P.off + (active ? col : 1);
+    const int last = t.nx + 1;  // rows 0 .. nx+1
+    double ch[K], chu[K], chv[K], cb[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const long long i = base + (long long)(k <= last ? k : last) * stride;
+      ch[k] = gl(h + i); chu[k] = gl(hu + i); chv[k] = gl(hv + i); cb[k] = gl(b + i);
+    }
+    for (int r0 = 0; r0 <= last; r0 += K) {
+      double nh[K], nhu[K], nhv[K], nbv[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int rr = r0 + K + k;
+        const long long i = base + (long long)(rr <= last ? rr : last) * stride;
+        nh[k] = gl(h + i); nhu[k] = gl(hu + i); nhv[k] = gl(hv + i); nbv[k] = gl(b + i);
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int r = r0 + k;
+        if (active && r >= 1 && r <= t.nx) {
+          const long long o = base + (long long)r * stride;
+          const double x = ch[k] + 0.5 * cb[k];
+          gs(oh + o, x);
+          gs(ohu + o, chu[k] * x);
+          gs(ohv + o, chv[k] * x);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        ch[k] = nh[k]; chu[k] = nhu[k]; chv[k] = nhv[k]; cb[k] = nbv[k];
+      }
+    }
+  }
+}
+
 // Edge buffers of the current state from the tiles (after uploads or
 // per-patch calls that changed the interior).
 __global__ void k_edges_init(const int* __restrict__ slots, int nslots,

@@ -231,5 +231,12 @@ class PatchPool:
         check(self.lib.pc_stream_probe(self.h, int(cells), int(reps), ctypes.byref(v)))
         return v.value
 
+    def march_probe(self, reps=5):
+        """Best rate in GB/s (56 B per interior cell) of the batched update's
+        traversal of the pool without the edge arithmetic."""
+        v = ctypes.c_double()
+        check(self.lib.pc_march_probe(self.h, int(reps), ctypes.byref(v)))
+        return v.value
+
 
 __all__ = ["PatchPool", "make_description", "iarr"]
