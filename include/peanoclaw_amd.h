@@ -275,6 +275,30 @@ int pc_stream_probe(pc_pool *pool, long long cells, int reps, double *best_gbs);
  * *best_gbs = 56 * interior cells / fastest launch. */
 int pc_march_probe(pc_pool *pool, int reps, double *best_gbs);
 
+/* ---- output (grid/plotter/VTUSubgridPlotter.cpp:9-241) ----
+ * One quadrilateral per subcell in a VTK unstructured grid (XML, appended raw
+ * binary, Float64 points and cell arrays), vertices shared between subcells
+ * and patches; cell arrays q0..q2 (uNew), pWG0 (b), SubdivisionFactor,
+ * GhostlayerWidth, timeOld, timeNew, demandedMeshWidth, age, as the
+ * reference's plotter writes them (plotSubcell, :97-150). */
+typedef struct {
+  int32_t subdivision_factor[2];
+  int32_t ghostlayer_width;
+  int32_t level;
+  double position[2];
+  double size[2];
+  double time, timestep_size;  /* timeOld = time, timeNew = time + size */
+  double demanded_mesh_width;  /* norm1(demanded width) / 2 */
+  double age;                  /* grid iterations since creation */
+  const double *q[3];          /* uNew h, hu, hv: nx*ny, x rows, y fastest */
+  const double *b;             /* b at the nx*ny interior cells, same order */
+} pc_vtu_patch;
+/* Write host data of `npatches` patches to `path` (no device involved). */
+int pc_vtu_write(const char *path, int npatches, const pc_vtu_patch *patches);
+/* Download the listed slots (uNew of the current state, b, descriptor) and
+ * write them to `path`; age is written as 0. */
+int pc_pool_write_vtu(pc_pool *pool, const char *path, const int32_t *slots, int nslots);
+
 /* ---- serving: a stream of independent dam-break scenarios ----
  *
  * No reference counterpart: PeanoClaw runs one simulation per process

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(PKG)
 SRC = os.path.join(PKG, "csrc", "pc_abi.hip")
 DEPS = [os.path.join(PKG, "csrc", f) for f in
         ("pc_abi.hip", "pc_kernels.hpp", "pc_patch_kernels.hpp", "pc_server.hpp",
-         "swe_device.hpp")] + \
+         "pc_amr.hpp", "pc_vtu.hpp", "swe_device.hpp")] + \
        [os.path.join(ROOT, "include", "peanoclaw_amd.h")]
 LIB = os.path.join(PKG, "libpeanoclaw_amd.so")
 ARCH = "gfx950"

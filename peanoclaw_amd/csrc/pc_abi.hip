@@ -1973,3 +1973,4 @@ int pc_pool_device_fields(pc_pool* p, int state, double** h, double** hu, double
 
 #include "pc_server.hpp"
 #include "pc_amr.hpp"
+#include "pc_vtu.hpp"

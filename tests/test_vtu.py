@@ -1,0 +1,69 @@
+"""VTU output (the content of PeanoClaw's VTUSubgridPlotter,
+grid/plotter/VTUSubgridPlotter.cpp:9-241) written by the native writer and
+read back: vertices shared between subcells and patches, one counter-clockwise
+quadrilateral per subcell, the cell arrays of the reference's plotter."""
+import numpy as np
+import pytest
+
+from peanoclaw_amd.vtu import read_vtu, write_vtu
+
+
+def _patch(pos, size, n, level, rng, t=0.25, dt=0.125):
+    return {"pos": pos, "size": size, "level": level, "time": t, "dt": dt,
+            "q": rng.standard_normal((3, n, n)), "b": rng.standard_normal((n, n)),
+            "demanded": size / n, "age": 7}
+
+
+def test_uniform_grid_shares_vertices(tmp_path):
+    rng = np.random.default_rng(0)
+    n, P = 4, 3
+    patches = [_patch((i / P, j / P), 1.0 / P, n, 1, rng) for i in range(P) for j in range(P)]
+    f = tmp_path / "u.vtu"
+    write_vtu(f, patches)
+    pts, conn, types, cell = read_vtu(f)
+    assert len(pts) == (P * n + 1) ** 2  # every interior vertex shared
+    assert conn.shape == (P * P * n * n, 4) and (types == 9).all()
+    # counter-clockwise quads of the subcell size
+    x, y = pts[conn, 0], pts[conn, 1]
+    area = 0.5 * np.sum(x * np.roll(y, -1, axis=1) - np.roll(x, -1, axis=1) * y, axis=1)
+    np.testing.assert_allclose(area, (1.0 / (P * n)) ** 2, rtol=1e-12)
+    # cell order: patches in call order, subcells x slowest, y fastest
+    for k, p in enumerate(patches):
+        s = slice(k * n * n, (k + 1) * n * n)
+        for q in range(3):
+            assert np.array_equal(cell[f"q{q}"][s], p["q"][q].ravel())
+        assert np.array_equal(cell["pWG0"][s], p["b"].ravel())
+        cx = x[s].mean(axis=1).reshape(n, n)
+        cy = y[s].mean(axis=1).reshape(n, n)
+        np.testing.assert_allclose(cx[:, 0], p["pos"][0] + (np.arange(n) + 0.5) / (P * n),
+                                   rtol=0, atol=1e-14)
+        np.testing.assert_allclose(cy[0, :], p["pos"][1] + (np.arange(n) + 0.5) / (P * n),
+                                   rtol=0, atol=1e-14)
+    assert (cell["SubdivisionFactor"] == n).all() and (cell["GhostlayerWidth"] == 1).all()
+    assert (cell["timeOld"] == 0.25).all() and (cell["timeNew"] == 0.375).all()
+    assert (cell["age"] == 7).all()
+
+
+def test_refined_patch_hanging_vertices(tmp_path):
+    """A coarse patch next to 3 x 3 fine patches: the fine vertices on the
+    coarse patch's edge that coincide with coarse vertices are shared, the
+    others (hanging) are not."""
+    rng = np.random.default_rng(1)
+    n = 3
+    coarse = _patch((0.0, 0.0), 1.0 / 3, n, 1, rng)
+    fine = [_patch((1.0 / 3 + a / 9, c / 9), 1.0 / 9, n, 2, rng) for a in range(3) for c in range(3)]
+    f = tmp_path / "h.vtu"
+    write_vtu(f, [coarse] + fine)
+    pts, conn, _, _ = read_vtu(f)
+    coarse_v = (n + 1) ** 2
+    fine_v = (3 * n + 1) ** 2
+    shared = n + 1  # coarse vertices on x = 1/3
+    assert len(pts) == coarse_v + fine_v - shared
+    assert conn.shape[0] == n * n + 9 * n * n
+
+
+def test_invalid_arguments(tmp_path):
+    from peanoclaw_amd._abi import PeanoClawAmdError
+    with pytest.raises(PeanoClawAmdError):
+        write_vtu(tmp_path / "missing_dir" / "x.vtu", [_patch((0, 0), 1.0, 2, 1,
+                                                              np.random.default_rng(2))])
