@@ -176,6 +176,9 @@ class AmrGrid:
                 B.release(c.handle)
             v.virtual = False
             v.age = 0
+        # all new patches are registered first, then filled: a device pool
+        # rebuilds its work lists once instead of once per new patch
+        created = []
         for p in refine:
             sf = p.size / 3.0
             lev = p.level + 1
@@ -185,11 +188,14 @@ class AmrGrid:
                     pos = (p.pos[0] + a * sf, p.pos[1] + c * sf)
                     ch = AmrPatch(B.new_patch(pos, sf, n, lev, False, self.time, self.dt), lev,
                                   pos, sf, n, False, ij)
-                    _, bb = self._fields(ch)
-                    B.set_b(ch.handle, bb)
-                    B.interpolate((n, n), (0, 0), p.handle, ch.handle, 0, 0, 0)
-                    B.interpolate((n, n), (0, 0), p.handle, ch.handle, 1, 1, 0)
+                    created.append((p, ch))
                     self.levels.setdefault(lev, {})[ij] = ch
+        for p, ch in created:
+            B.set_b(ch.handle, self._fields(ch)[1])
+        for p, ch in created:
+            B.interpolate((n, n), (0, 0), p.handle, ch.handle, 0, 0, 0)
+            B.interpolate((n, n), (0, 0), p.handle, ch.handle, 1, 1, 0)
+        for p in refine:
             B.set_virtual(p.handle, True)
             p.virtual = True
         if refine or coarsen:
@@ -225,8 +231,9 @@ class AmrGrid:
         k = 3 ** (lev - 1)
         if not (0 <= q[0] < self.px * k and 0 <= q[1] < self.py * k):
             return ("wall", None)
-        if q in self.levels[lev]:
-            return ("transfer", self.levels[lev][q])
+        same = self.levels[lev].get(q)
+        if same is not None:
+            return ("transfer", same)
         while lev > 1:
             lev -= 1
             q = (q[0] // 3, q[1] // 3)
@@ -236,25 +243,35 @@ class AmrGrid:
 
     def _ghost_plan(self):
         n = self.n
+        # region geometry per face: (size, source offset, destination offset)
+        geom = {(0, 0): ((1, n), (n - 1, 0), (-1, 0)), (0, 1): ((1, n), (0, 0), (n, 0)),
+                (1, 0): ((n, 1), (0, n - 1), (0, -1)), (1, 1): ((n, 1), (0, 0), (0, n))}
         plan = []
-        for p in self.leaves:
-            for dim, upper in FACES:
-                kind, src = self._face(p, dim, upper)
-                if kind == "wall":
-                    continue
-                size = (1, n) if dim == 0 else (n, 1)
-                if dim == 0:
-                    doff = (n if upper else -1, 0)
-                    soff = (0 if upper else n - 1, 0)
-                else:
-                    doff = (0, n if upper else -1)
-                    soff = (0, 0 if upper else n - 1)
-                plan.append((kind, src, p, size, soff, doff))
         walls = []
+        face = self._face
+        wall_rank = [FACES.index(f) for f in WALL_ORDER]
+        steps = [(-1, 0), (1, 0), (0, -1), (0, 1)]  # FACES order
         for p in self.leaves:
-            for dim, upper in WALL_ORDER:
-                if self._face(p, dim, upper)[0] == "wall":
-                    walls.append((p, dim, upper))
+            # same-level neighbours by direct lookup, the rest by _face
+            lv = self.levels[p.level]
+            k = 3 ** (p.level - 1)
+            nx_, ny_ = self.px * k, self.py * k
+            i, j = p.ij
+            res = []
+            for (di, dj), (dim, upper) in zip(steps, FACES):
+                qi, qj = i + di, j + dj
+                if not (0 <= qi < nx_ and 0 <= qj < ny_):
+                    res.append(("wall", None))
+                    continue
+                same = lv.get((qi, qj))
+                res.append(("transfer", same) if same is not None else face(p, dim, upper))
+            for f, (kind, src) in zip(FACES, res):
+                if kind != "wall":
+                    size, soff, doff = geom[f]
+                    plan.append((kind, src, p, size, soff, doff))
+            for k in wall_rank:
+                if res[k][0] == "wall":
+                    walls.append((p, FACES[k][0], FACES[k][1]))
         return plan, walls
 
     def step(self, end_time=np.inf):
