@@ -396,6 +396,24 @@ int pc_amr_create(pc_pool *pool, const int32_t *leaves, int nleaves,
  * skipped).  The leaves' descriptors carry the clock afterwards. */
 int pc_amr_step(pc_amr *amr, int k, double end_time, double *last_dt);
 int pc_amr_clock(pc_amr *amr, double *time, double *timestep_size, long long *steps);
+/* Local time stepping over the same plan (SolveTimestep::enterCell,
+ * SolveTimestep.cpp:591-751, with TimeIntervals::isAllowedToAdvanceInTime,
+ * TimeIntervals.cpp:24-84): every leaf keeps its own clock.  `nptr` (nleaves
+ * + 1 offsets, leaves in plan order) and `nbr` list for each leaf the plan
+ * patches sharing a face or a corner with it (a leaf, or the virtual patch
+ * covering that region); a leaf advances when end_time - (time + dt) > 1e-8
+ * and time + dt - (minimal end time of its neighbours; of a virtual patch:
+ * of the leaves below it) <= 1e-8.  Initialises the leaves' clocks from
+ * their descriptors. */
+int pc_amr_lts_init(pc_amr *amr, const int32_t *nptr, const int32_t *nbr);
+/* k grid iterations: the advance flags from the clocks at the start of each,
+ * restriction of every leaf into the virtual patches on its own interval,
+ * ghost faces of the advancing leaves (time factors from the clocks), uNew
+ * -> uOld, walls, and per advancing leaf solveTimestep with
+ * maximumTimestepSize = end_time - (time + dt); *leaf_steps = leaf steps
+ * taken (0: every leaf has reached end_time).  The leaves' descriptors
+ * carry their clocks afterwards; the plan no longer runs pc_amr_step. */
+int pc_amr_lts_step(pc_amr *amr, int k, double end_time, long long *leaf_steps);
 int pc_amr_destroy(pc_amr *amr);
 
 /* Raw device pointers of the pool (for zero-copy interop); state = 0/1. */

@@ -372,69 +372,76 @@ class AmrGrid:
             return self.b.clock(pt.handle)
         return (min(sum(self.b.clock(x.handle)) for x in self._lts[3][pt.handle]), 0.0)
 
-    def _restrict_one(self, v):
-        """Fresh content of virtual patch v from the current states of the
-        leaves below it (all of them, depth-first, then the SWE
-        post-processing)."""
-        B = self.b
-        restrict = B.restrict_pure if self.pure else B.restrict
-        B.clear(v.handle)
-        for leaf in self._lts[3][v.handle]:
-            restrict(leaf.handle, v.handle)
-        B.post_restrict(v.handle)
-
-    def lts_iteration(self, end_time):
-        """One grid iteration with local time stepping: every leaf, coarse
-        levels first, advances one step of its own CFL size if it may
-        (SolveTimestep::enterCell, SolveTimestep.cpp:591-751, with
-        TimeIntervals::isAllowedToAdvanceInTime, TimeIntervals.cpp:24-78):
-        its step must not pass the global end time, and its current end
-        time must not exceed the time constraint (end of the current
-        interval) of any patch sharing a face or a corner with it
-        (isBlockedByNeighbors, TimeIntervals.cpp:80-84; a virtual patch
-        constrains by the minimal end of its leaves).  A leaf that advances
-        takes its ghost layers from its neighbours' current intervals (the
-        time factors of DefaultTransfer / DefaultInterpolation; a virtual
-        source is restricted afresh from its leaves), copies uNew to uOld,
-        fills its walls and solves with maximumTimestepSize = end time -
-        (time + timestep size).  Leaves are visited one after the other, so
-        a neighbour that advanced earlier in the iteration is seen with its
-        new interval.  Returns the number of leaves that advanced."""
-        B = self.b
+    def lts_advancing(self, end_time):
+        """The leaves that may advance now (TimeIntervals::
+        isAllowedToAdvanceInTime, TimeIntervals.cpp:24-78): the step must
+        not pass the global end time, and the leaf's current end time must
+        not exceed the time constraint (end of the current interval) of any
+        patch sharing a face or a corner with it (isBlockedByNeighbors,
+        TimeIntervals.cpp:80-84; a virtual patch constrains by the minimal
+        end of its leaves), with tarch's 1e-8 tolerance."""
         if getattr(self, "_lts", None) is None or self._lts_key is not self._plan:
             self._lts = self._lts_tables()
             self._lts_key = self._plan
-        faces, walls, nbrs, below = self._lts
-        interpolate = B.interpolate_pure if self.pure else B.interpolate
-        dirty = {v.handle for v in self.virtuals}
-        ancestors = {leaf.handle: anc for leaf, anc in self._restrictions}
-        advanced = 0
+        nbrs = self._lts[2]
+        end = {p.handle: sum(self.b.clock(p.handle)) for p in self.leaves}
+        vend = {v.handle: min(end[x.handle] for x in self._lts[3][v.handle])
+                for v in self.virtuals}
+        out = []
         for p in self.leaves:
-            t, dt = B.clock(p.handle)
-            max_dt = end_time - (t + dt)
-            if not (max_dt - 0.0 > LA_EPS):
+            e = end[p.handle]
+            if not ((end_time - e) - 0.0 > LA_EPS):
                 continue
-            constraint = min((sum(self.clock(q)) for q in nbrs[p.handle]), default=np.inf)
-            if (t + dt) - constraint > LA_EPS:
+            c = min((vend[q.handle] if q.virtual else end[q.handle] for q in nbrs[p.handle]),
+                    default=np.inf)
+            if e - c > LA_EPS:
                 continue
+            out.append(p)
+        return out
+
+    def lts_iteration(self, end_time):
+        """One grid iteration with local time stepping: every leaf that may
+        advance (`lts_advancing`, decided on the clocks at the start of the
+        iteration) takes one step of its own CFL size
+        (SolveTimestep::enterCell, SolveTimestep.cpp:591-751), phase by
+        phase over those leaves: restriction of all leaves into the virtual
+        patches; the ghost faces of the advancing leaves from their
+        neighbours' current intervals (the time factors of DefaultTransfer /
+        DefaultInterpolation; every neighbour is read before any leaf
+        changes); uNew -> uOld; walls; solveTimestep with
+        maximumTimestepSize = end time - (time + timestep size).  Returns
+        the number of leaves that advanced."""
+        B = self.b
+        adv = self.lts_advancing(end_time)
+        if not adv:
+            return 0
+        faces, walls = self._lts[0], self._lts[1]
+        for v in self.virtuals:
+            B.clear(v.handle)
+        restrict = B.restrict_pure if self.pure else B.restrict
+        for leaf, ancestors in self._restrictions:
+            for v in ancestors:
+                restrict(leaf.handle, v.handle)
+        for v in self.virtuals:
+            B.post_restrict(v.handle)
+        interpolate = B.interpolate_pure if self.pure else B.interpolate
+        for p in adv:
             for kind, src, dst, size, soff, doff in faces[p.handle]:
-                if src.virtual and src.handle in dirty:
-                    self._restrict_one(src)
-                    dirty.discard(src.handle)
                 if kind == "transfer":
                     B.transfer(size, soff, doff, src.handle, dst.handle)
                 else:
                     interpolate(size, doff, src.handle, dst.handle, 1, 0, 1)
+        for p in adv:
             B.copy_unew_to_uold(p.handle)
+        for p in adv:
             for dim, upper in walls[p.handle]:
                 B.fill_boundary(p.handle, dim, upper)
-            B.solve(p.handle, max_dt)
+        for p in adv:
+            t, dt = B.clock(p.handle)
+            B.solve(p.handle, end_time - (t + dt))
             p.age += 1
-            advanced += 1
-            for v in ancestors.get(p.handle, ()):
-                dirty.add(v.handle)
         self.steps += 1
-        return advanced
+        return len(adv)
 
     def run_lts(self, end_time, max_iterations=100000):
         """Grid iterations with local time stepping until every leaf has
@@ -660,6 +667,40 @@ class BatchedAmr:
         for p in self.grid.leaves:
             p.age += taken
         return dt.value
+
+    def lts_init(self):
+        """Switch the plan to local time stepping (pc_amr_lts_init): the
+        face and corner neighbours of every leaf from the grid's tables,
+        the leaves' clocks from their descriptors."""
+        grid = self.grid
+        if getattr(grid, "_lts", None) is None or grid._lts_key is not grid._plan:
+            grid._lts = grid._lts_tables()
+            grid._lts_key = grid._plan
+        nbrs = grid._lts[2]
+        ptr, idx = [0], []
+        for p in grid.leaves:
+            idx += [q.handle for q in nbrs[p.handle]]
+            ptr.append(len(idx))
+        check(self.lib.pc_amr_lts_init(self.h, iarr_list(ptr), iarr_list(idx)))
+
+    def lts_iterations(self, k, end_time):
+        """k grid iterations with local time stepping (the semantics of
+        AmrGrid.lts_iteration with pure=True); returns the leaf steps taken."""
+        n = ctypes.c_longlong()
+        check(self.lib.pc_amr_lts_step(self.h, int(k), float(end_time), ctypes.byref(n)))
+        return n.value
+
+    def run_lts(self, end_time, batch=16, max_iterations=1000000):
+        """Iterations in batches of `batch` until every leaf has reached
+        `end_time`; returns (iterations launched, leaf steps)."""
+        its = steps = 0
+        while its < max_iterations:
+            k = self.lts_iterations(batch, end_time)
+            its += batch
+            steps += k
+            if k == 0:
+                break
+        return its, steps
 
     def clock(self):
         """(time, timestep_size, steps) of the leaves."""

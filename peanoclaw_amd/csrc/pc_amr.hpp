@@ -56,6 +56,25 @@ __device__ __forceinline__ bool amr_advance(const AmrClock& c) {
   return maxdt - 0.0 > kLaEps;
 }
 
+// Local time stepping: per-slot clocks of the leaves (indexed by slot id).
+struct AmrLts {
+  double* t;        // time (start of the current interval)
+  double* dt;       // timestep size
+  int* adv;         // the leaf advances in this iteration
+  double* vend;     // virtual patch: minimal end time of the leaves below it
+  double* stepdt;   // the step an advancing leaf takes
+  int* iter;        // leaves advancing in this iteration
+  unsigned long long* steps;  // leaf steps taken
+};
+
+// Gate of a phase: global steps run while the shared clock may advance;
+// local time stepping while some leaf advances in this iteration.
+template <bool LTS>
+__device__ __forceinline__ bool amr_phase_runs(const AmrClock* clock, const AmrLts& L) {
+  if (LTS) return *L.iter > 0;
+  return amr_advance(*clock);
+}
+
 // 64 consecutive cells of one virtual patch
 struct AmrVBlock {
   int slot;   // the virtual patch
@@ -104,12 +123,14 @@ __device__ __forceinline__ void amr_restrict_factors(const AmrClock& c, double& 
 
 // One thread per virtual-patch cell; `cptr`/`cpairs` list, per cell, the
 // restriction pairs whose destination region holds it, in plan order.
+template <bool LTS>
 __global__ void __launch_bounds__(64)
 k_amr_restrict(const AmrVBlock* __restrict__ blocks, const int* __restrict__ cptr,
                const int* __restrict__ cpairs, const AmrRestrict* __restrict__ pairs,
-               const PatchDev* __restrict__ pd, Fields F, int cur, const AmrClock* clock) {
+               const PatchDev* __restrict__ pd, Fields F, int cur, const AmrClock* clock,
+               AmrLts L) {
   const AmrClock c = *clock;
-  if (!amr_advance(c)) return;
+  if (!amr_phase_runs<LTS>(clock, L)) return;
   const AmrVBlock B = blocks[blockIdx.x];
   const PatchDev D = pd[B.slot];
   const int k = B.cell0 + (int)threadIdx.x;
@@ -126,6 +147,13 @@ k_amr_restrict(const AmrVBlock* __restrict__ blocks, const int* __restrict__ cpt
     for (int j = cptr[B.base + k]; j < pe; ++j) {
       const AmrRestrict a = pairs[cpairs[j]];
       const PatchDev S = pd[a.src];
+      if (LTS) {
+        // each leaf restricts on its own interval
+        AmrClock cs = c;
+        cs.time = L.t[a.src];
+        cs.dt = L.dt[a.src];
+        amr_restrict_factors(cs, tfOld, tfNew);
+      }
       const double dp[2] = {a.dpos[0] + (double)di0 * a.dstSub[0],
                             a.dpos[1] + (double)di1 * a.dstSub[1]};
       int cOff[2], cSize[2];
@@ -178,10 +206,12 @@ k_amr_restrict(const AmrVBlock* __restrict__ blocks, const int* __restrict__ cpt
   }
 }
 
+template <bool LTS>
 __global__ void __launch_bounds__(256)
 k_amr_copy(const int* __restrict__ leaves, const PatchDev* __restrict__ pd, Fields F, int cur,
-           const AmrClock* clock) {
-  if (!amr_advance(*clock)) return;
+           const AmrClock* clock, AmrLts L) {
+  if (!amr_phase_runs<LTS>(clock, L)) return;
+  if (LTS && !L.adv[leaves[blockIdx.x]]) return;
   const PatchDev P = pd[leaves[blockIdx.x]];
   for (int k = threadIdx.x; k < P.nx * P.ny; k += blockDim.x) {
     const long long i = tile_index(P, k / P.ny, k - (k / P.ny) * P.ny);
@@ -192,26 +222,32 @@ k_amr_copy(const int* __restrict__ leaves, const PatchDev* __restrict__ pd, Fiel
 }
 
 // One wavefront per face entry (4 per workgroup).
+template <bool LTS>
 __global__ void __launch_bounds__(256)
 k_amr_ghosts(const AmrGhost* __restrict__ ghosts, int nghosts, const PatchDev* __restrict__ pd,
-             Fields F, int cur, const AmrClock* clock) {
+             Fields F, int cur, const AmrClock* clock, AmrLts L) {
   const AmrClock c = *clock;
-  if (!amr_advance(c)) return;
+  if (!amr_phase_runs<LTS>(clock, L)) return;
   const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (e >= nghosts) return;
   const int lane = threadIdx.x & 63;
   const AmrGhost g = ghosts[e];
+  if (LTS && !L.adv[g.dst]) return;
   const PatchDev S = pd[g.src], D = pd[g.dst];
   const int oldSt = 1 - cur;
-  // every leaf runs on the clock (time, dt): time_unew = time + dt
-  const double dstTimeUNew = c.time + c.dt;
+  // global steps: every leaf runs on the clock (time, dt); local time
+  // stepping: each leaf on its own
+  const double dTime = LTS ? L.t[g.dst] : c.time, dDt = LTS ? L.dt[g.dst] : c.dt;
+  const double sTime = (LTS && g.kind != kAmrTransferVirtual) ? L.t[g.src] : c.time;
+  const double sDt = (LTS && g.kind != kAmrTransferVirtual) ? L.dt[g.src] : c.dt;
+  const double dstTimeUNew = dTime + dDt;
   if (g.kind != kAmrInterpolate) {
     // DefaultTransfer.cpph:27-51
     double tf;
     if (g.kind == kAmrTransferVirtual) {
       tf = (dstTimeUNew - 0.0) / 1.0;
     } else {
-      const double sNew = c.time + c.dt, sOld = c.time;
+      const double sNew = sTime + sDt, sOld = sTime;
       tf = (sNew - sOld - 0.0 > kLaEps) ? (dstTimeUNew - sOld) / (sNew - sOld) : 1.0;
     }
     const double inv = 1.0 - tf;
@@ -227,10 +263,10 @@ k_amr_ghosts(const AmrGhost* __restrict__ ghosts, int nghosts, const PatchDev* _
   }
   // DefaultInterpolation.cpph:45-67 (toCurrentTime = 0, useTimeUNewOrUOld = 1)
   double tf;
-  if (fabs(c.dt - 0.0) <= kLaEps) {
+  if (fabs(sDt - 0.0) <= kLaEps) {
     tf = 1.0;
   } else {
-    const double sOld = c.time, sNew = c.time + c.dt;
+    const double sOld = sTime, sNew = sTime + sDt;
     tf = (dstTimeUNew - sOld) / (sNew - sOld);
   }
   tf = dmax(0.0, dmin(1.0, tf));
@@ -275,11 +311,14 @@ k_amr_ghosts(const AmrGhost* __restrict__ ghosts, int nghosts, const PatchDev* _
 
 // SWEKernel::fillBoundaryLayer (SWEKernel.cpp:176-237) of one leaf, its
 // walls in call order (corner cells read what the previous wall wrote).
+template <bool LTS>
 __global__ void __launch_bounds__(64)
 k_amr_walls(const AmrWalls* __restrict__ walls, const int* __restrict__ dims,
-            const PatchDev* __restrict__ pd, Fields F, int cur, const AmrClock* clock) {
-  if (!amr_advance(*clock)) return;
+            const PatchDev* __restrict__ pd, Fields F, int cur, const AmrClock* clock,
+            AmrLts L) {
+  if (!amr_phase_runs<LTS>(clock, L)) return;
   const AmrWalls W = walls[blockIdx.x];
+  if (LTS && !L.adv[W.slot]) return;
   const PatchDev P = pd[W.slot];
   const int st = 1 - cur;
   for (int w = 0; w < W.count; ++w) {
@@ -306,13 +345,13 @@ k_amr_walls(const AmrWalls* __restrict__ walls, const int* __restrict__ dims,
   }
 }
 
-template <int SOLVER>
+template <int SOLVER, bool LTS>
 __global__ void __launch_bounds__(64)
 k_amr_speed(const Task* __restrict__ tasks, const int* __restrict__ plist,
             const PatchDev* __restrict__ pd, Fields F, int cur, double* __restrict__ smax,
-            const AmrClock* clock) {
+            const AmrClock* clock, AmrLts T) {
   __shared__ MarchLds L;
-  if (!amr_advance(*clock)) return;
+  if (!amr_phase_runs<LTS>(clock, T)) return;
   const Task t = tasks[blockIdx.x];
   EdgeBufs none{{nullptr, nullptr}, nullptr};
   MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
@@ -350,11 +389,87 @@ k_amr_dt(const int* __restrict__ leaves, int nleaves, const PatchDev* __restrict
   }
 }
 
-template <int SOLVER>
+// Local time stepping: each advancing leaf's step, then its clock
+// (SWEKernel.cpp:60-68 dt = fmin(CFL step, maximumTimestepSize), :83-87).
+__global__ void __launch_bounds__(256)
+k_lts_dt(const int* __restrict__ leaves, int nleaves, const PatchDev* __restrict__ pd,
+         const double* __restrict__ smax, const AmrClock* clock, AmrLts L) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nleaves || *L.iter == 0) return;
+  const int s = leaves[k];
+  if (!L.adv[s]) return;
+  const PatchDev P = pd[s];
+  const double md = clock->end_time - (L.t[s] + L.dt[s]);
+  L.stepdt[s] = fmin(max_timestep(smax[s], P.dx, P.dy), md);
+}
+
+__global__ void __launch_bounds__(256)
+k_lts_clock(const int* __restrict__ leaves, int nleaves, AmrLts L) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nleaves || *L.iter == 0) return;
+  const int s = leaves[k];
+  if (!L.adv[s]) return;
+  const double dt = L.stepdt[s];
+  if (dt - 0.0 > kLaEps) {
+    L.t[s] = L.t[s] + L.dt[s];
+    L.dt[s] = dt;
+  }
+  atomicAdd(L.steps, 1ull);
+}
+
+// Minimal end time of the leaves below each virtual patch (vptr/vleaf: CSR
+// over the plan's virtual patches); resets the iteration's advance count.
+__global__ void __launch_bounds__(256)
+k_lts_vend(const int* __restrict__ virtuals, int nvirtuals, const int* __restrict__ vptr,
+           const int* __restrict__ vleaf, AmrLts L) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0) *L.iter = 0;
+  if (k >= nvirtuals) return;
+  double m = 1.7976931348623157e308;
+  for (int j = vptr[k]; j < vptr[k + 1]; ++j) {
+    const int s = vleaf[j];
+    m = dmin(m, L.t[s] + L.dt[s]);
+  }
+  L.vend[virtuals[k]] = m;
+}
+
+// TimeIntervals::isAllowedToAdvanceInTime (TimeIntervals.cpp:24-84) of each
+// leaf: the step bound end - (t + dt) > 1e-8 and t + dt - constraint <= 1e-8,
+// the constraint being the minimal end time over the neighbours listed in
+// nptr/nbr (>= 0: a leaf slot; < 0: -1 - the slot of a virtual patch).
+__global__ void __launch_bounds__(256)
+k_lts_flags(const int* __restrict__ leaves, int nleaves, const int* __restrict__ nptr,
+            const int* __restrict__ nbr, const AmrClock* clock, AmrLts L) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nleaves) return;
+  const int s = leaves[k];
+  const double e = L.t[s] + L.dt[s];
+  const double md = clock->end_time - e;
+  double c = 1.7976931348623157e308;
+  for (int j = nptr[k]; j < nptr[k + 1]; ++j) {
+    const int q = nbr[j];
+    c = dmin(c, q >= 0 ? L.t[q] + L.dt[q] : L.vend[-1 - q]);
+  }
+  const int adv = (md - 0.0 > kLaEps) && !(e - c > kLaEps);
+  L.adv[s] = adv;
+  if (adv) atomicAdd(L.iter, 1);
+}
+
+template <int SOLVER, bool LTS>
 __global__ void __launch_bounds__(64)
 k_amr_solve(const Task* __restrict__ tasks, const int* __restrict__ plist,
-            const PatchDev* __restrict__ pd, Fields F, int cur, AmrClock* clock) {
+            const PatchDev* __restrict__ pd, Fields F, int cur, AmrClock* clock, AmrLts T) {
   __shared__ MarchLds L;
+  if (LTS) {
+    if (*T.iter == 0) return;
+    const Task t = tasks[blockIdx.x];
+    EdgeBufs none{{nullptr, nullptr}, nullptr};
+    MarchArgs A{plist, pd, F, nullptr, none, cur, 0.0, nullptr, nullptr, 1.0};
+    A.seg_dt = T.stepdt;
+    A.seg_adv = T.adv;
+    march_task<kUpdateSwap, false, 0, SOLVER, true>(t, A, L);
+    return;
+  }
   // the dt phase decided; this launch's blocks read only advance and bound
   if (!clock->advance) return;
   const Task t = tasks[blockIdx.x];
@@ -394,6 +509,23 @@ struct pc_amr {
   double* d_smax = nullptr;
   AmrClock* d_clock = nullptr;
   double last_dt = 0.0;
+  bool shared_clock = true;  // all leaves on one clock (global steps allowed)
+  // local time stepping: per-slot clocks and flags, neighbour lists of the
+  // leaves, leaves below each virtual patch
+  std::vector<std::vector<int>> vleaves;  // per virtual patch, plan order
+  bool lts_ready = false;
+  size_t lts_ns = 0;  // slots covered by the per-slot arrays
+  AmrLts lts{};
+  int* d_virtuals = nullptr;
+  int* d_vptr = nullptr;
+  int* d_vleaf = nullptr;
+  int* d_nptr = nullptr;
+  int* d_nbr = nullptr;
+  hipGraph_t lgraph = nullptr;
+  hipGraphExec_t lgexec = nullptr;
+  const PatchDev* lg_pd = nullptr;
+  Fields lg_F{};
+  int lg_solver = -1, lg_cur = -1;
   // captured step and what it was captured with
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -426,6 +558,11 @@ void amr_free(pc_amr* a) {
   hipFree(a->d_clock);
   if (a->gexec) hipGraphExecDestroy(a->gexec);
   if (a->graph) hipGraphDestroy(a->graph);
+  void* lts[] = {a->lts.t, a->lts.dt, a->lts.adv, a->lts.vend, a->lts.stepdt, a->lts.iter,
+                 a->lts.steps, a->d_virtuals, a->d_vptr, a->d_vleaf, a->d_nptr, a->d_nbr};
+  for (void* q : lts) hipFree(q);
+  if (a->lgexec) hipGraphExecDestroy(a->lgexec);
+  if (a->lgraph) hipGraphDestroy(a->lgraph);
 }
 
 // The plan's slots must still be live with the same role and size.
@@ -493,10 +630,7 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
   for (int s : a->leaves) {
     double t, dt;
     slot_clock(p, s, &t, &dt);
-    if (t != t0 || dt != dt0) {
-      delete a;
-      return fail(PC_E_INVALID, "AMR leaves must share one clock");
-    }
+    if (t != t0 || dt != dt0) a->shared_clock = false;
   }
   auto bail = [&](const char* msg) {
     amr_free(a);
@@ -561,6 +695,8 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
       R.dstRegion = R.dstSub[0] * R.dstSub[1];
       pairs.push_back(R);
     }
+    a->vleaves.emplace_back();
+    for (int pi = pbeg; pi < (int)pairs.size(); ++pi) a->vleaves.back().push_back(pairs[pi].src);
     const int nx = dd.subdivision_factor[0], ny = dd.subdivision_factor[1];
     const int base = (int)cptr.size() - 1;
     for (int c0 = 0; c0 < nx * ny; c0 += 64) vblocks.push_back(AmrVBlock{v, c0, base, 0});
@@ -702,31 +838,76 @@ namespace {
 void amr_launch_step(pc_amr* a, Fields F, int cur) {
   pc_pool* p = a->pool;
   const int nl = (int)a->leaves.size();
+  const AmrLts none{};
   if (a->nvblocks > 0)
-    hipLaunchKernelGGL(k_amr_restrict, dim3(a->nvblocks), dim3(64), 0, p->stream, a->d_vblocks,
-                       a->d_cptr, a->d_cpairs, a->d_pairs, p->d_pd, F, cur, a->d_clock);
-  hipLaunchKernelGGL(k_amr_copy, dim3(nl), dim3(256), 0, p->stream, a->d_leaves, p->d_pd, F,
-                     cur, a->d_clock);
+    hipLaunchKernelGGL(k_amr_restrict<false>, dim3(a->nvblocks), dim3(64), 0, p->stream,
+                       a->d_vblocks, a->d_cptr, a->d_cpairs, a->d_pairs, p->d_pd, F, cur,
+                       a->d_clock, none);
+  hipLaunchKernelGGL(k_amr_copy<false>, dim3(nl), dim3(256), 0, p->stream, a->d_leaves, p->d_pd,
+                     F, cur, a->d_clock, none);
   if (a->nghosts > 0)
-    hipLaunchKernelGGL(k_amr_ghosts, dim3((a->nghosts + 3) / 4), dim3(256), 0, p->stream,
-                       a->d_ghosts, a->nghosts, p->d_pd, F, cur, a->d_clock);
+    hipLaunchKernelGGL(k_amr_ghosts<false>, dim3((a->nghosts + 3) / 4), dim3(256), 0, p->stream,
+                       a->d_ghosts, a->nghosts, p->d_pd, F, cur, a->d_clock, none);
   if (a->nwall_leaves > 0)
-    hipLaunchKernelGGL(k_amr_walls, dim3(a->nwall_leaves), dim3(64), 0, p->stream, a->d_walls,
-                       a->d_wall_dims, p->d_pd, F, cur, a->d_clock);
+    hipLaunchKernelGGL(k_amr_walls<false>, dim3(a->nwall_leaves), dim3(64), 0, p->stream,
+                       a->d_walls, a->d_wall_dims, p->d_pd, F, cur, a->d_clock, none);
   if (p->solver)
-    hipLaunchKernelGGL(k_amr_speed<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
-                       a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock);
+    hipLaunchKernelGGL((k_amr_speed<1, false>), dim3(a->ntasks), dim3(64), 0, p->stream,
+                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, none);
   else
-    hipLaunchKernelGGL(k_amr_speed<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
-                       a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock);
+    hipLaunchKernelGGL((k_amr_speed<0, false>), dim3(a->ntasks), dim3(64), 0, p->stream,
+                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, none);
   hipLaunchKernelGGL(k_amr_dt, dim3(1), dim3(256), 0, p->stream, a->d_leaves, nl, p->d_pd,
                      a->d_smax, a->d_clock);
   if (p->solver)
-    hipLaunchKernelGGL(k_amr_solve<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
-                       a->d_plist, p->d_pd, F, cur, a->d_clock);
+    hipLaunchKernelGGL((k_amr_solve<1, false>), dim3(a->ntasks), dim3(64), 0, p->stream,
+                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_clock, none);
   else
-    hipLaunchKernelGGL(k_amr_solve<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
-                       a->d_plist, p->d_pd, F, cur, a->d_clock);
+    hipLaunchKernelGGL((k_amr_solve<0, false>), dim3(a->ntasks), dim3(64), 0, p->stream,
+                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_clock, none);
+}
+
+// One grid iteration with local time stepping: advance flags from the
+// clocks at its start, restriction of every leaf on its own interval, the
+// ghost faces of the advancing leaves (every neighbour read before any leaf
+// changes), uNew -> uOld, walls, CFL speeds, each leaf's step, update, clocks.
+void amr_launch_lts(pc_amr* a, Fields F, int cur) {
+  pc_pool* p = a->pool;
+  const int nl = (int)a->leaves.size();
+  const int nv = (int)a->virtuals.size();
+  const AmrLts& L = a->lts;
+  hipLaunchKernelGGL(k_lts_vend, dim3(std::max(1, (nv + 255) / 256)), dim3(256), 0, p->stream,
+                     a->d_virtuals, nv, a->d_vptr, a->d_vleaf, L);
+  hipLaunchKernelGGL(k_lts_flags, dim3((nl + 255) / 256), dim3(256), 0, p->stream, a->d_leaves,
+                     nl, a->d_nptr, a->d_nbr, a->d_clock, L);
+  if (a->nvblocks > 0)
+    hipLaunchKernelGGL(k_amr_restrict<true>, dim3(a->nvblocks), dim3(64), 0, p->stream,
+                       a->d_vblocks, a->d_cptr, a->d_cpairs, a->d_pairs, p->d_pd, F, cur,
+                       a->d_clock, L);
+  if (a->nghosts > 0)
+    hipLaunchKernelGGL(k_amr_ghosts<true>, dim3((a->nghosts + 3) / 4), dim3(256), 0, p->stream,
+                       a->d_ghosts, a->nghosts, p->d_pd, F, cur, a->d_clock, L);
+  hipLaunchKernelGGL(k_amr_copy<true>, dim3(nl), dim3(256), 0, p->stream, a->d_leaves, p->d_pd,
+                     F, cur, a->d_clock, L);
+  if (a->nwall_leaves > 0)
+    hipLaunchKernelGGL(k_amr_walls<true>, dim3(a->nwall_leaves), dim3(64), 0, p->stream,
+                       a->d_walls, a->d_wall_dims, p->d_pd, F, cur, a->d_clock, L);
+  if (p->solver)
+    hipLaunchKernelGGL((k_amr_speed<1, true>), dim3(a->ntasks), dim3(64), 0, p->stream,
+                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, L);
+  else
+    hipLaunchKernelGGL((k_amr_speed<0, true>), dim3(a->ntasks), dim3(64), 0, p->stream,
+                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, L);
+  hipLaunchKernelGGL(k_lts_dt, dim3((nl + 255) / 256), dim3(256), 0, p->stream, a->d_leaves, nl,
+                     p->d_pd, a->d_smax, a->d_clock, L);
+  if (p->solver)
+    hipLaunchKernelGGL((k_amr_solve<1, true>), dim3(a->ntasks), dim3(64), 0, p->stream,
+                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_clock, L);
+  else
+    hipLaunchKernelGGL((k_amr_solve<0, true>), dim3(a->ntasks), dim3(64), 0, p->stream,
+                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_clock, L);
+  hipLaunchKernelGGL(k_lts_clock, dim3((nl + 255) / 256), dim3(256), 0, p->stream, a->d_leaves,
+                     nl, L);
 }
 
 bool same_fields(const Fields& x, const Fields& y) {
@@ -760,12 +941,130 @@ int amr_graph(pc_amr* a, Fields F, int cur) {
   return PC_OK;
 }
 
+int amr_lts_graph(pc_amr* a, Fields F, int cur) {
+  pc_pool* p = a->pool;
+  if (a->lgexec && a->lg_pd == p->d_pd && same_fields(a->lg_F, F) &&
+      a->lg_solver == p->solver && a->lg_cur == cur)
+    return PC_OK;
+  if (a->lgexec) HIPCHK(hipGraphExecDestroy(a->lgexec));
+  if (a->lgraph) HIPCHK(hipGraphDestroy(a->lgraph));
+  a->lgexec = nullptr;
+  a->lgraph = nullptr;
+  HIPCHK(hipStreamSynchronize(p->stream));
+  HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+  amr_launch_lts(a, F, cur);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(p->stream, &g);
+  if (e != hipSuccess) return fail(PC_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+  a->lgraph = g;
+  HIPCHK(hipGraphInstantiate(&a->lgexec, g, nullptr, nullptr, 0));
+  a->lg_pd = p->d_pd;
+  a->lg_F = F;
+  a->lg_solver = p->solver;
+  a->lg_cur = cur;
+  return PC_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
+int pc_amr_lts_init(pc_amr* a, const int32_t* nptr, const int32_t* nbr) {
+  if (!a || !nptr || (nptr[a->leaves.size()] > 0 && !nbr))
+    return fail(PC_E_INVALID, "invalid argument");
+  pc_pool* p = a->pool;
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  CHK(amr_check_slots(a));
+  const int nl = (int)a->leaves.size();
+  std::vector<char> role(p->slots.size(), 0);
+  for (int s : a->leaves) role[s] = 1;
+  for (int s : a->virtuals) role[s] = 2;
+  std::vector<int> np(nptr, nptr + nl + 1), nb;
+  if (np[0] != 0) return fail(PC_E_INVALID, "neighbour lists must start at 0");
+  for (int k = 0; k < nl; ++k) {
+    if (np[k + 1] < np[k]) return fail(PC_E_INVALID, "neighbour list offsets must not decrease");
+    for (int j = np[k]; j < np[k + 1]; ++j) {
+      const int q = nbr[j];
+      if (q < 0 || q >= (int)role.size() || !role[q])
+        return fail(PC_E_INVALID, "a neighbour is not a patch of the plan");
+      nb.push_back(role[q] == 1 ? q : -1 - q);
+    }
+  }
+  std::vector<int> vptr{0}, vleaf;
+  for (const auto& v : a->vleaves) {
+    vleaf.insert(vleaf.end(), v.begin(), v.end());
+    vptr.push_back((int)vleaf.size());
+  }
+  const size_t ns = p->slots.size();
+  std::vector<double> t(ns, 0.0), dt(ns, 0.0);
+  for (int s : a->leaves) slot_clock(p, s, &t[s], &dt[s]);
+  void* old[] = {a->lts.t, a->lts.dt, a->lts.adv, a->lts.vend, a->lts.stepdt, a->lts.iter,
+                 a->lts.steps, a->d_virtuals, a->d_vptr, a->d_vleaf, a->d_nptr, a->d_nbr};
+  for (void* q : old) hipFree(q);
+  a->lts = AmrLts{};
+  a->d_virtuals = a->d_vptr = a->d_vleaf = a->d_nptr = a->d_nbr = nullptr;
+  if (a->lgexec) hipGraphExecDestroy(a->lgexec);
+  if (a->lgraph) hipGraphDestroy(a->lgraph);
+  a->lgexec = nullptr;
+  a->lgraph = nullptr;
+  CHK(amr_upload(&a->lts.t, t));
+  CHK(amr_upload(&a->lts.dt, dt));
+  CHK(amr_upload(&a->lts.adv, std::vector<int>(ns, 0)));
+  CHK(amr_upload(&a->lts.vend, std::vector<double>(ns, 0.0)));
+  CHK(amr_upload(&a->lts.stepdt, std::vector<double>(ns, 0.0)));
+  CHK(amr_upload(&a->lts.iter, std::vector<int>(1, 0)));
+  CHK(amr_upload(&a->lts.steps, std::vector<unsigned long long>(1, 0)));
+  CHK(amr_upload(&a->d_virtuals, a->virtuals));
+  CHK(amr_upload(&a->d_vptr, vptr));
+  CHK(amr_upload(&a->d_vleaf, vleaf));
+  CHK(amr_upload(&a->d_nptr, np));
+  CHK(amr_upload(&a->d_nbr, nb));
+  a->lts_ns = ns;
+  a->lts_ready = true;
+  return PC_OK;
+}
+
+int pc_amr_lts_step(pc_amr* a, int k, double end_time, long long* leaf_steps) {
+  if (!a || k < 0) return fail(PC_E_INVALID, "invalid argument");
+  if (!a->lts_ready) return fail(PC_E_INVALID, "pc_amr_lts_init has not been called");
+  pc_pool* p = a->pool;
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  CHK(amr_check_slots(a));
+  int cur;
+  CHK(amr_cur(a, &cur));
+  mark_state_changed(p);
+  a->shared_clock = false;
+  Fields F = fields_of(p);
+  CHK(amr_lts_graph(a, F, cur));
+  HIPCHK(hipMemcpyAsync(&a->d_clock->end_time, &end_time, sizeof(double),
+                        hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipMemsetAsync(a->lts.steps, 0, sizeof(unsigned long long), p->stream));
+  for (int it = 0; it < k; ++it) HIPCHK(hipGraphLaunch(a->lgexec, p->stream));
+  unsigned long long n = 0;
+  const size_t ns = a->lts_ns;
+  std::vector<double> t(ns), dt(ns);
+  HIPCHK(hipMemcpyAsync(&n, a->lts.steps, sizeof(n), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipMemcpyAsync(t.data(), a->lts.t, ns * sizeof(double), hipMemcpyDeviceToHost,
+                        p->stream));
+  HIPCHK(hipMemcpyAsync(dt.data(), a->lts.dt, ns * sizeof(double), hipMemcpyDeviceToHost,
+                        p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  // the leaves' descriptors follow their device clocks
+  for (int s : a->leaves) {
+    p->slots[s].desc.time = t[s];
+    p->slots[s].desc.timestep_size = dt[s];
+    p->slots[s].clock_from_scenario = false;
+  }
+  if (leaf_steps) *leaf_steps = (long long)n;
+  return PC_OK;
+}
+
 int pc_amr_step(pc_amr* a, int k, double end_time, double* last_dt) {
   if (!a || k < 0) return fail(PC_E_INVALID, "invalid argument");
+  if (!a->shared_clock)
+    return fail(PC_E_INVALID, "global steps need every leaf of the plan on one clock");
   pc_pool* p = a->pool;
   hipSetDevice(p->device);
   CHK(prepare(p));

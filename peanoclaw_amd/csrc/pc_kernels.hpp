@@ -194,11 +194,16 @@ struct MarchArgs {
   double* smax;       // kSpeed: full per-patch max; fused: next-state interior max
   const ScnDev* scn;  // pruned mode: the scenario's last CFL step (threshold)
   double slack;       // pruned mode: threshold slack
+  // per-patch steps (PERSEG): the step of patch p is seg_dt[p], and only
+  // patches with seg_adv[p] != 0 are updated (local time stepping)
+  const double* seg_dt = nullptr;
+  const int* seg_adv = nullptr;
 };
 
 // FUSE: 0 = no next-state speeds, 1 = exact speeds of every interior edge of
 // the new state, 2 = pruned (exact speeds only for bound candidates).
-template <int MODE, bool SIDES, int FUSE, int SOLVER>
+// PERSEG: every patch of the task has its own step and advance flag.
+template <int MODE, bool SIDES, int FUSE, int SOLVER, bool PERSEG = false>
 __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, MarchLds& L) {
   constexpr bool FUSED = FUSE != 0;
   constexpr bool PRUNE = FUSE == 2;
@@ -213,8 +218,8 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   const int seg = lane / W;
   const int col = lane - seg * W + 1;  // 1-based interior column
   const bool segok = seg < t.nseg;
-  const bool active = segok && (col <= ny);
   const int p = gl(A.plist + t.first + (segok ? seg : 0));
+  const bool active = segok && (col <= ny) && (!PERSEG || gl(A.seg_adv + p) != 0);
   const PatchDev P = gl(A.pd + p);
   const int stride = ny + 2;
   const int par = A.par;
@@ -300,7 +305,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
 
   const long long base = P.off + col;
   const double dxinv = P.dxinv, dyinv = P.dyinv;
-  const double dt = A.dt;
+  const double dt = PERSEG ? gl(A.seg_dt + p) : A.dt;
   double smaxl = 0.0;
   // pruned mode: speed threshold of this lane's patch (0 = every edge exact)
   float thr = 0.0f;

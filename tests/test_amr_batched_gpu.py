@@ -263,3 +263,46 @@ def test_regrid_breaking_dam_500_steps():
         amr.close()
     finally:
         pool.close()
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_batched_local_time_stepping_bitwise(levels):
+    """Local time stepping as a batched device plan (pc_amr_lts_*: advance
+    flags, restriction on each leaf's interval, ghosts with time factors,
+    per-leaf steps, ten launches per iteration replayed from a graph)
+    against the oracle's AmrGrid.lts_iteration with the same (pure)
+    semantics: advancing leaves and clocks equal every iteration, states
+    bitwise."""
+    from peanoclaw_amd.amr import AmrGrid, BatchedAmr, DeviceBackend, dam_break_fields
+    from peanoclaw_amd.pool import PatchPool
+    fn = dam_break_fields(centre=(0.5, 0.5), radius=0.2, hills=[(0.3, 0.6, 0.1, 0.15)])
+
+    def refine(lev, ij):
+        return (lev == 1 and abs(ij[0] - 2) <= 1 and abs(ij[1] - 2) <= 1) or (
+            lev == 2 and ij == (7, 7))
+    pool = PatchPool(0)
+    try:
+        dev = AmrGrid(DeviceBackend(pool), 5, 5, 8, refine, fn, max_level=levels)
+        ora = AmrGrid(OracleBackend(), 5, 5, 8, refine, fn, max_level=levels, pure=True)
+        amr = BatchedAmr(dev)
+        amr.lts_init()
+        end = 0.006
+        its = steps = 0
+        while True:
+            k1 = amr.lts_iterations(1, end)
+            k0 = ora.lts_iteration(end)
+            assert k1 == k0, f"leaves advanced differ in iteration {its}"
+            for p, q in zip(dev.leaves, ora.leaves):
+                assert dev.b.clock(p.handle) == ora.b.clock(q.handle), f"clock, iteration {its}"
+            its += 1
+            steps += k1
+            if its in (1, 10) or k1 == 0:
+                same, total = _equal(dev, ora)
+                assert same == total, f"iteration {its}: bitwise {same}/{total}"
+            if k1 == 0:
+                break
+        print(f"batched LTS {levels}-level, {len(dev.leaves)} leaves: {its} iterations, "
+              f"{steps} leaf steps to t={end}; bitwise {same}/{total}")
+        amr.close()
+    finally:
+        pool.close()
