@@ -61,7 +61,6 @@ struct AmrLts {
   double* t;        // time (start of the current interval)
   double* dt;       // timestep size
   int* adv;         // the leaf advances in this iteration
-  double* vend;     // virtual patch: minimal end time of the leaves below it
   double* stepdt;   // the step an advancing leaf takes
   int* iter;        // leaves advancing in this iteration
   unsigned long long* steps;  // leaf steps taken
@@ -206,12 +205,10 @@ k_amr_restrict(const AmrVBlock* __restrict__ blocks, const int* __restrict__ cpt
   }
 }
 
-template <bool LTS>
 __global__ void __launch_bounds__(256)
 k_amr_copy(const int* __restrict__ leaves, const PatchDev* __restrict__ pd, Fields F, int cur,
-           const AmrClock* clock, AmrLts L) {
-  if (!amr_phase_runs<LTS>(clock, L)) return;
-  if (LTS && !L.adv[leaves[blockIdx.x]]) return;
+           const AmrClock* clock) {
+  if (!amr_advance(*clock)) return;
   const PatchDev P = pd[leaves[blockIdx.x]];
   for (int k = threadIdx.x; k < P.nx * P.ny; k += blockDim.x) {
     const long long i = tile_index(P, k / P.ny, k - (k / P.ny) * P.ny);
@@ -311,14 +308,11 @@ k_amr_ghosts(const AmrGhost* __restrict__ ghosts, int nghosts, const PatchDev* _
 
 // SWEKernel::fillBoundaryLayer (SWEKernel.cpp:176-237) of one leaf, its
 // walls in call order (corner cells read what the previous wall wrote).
-template <bool LTS>
 __global__ void __launch_bounds__(64)
 k_amr_walls(const AmrWalls* __restrict__ walls, const int* __restrict__ dims,
-            const PatchDev* __restrict__ pd, Fields F, int cur, const AmrClock* clock,
-            AmrLts L) {
-  if (!amr_phase_runs<LTS>(clock, L)) return;
+            const PatchDev* __restrict__ pd, Fields F, int cur, const AmrClock* clock) {
+  if (!amr_advance(*clock)) return;
   const AmrWalls W = walls[blockIdx.x];
-  if (LTS && !L.adv[W.slot]) return;
   const PatchDev P = pd[W.slot];
   const int st = 1 - cur;
   for (int w = 0; w < W.count; ++w) {
@@ -345,13 +339,13 @@ k_amr_walls(const AmrWalls* __restrict__ walls, const int* __restrict__ dims,
   }
 }
 
-template <int SOLVER, bool LTS>
+template <int SOLVER>
 __global__ void __launch_bounds__(64)
 k_amr_speed(const Task* __restrict__ tasks, const int* __restrict__ plist,
             const PatchDev* __restrict__ pd, Fields F, int cur, double* __restrict__ smax,
-            const AmrClock* clock, AmrLts T) {
+            const AmrClock* clock) {
   __shared__ MarchLds L;
-  if (!amr_phase_runs<LTS>(clock, T)) return;
+  if (!amr_advance(*clock)) return;
   const Task t = tasks[blockIdx.x];
   EdgeBufs none{{nullptr, nullptr}, nullptr};
   MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
@@ -389,57 +383,18 @@ k_amr_dt(const int* __restrict__ leaves, int nleaves, const PatchDev* __restrict
   }
 }
 
-// Local time stepping: each advancing leaf's step, then its clock
-// (SWEKernel.cpp:60-68 dt = fmin(CFL step, maximumTimestepSize), :83-87).
-__global__ void __launch_bounds__(256)
-k_lts_dt(const int* __restrict__ leaves, int nleaves, const PatchDev* __restrict__ pd,
-         const double* __restrict__ smax, const AmrClock* clock, AmrLts L) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nleaves || *L.iter == 0) return;
-  const int s = leaves[k];
-  if (!L.adv[s]) return;
-  const PatchDev P = pd[s];
-  const double md = clock->end_time - (L.t[s] + L.dt[s]);
-  L.stepdt[s] = fmin(max_timestep(smax[s], P.dx, P.dy), md);
-}
-
-__global__ void __launch_bounds__(256)
-k_lts_clock(const int* __restrict__ leaves, int nleaves, AmrLts L) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nleaves || *L.iter == 0) return;
-  const int s = leaves[k];
-  if (!L.adv[s]) return;
-  const double dt = L.stepdt[s];
-  if (dt - 0.0 > kLaEps) {
-    L.t[s] = L.t[s] + L.dt[s];
-    L.dt[s] = dt;
-  }
-  atomicAdd(L.steps, 1ull);
-}
-
-// Minimal end time of the leaves below each virtual patch (vptr/vleaf: CSR
-// over the plan's virtual patches); resets the iteration's advance count.
-__global__ void __launch_bounds__(256)
-k_lts_vend(const int* __restrict__ virtuals, int nvirtuals, const int* __restrict__ vptr,
-           const int* __restrict__ vleaf, AmrLts L) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k == 0) *L.iter = 0;
-  if (k >= nvirtuals) return;
-  double m = 1.7976931348623157e308;
-  for (int j = vptr[k]; j < vptr[k + 1]; ++j) {
-    const int s = vleaf[j];
-    m = dmin(m, L.t[s] + L.dt[s]);
-  }
-  L.vend[virtuals[k]] = m;
-}
-
 // TimeIntervals::isAllowedToAdvanceInTime (TimeIntervals.cpp:24-84) of each
 // leaf: the step bound end - (t + dt) > 1e-8 and t + dt - constraint <= 1e-8,
 // the constraint being the minimal end time over the neighbours listed in
-// nptr/nbr (>= 0: a leaf slot; < 0: -1 - the slot of a virtual patch).
+// nptr/nbr (>= 0: a leaf slot; < 0: -1 - the slot of a virtual patch, whose
+// end time is the minimum over the leaves below it: vleaf[vptr[v]..vptr[v+1])
+// with v = vidx[slot]).  Counts the advancing leaves into *iter (zeroed by
+// the last launch of the previous iteration).
 __global__ void __launch_bounds__(256)
 k_lts_flags(const int* __restrict__ leaves, int nleaves, const int* __restrict__ nptr,
-            const int* __restrict__ nbr, const AmrClock* clock, AmrLts L) {
+            const int* __restrict__ nbr, const int* __restrict__ vptr,
+            const int* __restrict__ vleaf, const int* __restrict__ vidx,
+            const AmrClock* clock, AmrLts L) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nleaves) return;
   const int s = leaves[k];
@@ -448,28 +403,120 @@ k_lts_flags(const int* __restrict__ leaves, int nleaves, const int* __restrict__
   double c = 1.7976931348623157e308;
   for (int j = nptr[k]; j < nptr[k + 1]; ++j) {
     const int q = nbr[j];
-    c = dmin(c, q >= 0 ? L.t[q] + L.dt[q] : L.vend[-1 - q]);
+    if (q >= 0) {
+      c = dmin(c, L.t[q] + L.dt[q]);
+    } else {
+      const int v = vidx[-1 - q];
+      double m = 1.7976931348623157e308;
+      for (int i = vptr[v]; i < vptr[v + 1]; ++i) {
+        const int x = vleaf[i];
+        m = dmin(m, L.t[x] + L.dt[x]);
+      }
+      c = dmin(c, m);
+    }
   }
   const int adv = (md - 0.0 > kLaEps) && !(e - c > kLaEps);
   L.adv[s] = adv;
   if (adv) atomicAdd(L.iter, 1);
 }
 
-template <int SOLVER, bool LTS>
+// uNew -> uOld and the walls (in call order) of one advancing leaf
+// (DefaultTransfer::copyUNewToUOld; SWEKernel::fillBoundaryLayer,
+// SWEKernel.cpp:176-237); widx[slot] = its entry in `walls`, -1 for none.
+__global__ void __launch_bounds__(256)
+k_lts_prep(const int* __restrict__ leaves, const int* __restrict__ widx,
+           const AmrWalls* __restrict__ walls, const int* __restrict__ dims,
+           const PatchDev* __restrict__ pd, Fields F, int cur, AmrLts L) {
+  if (*L.iter == 0) return;
+  const int s = leaves[blockIdx.x];
+  if (!L.adv[s]) return;
+  const PatchDev P = pd[s];
+  for (int k = threadIdx.x; k < P.nx * P.ny; k += blockDim.x) {
+    const long long i = tile_index(P, k / P.ny, k - (k / P.ny) * P.ny);
+    F.H(1 - cur)[i] = F.H(cur)[i];
+    F.HU(1 - cur)[i] = F.HU(cur)[i];
+    F.HV(1 - cur)[i] = F.HV(cur)[i];
+  }
+  const int w = widx[s];
+  if (w < 0) return;
+  __syncthreads();
+  const AmrWalls W = walls[w];
+  const int st = 1 - cur;
+  for (int q = 0; q < W.count; ++q) {
+    const int dim = dims[2 * (W.first + q)], upper = dims[2 * (W.first + q) + 1];
+    const int n_other = dim == 0 ? P.ny : P.nx;
+    for (int k = -1 + (int)threadIdx.x; k < n_other + 1; k += blockDim.x) {
+      int dx, dy, sx, sy;
+      if (dim == 0) {
+        dx = upper ? P.nx : -1;
+        sx = dx + (upper ? -1 : 1);
+        dy = sy = k;
+      } else {
+        dy = upper ? P.ny : -1;
+        sy = dy + (upper ? -1 : 1);
+        dx = sx = k;
+      }
+      const long long d = tile_index(P, dx, dy), sidx = tile_index(P, sx, sy);
+      const double h = F.H(st)[sidx], hu = F.HU(st)[sidx], hv = F.HV(st)[sidx];
+      F.H(st)[d] = h;
+      F.HU(st)[d] = dim == 0 ? -hu : hu;
+      F.HV(st)[d] = dim == 1 ? -hv : hv;
+    }
+    __syncthreads();
+  }
+}
+
+// One task of leaves: CFL speeds from uOld with its ghost layer, each
+// advancing leaf's step dt = fmin(CFL step, end - (t + dt))
+// (SWEKernel.cpp:60-68), the update of the advancing leaves with their own
+// steps, then their clocks (:83-87).  Zeroes *iter for the next iteration
+// (no block of this launch reads it).
+template <int SOLVER>
+__global__ void __launch_bounds__(64)
+k_lts_task(const Task* __restrict__ tasks, const int* __restrict__ plist,
+           const PatchDev* __restrict__ pd, Fields F, int cur, double* __restrict__ smax,
+           const AmrClock* clock, AmrLts T) {
+  __shared__ MarchLds L;
+  const Task t = tasks[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int W = pow2ceil(t.ny);
+  const int seg = lane / W;
+  const int col = lane - seg * W + 1;
+  const bool segok = seg < t.nseg;
+  const int p = plist[t.first + (segok ? seg : 0)];
+  const bool lead = segok && col == 1 && T.adv[p] != 0;
+  if (__any(lead ? 1 : 0)) {
+    EdgeBufs none{{nullptr, nullptr}, nullptr};
+    MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
+    march_task<kSpeed, false, 0, SOLVER>(t, A, L);
+    __syncthreads();
+    if (lead) {
+      const PatchDev P = pd[p];
+      const double md = clock->end_time - (T.t[p] + T.dt[p]);
+      T.stepdt[p] = fmin(max_timestep(smax[p], P.dx, P.dy), md);
+    }
+    __syncthreads();
+    MarchArgs B{plist, pd, F, nullptr, none, cur, 0.0, nullptr, nullptr, 1.0};
+    B.seg_dt = T.stepdt;
+    B.seg_adv = T.adv;
+    march_task<kUpdateSwap, false, 0, SOLVER, true>(t, B, L);
+    if (lead) {
+      const double dt = T.stepdt[p];
+      if (dt - 0.0 > kLaEps) {
+        T.t[p] = T.t[p] + T.dt[p];
+        T.dt[p] = dt;
+      }
+      atomicAdd(T.steps, 1ull);
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *T.iter = 0;
+}
+
+template <int SOLVER>
 __global__ void __launch_bounds__(64)
 k_amr_solve(const Task* __restrict__ tasks, const int* __restrict__ plist,
-            const PatchDev* __restrict__ pd, Fields F, int cur, AmrClock* clock, AmrLts T) {
+            const PatchDev* __restrict__ pd, Fields F, int cur, AmrClock* clock) {
   __shared__ MarchLds L;
-  if (LTS) {
-    if (*T.iter == 0) return;
-    const Task t = tasks[blockIdx.x];
-    EdgeBufs none{{nullptr, nullptr}, nullptr};
-    MarchArgs A{plist, pd, F, nullptr, none, cur, 0.0, nullptr, nullptr, 1.0};
-    A.seg_dt = T.stepdt;
-    A.seg_adv = T.adv;
-    march_task<kUpdateSwap, false, 0, SOLVER, true>(t, A, L);
-    return;
-  }
   // the dt phase decided; this launch's blocks read only advance and bound
   if (!clock->advance) return;
   const Task t = tasks[blockIdx.x];
@@ -516,7 +563,9 @@ struct pc_amr {
   bool lts_ready = false;
   size_t lts_ns = 0;  // slots covered by the per-slot arrays
   AmrLts lts{};
-  int* d_virtuals = nullptr;
+  int* d_vidx = nullptr;   // per slot: index of the virtual patch, -1 otherwise
+  int* d_widx = nullptr;   // per slot: entry of the leaf's walls, -1 for none
+  std::vector<AmrWalls> h_walls;
   int* d_vptr = nullptr;
   int* d_vleaf = nullptr;
   int* d_nptr = nullptr;
@@ -558,8 +607,8 @@ void amr_free(pc_amr* a) {
   hipFree(a->d_clock);
   if (a->gexec) hipGraphExecDestroy(a->gexec);
   if (a->graph) hipGraphDestroy(a->graph);
-  void* lts[] = {a->lts.t, a->lts.dt, a->lts.adv, a->lts.vend, a->lts.stepdt, a->lts.iter,
-                 a->lts.steps, a->d_virtuals, a->d_vptr, a->d_vleaf, a->d_nptr, a->d_nbr};
+  void* lts[] = {a->lts.t,    a->lts.dt, a->lts.adv, a->lts.stepdt, a->lts.iter, a->lts.steps,
+                 a->d_vidx,   a->d_widx, a->d_vptr,  a->d_vleaf,   a->d_nptr,   a->d_nbr};
   for (void* q : lts) hipFree(q);
   if (a->lgexec) hipGraphExecDestroy(a->lgexec);
   if (a->lgraph) hipGraphDestroy(a->lgraph);
@@ -807,6 +856,7 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
     wl.push_back(W);
   }
   a->nwall_leaves = (int)wl.size();
+  a->h_walls = wl;
   AmrClock c{};
   c.time = t0;
   c.dt = dt0;
@@ -838,48 +888,46 @@ namespace {
 void amr_launch_step(pc_amr* a, Fields F, int cur) {
   pc_pool* p = a->pool;
   const int nl = (int)a->leaves.size();
-  const AmrLts none{};
+  const AmrLts none{};  // unused by the global phases
   if (a->nvblocks > 0)
     hipLaunchKernelGGL(k_amr_restrict<false>, dim3(a->nvblocks), dim3(64), 0, p->stream,
                        a->d_vblocks, a->d_cptr, a->d_cpairs, a->d_pairs, p->d_pd, F, cur,
                        a->d_clock, none);
-  hipLaunchKernelGGL(k_amr_copy<false>, dim3(nl), dim3(256), 0, p->stream, a->d_leaves, p->d_pd,
-                     F, cur, a->d_clock, none);
+  hipLaunchKernelGGL(k_amr_copy, dim3(nl), dim3(256), 0, p->stream, a->d_leaves, p->d_pd, F,
+                     cur, a->d_clock);
   if (a->nghosts > 0)
     hipLaunchKernelGGL(k_amr_ghosts<false>, dim3((a->nghosts + 3) / 4), dim3(256), 0, p->stream,
                        a->d_ghosts, a->nghosts, p->d_pd, F, cur, a->d_clock, none);
   if (a->nwall_leaves > 0)
-    hipLaunchKernelGGL(k_amr_walls<false>, dim3(a->nwall_leaves), dim3(64), 0, p->stream,
-                       a->d_walls, a->d_wall_dims, p->d_pd, F, cur, a->d_clock, none);
+    hipLaunchKernelGGL(k_amr_walls, dim3(a->nwall_leaves), dim3(64), 0, p->stream, a->d_walls,
+                       a->d_wall_dims, p->d_pd, F, cur, a->d_clock);
   if (p->solver)
-    hipLaunchKernelGGL((k_amr_speed<1, false>), dim3(a->ntasks), dim3(64), 0, p->stream,
-                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, none);
+    hipLaunchKernelGGL(k_amr_speed<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock);
   else
-    hipLaunchKernelGGL((k_amr_speed<0, false>), dim3(a->ntasks), dim3(64), 0, p->stream,
-                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, none);
+    hipLaunchKernelGGL(k_amr_speed<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock);
   hipLaunchKernelGGL(k_amr_dt, dim3(1), dim3(256), 0, p->stream, a->d_leaves, nl, p->d_pd,
                      a->d_smax, a->d_clock);
   if (p->solver)
-    hipLaunchKernelGGL((k_amr_solve<1, false>), dim3(a->ntasks), dim3(64), 0, p->stream,
-                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_clock, none);
+    hipLaunchKernelGGL(k_amr_solve<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_clock);
   else
-    hipLaunchKernelGGL((k_amr_solve<0, false>), dim3(a->ntasks), dim3(64), 0, p->stream,
-                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_clock, none);
+    hipLaunchKernelGGL(k_amr_solve<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_clock);
 }
 
-// One grid iteration with local time stepping: advance flags from the
-// clocks at its start, restriction of every leaf on its own interval, the
-// ghost faces of the advancing leaves (every neighbour read before any leaf
-// changes), uNew -> uOld, walls, CFL speeds, each leaf's step, update, clocks.
+// One grid iteration with local time stepping, five launches: advance flags
+// from the clocks at its start; restriction of every leaf on its own
+// interval; the ghost faces of the advancing leaves (every neighbour read
+// before any leaf changes); uNew -> uOld and walls; per task CFL speeds,
+// steps, update and clocks.
 void amr_launch_lts(pc_amr* a, Fields F, int cur) {
   pc_pool* p = a->pool;
   const int nl = (int)a->leaves.size();
-  const int nv = (int)a->virtuals.size();
   const AmrLts& L = a->lts;
-  hipLaunchKernelGGL(k_lts_vend, dim3(std::max(1, (nv + 255) / 256)), dim3(256), 0, p->stream,
-                     a->d_virtuals, nv, a->d_vptr, a->d_vleaf, L);
   hipLaunchKernelGGL(k_lts_flags, dim3((nl + 255) / 256), dim3(256), 0, p->stream, a->d_leaves,
-                     nl, a->d_nptr, a->d_nbr, a->d_clock, L);
+                     nl, a->d_nptr, a->d_nbr, a->d_vptr, a->d_vleaf, a->d_vidx, a->d_clock, L);
   if (a->nvblocks > 0)
     hipLaunchKernelGGL(k_amr_restrict<true>, dim3(a->nvblocks), dim3(64), 0, p->stream,
                        a->d_vblocks, a->d_cptr, a->d_cpairs, a->d_pairs, p->d_pd, F, cur,
@@ -887,27 +935,14 @@ void amr_launch_lts(pc_amr* a, Fields F, int cur) {
   if (a->nghosts > 0)
     hipLaunchKernelGGL(k_amr_ghosts<true>, dim3((a->nghosts + 3) / 4), dim3(256), 0, p->stream,
                        a->d_ghosts, a->nghosts, p->d_pd, F, cur, a->d_clock, L);
-  hipLaunchKernelGGL(k_amr_copy<true>, dim3(nl), dim3(256), 0, p->stream, a->d_leaves, p->d_pd,
-                     F, cur, a->d_clock, L);
-  if (a->nwall_leaves > 0)
-    hipLaunchKernelGGL(k_amr_walls<true>, dim3(a->nwall_leaves), dim3(64), 0, p->stream,
-                       a->d_walls, a->d_wall_dims, p->d_pd, F, cur, a->d_clock, L);
+  hipLaunchKernelGGL(k_lts_prep, dim3(nl), dim3(256), 0, p->stream, a->d_leaves, a->d_widx,
+                     a->d_walls, a->d_wall_dims, p->d_pd, F, cur, L);
   if (p->solver)
-    hipLaunchKernelGGL((k_amr_speed<1, true>), dim3(a->ntasks), dim3(64), 0, p->stream,
-                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, L);
+    hipLaunchKernelGGL(k_lts_task<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, L);
   else
-    hipLaunchKernelGGL((k_amr_speed<0, true>), dim3(a->ntasks), dim3(64), 0, p->stream,
-                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, L);
-  hipLaunchKernelGGL(k_lts_dt, dim3((nl + 255) / 256), dim3(256), 0, p->stream, a->d_leaves, nl,
-                     p->d_pd, a->d_smax, a->d_clock, L);
-  if (p->solver)
-    hipLaunchKernelGGL((k_amr_solve<1, true>), dim3(a->ntasks), dim3(64), 0, p->stream,
-                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_clock, L);
-  else
-    hipLaunchKernelGGL((k_amr_solve<0, true>), dim3(a->ntasks), dim3(64), 0, p->stream,
-                       a->d_tasks, a->d_plist, p->d_pd, F, cur, a->d_clock, L);
-  hipLaunchKernelGGL(k_lts_clock, dim3((nl + 255) / 256), dim3(256), 0, p->stream, a->d_leaves,
-                     nl, L);
+    hipLaunchKernelGGL(k_lts_task<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, L);
 }
 
 bool same_fields(const Fields& x, const Fields& y) {
@@ -999,11 +1034,11 @@ int pc_amr_lts_init(pc_amr* a, const int32_t* nptr, const int32_t* nbr) {
   const size_t ns = p->slots.size();
   std::vector<double> t(ns, 0.0), dt(ns, 0.0);
   for (int s : a->leaves) slot_clock(p, s, &t[s], &dt[s]);
-  void* old[] = {a->lts.t, a->lts.dt, a->lts.adv, a->lts.vend, a->lts.stepdt, a->lts.iter,
-                 a->lts.steps, a->d_virtuals, a->d_vptr, a->d_vleaf, a->d_nptr, a->d_nbr};
+  void* old[] = {a->lts.t,    a->lts.dt, a->lts.adv, a->lts.stepdt, a->lts.iter, a->lts.steps,
+                 a->d_vidx,   a->d_widx, a->d_vptr,  a->d_vleaf,   a->d_nptr,   a->d_nbr};
   for (void* q : old) hipFree(q);
   a->lts = AmrLts{};
-  a->d_virtuals = a->d_vptr = a->d_vleaf = a->d_nptr = a->d_nbr = nullptr;
+  a->d_vidx = a->d_widx = a->d_vptr = a->d_vleaf = a->d_nptr = a->d_nbr = nullptr;
   if (a->lgexec) hipGraphExecDestroy(a->lgexec);
   if (a->lgraph) hipGraphDestroy(a->lgraph);
   a->lgexec = nullptr;
@@ -1011,11 +1046,14 @@ int pc_amr_lts_init(pc_amr* a, const int32_t* nptr, const int32_t* nbr) {
   CHK(amr_upload(&a->lts.t, t));
   CHK(amr_upload(&a->lts.dt, dt));
   CHK(amr_upload(&a->lts.adv, std::vector<int>(ns, 0)));
-  CHK(amr_upload(&a->lts.vend, std::vector<double>(ns, 0.0)));
   CHK(amr_upload(&a->lts.stepdt, std::vector<double>(ns, 0.0)));
   CHK(amr_upload(&a->lts.iter, std::vector<int>(1, 0)));
   CHK(amr_upload(&a->lts.steps, std::vector<unsigned long long>(1, 0)));
-  CHK(amr_upload(&a->d_virtuals, a->virtuals));
+  std::vector<int> vidx(ns, -1), widx(ns, -1);
+  for (size_t v = 0; v < a->virtuals.size(); ++v) vidx[a->virtuals[v]] = (int)v;
+  for (size_t w = 0; w < a->h_walls.size(); ++w) widx[a->h_walls[w].slot] = (int)w;
+  CHK(amr_upload(&a->d_vidx, vidx));
+  CHK(amr_upload(&a->d_widx, widx));
   CHK(amr_upload(&a->d_vptr, vptr));
   CHK(amr_upload(&a->d_vleaf, vleaf));
   CHK(amr_upload(&a->d_nptr, np));

@@ -269,7 +269,7 @@ def test_regrid_breaking_dam_500_steps():
 def test_batched_local_time_stepping_bitwise(levels):
     """Local time stepping as a batched device plan (pc_amr_lts_*: advance
     flags, restriction on each leaf's interval, ghosts with time factors,
-    per-leaf steps, ten launches per iteration replayed from a graph)
+    per-leaf steps, five launches per iteration replayed from a graph)
     against the oracle's AmrGrid.lts_iteration with the same (pure)
     semantics: advancing leaves and clocks equal every iteration, states
     bitwise."""
