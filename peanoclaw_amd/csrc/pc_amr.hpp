@@ -64,6 +64,8 @@ struct AmrLts {
   double* stepdt;   // the step an advancing leaf takes
   int* iter;        // leaves advancing in this iteration
   unsigned long long* steps;  // leaf steps taken
+  int* itn;         // iteration number
+  int* need;        // virtual patch: iteration in which an advancing leaf reads it
 };
 
 // Gate of a phase: global steps run while the shared clock may advance;
@@ -131,6 +133,8 @@ k_amr_restrict(const AmrVBlock* __restrict__ blocks, const int* __restrict__ cpt
   const AmrClock c = *clock;
   if (!amr_phase_runs<LTS>(clock, L)) return;
   const AmrVBlock B = blocks[blockIdx.x];
+  // local time stepping: only the virtual patches an advancing leaf reads
+  if (LTS && L.need[B.slot] != *L.itn) return;
   const PatchDev D = pd[B.slot];
   const int k = B.cell0 + (int)threadIdx.x;
   if (k >= D.nx * D.ny) return;
@@ -389,11 +393,13 @@ k_amr_dt(const int* __restrict__ leaves, int nleaves, const PatchDev* __restrict
 // nptr/nbr (>= 0: a leaf slot; < 0: -1 - the slot of a virtual patch, whose
 // end time is the minimum over the leaves below it: vleaf[vptr[v]..vptr[v+1])
 // with v = vidx[slot]).  Counts the advancing leaves into *iter (zeroed by
-// the last launch of the previous iteration).
+// the last launch of the previous iteration) and stamps the virtual patches
+// they read with the iteration number.
 __global__ void __launch_bounds__(256)
 k_lts_flags(const int* __restrict__ leaves, int nleaves, const int* __restrict__ nptr,
             const int* __restrict__ nbr, const int* __restrict__ vptr,
             const int* __restrict__ vleaf, const int* __restrict__ vidx,
+            const int* __restrict__ rptr, const int* __restrict__ rsrc,
             const AmrClock* clock, AmrLts L) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nleaves) return;
@@ -417,7 +423,12 @@ k_lts_flags(const int* __restrict__ leaves, int nleaves, const int* __restrict__
   }
   const int adv = (md - 0.0 > kLaEps) && !(e - c > kLaEps);
   L.adv[s] = adv;
-  if (adv) atomicAdd(L.iter, 1);
+  if (adv) {
+    atomicAdd(L.iter, 1);
+    // the virtual patches its ghost faces read (rsrc[rptr[k] .. rptr[k+1]))
+    const int stamp = *L.itn;
+    for (int j = rptr[k]; j < rptr[k + 1]; ++j) L.need[rsrc[j]] = stamp;
+  }
 }
 
 // uNew -> uOld and the walls (in call order) of one advancing leaf
@@ -509,7 +520,10 @@ k_lts_task(const Task* __restrict__ tasks, const int* __restrict__ plist,
       atomicAdd(T.steps, 1ull);
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *T.iter = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *T.iter = 0;
+    *T.itn = *T.itn + 1;
+  }
 }
 
 template <int SOLVER>
@@ -564,6 +578,9 @@ struct pc_amr {
   size_t lts_ns = 0;  // slots covered by the per-slot arrays
   AmrLts lts{};
   int* d_vidx = nullptr;   // per slot: index of the virtual patch, -1 otherwise
+  int* d_rptr = nullptr;   // per leaf (plan order): the virtual patches its faces read
+  int* d_rsrc = nullptr;
+  std::vector<AmrGhost> h_ghosts;
   int* d_widx = nullptr;   // per slot: entry of the leaf's walls, -1 for none
   std::vector<AmrWalls> h_walls;
   int* d_vptr = nullptr;
@@ -607,8 +624,9 @@ void amr_free(pc_amr* a) {
   hipFree(a->d_clock);
   if (a->gexec) hipGraphExecDestroy(a->gexec);
   if (a->graph) hipGraphDestroy(a->graph);
-  void* lts[] = {a->lts.t,    a->lts.dt, a->lts.adv, a->lts.stepdt, a->lts.iter, a->lts.steps,
-                 a->d_vidx,   a->d_widx, a->d_vptr,  a->d_vleaf,   a->d_nptr,   a->d_nbr};
+  void* lts[] = {a->lts.t,    a->lts.dt,   a->lts.adv, a->lts.stepdt, a->lts.iter, a->lts.steps,
+                 a->lts.itn,  a->lts.need, a->d_vidx,  a->d_widx,     a->d_vptr,   a->d_vleaf,
+                 a->d_nptr,   a->d_nbr,    a->d_rptr,  a->d_rsrc};
   for (void* q : lts) hipFree(q);
   if (a->lgexec) hipGraphExecDestroy(a->lgexec);
   if (a->lgraph) hipGraphDestroy(a->lgraph);
@@ -837,6 +855,7 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
     gh.push_back(g);
   }
   a->nghosts = (int)gh.size();
+  a->h_ghosts = gh;
   // walls grouped per leaf in call order
   std::vector<AmrWalls> wl;
   std::vector<int> wdims;
@@ -927,7 +946,8 @@ void amr_launch_lts(pc_amr* a, Fields F, int cur) {
   const int nl = (int)a->leaves.size();
   const AmrLts& L = a->lts;
   hipLaunchKernelGGL(k_lts_flags, dim3((nl + 255) / 256), dim3(256), 0, p->stream, a->d_leaves,
-                     nl, a->d_nptr, a->d_nbr, a->d_vptr, a->d_vleaf, a->d_vidx, a->d_clock, L);
+                     nl, a->d_nptr, a->d_nbr, a->d_vptr, a->d_vleaf, a->d_vidx, a->d_rptr,
+                     a->d_rsrc, a->d_clock, L);
   if (a->nvblocks > 0)
     hipLaunchKernelGGL(k_amr_restrict<true>, dim3(a->nvblocks), dim3(64), 0, p->stream,
                        a->d_vblocks, a->d_cptr, a->d_cpairs, a->d_pairs, p->d_pd, F, cur,
@@ -1034,11 +1054,13 @@ int pc_amr_lts_init(pc_amr* a, const int32_t* nptr, const int32_t* nbr) {
   const size_t ns = p->slots.size();
   std::vector<double> t(ns, 0.0), dt(ns, 0.0);
   for (int s : a->leaves) slot_clock(p, s, &t[s], &dt[s]);
-  void* old[] = {a->lts.t,    a->lts.dt, a->lts.adv, a->lts.stepdt, a->lts.iter, a->lts.steps,
-                 a->d_vidx,   a->d_widx, a->d_vptr,  a->d_vleaf,   a->d_nptr,   a->d_nbr};
+  void* old[] = {a->lts.t,    a->lts.dt,   a->lts.adv, a->lts.stepdt, a->lts.iter, a->lts.steps,
+                 a->lts.itn,  a->lts.need, a->d_vidx,  a->d_widx,     a->d_vptr,   a->d_vleaf,
+                 a->d_nptr,   a->d_nbr,    a->d_rptr,  a->d_rsrc};
   for (void* q : old) hipFree(q);
   a->lts = AmrLts{};
   a->d_vidx = a->d_widx = a->d_vptr = a->d_vleaf = a->d_nptr = a->d_nbr = nullptr;
+  a->d_rptr = a->d_rsrc = nullptr;
   if (a->lgexec) hipGraphExecDestroy(a->lgexec);
   if (a->lgraph) hipGraphDestroy(a->lgraph);
   a->lgexec = nullptr;
@@ -1052,6 +1074,20 @@ int pc_amr_lts_init(pc_amr* a, const int32_t* nptr, const int32_t* nbr) {
   std::vector<int> vidx(ns, -1), widx(ns, -1);
   for (size_t v = 0; v < a->virtuals.size(); ++v) vidx[a->virtuals[v]] = (int)v;
   for (size_t w = 0; w < a->h_walls.size(); ++w) widx[a->h_walls[w].slot] = (int)w;
+  // per leaf the virtual patches its ghost faces read
+  std::vector<int> lidx(ns, -1), rptr{0}, rsrc;
+  for (int k = 0; k < nl; ++k) lidx[a->leaves[k]] = k;
+  std::vector<std::vector<int>> reads(nl);
+  for (const AmrGhost& g : a->h_ghosts)
+    if (g.kind == kAmrTransferVirtual) reads[lidx[g.dst]].push_back(g.src);
+  for (int k = 0; k < nl; ++k) {
+    rsrc.insert(rsrc.end(), reads[k].begin(), reads[k].end());
+    rptr.push_back((int)rsrc.size());
+  }
+  CHK(amr_upload(&a->d_rptr, rptr));
+  CHK(amr_upload(&a->d_rsrc, rsrc));
+  CHK(amr_upload(&a->lts.itn, std::vector<int>(1, 1)));
+  CHK(amr_upload(&a->lts.need, std::vector<int>(ns, 0)));
   CHK(amr_upload(&a->d_vidx, vidx));
   CHK(amr_upload(&a->d_widx, widx));
   CHK(amr_upload(&a->d_vptr, vptr));
