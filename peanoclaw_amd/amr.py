@@ -107,7 +107,7 @@ class AmrGrid:
         return (np.ascontiguousarray(h, dtype=np.float64),
                 np.ascontiguousarray(bb, dtype=np.float64))
 
-    def _rebuild(self):
+    def _rebuild(self, changed=None):
         """Children, leaves, virtual patches, restriction and ghost plans
         from the current tree."""
         for lev in sorted(self.levels):
@@ -130,7 +130,7 @@ class AmrGrid:
         for i in range(self.px):
             for j in range(self.py):
                 walk(self.levels[1][(i, j)], [])
-        self._plan = self._ghost_plan()
+        self._plan = self._ghost_plan(changed)
 
     def regrid(self, threshold=0.05, minimal_width=None, maximal_width=None):
         """One remeshing pass on the current state with the BreakingDam
@@ -165,7 +165,10 @@ class AmrGrid:
         coarsen = [v for v in self.virtuals
                    if all(not c.virtual and flag.get(c.handle) == 0 and c.age >= 5
                           for c in v.children)]
+        changed = set()
         for v in coarsen:
+            changed.add(v.handle)
+            changed.update(c.handle for c in v.children)
             B.clear(v.handle)
             for c in v.children:
                 B.restrict(c.handle, v.handle)
@@ -189,6 +192,7 @@ class AmrGrid:
                     ch = AmrPatch(B.new_patch(pos, sf, n, lev, False, self.time, self.dt), lev,
                                   pos, sf, n, False, ij)
                     created.append((p, ch))
+                    changed.add(ch.handle)
                     self.levels.setdefault(lev, {})[ij] = ch
         for p, ch in created:
             B.set_b(ch.handle, self._fields(ch)[1])
@@ -198,8 +202,9 @@ class AmrGrid:
         for p in refine:
             B.set_virtual(p.handle, True)
             p.virtual = True
+            changed.add(p.handle)
         if refine or coarsen:
-            self._rebuild()
+            self._rebuild(changed)
         return len(refine), len(coarsen)
 
     @property
@@ -241,37 +246,46 @@ class AmrGrid:
                 return ("interp", self.levels[lev][q])
         raise RuntimeError("face region not covered")
 
-    def _ghost_plan(self):
+    def _ghost_plan(self, changed=None):
+        """Face entries and walls of every leaf.  With `changed` (handles of
+        the patches a regrid created, refined, coarsened or released), only
+        the leaves that are new or whose face sources changed are looked up
+        again; the others keep their entries."""
         n = self.n
         # region geometry per face: (size, source offset, destination offset)
         geom = {(0, 0): ((1, n), (n - 1, 0), (-1, 0)), (0, 1): ((1, n), (0, 0), (n, 0)),
                 (1, 0): ((n, 1), (0, n - 1), (0, -1)), (1, 1): ((n, 1), (0, 0), (0, n))}
-        plan = []
-        walls = []
         face = self._face
         wall_rank = [FACES.index(f) for f in WALL_ORDER]
         steps = [(-1, 0), (1, 0), (0, -1), (0, 1)]  # FACES order
+        old = getattr(self, "_face_cache", {}) if changed is not None else {}
+        cache = {}
+        plan = []
+        walls = []
         for p in self.leaves:
-            # same-level neighbours by direct lookup, the rest by _face
-            lv = self.levels[p.level]
-            k = 3 ** (p.level - 1)
-            nx_, ny_ = self.px * k, self.py * k
-            i, j = p.ij
-            res = []
-            for (di, dj), (dim, upper) in zip(steps, FACES):
-                qi, qj = i + di, j + dj
-                if not (0 <= qi < nx_ and 0 <= qj < ny_):
-                    res.append(("wall", None))
-                    continue
-                same = lv.get((qi, qj))
-                res.append(("transfer", same) if same is not None else face(p, dim, upper))
-            for f, (kind, src) in zip(FACES, res):
-                if kind != "wall":
-                    size, soff, doff = geom[f]
-                    plan.append((kind, src, p, size, soff, doff))
-            for k in wall_rank:
-                if res[k][0] == "wall":
-                    walls.append((p, FACES[k][0], FACES[k][1]))
+            c = old.get(p.handle)
+            if c is None or c[0] is not p or p.handle in changed or not c[3].isdisjoint(changed):
+                # same-level neighbours by direct lookup, the rest by _face
+                lv = self.levels[p.level]
+                k = 3 ** (p.level - 1)
+                nx_, ny_ = self.px * k, self.py * k
+                i, j = p.ij
+                res = []
+                for (di, dj), (dim, upper) in zip(steps, FACES):
+                    qi, qj = i + di, j + dj
+                    if not (0 <= qi < nx_ and 0 <= qj < ny_):
+                        res.append(("wall", None))
+                        continue
+                    same = lv.get((qi, qj))
+                    res.append(("transfer", same) if same is not None else face(p, dim, upper))
+                pe = [(kind, src, p) + geom[f] for f, (kind, src) in zip(FACES, res)
+                      if kind != "wall"]
+                we = [(p, FACES[k][0], FACES[k][1]) for k in wall_rank if res[k][0] == "wall"]
+                c = (p, pe, we, {e[1].handle for e in pe})
+            cache[p.handle] = c
+            plan.extend(c[1])
+            walls.extend(c[2])
+        self._face_cache = cache
         return plan, walls
 
     def step(self, end_time=np.inf):
