@@ -54,5 +54,27 @@ def build(verbose=False, out=None, defines=()):
     return lib
 
 
+REPLAY_SRC = os.path.join(ROOT, "tests", "cpp", "numerics_replay.cpp")
+REPLAY = os.path.join(ROOT, "tests", "cpp", "numerics_replay")
+
+
+def build_replay(verbose=False):
+    """Compile the C++ Numerics-adapter replay program of the tests (host
+    code only, g++, linked against the library built by `build`)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-Werror",
+           "-I", os.path.join(ROOT, "include"), "-o", REPLAY + ".tmp", REPLAY_SRC,
+           "-L", PKG, "-lpeanoclaw_amd", "-Wl,-rpath,$ORIGIN/../../peanoclaw_amd",
+           "-Wl,-rpath-link,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed for " + REPLAY_SRC)
+    os.replace(REPLAY + ".tmp", REPLAY)
+    return REPLAY
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_replay(verbose=True))
