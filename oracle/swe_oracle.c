@@ -1266,10 +1266,11 @@ double or_pool_step(or_pool *pl, double maxDt, int nthreads) {
  * the domain walls in the order left, top, right, bottom
  * (SolveTimestep.cpp:87-142). */
 /* Same, for a px x py grid whose patch (i, j) has pool offset
- * off[slot_of[i*py+j]] (slot_of < 0: hole; sides facing a hole are walls). */
-void or_grid_fill(int gpx, int gpy, int n, const int *slot_of,
-                  const long long *off, double *h, double *hu, double *hv) {
-  int s = n + 2;
+ * off[slot_of[i*py+j]] (slot_of < 0: hole; sides facing a hole are walls),
+ * every patch of nx x ny cells. */
+void or_grid_fill_rect(int gpx, int gpy, int nx, int ny, const int *slot_of,
+                       const long long *off, double *h, double *hu, double *hv) {
+  int s = ny + 2;
   for (int P = 0; P < gpx * gpy; P++) {
     if (slot_of[P] < 0) continue;
     int px = P / gpy, py = P % gpy;
@@ -1285,12 +1286,12 @@ void or_grid_fill(int gpx, int gpy, int n, const int *slot_of,
         const double *SH = h + off[q];
         const double *SHU = hu + off[q];
         const double *SHV = hv + off[q];
-        int x0 = di < 0 ? 0 : (di > 0 ? n + 1 : 1), x1 = di ? x0 : n;
-        int y0 = dj < 0 ? 0 : (dj > 0 ? n + 1 : 1), y1 = dj ? y0 : n;
+        int x0 = di < 0 ? 0 : (di > 0 ? nx + 1 : 1), x1 = di ? x0 : nx;
+        int y0 = dj < 0 ? 0 : (dj > 0 ? ny + 1 : 1), y1 = dj ? y0 : ny;
         for (int x = x0; x <= x1; x++)
           for (int y = y0; y <= y1; y++) {
-            int sx = di < 0 ? n : (di > 0 ? 1 : x);
-            int sy = dj < 0 ? n : (dj > 0 ? 1 : y);
+            int sx = di < 0 ? nx : (di > 0 ? 1 : x);
+            int sy = dj < 0 ? ny : (dj > 0 ? 1 : y);
             H[x * s + y] = SH[sx * s + sy];
             HU[x * s + y] = SHU[sx * s + sy];
             HV[x * s + y] = SHV[sx * s + sy];
@@ -1303,12 +1304,13 @@ void or_grid_fill(int gpx, int gpy, int n, const int *slot_of,
       int onwall = dim == 0 ? OR_AT(px + (up ? 1 : -1), py) < 0
                             : OR_AT(px, py + (up ? 1 : -1)) < 0;
       if (!onwall) continue;
-      for (int k = 0; k < n + 2; k++) {
+      int len = dim == 0 ? ny + 2 : nx + 2;
+      for (int k = 0; k < len; k++) {
         int dx_, dy_, sx, sy;
         if (dim == 0) {
-          dx_ = up ? n + 1 : 0; sx = up ? n : 1; dy_ = sy = k;
+          dx_ = up ? nx + 1 : 0; sx = up ? nx : 1; dy_ = sy = k;
         } else {
-          dy_ = up ? n + 1 : 0; sy = up ? n : 1; dx_ = sx = k;
+          dy_ = up ? ny + 1 : 0; sy = up ? ny : 1; dx_ = sx = k;
         }
         H[dx_ * s + dy_] = H[sx * s + sy];
         double hu = HU[sx * s + sy], hv = HV[sx * s + sy];
@@ -1318,6 +1320,11 @@ void or_grid_fill(int gpx, int gpy, int n, const int *slot_of,
     }
 #undef OR_AT
   }
+}
+
+void or_grid_fill(int gpx, int gpy, int n, const int *slot_of,
+                  const long long *off, double *h, double *hu, double *hv) {
+  or_grid_fill_rect(gpx, gpy, n, n, slot_of, off, h, hu, hv);
 }
 
 void or_grid_fill_ghosts(or_grid *gr) {

@@ -178,6 +178,8 @@ typedef struct {
 void or_grid_fill_ghosts(or_grid *gr);
 void or_grid_fill(int px, int py, int n, const int *slot_of,
                   const long long *off, double *h, double *hu, double *hv);
+void or_grid_fill_rect(int px, int py, int nx, int ny, const int *slot_of,
+                       const long long *off, double *h, double *hu, double *hv);
 double or_grid_step(or_grid *gr, double maxDt, double *out_dtcfl);
 double or_grid_max_speed(const or_grid *gr);
 

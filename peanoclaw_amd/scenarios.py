@@ -25,10 +25,12 @@ SIZES = (8, 16, 32, 64)
 
 
 class GridLayout:
-    """Slots of one equal-size patch grid: slot_of[i, j] (-1 = hole)."""
+    """Slots of one equal-size patch grid: slot_of[i, j] (-1 = hole); its
+    patches have n x ny cells (ny = n unless given)."""
 
-    def __init__(self, n, slot_of):
+    def __init__(self, n, slot_of, ny=None):
         self.n = n
+        self.ny = n if ny is None else ny
         self.slot_of = slot_of
 
 

@@ -134,6 +134,8 @@ def lib():
             "or_refinement_decision": (ctypes.c_int, [_dp, _dp]),
             "or_grid_fill": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
                              ctypes.POINTER(ctypes.c_longlong), _dp, _dp, _dp]),
+            "or_grid_fill_rect": (None, [ctypes.c_int] * 4 + [_ip,
+                                  ctypes.POINTER(ctypes.c_longlong), _dp, _dp, _dp]),
             "or_pool_fill_ghosts": (None, [ctypes.POINTER(OrPool),
                                     ctypes.POINTER(OrGhostEntry),
                                     ctypes.c_longlong]),

@@ -211,20 +211,23 @@ class OracleRun:
         nslots = max(wl.slots) + 1
         self.off = np.zeros(nslots, dtype=np.int64)
         nx = np.zeros(nslots, dtype=np.int32)
+        ny = np.zeros(nslots, dtype=np.int32)
         dx = np.zeros(nslots)
+        dy = np.zeros(nslots)
         for s in wl.slots:
             self.off[s] = pool.offset(s)
             d = pool.description(s)
-            nx[s] = d.subdivision_factor[0]
+            nx[s], ny[s] = d.subdivision_factor[0], d.subdivision_factor[1]
             dx[s] = d.size[0] / d.subdivision_factor[0]
-        self.nx, self.dx = nx, dx
+            dy[s] = d.size[1] / d.subdivision_factor[1]
+        self.nx, self.ny, self.dx, self.dy = nx, ny, dx, dy
         self.pl = OrPool()
         self.pl.np = nslots
         self.pl.nx = nx.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
-        self.pl.ny = nx.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        self.pl.ny = ny.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
         self.pl.off = self.off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
         self.pl.dx = dx.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-        self.pl.dy = dx.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        self.pl.dy = dy.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         for k in range(2):
             self.pl.h[k] = odptr(self.H[k])
             self.pl.hu[k] = odptr(self.HU[k])
@@ -236,20 +239,20 @@ class OracleRun:
         c = self.pl.cur
         for L in self.wl.layouts:
             so = np.ascontiguousarray(L.slot_of, dtype=np.int32)
-            olib().or_grid_fill(so.shape[0], so.shape[1], L.n,
-                                so.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                self.off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
-                                odptr(self.H[c]), odptr(self.HU[c]), odptr(self.HV[c]))
+            olib().or_grid_fill_rect(so.shape[0], so.shape[1], L.n, L.ny,
+                                     so.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                     self.off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                                     odptr(self.H[c]), odptr(self.HU[c]), odptr(self.HV[c]))
 
     def step(self):
         self.fill()
         return olib().or_pool_step(ctypes.byref(self.pl), math.inf, 8)
 
     def interior(self, pool, slot):
-        n = self.nx[slot]
+        nx, ny = self.nx[slot], self.ny[slot]
         o = self.off[slot]
         c = self.pl.cur
-        t = lambda a: a[o:o + (n + 2) ** 2].reshape(n + 2, n + 2)[1:-1, 1:-1]
+        t = lambda a: a[o:o + (nx + 2) * (ny + 2)].reshape(nx + 2, ny + 2)[1:-1, 1:-1]
         return np.stack([t(self.H[c]), t(self.HU[c]), t(self.HV[c])])
 
 
@@ -354,6 +357,83 @@ def test_uneven_patches_parity_and_forms(mode):
         orun.step()
     worst, _, _ = _compare(pool, orun, wl.slots)
     assert worst <= REL_STEP
+    pool.close()
+
+
+# (nx, ny, grid px, grid py): the smallest and largest sizes, both aspect
+# ratios, odd sizes, a lone patch; every wavefront packing (W = 2 ... 64)
+RAGGED = [(2, 2, 5, 4), (2, 64, 3, 2), (64, 2, 2, 3), (5, 3, 4, 4), (13, 7, 3, 5),
+          (64, 64, 2, 2), (33, 17, 2, 3), (9, 41, 1, 1), (7, 7, 3, 3)]
+
+
+def _ragged_workload(pool, seed=11):
+    """Grids of rectangular patches (dx = dy = 1/64, one scenario, global
+    dt); the 7 x 7 grid has a hole.  h from smooth noise in [0.6, 1.8],
+    hu, hv ~ U[-0.1, 0.1], b = two Gaussian hills at every cell centre
+    including the ghost ring."""
+    from peanoclaw_amd.pool import make_description
+    from peanoclaw_amd.scenarios import GridLayout, Workload
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / 64
+    layouts, slots, cells = [], [], 0
+    x0 = 0.0
+    for nx, ny, gpx, gpy in RAGGED:
+        slot_of = -np.ones((gpx, gpy), dtype=np.int32)
+        hills = [(rng.uniform(0, gpx * nx * dx), rng.uniform(0, gpy * ny * dx),
+                  rng.uniform(0.05, 0.2), rng.uniform(0.0, 0.3)) for _ in range(2)]
+        for i in range(gpx):
+            for j in range(gpy):
+                if (nx, ny) == (7, 7) and (i, j) == (1, 1):
+                    continue  # hole: its neighbours see walls
+                pos = (x0 + i * nx * dx, j * ny * dx)
+                s = pool.register(make_description(pos, (nx * dx, ny * dx), nx, ny=ny))
+                slot_of[i, j] = s
+                X = pos[0] + (np.arange(-1, nx + 1)[:, None] + 0.5) * dx
+                Y = pos[1] + (np.arange(-1, ny + 1)[None, :] + 0.5) * dx
+                X, Y = np.broadcast_arrays(X, Y)
+                b = np.zeros(X.shape)
+                for hx_, hy_, sg, a in hills:
+                    b = b + a * np.exp(-((X - hx_) ** 2 + (Y - hy_) ** 2) / (sg * sg))
+                unew = np.empty((3, nx, ny))
+                unew[0] = 1.2 + 0.3 * np.sin(7 * X[1:-1, 1:-1]) * np.cos(5 * Y[1:-1, 1:-1]) \
+                    + 0.3 * rng.uniform(-1, 1, (nx, ny))
+                unew[1:] = rng.uniform(-0.1, 0.1, (2, nx, ny))
+                pool.upload(s, unew=unew, b=b)
+                slots.append(s)
+                cells += nx * ny
+        layouts.append(GridLayout(nx, slot_of, ny=ny))
+        x0 += gpx * nx * dx + 0.25
+    pool.scenario_set(0, 0.0, 0.0, np.inf)
+    return Workload(pool, layouts, 0, cells, slots)
+
+
+@pytest.mark.parametrize("mode", ["strips", "sides", "sides_fused", "sides_pruned",
+                                  "sides_pruned_fallback"])
+def test_ragged_patch_grids_bitwise(mode):
+    """Rectangular patches of every size class from 2 to 64 cells per side
+    (grids, a lone patch and a grid with a hole) in one batched step:
+    bitwise against the oracle every step, then the per-patch form and
+    graph replays continue bitwise."""
+    pool = new_pool()
+    wl = _ragged_workload(pool)
+    _apply_mode(pool, wl, mode)
+    orun = OracleRun(pool, wl)
+    for it in range(6):
+        pool.step(1)
+        orun.step()
+        worst, exact, total = _compare(pool, orun, wl.slots)
+        if it in (0, 5):
+            print(f"ragged step {it + 1} ({mode}): bitwise {exact}/{total}")
+        assert exact == total, (it, worst)
+    pool.step_per_patch(2)
+    pool.graph_capture()
+    pool.graph_replay(2)
+    pool.synchronize()
+    for _ in range(4):
+        orun.step()
+    worst, exact, total = _compare(pool, orun, wl.slots)
+    print(f"ragged after per-patch steps and graph replays ({mode}): bitwise {exact}/{total}")
+    assert exact == total, worst
     pool.close()
 
 

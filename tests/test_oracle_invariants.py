@@ -271,3 +271,68 @@ def test_breaking_dam_refinement_closed_forms():
     # a fine patch (dx = 1/1296) in flat water may coarsen
     sub = (ctypes.c_double * 2)(1 / 1296, 1 / 1296)
     assert lib().or_refinement_decision(sub, out) == 0
+
+
+def test_rectangular_grid_ghost_fill_brute_force():
+    """or_grid_fill_rect on a 3 x 2 grid of 3 x 5 patches with a hole: every
+    face ghost cell holds the facing neighbour's boundary cell (same-level
+    transfer), or at a wall (outer border or hole) the patch's own boundary
+    cell with the normal momentum negated (SWEKernel.cpp:176-237); checked
+    cell by cell in plain Python."""
+    gpx, gpy, nx, ny = 3, 2, 3, 5
+    grid = [[0, 1], [2, -1], [3, 4]]
+    area = (nx + 2) * (ny + 2)
+    r = rng()
+    H, HU, HV = (r.uniform(1, 2, 5 * area), r.uniform(-1, 1, 5 * area),
+                 r.uniform(-1, 1, 5 * area))
+    h0, hu0, hv0 = H.copy(), HU.copy(), HV.copy()
+    slot_of = np.array(grid, dtype=np.int32).ravel()
+    off = np.arange(5, dtype=np.int64) * area
+    lib().or_grid_fill_rect(gpx, gpy, nx, ny,
+                            slot_of.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                            off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                            dptr(H), dptr(HU), dptr(HV))
+
+    def at(i, j):
+        return grid[i][j] if 0 <= i < gpx and 0 <= j < gpy else -1
+
+    def idx(s, x, y):
+        return s * area + x * (ny + 2) + y
+
+    checked = 0
+    for i in range(gpx):
+        for j in range(gpy):
+            s = at(i, j)
+            if s < 0:
+                continue
+            # interior untouched
+            for x in range(1, nx + 1):
+                for y in range(1, ny + 1):
+                    k = idx(s, x, y)
+                    assert (H[k], HU[k], HV[k]) == (h0[k], hu0[k], hv0[k])
+            faces = [(0, -1, 0), (nx + 1, 1, 0)]  # x faces: ghost row, di
+            for gx, di, _ in faces:
+                q = at(i + di, j)
+                for y in range(1, ny + 1):
+                    k = idx(s, gx, y)
+                    if q >= 0:
+                        src = idx(q, nx if di < 0 else 1, y)
+                        want = (h0[src], hu0[src], hv0[src])
+                    else:
+                        src = idx(s, 1 if di < 0 else nx, y)
+                        want = (h0[src], -hu0[src], hv0[src])
+                    assert (H[k], HU[k], HV[k]) == want, (s, gx, y)
+                    checked += 1
+            for gy, dj in ((0, -1), (ny + 1, 1)):
+                q = at(i, j + dj)
+                for x in range(1, nx + 1):
+                    k = idx(s, x, gy)
+                    if q >= 0:
+                        src = idx(q, x, ny if dj < 0 else 1)
+                        want = (h0[src], hu0[src], hv0[src])
+                    else:
+                        src = idx(s, x, 1 if dj < 0 else ny)
+                        want = (h0[src], hu0[src], -hv0[src])
+                    assert (H[k], HU[k], HV[k]) == want, (s, x, gy)
+                    checked += 1
+    assert checked == 5 * 2 * (nx + ny)
