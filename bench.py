@@ -181,7 +181,7 @@ def main():
                     help="requests of the served stream (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-mib", type=int, default=512)
-    ap.add_argument("--serve-steps-per-round", type=int, default=16)
+    ap.add_argument("--serve-steps-per-round", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

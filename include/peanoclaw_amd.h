@@ -192,7 +192,8 @@ int pc_scenario_get(pc_pool *pool, int scenario, double *time,
                     double *timestep_size, double *estimated_next_dt,
                     long long *steps);
 /* Execution options: tasks per launch (0 = one task per workgroup,
- * > 0 = persistent grid of that many workgroups), and how the CFL edge
+ * > 0 = persistent grid of that many workgroups when the step has more
+ * tasks than that, else one task per workgroup), and how the CFL edge
  * speeds of the next step are obtained (grids with side sources only):
  *   fused = 0: a separate exact speed pass before every dt reduction;
  *   fused = 1: exact speeds of every interior edge inside the update kernel;

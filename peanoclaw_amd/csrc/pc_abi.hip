@@ -625,8 +625,11 @@ int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
   }
   if (p->ntasks > 0) {
     if (timed_update) HIPCHK(hipEventRecord(eb ? eb : p->ev2, p->stream));
-    const int grid = p->persistent_wgs > 0 ? p->persistent_wgs : p->ntasks;
-    const int pers = p->persistent_wgs > 0 ? 1 : 0;
+    // a persistent grid only pays when there are more tasks than workgroups:
+    // below that every workgroup would take one task through the shared
+    // counter (contended atomics) instead of its own block index
+    const int pers = (p->persistent_wgs > 0 && p->ntasks > p->persistent_wgs) ? 1 : 0;
+    const int grid = pers ? p->persistent_wgs : p->ntasks;
     if (prune)
       hipLaunchKernelGGL((k_update<2, true, 0>), dim3(grid), dim3(64), 0, p->stream,
                          p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
