@@ -46,6 +46,16 @@ def read_traffic():
     return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(num_patches, seconds, threads):
     """The oracle (FP64, C, OpenMP over patches) on a sample of the same
     workload generator: ghost fill of every size-class grid + global-dt
@@ -293,7 +303,14 @@ def main():
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(args.cpu_patches, args.cpu_seconds, threads)
-        log(f"cpu baseline: {cpu['value'] / 1e6:.1f} M cell-updates/s on {threads} threads")
+        # the same sample on one core (a shorter run: the rate is steady)
+        one = cpu_baseline(args.cpu_patches, max(args.cpu_seconds / 2, 1.0), 1)
+        cpu["single_core_value"] = one["value"]
+        cpu["single_core_sample"] = one["sample"]
+        cpu["host_cpus"] = os.cpu_count()
+        cpu["cpu_model"] = cpu_model()
+        log(f"cpu baseline: {cpu['value'] / 1e6:.1f} M cell-updates/s on {threads} threads, "
+            f"{one['value'] / 1e6:.1f} M on one core ({cpu['cpu_model']})")
 
     out = {
         "metric": METRIC,
