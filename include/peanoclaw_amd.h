@@ -220,6 +220,12 @@ int pc_set_wave_propagation_solver(pc_pool *pool, int solver);
  * est * slack * (1 - 1/256), else the exact fallback pass runs.  Results do
  * not depend on slack; slack < 1 makes every step take the fallback. */
 int pc_set_prune_slack(pc_pool *pool, double slack);
+/* Row segments of the batched update (no reference counterpart): 0 =
+ * automatic (a step with fewer tasks than the GPU holds splits the rows of
+ * each patch into 2 or 4 segments of at least 8 rows, each marched by its
+ * own wavefront), 1 = whole patches, 2 or 4 = that many segments where the
+ * patch has at least 8 rows per segment.  Results do not depend on it. */
+int pc_set_row_segments(pc_pool *pool, int segments);
 /* Run k global-dt steps of every scenario that has patches. */
 int pc_step_all(pc_pool *pool, int k);
 /* Plain per-patch-launch form of the same step (reference call pattern:

@@ -171,6 +171,7 @@ _SIGS = {
     "pc_set_launch_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int]),
     "pc_set_prune_slack": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "pc_set_row_segments": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pc_set_wave_propagation_solver": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pc_step_all": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pc_step_all_per_patch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),

@@ -91,6 +91,12 @@ struct pc_pool {
   Task* d_tasks = nullptr;
   int ntasks = 0;
   std::vector<Task> h_tasks;
+  // the update's work list: h_tasks, with the rows of every patch split into
+  // row segments when the step has fewer tasks than the GPU holds
+  Task* d_utasks = nullptr;
+  int nutasks = 0;
+  int row_segments = 0;  // 0 automatic, 1 off, 2 or 4 forced
+  std::vector<int> h_rseg;  // row segments per slot of the current list
   int* d_plist = nullptr;
   int* d_plist_scn = nullptr;
   size_t plist_cap = 0;
@@ -139,21 +145,25 @@ inline int fused_mode(const pc_pool* p) { return p->solver ? 0 : p->fused; }
 // instance for the ghost-source mode and the edge solver.
 void launch_speed(pc_pool* p, dim3 g, const Task* tasks, int ntasks, const int* plist,
                   Fields F, const SideDev* sides, EdgeBufs E, double* smax,
-                  const int* only_if) {
+                  const int* only_if, int smax_stride = 1) {
   if (sides) {
     if (p->solver)
       hipLaunchKernelGGL((k_speed<true, 1>), g, dim3(64), 0, p->stream, tasks, ntasks, plist,
-                         p->d_pd, F, p->d_scn, sides, E, smax, only_if);
+                         p->d_pd, F, p->d_scn, sides, E, smax, only_if,
+                         smax_stride);
     else
       hipLaunchKernelGGL((k_speed<true, 0>), g, dim3(64), 0, p->stream, tasks, ntasks, plist,
-                         p->d_pd, F, p->d_scn, sides, E, smax, only_if);
+                         p->d_pd, F, p->d_scn, sides, E, smax, only_if,
+                         smax_stride);
   } else {
     if (p->solver)
       hipLaunchKernelGGL((k_speed<false, 1>), g, dim3(64), 0, p->stream, tasks, ntasks, plist,
-                         p->d_pd, F, p->d_scn, sides, E, smax, only_if);
+                         p->d_pd, F, p->d_scn, sides, E, smax, only_if,
+                         smax_stride);
     else
       hipLaunchKernelGGL((k_speed<false, 0>), g, dim3(64), 0, p->stream, tasks, ntasks, plist,
-                         p->d_pd, F, p->d_scn, sides, E, smax, only_if);
+                         p->d_pd, F, p->d_scn, sides, E, smax, only_if,
+                         smax_stride);
   }
 }
 
@@ -361,7 +371,8 @@ int rebuild_topology(pc_pool* p) {
     void** arrs[] = {(void**)&p->d_smax, (void**)&p->d_smax_int, (void**)&p->d_est,
                      (void**)&p->d_ebuf[0], (void**)&p->d_ebuf[1], (void**)&p->d_bcol,
                      (void**)&p->d_sides};
-    const size_t sizes[] = {cap * sizeof(double), cap * sizeof(double), cap * sizeof(double),
+    const size_t sizes[] = {cap * sizeof(double), cap * kMaxRowSeg * sizeof(double),
+                            cap * sizeof(double),
                             cap * 4 * kERows * sizeof(double), cap * 4 * kERows * sizeof(double),
                             cap * 4 * kERows * sizeof(double), cap * sizeof(SideDev)};
     for (int k = 0; k < 7; ++k) {
@@ -425,6 +436,41 @@ int rebuild_topology(pc_pool* p) {
     p->h_tasks.push_back(t);
   }
   p->ntasks = (int)p->h_tasks.size();
+  // the update's tasks: below the number of wavefronts the GPU holds at the
+  // update's occupancy, the rows of each patch are split into row segments
+  // (at least 8 rows each), so a small step is not bound by the march of
+  // whole patches
+  std::vector<Task> utasks;
+  std::vector<int> rseg_of(ns, 1);
+  {
+    constexpr int kFill = 3 * 1024;  // 3 waves on each of the 1024 SIMDs
+    int want = 1;
+    if (p->row_segments == 0) {
+      while (want < kMaxRowSeg && (long long)p->ntasks * want * 2 <= kFill) want *= 2;
+    } else {
+      want = p->row_segments;
+    }
+    for (const Task& t : p->h_tasks) {
+      int r = want;
+      while (r > 1 && t.nx / r < 8) r /= 2;
+      for (int k = 0; k < r; ++k) {
+        Task u = t;
+        u.r0 = seg_first_row(k, t.nx, r);
+        u.r1 = seg_first_row(k + 1, t.nx, r) - 1;
+        u.sidx = k;
+        utasks.push_back(u);
+      }
+      for (int k = 0; k < t.nseg; ++k) rseg_of[p->h_plist[t.first + k]] = r;
+    }
+  }
+  p->nutasks = (int)utasks.size();
+  // interior maxima kept per row segment are only valid for the split they
+  // were computed with: a changed split re-derives them all
+  bool resplit = false;
+  for (int i = 0; i < ns; ++i)
+    if (p->slots[i].live && (i >= (int)p->h_rseg.size() || p->h_rseg[i] != rseg_of[i]))
+      resplit = true;
+  p->h_rseg = rseg_of;
   // scenario patch lists
   int nscn = 0;
   for (int s : live) nscn = std::max(nscn, p->slots[s].desc.scenario + 1);
@@ -464,6 +510,11 @@ int rebuild_topology(pc_pool* p) {
   if (!p->h_tasks.empty())
     HIPCHK(hipMemcpyAsync(p->d_tasks, p->h_tasks.data(), p->h_tasks.size() * sizeof(Task),
                           hipMemcpyHostToDevice, p->stream));
+  if (p->d_utasks) HIPCHK(hipFree(p->d_utasks));
+  HIPCHK(hipMalloc(&p->d_utasks, std::max<size_t>(utasks.size(), 1) * sizeof(Task)));
+  if (!utasks.empty())
+    HIPCHK(hipMemcpyAsync(p->d_utasks, utasks.data(), utasks.size() * sizeof(Task),
+                          hipMemcpyHostToDevice, p->stream));
   {
     // ghost-edge sources of every live patch for the boundary speed pass
     std::vector<BoundaryDesc> bdv(live.size());
@@ -480,6 +531,7 @@ int rebuild_topology(pc_pool* p) {
       D.slot = i;
       D.dx = P.dx;
       D.dy = P.dy;
+      D.rseg = rseg_of[i];
       for (int side = 0; side < 2; ++side) {
         const int kx = sd.kind[side];
         D.xwall[side] = kx == kSideWall;
@@ -530,7 +582,7 @@ int rebuild_topology(pc_pool* p) {
   // The interior speed maxima of unchanged patches stay valid; the pruned
   // mode re-derives those of scenarios whose patch set changed through its
   // exact fallback (their estimate is reset on register / release).
-  if (fused_mode(p) != 2 || realloc) p->fused_ready = false;
+  if (fused_mode(p) != 2 || realloc || resplit) p->fused_ready = false;
   CHK(invalidate_graph(p));
   return PC_OK;
 }
@@ -568,7 +620,8 @@ int prepare_batched(pc_pool* p) {
       hipLaunchKernelGGL(k_gather, dim3((p->nstrips + 3) / 4), dim3(256), 0, p->stream,
                          p->d_strips, p->nstrips, p->d_pd, fields_of(p), p->d_scn);
     launch_speed(p, dim3(p->ntasks), p->d_tasks, p->ntasks, p->d_plist, fields_of(p),
-                 p->sides_any ? p->d_sides : nullptr, edges_of(p), p->d_smax_int, nullptr);
+                 p->sides_any ? p->d_sides : nullptr, edges_of(p), p->d_smax_int, nullptr,
+                 kMaxRowSeg);
     HIPCHK(hipGetLastError());
     p->fused_ready = true;
   }
@@ -614,7 +667,7 @@ int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
       if (p->ntasks > 0)
         hipLaunchKernelGGL((k_speed<true, 0>), dim3(std::min(p->ntasks, kFallbackWgs)),
                            dim3(64), 0, p->stream, p->d_tasks, p->ntasks, p->d_plist, p->d_pd,
-                           F, p->d_scn, sides, E, p->d_smax, p->d_redo);
+                           F, p->d_scn, sides, E, p->d_smax, p->d_redo, 1);
       if (p->nentries > 0)
         hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
                            p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
@@ -628,18 +681,19 @@ int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
     // a persistent grid only pays when there are more tasks than workgroups:
     // below that every workgroup would take one task through the shared
     // counter (contended atomics) instead of its own block index
-    const int pers = (p->persistent_wgs > 0 && p->ntasks > p->persistent_wgs) ? 1 : 0;
-    const int grid = pers ? p->persistent_wgs : p->ntasks;
+    const int pers = (p->persistent_wgs > 0 && p->nutasks > p->persistent_wgs) ? 1 : 0;
+    const int grid = pers ? p->persistent_wgs : p->nutasks;
     if (prune)
       hipLaunchKernelGGL((k_update<2, true, 0>), dim3(grid), dim3(64), 0, p->stream,
-                         p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
+                         p->d_utasks, p->nutasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
                          p->d_smax_int, p->d_counter, pers, p->prune_slack);
     else if (fused)
       hipLaunchKernelGGL((k_update<1, true, 0>), dim3(grid), dim3(64), 0, p->stream,
-                         p->d_tasks, p->ntasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
+                         p->d_utasks, p->nutasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
                          p->d_smax_int, p->d_counter, pers, p->prune_slack);
     else
-      launch_update_plain(p, dim3(grid), p->d_tasks, p->ntasks, p->d_plist, F, sides, E, pers);
+      launch_update_plain(p, dim3(grid), p->d_utasks, p->nutasks, p->d_plist, F, sides, E,
+                          pers);
     if (timed_update) HIPCHK(hipEventRecord(ee ? ee : p->ev3, p->stream));
   }
   HIPCHK(hipGetLastError());
@@ -690,7 +744,7 @@ int pc_pool_destroy(pc_pool* p) {
   invalidate_graph(p);
   for (int k = 0; k < kNumFields; ++k)
     if (p->f[k]) hipFree(p->f[k]);
-  void* ptrs[] = {p->d_pd, p->d_smax, p->d_smax_int, p->d_est, p->d_scn, p->d_tasks,
+  void* ptrs[] = {p->d_pd, p->d_smax, p->d_smax_int, p->d_est, p->d_scn, p->d_tasks, p->d_utasks,
                   p->d_plist, p->d_plist_scn, p->d_scn_of, p->d_live, p->d_bdesc, p->d_strips,
                   p->d_counter, p->d_slot1, p->d_result, p->d_aux, p->d_flags, p->d_ebuf[0],
                   p->d_ebuf[1], p->d_bcol, p->d_sides};
@@ -1629,6 +1683,16 @@ int pc_set_launch_config(pc_pool* p, int persistent_wgs, int fused, int cap) {
   p->fused = fused;
   if (cap != p->task_cap) p->topo_dirty = true;
   p->task_cap = cap;
+  CHK(invalidate_graph(p));
+  return PC_OK;
+}
+
+int pc_set_row_segments(pc_pool* p, int segments) {
+  if (!p) return fail(PC_E_INVALID, "pool is NULL");
+  if (segments != 0 && segments != 1 && segments != 2 && segments != 4)
+    return fail(PC_E_INVALID, "row segments must be 0 (automatic), 1, 2 or 4");
+  if (segments != p->row_segments) p->topo_dirty = true;
+  p->row_segments = segments;
   CHK(invalidate_graph(p));
   return PC_OK;
 }

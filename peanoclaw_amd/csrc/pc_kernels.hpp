@@ -108,6 +108,7 @@ struct BoundaryDesc {
   int nx, ny, scn, slot;
   int xwall[2];
   int shared[2];
+  int rseg;  // row segments of the update (seams: seg_first_row(k >= 1))
   double dx, dy;
 };
 
@@ -124,7 +125,20 @@ struct Task {
   int first;  // index into the task patch list
   int nseg;   // patches in this task (same nx, ny)
   int nx, ny;
+  // row segment: the task finalises interior rows r0..r1 (r1 = 0: nx) of its
+  // patches; segment sidx of the patch's row segments (see seg_first_row)
+  int r0 = 1, r1 = 0, sidx = 0;
 };
+
+// Row segments of a patch: a launch with fewer tasks than the GPU holds
+// splits the rows of each patch into rseg segments (1, 2 or 4), each marched
+// by its own wavefront.  Segment k starts at row seg_first_row(k, nx, rseg);
+// the update's next-step interior speed maximum of segment k of patch p is
+// kept in smax_int[p * kMaxRowSeg + k].
+constexpr int kMaxRowSeg = 4;
+__host__ __device__ __forceinline__ int seg_first_row(int k, int nx, int rseg) {
+  return 1 + (k * nx) / rseg;
+}
 
 struct ScnDev {
   double time;        // TimeIntervals "time" (start of the last interval)
@@ -198,6 +212,10 @@ struct MarchArgs {
   // patches with seg_adv[p] != 0 are updated (local time stepping)
   const double* seg_dt = nullptr;
   const int* seg_adv = nullptr;
+  // slots per patch of `smax` (kMaxRowSeg for the update's interior maxima
+  // of row segments; a whole-patch kSpeed pass fills slot 0 and clears the
+  // others)
+  int smax_stride = 1;
 };
 
 // FUSE: 0 = no next-state speeds, 1 = exact speeds of every interior edge of
@@ -245,12 +263,16 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   const double* Ecur = SIDES ? (par ? A.E.e[1] : A.E.e[0]) : nullptr;
   double* const Edst = SIDES ? (dst ? A.E.e[1] : A.E.e[0]) : nullptr;
 
+  // rows finalised by this task (a row segment, or all rows)
+  const int ra = t.r0, rb = t.r1 ? t.r1 : nx;
+  const int nrows = rb - ra + 1;
+
   // ---- pre-pass: the y-edges (row, 0|1) and (row, ny|ny+1) ----
-  const int npre = t.nseg * nx * 2;
+  const int npre = t.nseg * nrows * 2;
   for (int e = lane; e < npre; e += 64) {
-    const int s = e / (2 * nx);
-    const int rem = e - s * 2 * nx;
-    const int row = (rem >> 1) + 1;
+    const int s = e / (2 * nrows);
+    const int rem = e - s * 2 * nrows;
+    const int row = (rem >> 1) + ra;
     const int side = rem & 1;
     const int ps = gl(A.plist + t.first + s);
     const PatchDev Q = gl(A.pd + ps);
@@ -374,21 +396,21 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   float2* const nb = reinterpret_cast<float2*>(&L.nc[0][0]);
 
   double h, hu, hv, b;
-  load_row(0, h, hu, hv, b);
-  for (int r = 0; r <= nx + 1; ++r) {
+  load_row(ra - 1, h, hu, hv, b);
+  for (int r = ra - 1; r <= rb + 1; ++r) {
     const double ch = h, chu = hu, chv = hv, cbv = b;
-    if (r < nx + 1) load_row(r + 1, h, hu, hv, b);  // prefetch the next row
+    if (r < rb + 1) load_row(r + 1, h, hu, hv, b);  // prefetch the next row
     const CellBase cb = cell_base(ch, cbv);
     const CellQ cx = cell_dir(cb, chu);
 
-    if (r >= 1) {
+    if (r >= ra) {
       // x-edge (r-1 | r)
       if (MODE == kSpeed) {
         smaxl = dmax(smaxl, ES::speed(px, cx));
       } else {
         double dhL, dhR, dqL, dqR;
         ES::solve(px, cx, dhL, dhR, dqL, dqR);
-        if (r >= 2) {
+        if (r >= ra + 1) {
           // finalise row r-1: x-left (already in acc), x-right, y-up, y-down
           double nH = accH + dxinv * dhL;
           double nHu = accHu + dxinv * dqL;
@@ -430,7 +452,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
           }
           if (PRUNE) {
             const BoundTerms bt = bound_terms(h1, hu1, hv1);
-            if (r >= 3) {
+            if (r >= ra + 2) {
               const bool cand = active && bound_candidate(pbt.ax, pbt.c, bt.ax, bt.c, thr);
               if (__any(cand ? 1 : 0)) {
                 const CellQ ea = cell_dir(cell_base(pnh, 0.0), pnhu);
@@ -447,7 +469,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
           } else if (FUSED) {
             nbase = cell_base(h1, 0.0);
             const CellQ ncx = cell_dir(nbase, hu1);
-            if (r >= 3) {
+            if (r >= ra + 2) {
               const double sp = ES::speed(pnx, ncx);
               if (active) smaxl = dmax(smaxl, sp);
             }
@@ -461,7 +483,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
       }
     }
 
-    if (r >= 1 && r <= nx) {
+    if (r >= ra && r <= rb) {
       // y-edges of row r: lane col solves (col-1 | col) for col >= 2
       const CellQ cy = cell_dir(cb, chv);
       L.yc[0][lane] = cy.h;
@@ -588,7 +610,13 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
     // segmented max over the W lanes of each patch
     double v = active ? smaxl : 0.0;
     for (int m = W >> 1; m >= 1; m >>= 1) v = dmax(v, __shfl_xor(v, m));
-    if (segok && col == 1) gs(A.smax + p, v);
+    double* const out = A.smax + (long long)p * A.smax_stride;
+    if (segok && col == 1) {
+      gs(out + t.sidx, v);
+      // a whole-patch pass clears the slots of the other row segments
+      if (nrows == nx)
+        for (int k = 1; k < A.smax_stride; ++k) gs(out + k, 0.0);
+    }
   }
 }
 
@@ -601,7 +629,7 @@ __global__ void __launch_bounds__(64)
 k_speed(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plist,
         const PatchDev* __restrict__ pd, Fields F, const ScnDev* __restrict__ scn,
         const SideDev* __restrict__ sides, EdgeBufs E, double* __restrict__ smax,
-        const int* __restrict__ only_if) {
+        const int* __restrict__ only_if, int smax_stride) {
   __shared__ MarchLds L;
   if (only_if && gl(only_if) == 0) return;
   for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
@@ -611,6 +639,7 @@ k_speed(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plis
     if (only_if && gl(&S->redo) == 0) continue;
     const int par = gl(&S->stepc) & 1;
     MarchArgs A{plist, pd, F, sides, E, par, 0.0, smax, scn, 1.0};
+    A.smax_stride = smax_stride;
     march_task<kSpeed, SIDES, 0, SOLVER>(t, A, L);
   }
 }
@@ -647,6 +676,7 @@ k_update(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ pli
       // k_dt_final already moved the parity counter past this step
       MarchArgs A{plist, pd, F, sides, E, (gl(&S->stepc) - 1) & 1, gl(&S->dt_applied),
                   smax_next, scn, slack};
+      A.smax_stride = kMaxRowSeg;
       march_task<kUpdate, SIDES, SIDES ? FUSE : 0, SOLVER>(t, A, L);
     }
     if (!persistent) return;
@@ -762,8 +792,43 @@ k_boundary_speed(const ScnDev* __restrict__ scn, const BoundaryDesc* __restrict_
       if (valid) sm = dmax(sm, sp);
     }
   }
+  // seams of the update's row segments: the x-edges (a-1 | a) of the new
+  // state at the first row a of every segment after the first (each segment
+  // of the update saw only one side of them)
+  if (D.rseg > 1) {
+    const int nse = (D.rseg - 1) * ny;
+    for (int e = lane; e - lane < nse; e += 64) {  // wave-uniform trip count
+      const bool valid = e < nse;
+      double hL = 1.0, qL = 0.0, hR = 1.0, qR = 0.0;
+      if (valid) {
+        const int k = e / ny + 1;
+        const int c = e - (k - 1) * ny + 1;
+        const long long i = D.off + (long long)seg_first_row(k, nx, D.rseg) * stride + c;
+        hL = gl(H + i - stride);
+        qL = gl(HU + i - stride);
+        hR = gl(H + i);
+        qR = gl(HU + i);
+      }
+      bool need = valid;
+      if (PRUNE) {
+        const BoundTerms bl = bound_terms(hL, qL, 0.0), br = bound_terms(hR, qR, 0.0);
+        need = valid && bound_candidate(bl.ax, bl.c, br.ax, br.c, thr);
+      }
+      if (__any(need ? 1 : 0)) {
+        const CellQ cl = cell_dir(cell_base(hL, 0.0), qL);
+        const CellQ cr = cell_dir(cell_base(hR, 0.0), qR);
+        const double sp = edge_speed_any(cl, cr);
+        if (valid) sm = dmax(sm, sp);
+      }
+    }
+  }
   for (int o = 32; o >= 1; o >>= 1) sm = dmax(sm, __shfl_xor(sm, o));
-  if (lane == 0) smax[D.slot] = dmax(gl(smax_int + D.slot), sm);
+  if (lane == 0) {
+    const double* si = smax_int + (long long)D.slot * kMaxRowSeg;
+    double m = gl(si);
+    for (int k = 1; k < D.rseg; ++k) m = dmax(m, gl(si + k));
+    smax[D.slot] = dmax(m, sm);
+  }
 }
 
 // Memory-pattern probe of the row march: the update kernel's traversal

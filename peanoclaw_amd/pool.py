@@ -165,6 +165,11 @@ class PatchPool:
         """Threshold slack of the pruned speed mode (fused=2); see the C header."""
         check(self.lib.pc_set_prune_slack(self.h, float(slack)))
 
+    def row_segments(self, segments):
+        """Row segments of the batched update: 0 automatic, 1 whole patches,
+        2 or 4 forced (see the C header)."""
+        check(self.lib.pc_set_row_segments(self.h, int(segments)))
+
     def step(self, k=1):
         check(self.lib.pc_step_all(self.h, k))
 
