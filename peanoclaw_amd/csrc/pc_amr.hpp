@@ -353,6 +353,8 @@ k_amr_speed(const Task* __restrict__ tasks, const int* __restrict__ plist,
   const Task t = tasks[blockIdx.x];
   EdgeBufs none{{nullptr, nullptr}, nullptr};
   MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
+  // one slot per row segment (tasks may be row segments of their leaves)
+  A.smax_stride = kMaxRowSeg;
   march_task<kSpeed, false, 0, SOLVER>(t, A, L);
 }
 
@@ -360,7 +362,7 @@ k_amr_speed(const Task* __restrict__ tasks, const int* __restrict__ plist,
 // left to the end time (AMR driver: min(min over leaves, maxdt)).
 __global__ void __launch_bounds__(256)
 k_amr_dt(const int* __restrict__ leaves, int nleaves, const PatchDev* __restrict__ pd,
-         const double* __restrict__ smax, AmrClock* clock) {
+         const double* __restrict__ smax, const int* __restrict__ rseg, AmrClock* clock) {
   __shared__ double red[256];
   const bool adv = amr_advance(*clock);
   if (!adv) {
@@ -371,7 +373,11 @@ k_amr_dt(const int* __restrict__ leaves, int nleaves, const PatchDev* __restrict
   for (int k = threadIdx.x; k < nleaves; k += blockDim.x) {
     const int s = leaves[k];
     const PatchDev P = pd[s];
-    m = dmin(m, max_timestep(smax[s], P.dx, P.dy));
+    // the speed maximum of the leaf: over its row segments' slots
+    const double* sm = smax + (long long)s * kMaxRowSeg;
+    double v = sm[0];
+    for (int j = 1; j < rseg[k]; ++j) v = dmax(v, sm[j]);
+    m = dmin(m, max_timestep(v, P.dx, P.dy));
   }
   red[threadIdx.x] = m;
   __syncthreads();
@@ -537,9 +543,11 @@ k_amr_solve(const Task* __restrict__ tasks, const int* __restrict__ plist,
   const double dt = clock->bound;
   EdgeBufs none{{nullptr, nullptr}, nullptr};
   // every leaf's step fmin(own CFL step, bound) equals the bound, which is
-  // at most the minimum of the CFL steps
+  // at most the minimum of the CFL steps.  uNew -> uOld ran before (k_amr_copy),
+  // so the update reads uOld and writes uNew without the swap's uOld store
+  // (the swap of SWEKernel.cpp:72-73 would store the values already there).
   MarchArgs A{plist, pd, F, nullptr, none, cur, dt, nullptr, nullptr, 1.0};
-  march_task<kUpdateSwap, false, 0, SOLVER>(t, A, L);
+  march_task<kUpdateFromOld, false, 0, SOLVER>(t, A, L);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     // SWEKernel.cpp:83-87 for every leaf
     if (dt - 0.0 > kLaEps) {
@@ -558,6 +566,13 @@ struct pc_amr {
   std::vector<int> leaf_nx, leaf_ny;
   int ntasks = 0, nvblocks = 0, nghosts = 0, nwall_leaves = 0;
   Task* d_tasks = nullptr;
+  // the global step's speed and update tasks: d_tasks with the rows of
+  // every leaf split into row segments when the plan has few tasks; per
+  // leaf (plan order) its number of row segments
+  int nutasks = 0;
+  Task* d_utasks = nullptr;
+  int* d_rseg = nullptr;
+  double* d_smax_seg = nullptr;
   int* d_plist = nullptr;
   int* d_leaves = nullptr;
   AmrVBlock* d_vblocks = nullptr;
@@ -611,6 +626,9 @@ int amr_upload(T** d, const std::vector<T>& h) {
 
 void amr_free(pc_amr* a) {
   hipFree(a->d_tasks);
+  hipFree(a->d_utasks);
+  hipFree(a->d_rseg);
+  hipFree(a->d_smax_seg);
   hipFree(a->d_plist);
   hipFree(a->d_leaves);
   hipFree(a->d_vblocks);
@@ -727,6 +745,30 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
     tasks.push_back(t);
   }
   a->ntasks = (int)tasks.size();
+  // row segments of the global step's tasks: as in the pool's batched step,
+  // a plan with fewer tasks than the GPU holds splits each leaf's rows into
+  // 2 or 4 segments of at least 8 rows
+  std::vector<Task> utasks;
+  std::vector<int> rseg_slot(p->slots.size(), 1), rseg;
+  {
+    constexpr int kFill = 3 * 1024;
+    int want = 1;
+    while (want < kMaxRowSeg && (long long)a->ntasks * want * 2 <= kFill) want *= 2;
+    for (const Task& t : tasks) {
+      int r = want;
+      while (r > 1 && t.nx / r < 8) r /= 2;
+      for (int k = 0; k < r; ++k) {
+        Task u = t;
+        u.r0 = seg_first_row(k, t.nx, r);
+        u.r1 = seg_first_row(k + 1, t.nx, r) - 1;
+        u.sidx = k;
+        utasks.push_back(u);
+      }
+      for (int k = 0; k < t.nseg; ++k) rseg_slot[plist[t.first + k]] = r;
+    }
+    for (int s : a->leaves) rseg.push_back(rseg_slot[s]);
+  }
+  a->nutasks = (int)utasks.size();
   // restriction pairs grouped by virtual patch, in the given order; per
   // virtual cell the list of pairs covering it
   std::vector<AmrVBlock> vblocks;
@@ -881,6 +923,10 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
   c.dt = dt0;
   int rc = PC_OK;
   if ((rc = amr_upload(&a->d_tasks, tasks)) != PC_OK ||
+      (rc = amr_upload(&a->d_utasks, utasks)) != PC_OK ||
+      (rc = amr_upload(&a->d_rseg, rseg)) != PC_OK ||
+      (rc = amr_upload(&a->d_smax_seg,
+                       std::vector<double>(p->slots.size() * kMaxRowSeg, 0.0))) != PC_OK ||
       (rc = amr_upload(&a->d_plist, plist)) != PC_OK ||
       (rc = amr_upload(&a->d_leaves, a->leaves)) != PC_OK ||
       (rc = amr_upload(&a->d_vblocks, vblocks)) != PC_OK ||
@@ -921,18 +967,18 @@ void amr_launch_step(pc_amr* a, Fields F, int cur) {
     hipLaunchKernelGGL(k_amr_walls, dim3(a->nwall_leaves), dim3(64), 0, p->stream, a->d_walls,
                        a->d_wall_dims, p->d_pd, F, cur, a->d_clock);
   if (p->solver)
-    hipLaunchKernelGGL(k_amr_speed<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
-                       a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock);
+    hipLaunchKernelGGL(k_amr_speed<1>, dim3(a->nutasks), dim3(64), 0, p->stream, a->d_utasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_smax_seg, a->d_clock);
   else
-    hipLaunchKernelGGL(k_amr_speed<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
-                       a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock);
+    hipLaunchKernelGGL(k_amr_speed<0>, dim3(a->nutasks), dim3(64), 0, p->stream, a->d_utasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_smax_seg, a->d_clock);
   hipLaunchKernelGGL(k_amr_dt, dim3(1), dim3(256), 0, p->stream, a->d_leaves, nl, p->d_pd,
-                     a->d_smax, a->d_clock);
+                     a->d_smax_seg, a->d_rseg, a->d_clock);
   if (p->solver)
-    hipLaunchKernelGGL(k_amr_solve<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+    hipLaunchKernelGGL(k_amr_solve<1>, dim3(a->nutasks), dim3(64), 0, p->stream, a->d_utasks,
                        a->d_plist, p->d_pd, F, cur, a->d_clock);
   else
-    hipLaunchKernelGGL(k_amr_solve<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+    hipLaunchKernelGGL(k_amr_solve<0>, dim3(a->nutasks), dim3(64), 0, p->stream, a->d_utasks,
                        a->d_plist, p->d_pd, F, cur, a->d_clock);
 }
 

@@ -172,7 +172,11 @@ struct GhostStrip {
   int length, negate;
 };
 
-enum MarchMode { kSpeed = 0, kUpdate = 1, kUpdateSwap = 2 };
+// kUpdateFromOld: the reads and writes of kUpdateSwap without its uOld
+// store, for callers that copied uNew -> uOld before the update (the store
+// would write the values already there); with nothing written into the tile
+// it reads, row segments of one patch may run in parallel.
+enum MarchMode { kSpeed = 0, kUpdate = 1, kUpdateSwap = 2, kUpdateFromOld = 3 };
 
 __device__ __forceinline__ int pow2ceil(int v) {
   int w = 1;
@@ -241,8 +245,9 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   const PatchDev P = gl(A.pd + p);
   const int stride = ny + 2;
   const int par = A.par;
-  const int src = (MODE == kUpdateSwap) ? 1 - par : par;
-  const int dst = (MODE == kUpdateSwap) ? par : 1 - par;
+  constexpr bool FROM_OLD = MODE == kUpdateSwap || MODE == kUpdateFromOld;
+  const int src = FROM_OLD ? 1 - par : par;
+  const int dst = FROM_OLD ? par : 1 - par;
   const Fields& F = A.F;
   // kUpdateSwap writes into the tile it reads (rows already consumed), so
   // its source pointers must not be declared non-aliasing.

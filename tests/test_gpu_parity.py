@@ -360,6 +360,29 @@ def test_uneven_patches_parity_and_forms(mode):
     pool.close()
 
 
+@pytest.mark.parametrize("fused", [2, 1, 0])
+def test_row_segments_do_not_change_results(fused):
+    """The update's row segments (whole patches, 2 or 4 segments, automatic)
+    switched between steps of one run, in each speed mode: every step
+    bitwise against the oracle (a changed split re-derives the per-segment
+    interior speed maxima)."""
+    from peanoclaw_amd.scenarios import uneven_patches
+    pool = new_pool()
+    wl = uneven_patches(pool, num_patches=240, seed=7, field_seed=8)
+    wl.build_ghost_table(fused_sides=True)
+    pool.launch_config(fused=fused)
+    orun = OracleRun(pool, wl)
+    for it, seg in enumerate([1, 4, 4, 2, 0, 1, 2, 4]):
+        pool.row_segments(seg)
+        pool.step(1)
+        orun.step()
+        worst, exact, total = _compare(pool, orun, wl.slots)
+        assert exact == total, (it, seg, worst)
+    t, dt, est, n = pool.scenario_get(0)
+    print(f"row segments 1/4/4/2/auto/1/2/4 (fused {fused}): bitwise {exact}/{total}, dt {dt:.6e}")
+    pool.close()
+
+
 # (nx, ny, grid px, grid py): the smallest and largest sizes, both aspect
 # ratios, odd sizes, a lone patch; every wavefront packing (W = 2 ... 64)
 RAGGED = [(2, 2, 5, 4), (2, 64, 3, 2), (64, 2, 2, 3), (5, 3, 4, 4), (13, 7, 3, 5),
