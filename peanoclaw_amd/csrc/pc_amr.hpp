@@ -77,6 +77,11 @@ __device__ __forceinline__ bool amr_phase_runs(const AmrClock* clock, const AmrL
 }
 
 // 64 consecutive cells of one virtual patch
+// the restriction kernel's lanes per virtual-patch cell, and cells per
+// wavefront (one AmrVBlock)
+constexpr int kRestrictLanes = 8;
+constexpr int kRestrictCells = 64 / kRestrictLanes;
+
 struct AmrVBlock {
   int slot;   // the virtual patch
   int cell0;  // first cell (x * ny + y)
@@ -122,81 +127,106 @@ __device__ __forceinline__ void amr_restrict_factors(const AmrClock& c, double& 
   }
 }
 
-// One thread per virtual-patch cell; `cptr`/`cpairs` list, per cell, the
-// restriction pairs whose destination region holds it, in plan order.
+// One virtual-patch cell per group of kRestrictLanes lanes (kRestrictCells
+// cells per wavefront): the contributions of its fine cells are computed
+// kRestrictLanes at a time in parallel and summed by the group's first lane
+// in the order of the per-call sequence (pairs in plan order, fine cells x
+// outer, y inner), so the sums equal the sequential ones bit for bit.
+// `cptr`/`cpairs` list, per cell, the restriction pairs whose destination
+// region holds it, in plan order.
 template <bool LTS>
 __global__ void __launch_bounds__(64)
 k_amr_restrict(const AmrVBlock* __restrict__ blocks, const int* __restrict__ cptr,
                const int* __restrict__ cpairs, const AmrRestrict* __restrict__ pairs,
                const PatchDev* __restrict__ pd, Fields F, int cur, const AmrClock* clock,
                AmrLts L) {
+  constexpr int K = kRestrictLanes;
+  __shared__ double con[kRestrictCells][6][K];
   const AmrClock c = *clock;
   if (!amr_phase_runs<LTS>(clock, L)) return;
   const AmrVBlock B = blocks[blockIdx.x];
   // local time stepping: only the virtual patches an advancing leaf reads
   if (LTS && L.need[B.slot] != *L.itn) return;
   const PatchDev D = pd[B.slot];
-  const int k = B.cell0 + (int)threadIdx.x;
-  if (k >= D.nx * D.ny) return;
+  const int g = (int)threadIdx.x / K, sub = (int)threadIdx.x - g * K;
+  const int k = B.cell0 + g;
+  const bool cellok = k < D.nx * D.ny;
   double tfOld, tfNew;
   amr_restrict_factors(c, tfOld, tfNew);
   const double eps = 1e-12;
   const int oldSt = 1 - cur;
-  {
-    const int di0 = k / D.ny, di1 = k - (k / D.ny) * D.ny;
-    double accO[3] = {0.0, 0.0, 0.0};
-    double accN[3] = {0.0, 0.0, 0.0};
-    const int pe = cptr[B.base + k + 1];
-    for (int j = cptr[B.base + k]; j < pe; ++j) {
-      const AmrRestrict a = pairs[cpairs[j]];
-      const PatchDev S = pd[a.src];
-      if (LTS) {
-        // each leaf restricts on its own interval
-        AmrClock cs = c;
-        cs.time = L.t[a.src];
-        cs.dt = L.dt[a.src];
-        amr_restrict_factors(cs, tfOld, tfNew);
-      }
-      const double dp[2] = {a.dpos[0] + (double)di0 * a.dstSub[0],
-                            a.dpos[1] + (double)di1 * a.dstSub[1]};
-      int cOff[2], cSize[2];
-      for (int d = 0; d < 2; ++d) {
-        double ot = (dp[d] - a.spos[d] + eps);
-        ot /= a.srcSub[d];
-        const int o = (int)ot;
-        double st = dp[d] + a.dstSub[d] - a.spos[d] - eps;
-        st /= a.srcSub[d];
-        st -= ot;
-        st += 1.0;
-        const int s = (int)st;
-        const int upper = o + s, rUpper = a.rSize[d];
-        cOff[d] = o > 0 ? o : 0;
-        cSize[d] = (upper < rUpper ? upper : rUpper) - cOff[d];
-      }
-      for (int sx = 0; sx < cSize[0]; ++sx)
-        for (int sy = 0; sy < cSize[1]; ++sy) {
-          const int s0 = cOff[0] + sx, s1 = cOff[1] + sy;
-          const double sp[2] = {a.spos[0] + (double)s0 * a.srcSub[0],
-                                a.spos[1] + (double)s1 * a.srcSub[1]};
-          double region = 1.0;
-          for (int d = 0; d < 2; ++d) {
-            const double hi = dmin(dp[d] + a.dstSub[d], sp[d] + a.srcSub[d]);
-            const double lo = dmax(dp[d], sp[d]);
-            region *= hi - lo;
-            region = dmax(region, 0.0);
-          }
-          const long long si = tile_index(S, s0, s1);
-          const double bv = F.b[si];
-          const double vo[3] = {F.H(oldSt)[si] + 1.0 * bv, F.HU(oldSt)[si], F.HV(oldSt)[si]};
-          const double vn[3] = {F.H(cur)[si] + 1.0 * bv, F.HU(cur)[si], F.HV(cur)[si]};
-          for (int q = 0; q < 3; ++q) {
-            const double dO = (vo[q] * (1.0 - tfOld) + vn[q] * tfOld) * region / a.dstRegion;
-            const double dN = (vo[q] * (1.0 - tfNew) + vn[q] * tfNew) * region / a.dstRegion;
-            accO[q] = accO[q] + dO;
-            accN[q] = accN[q] + dN;
-          }
-        }
+  const int di0 = cellok ? k / D.ny : 0, di1 = cellok ? k - di0 * D.ny : 0;
+  double accO[3] = {0.0, 0.0, 0.0};
+  double accN[3] = {0.0, 0.0, 0.0};
+  const int jb = cellok ? cptr[B.base + k] : 0, je = cellok ? cptr[B.base + k + 1] : 0;
+  for (int j = jb; j < je; ++j) {  // uniform within the group
+    const AmrRestrict a = pairs[cpairs[j]];
+    const PatchDev S = pd[a.src];
+    if (LTS) {
+      // each leaf restricts on its own interval
+      AmrClock cs = c;
+      cs.time = L.t[a.src];
+      cs.dt = L.dt[a.src];
+      amr_restrict_factors(cs, tfOld, tfNew);
     }
+    const double dp[2] = {a.dpos[0] + (double)di0 * a.dstSub[0],
+                          a.dpos[1] + (double)di1 * a.dstSub[1]};
+    int cOff[2], cSize[2];
+    for (int d = 0; d < 2; ++d) {
+      double ot = (dp[d] - a.spos[d] + eps);
+      ot /= a.srcSub[d];
+      const int o = (int)ot;
+      double st = dp[d] + a.dstSub[d] - a.spos[d] - eps;
+      st /= a.srcSub[d];
+      st -= ot;
+      st += 1.0;
+      const int s = (int)st;
+      const int upper = o + s, rUpper = a.rSize[d];
+      cOff[d] = o > 0 ? o : 0;
+      cSize[d] = (upper < rUpper ? upper : rUpper) - cOff[d];
+    }
+    const int nfine = cSize[0] > 0 && cSize[1] > 0 ? cSize[0] * cSize[1] : 0;
+    for (int r0 = 0; r0 < nfine; r0 += K) {  // uniform within the group
+      const int m = r0 + sub;
+      double dO[3] = {0.0, 0.0, 0.0}, dN[3] = {0.0, 0.0, 0.0};
+      if (m < nfine) {
+        const int sx = m / cSize[1];
+        const int s0 = cOff[0] + sx, s1 = cOff[1] + (m - sx * cSize[1]);
+        const double sp[2] = {a.spos[0] + (double)s0 * a.srcSub[0],
+                              a.spos[1] + (double)s1 * a.srcSub[1]};
+        double region = 1.0;
+        for (int d = 0; d < 2; ++d) {
+          const double hi = dmin(dp[d] + a.dstSub[d], sp[d] + a.srcSub[d]);
+          const double lo = dmax(dp[d], sp[d]);
+          region *= hi - lo;
+          region = dmax(region, 0.0);
+        }
+        const long long si = tile_index(S, s0, s1);
+        const double bv = F.b[si];
+        const double vo[3] = {F.H(oldSt)[si] + 1.0 * bv, F.HU(oldSt)[si], F.HV(oldSt)[si]};
+        const double vn[3] = {F.H(cur)[si] + 1.0 * bv, F.HU(cur)[si], F.HV(cur)[si]};
+        for (int q = 0; q < 3; ++q) {
+          dO[q] = (vo[q] * (1.0 - tfOld) + vn[q] * tfOld) * region / a.dstRegion;
+          dN[q] = (vo[q] * (1.0 - tfNew) + vn[q] * tfNew) * region / a.dstRegion;
+        }
+      }
+      for (int q = 0; q < 3; ++q) {
+        con[g][q][sub] = dO[q];
+        con[g][3 + q][sub] = dN[q];
+      }
+      wave_sync();
+      if (sub == 0) {
+        const int cnt = nfine - r0 < K ? nfine - r0 : K;
+        for (int i = 0; i < cnt; ++i)
+          for (int q = 0; q < 3; ++q) {
+            accO[q] = accO[q] + con[g][q][i];
+            accN[q] = accN[q] + con[g][3 + q][i];
+          }
+      }
+      wave_sync();
+    }
+  }
+  if (cellok && sub == 0) {
     // SWEKernel::postProcessRestriction: h + b -> h with the virtual's b
     const long long di = tile_index(D, di0, di1);
     const double bd = F.b[di];
@@ -808,7 +838,8 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
     for (int pi = pbeg; pi < (int)pairs.size(); ++pi) a->vleaves.back().push_back(pairs[pi].src);
     const int nx = dd.subdivision_factor[0], ny = dd.subdivision_factor[1];
     const int base = (int)cptr.size() - 1;
-    for (int c0 = 0; c0 < nx * ny; c0 += 64) vblocks.push_back(AmrVBlock{v, c0, base, 0});
+    for (int c0 = 0; c0 < nx * ny; c0 += kRestrictCells)
+      vblocks.push_back(AmrVBlock{v, c0, base, 0});
     for (int c = 0; c < nx * ny; ++c) {
       const int x = c / ny, y = c % ny;
       for (int pi = pbeg; pi < (int)pairs.size(); ++pi) {
