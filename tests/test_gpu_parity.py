@@ -360,6 +360,60 @@ def test_uneven_patches_parity_and_forms(mode):
     pool.close()
 
 
+def test_uneven_50k_full_size_bitwise():
+    """The measured workload at its full size (50,000 uneven patches,
+    68,241,728 cells) in the autotuned configuration (pruned speeds, 4096
+    persistent workgroups): every interior cell of the pool bitwise equal
+    to the oracle after two eager steps and after two more graph replays,
+    and the same dt."""
+    import time
+
+    from peanoclaw_amd.scenarios import uneven_patches
+    t0 = time.perf_counter()
+    pool = new_pool()
+    wl = uneven_patches(pool, 50000)
+    wl.build_ghost_table(fused_sides=True)
+    pool.launch_config(persistent_workgroups=4096, fused=2)
+    orun = OracleRun(pool, wl)
+    top = orun.H[0].size
+    mask = np.zeros(top, dtype=bool)
+    for s in wl.slots:
+        nx, ny, o = int(orun.nx[s]), int(orun.ny[s]), int(orun.off[s])
+        mask[o:o + (nx + 2) * (ny + 2)].reshape(nx + 2, ny + 2)[1:-1, 1:-1] = True
+    assert int(mask.sum()) == wl.cells_per_step == 68241728
+    print(f"50k workload set up in {time.perf_counter() - t0:.1f} s", flush=True)
+
+    def compare():
+        st = pool.current_state(0)
+        got = [np.empty(top) for _ in range(3)]
+        pool.copy_fields(False, st, 0, *got)
+        c = orun.pl.cur
+        same = 0
+        for g, ref in zip(got, (orun.H[c], orun.HU[c], orun.HV[c])):
+            same += int((g[mask].view(np.uint64) == ref[mask].view(np.uint64)).sum())
+        return same, 3 * int(mask.sum())
+
+    dts = []
+    for _ in range(2):
+        pool.step(1)
+        dts.append(orun.step())
+    same, total = compare()
+    print(f"50k uneven, 2 steps: bitwise {same}/{total}", flush=True)
+    assert same == total
+    pool.graph_capture()
+    pool.graph_replay(2)
+    pool.synchronize()
+    for _ in range(2):
+        dts.append(orun.step())
+    same, total = compare()
+    t, dt, est, n = pool.scenario_get(0)
+    print(f"50k uneven, 2 more graph replays: bitwise {same}/{total}; dt {dt:.9e} (oracle "
+          f"{dts[-1]:.9e}); {time.perf_counter() - t0:.1f} s in all", flush=True)
+    assert same == total
+    assert dt == dts[-1] and n == 4
+    pool.close()
+
+
 @pytest.mark.parametrize("fused", [2, 1, 0])
 def test_row_segments_do_not_change_results(fused):
     """The update's row segments (whole patches, 2 or 4 segments, automatic)
