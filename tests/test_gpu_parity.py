@@ -320,6 +320,38 @@ def test_dam_break_grid_200_steps(mode):
     pool.close()
 
 
+def test_dam_break_grid_1000_steps_graph():
+    """C0 for 1000 steps in the autotuned configuration (pruned speeds,
+    row segments, graph replays): bitwise against the oracle after 1000
+    steps (the bar is 1e-10), the same dt, mass to 1e-13."""
+    from peanoclaw_amd.scenarios import dam_break_grid
+    pool = new_pool()
+    wl = dam_break_grid(pool, 9, 9, 16)
+    _apply_mode(pool, wl, "sides_pruned")
+    pool.launch_config(persistent_workgroups=4096, fused=2)
+    orun = OracleRun(pool, wl)
+    m0 = pool.mass()
+    pool.step(1)
+    orun.step()
+    pool.graph_capture()
+    dts = []
+    for _ in range(9):
+        pool.graph_replay(111)
+        pool.synchronize()
+        for _ in range(111):
+            dts.append(orun.step())
+    t, dt, est, nsteps = pool.scenario_get(0)
+    assert nsteps == 1000
+    assert dt == dts[-1]
+    worst, exact, total = _compare(pool, orun, wl.slots)
+    m1 = pool.mass()
+    print(f"C0 1000 steps (graph): max rel {worst:.3e}, bitwise {exact}/{total}, "
+          f"mass drift {(m1 - m0) / m0:.2e}")
+    assert worst <= REL_LONG and exact == total
+    assert abs(m1 - m0) / m0 < 1e-13
+    pool.close()
+
+
 @pytest.mark.parametrize("mode", ["strips", "sides", "sides_fused", "sides_pruned",
                                   "sides_pruned_fallback"])
 def test_uneven_patches_parity_and_forms(mode):
