@@ -467,6 +467,51 @@ k_lts_flags(const int* __restrict__ leaves, int nleaves, const int* __restrict__
   }
 }
 
+// The same decision in one workgroup of 1024 threads for plans of up to
+// kLtsBlockLeaves leaves and kLtsBlockVirtuals virtual patches: first the end time of every
+// virtual patch (one wavefront per virtual patch, its leaves over the lanes,
+// an exact minimum), kept in LDS, then the leaves' flags reading it, so no
+// thread walks the leaves below a virtual neighbour itself.
+constexpr int kLtsBlockLeaves = 8192;
+constexpr int kLtsBlockVirtuals = 4096;  // 32 KiB of LDS
+__global__ void __launch_bounds__(1024)
+k_lts_flags_block(const int* __restrict__ leaves, int nleaves, const int* __restrict__ nptr,
+                  const int* __restrict__ nbr, const int* __restrict__ vptr, int nvirt,
+                  const int* __restrict__ vleaf, const int* __restrict__ vidx,
+                  const int* __restrict__ rptr, const int* __restrict__ rsrc,
+                  const AmrClock* clock, AmrLts L) {
+  extern __shared__ double vend[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  for (int v = wave; v < nvirt; v += nwaves) {
+    double m = 1.7976931348623157e308;
+    for (int i = vptr[v] + lane; i < vptr[v + 1]; i += 64) {
+      const int x = vleaf[i];
+      m = dmin(m, L.t[x] + L.dt[x]);
+    }
+    for (int o = 32; o >= 1; o >>= 1) m = dmin(m, __shfl_xor(m, o));
+    if (lane == 0) vend[v] = m;
+  }
+  __syncthreads();
+  const double end_time = clock->end_time;
+  const int stamp = *L.itn;
+  for (int k = threadIdx.x; k < nleaves; k += blockDim.x) {
+    const int s = leaves[k];
+    const double e = L.t[s] + L.dt[s];
+    const double md = end_time - e;
+    double c = 1.7976931348623157e308;
+    for (int j = nptr[k]; j < nptr[k + 1]; ++j) {
+      const int q = nbr[j];
+      c = dmin(c, q >= 0 ? L.t[q] + L.dt[q] : vend[vidx[-1 - q]]);
+    }
+    const int adv = (md - 0.0 > kLaEps) && !(e - c > kLaEps);
+    L.adv[s] = adv;
+    if (adv) {
+      atomicAdd(L.iter, 1);
+      for (int j = rptr[k]; j < rptr[k + 1]; ++j) L.need[rsrc[j]] = stamp;
+    }
+  }
+}
+
 // uNew -> uOld and the walls (in call order) of one advancing leaf
 // (DefaultTransfer::copyUNewToUOld; SWEKernel::fillBoundaryLayer,
 // SWEKernel.cpp:176-237); widx[slot] = its entry in `walls`, -1 for none.
@@ -548,6 +593,84 @@ k_lts_task(const Task* __restrict__ tasks, const int* __restrict__ plist,
     B.seg_adv = T.adv;
     march_task<kUpdateSwap, false, 0, SOLVER, true>(t, B, L);
     if (lead) {
+      const double dt = T.stepdt[p];
+      if (dt - 0.0 > kLaEps) {
+        T.t[p] = T.t[p] + T.dt[p];
+        T.dt[p] = dt;
+      }
+      atomicAdd(T.steps, 1ull);
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *T.iter = 0;
+    *T.itn = *T.itn + 1;
+  }
+}
+
+// The same iteration in row segments (the global step's segmented tasks):
+// exact speeds of every task with an advancing leaf into one slot per row
+// segment; the step of every advancing leaf, fmin(own CFL step, end - (t +
+// dt)), from its segments' slots; the update of the advancing leaves (their
+// uNew -> uOld ran in k_lts_prep, so it reads uOld and writes uNew) and
+// their clocks.
+template <int SOLVER>
+__global__ void __launch_bounds__(64)
+k_lts_speed(const Task* __restrict__ tasks, const int* __restrict__ plist,
+            const PatchDev* __restrict__ pd, Fields F, int cur, double* __restrict__ smax,
+            AmrLts T) {
+  __shared__ MarchLds L;
+  const Task t = tasks[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int W = pow2ceil(t.ny);
+  const int seg = lane / W;
+  const int col = lane - seg * W + 1;
+  const bool segok = seg < t.nseg;
+  const int p = plist[t.first + (segok ? seg : 0)];
+  const bool lead = segok && col == 1 && T.adv[p] != 0;
+  if (!__any(lead ? 1 : 0)) return;
+  EdgeBufs none{{nullptr, nullptr}, nullptr};
+  MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
+  A.smax_stride = kMaxRowSeg;
+  march_task<kSpeed, false, 0, SOLVER>(t, A, L);
+}
+
+__global__ void __launch_bounds__(256)
+k_lts_dt(const int* __restrict__ leaves, int nleaves, const int* __restrict__ rseg,
+         const PatchDev* __restrict__ pd, const double* __restrict__ smax,
+         const AmrClock* clock, AmrLts T) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nleaves) return;
+  const int p = leaves[k];
+  if (!T.adv[p]) return;
+  const double* sm = smax + (long long)p * kMaxRowSeg;
+  double v = sm[0];
+  for (int j = 1; j < rseg[k]; ++j) v = dmax(v, sm[j]);
+  const PatchDev P = pd[p];
+  const double md = clock->end_time - (T.t[p] + T.dt[p]);
+  T.stepdt[p] = fmin(max_timestep(v, P.dx, P.dy), md);
+}
+
+template <int SOLVER>
+__global__ void __launch_bounds__(64)
+k_lts_update(const Task* __restrict__ tasks, const int* __restrict__ plist,
+             const PatchDev* __restrict__ pd, Fields F, int cur, AmrLts T) {
+  __shared__ MarchLds L;
+  const Task t = tasks[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int W = pow2ceil(t.ny);
+  const int seg = lane / W;
+  const int col = lane - seg * W + 1;
+  const bool segok = seg < t.nseg;
+  const int p = plist[t.first + (segok ? seg : 0)];
+  const bool lead = segok && col == 1 && T.adv[p] != 0;
+  if (__any(lead ? 1 : 0)) {
+    EdgeBufs none{{nullptr, nullptr}, nullptr};
+    MarchArgs B{plist, pd, F, nullptr, none, cur, 0.0, nullptr, nullptr, 1.0};
+    B.seg_dt = T.stepdt;
+    B.seg_adv = T.adv;
+    march_task<kUpdateFromOld, false, 0, SOLVER, true>(t, B, L);
+    if (lead && t.sidx == 0) {
+      // SWEKernel.cpp:83-87 for the leaf (once, by its first row segment)
       const double dt = T.stepdt[p];
       if (dt - 0.0 > kLaEps) {
         T.t[p] = T.t[p] + T.dt[p];
@@ -1013,18 +1136,24 @@ void amr_launch_step(pc_amr* a, Fields F, int cur) {
                        a->d_plist, p->d_pd, F, cur, a->d_clock);
 }
 
-// One grid iteration with local time stepping, five launches: advance flags
-// from the clocks at its start; restriction of every leaf on its own
-// interval; the ghost faces of the advancing leaves (every neighbour read
-// before any leaf changes); uNew -> uOld and walls; per task CFL speeds,
-// steps, update and clocks.
+// One grid iteration with local time stepping: advance flags from the
+// clocks at its start; restriction of every leaf on its own interval; the
+// ghost faces of the advancing leaves (every neighbour read before any leaf
+// changes); uNew -> uOld and walls; then CFL speeds, steps, update and
+// clocks, in three launches over row segments or one over whole leaves.
 void amr_launch_lts(pc_amr* a, Fields F, int cur) {
   pc_pool* p = a->pool;
   const int nl = (int)a->leaves.size();
   const AmrLts& L = a->lts;
-  hipLaunchKernelGGL(k_lts_flags, dim3((nl + 255) / 256), dim3(256), 0, p->stream, a->d_leaves,
-                     nl, a->d_nptr, a->d_nbr, a->d_vptr, a->d_vleaf, a->d_vidx, a->d_rptr,
-                     a->d_rsrc, a->d_clock, L);
+  const int nv = (int)a->vleaves.size();
+  if (nl <= kLtsBlockLeaves && nv <= kLtsBlockVirtuals)
+    hipLaunchKernelGGL(k_lts_flags_block, dim3(1), dim3(1024), nv * sizeof(double), p->stream,
+                       a->d_leaves, nl, a->d_nptr, a->d_nbr, a->d_vptr, nv, a->d_vleaf,
+                       a->d_vidx, a->d_rptr, a->d_rsrc, a->d_clock, L);
+  else
+    hipLaunchKernelGGL(k_lts_flags, dim3((nl + 255) / 256), dim3(256), 0, p->stream,
+                       a->d_leaves, nl, a->d_nptr, a->d_nbr, a->d_vptr, a->d_vleaf, a->d_vidx,
+                       a->d_rptr, a->d_rsrc, a->d_clock, L);
   if (a->nvblocks > 0)
     hipLaunchKernelGGL(k_amr_restrict<true>, dim3(a->nvblocks), dim3(64), 0, p->stream,
                        a->d_vblocks, a->d_cptr, a->d_cpairs, a->d_pairs, p->d_pd, F, cur,
@@ -1034,12 +1163,30 @@ void amr_launch_lts(pc_amr* a, Fields F, int cur) {
                        a->d_ghosts, a->nghosts, p->d_pd, F, cur, a->d_clock, L);
   hipLaunchKernelGGL(k_lts_prep, dim3(nl), dim3(256), 0, p->stream, a->d_leaves, a->d_widx,
                      a->d_walls, a->d_wall_dims, p->d_pd, F, cur, L);
+  if (a->nutasks == a->ntasks) {
+    // whole leaves: speeds, steps, update and clocks in one launch
+    if (p->solver)
+      hipLaunchKernelGGL(k_lts_task<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                         a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, L);
+    else
+      hipLaunchKernelGGL(k_lts_task<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
+                         a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, L);
+    return;
+  }
   if (p->solver)
-    hipLaunchKernelGGL(k_lts_task<1>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
-                       a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, L);
+    hipLaunchKernelGGL(k_lts_speed<1>, dim3(a->nutasks), dim3(64), 0, p->stream, a->d_utasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_smax_seg, L);
   else
-    hipLaunchKernelGGL(k_lts_task<0>, dim3(a->ntasks), dim3(64), 0, p->stream, a->d_tasks,
-                       a->d_plist, p->d_pd, F, cur, a->d_smax, a->d_clock, L);
+    hipLaunchKernelGGL(k_lts_speed<0>, dim3(a->nutasks), dim3(64), 0, p->stream, a->d_utasks,
+                       a->d_plist, p->d_pd, F, cur, a->d_smax_seg, L);
+  hipLaunchKernelGGL(k_lts_dt, dim3((nl + 255) / 256), dim3(256), 0, p->stream, a->d_leaves, nl,
+                     a->d_rseg, p->d_pd, a->d_smax_seg, a->d_clock, L);
+  if (p->solver)
+    hipLaunchKernelGGL(k_lts_update<1>, dim3(a->nutasks), dim3(64), 0, p->stream, a->d_utasks,
+                       a->d_plist, p->d_pd, F, cur, L);
+  else
+    hipLaunchKernelGGL(k_lts_update<0>, dim3(a->nutasks), dim3(64), 0, p->stream, a->d_utasks,
+                       a->d_plist, p->d_pd, F, cur, L);
 }
 
 bool same_fields(const Fields& x, const Fields& y) {
