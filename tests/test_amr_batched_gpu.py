@@ -265,14 +265,14 @@ def test_regrid_breaking_dam_500_steps():
         pool.close()
 
 
-@pytest.mark.parametrize("levels", [2, 3])
-def test_batched_local_time_stepping_bitwise(levels):
+@pytest.mark.parametrize("levels,n", [(2, 8), (3, 8), (2, 16), (3, 16)])
+def test_batched_local_time_stepping_bitwise(levels, n):
     """Local time stepping as a batched device plan (pc_amr_lts_*: advance
     flags, restriction on each leaf's interval, ghosts with time factors,
-    per-leaf steps, five launches per iteration replayed from a graph)
-    against the oracle's AmrGrid.lts_iteration with the same (pure)
-    semantics: advancing leaves and clocks equal every iteration, states
-    bitwise."""
+    per-leaf steps replayed from a graph; whole leaves for 8^2 patches, row
+    segments for 16^2) against the oracle's AmrGrid.lts_iteration with the
+    same (pure) semantics: advancing leaves and clocks equal every
+    iteration, states bitwise."""
     from peanoclaw_amd.amr import AmrGrid, BatchedAmr, DeviceBackend, dam_break_fields
     from peanoclaw_amd.pool import PatchPool
     fn = dam_break_fields(centre=(0.5, 0.5), radius=0.2, hills=[(0.3, 0.6, 0.1, 0.15)])
@@ -282,11 +282,11 @@ def test_batched_local_time_stepping_bitwise(levels):
             lev == 2 and ij == (7, 7))
     pool = PatchPool(0)
     try:
-        dev = AmrGrid(DeviceBackend(pool), 5, 5, 8, refine, fn, max_level=levels)
-        ora = AmrGrid(OracleBackend(), 5, 5, 8, refine, fn, max_level=levels, pure=True)
+        dev = AmrGrid(DeviceBackend(pool), 5, 5, n, refine, fn, max_level=levels)
+        ora = AmrGrid(OracleBackend(), 5, 5, n, refine, fn, max_level=levels, pure=True)
         amr = BatchedAmr(dev)
         amr.lts_init()
-        end = 0.006
+        end = 0.006 * 8 / n
         its = steps = 0
         while True:
             k1 = amr.lts_iterations(1, end)
@@ -301,7 +301,7 @@ def test_batched_local_time_stepping_bitwise(levels):
                 assert same == total, f"iteration {its}: bitwise {same}/{total}"
             if k1 == 0:
                 break
-        print(f"batched LTS {levels}-level, {len(dev.leaves)} leaves: {its} iterations, "
+        print(f"batched LTS {levels}-level, {len(dev.leaves)} leaves of {n}^2: {its} iterations, "
               f"{steps} leaf steps to t={end}; bitwise {same}/{total}")
         amr.close()
     finally:
