@@ -281,12 +281,28 @@ class AmrGrid:
                 pe = [(kind, src, p) + geom[f] for f, (kind, src) in zip(FACES, res)
                       if kind != "wall"]
                 we = [(p, FACES[k][0], FACES[k][1]) for k in wall_rank if res[k][0] == "wall"]
-                c = (p, pe, we, {e[1].handle for e in pe})
+                # the same entries packed as the rows of the device plan
+                # (pc_amr_ghost: kind, src, dst, size, src_offset, dst_offset;
+                # pc_amr_wall: slot, dimension, upper)
+                rg = np.array([(0 if e[0] == "transfer" else 1, e[1].handle, p.handle,
+                                e[3][0], e[3][1], e[4][0], e[4][1], e[5][0], e[5][1])
+                               for e in pe], dtype=np.int32).reshape(-1, 9)
+                rw = np.array([(p.handle, d, u) for _, d, u in we], dtype=np.int32).reshape(-1, 3)
+                c = (p, pe, we, {e[1].handle for e in pe}, rg, rw)
             cache[p.handle] = c
             plan.extend(c[1])
             walls.extend(c[2])
         self._face_cache = cache
         return plan, walls
+
+    def packed_plan(self):
+        """The face entries and walls of the current plan as int32 rows in
+        the layout of pc_amr_ghost and pc_amr_wall (plan order)."""
+        cache = self._face_cache
+        rows = [cache[p.handle] for p in self.leaves]
+        gh = np.concatenate([c[4] for c in rows]) if rows else np.zeros((0, 9), np.int32)
+        wl = np.concatenate([c[5] for c in rows]) if rows else np.zeros((0, 3), np.int32)
+        return gh, wl
 
     def step(self, end_time=np.inf):
         """One global step; returns the applied dt (0 when the end time is
@@ -654,21 +670,19 @@ class BatchedAmr:
                                                                                 v.handle)]
         rest = iarr_list(pairs)
         plan, walls = grid._plan
-        gh = (AmrGhost * max(len(plan), 1))()
-        for k, (kind, src, dst, size, soff, doff) in enumerate(plan):
-            g = gh[k]
-            g.kind = 0 if kind == "transfer" else 1
-            g.src, g.dst = src.handle, dst.handle
-            g.size[:] = size
-            g.src_offset[:] = soff
-            g.dst_offset[:] = doff
-        wl = (AmrWall * max(len(walls), 1))()
-        for k, (p, dim, upper) in enumerate(walls):
-            wl[k].slot, wl[k].dimension, wl[k].upper = p.handle, dim, upper
+        # the face entries and walls as int32 rows in the layout of
+        # pc_amr_ghost and pc_amr_wall
+        assert ctypes.sizeof(AmrGhost) == 9 * 4 and ctypes.sizeof(AmrWall) == 3 * 4
+        gh, wl = grid.packed_plan()
+        assert len(gh) == len(plan) and len(wl) == len(walls)
+        gh = np.ascontiguousarray(gh if len(gh) else np.zeros((1, 9), np.int32))
+        wl = np.ascontiguousarray(wl if len(wl) else np.zeros((1, 3), np.int32))
         h = ctypes.c_void_p()
         check(self.lib.pc_amr_create(self.pool.h, leaves, len(grid.leaves), virtuals,
-                                     len(grid.virtuals), rest, len(pairs) // 2, gh, len(plan),
-                                     wl, len(walls), ctypes.byref(h)))
+                                     len(grid.virtuals), rest, len(pairs) // 2,
+                                     gh.ctypes.data_as(ctypes.c_void_p), len(plan),
+                                     wl.ctypes.data_as(ctypes.c_void_p), len(walls),
+                                     ctypes.byref(h)))
         self.h = h
 
     def step(self, k=1, end_time=np.inf):
