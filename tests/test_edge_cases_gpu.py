@@ -1,0 +1,147 @@
+"""Edge cases of the C ABI on the GPU: empty pools, invalid arguments that
+must fail with PC_E_INVALID (and leave the pool usable), the smallest and
+largest patch sizes, slot reuse after release, and independent scenarios
+of which one reaches its end time while the other continues (each bitwise
+against the oracle)."""
+import numpy as np
+import pytest
+
+from peanoclaw_amd._abi import PeanoClawAmdError
+
+pytestmark = pytest.mark.gpu
+
+
+def _pool():
+    from peanoclaw_amd.pool import PatchPool
+    return PatchPool(0)
+
+
+def test_empty_pool_steps_and_graphs():
+    pool = _pool()
+    try:
+        pool.step(3)
+        pool.graph_capture()
+        pool.graph_replay(2)
+        pool.synchronize()
+        assert pool.mass() == 0.0
+    finally:
+        pool.close()
+
+
+def test_invalid_arguments_fail_loudly_and_leave_the_pool_usable():
+    from peanoclaw_amd.pool import make_description
+    from peanoclaw_amd.scenarios import dam_break_grid
+    pool = _pool()
+    try:
+        for n, ny in ((1, 8), (8, 1), (65, 8), (8, 65)):
+            with pytest.raises(PeanoClawAmdError) as e:
+                pool.register(make_description((0.0, 0.0), 1.0, n, ny=ny))
+            assert e.value.code == 1, (n, ny)
+        d = make_description((0.0, 0.0), 1.0, 8)
+        d.ghostlayer_width = 2
+        with pytest.raises(PeanoClawAmdError):
+            pool.register(d)
+        with pytest.raises(PeanoClawAmdError):
+            pool.release(12345)
+        with pytest.raises(PeanoClawAmdError):
+            pool.description(-1)
+        with pytest.raises(PeanoClawAmdError):
+            pool.launch_config(fused=3)
+        with pytest.raises(PeanoClawAmdError):
+            pool.row_segments(3)
+        with pytest.raises(PeanoClawAmdError):
+            pool.wave_propagation_solver(3)
+        with pytest.raises(PeanoClawAmdError):
+            pool.prune_slack(0.0)
+        # the pool still works after the rejected calls
+        wl = dam_break_grid(pool, 3, 3, 8)
+        m0 = pool.mass()
+        pool.step(5)
+        assert pool.scenario_get(0)[3] == 5
+        assert abs(pool.mass() - m0) / m0 < 1e-13
+        assert wl.slots
+    finally:
+        pool.close()
+
+
+def test_smallest_and_largest_patches_and_slot_reuse():
+    """A 2 x 2 and a 64 x 64 patch, each alone with reflective walls: five
+    batched steps bitwise against the per-patch solveTimestep sequence of
+    the oracle; then both released and their slots reused by new patches
+    of other sizes, stepped again."""
+    from oracle_lib import Patch as OPatch, lib as olib
+    from peanoclaw_amd.pool import make_description
+    pool = _pool()
+    try:
+        rng = np.random.default_rng(3)
+        slots, refs = [], []
+        for k, n in enumerate((2, 64)):
+            s = pool.register(make_description((2.0 * k, 0.0), 1.0, n, scenario=k))
+            unew = np.empty((3, n, n))
+            unew[0] = rng.uniform(0.5, 2.0, (n, n))
+            unew[1:] = rng.uniform(-0.2, 0.2, (2, n, n))
+            b = rng.uniform(-0.1, 0.1, (n + 2, n + 2))
+            pool.upload(s, unew=unew, b=b)
+            pool.scenario_set(k, 0.0, 0.0, np.inf)
+            pool.set_grid_sides(np.array([[s]], dtype=np.int32))
+            P = OPatch(3, 1, n, 1, (2.0 * k, 0.0), 1.0, 0, 0.0, 0.0)
+            P.unew[:] = unew
+            P.pwg[0] = b
+            slots.append(s)
+            refs.append(P)
+        pool.launch_config(fused=2)
+        for _ in range(5):
+            pool.step(1)
+            for P in refs:
+                # one global step of a lone patch: uNew -> uOld, the walls in
+                # call order, then solveTimestep without a bound
+                olib().or_copy_unew_to_uold(P.ref)
+                for dim, up in ((0, 0), (1, 1), (0, 1), (1, 0)):
+                    olib().or_fill_boundary_layer(P.ref, dim, up)
+                olib().or_solve_timestep(P.ref, float("inf"))
+        for s, P in zip(slots, refs):
+            got = pool.download(s)[0]
+            ref = np.array(P.unew)
+            assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), s
+        for s in slots:
+            pool.release(s)
+        reused = [pool.register(make_description((0.0, 0.0), 1.0, n, ny=m))
+                  for n, m in ((5, 3), (33, 17))]
+        assert sorted(reused) == sorted(slots)
+        for s in reused:
+            nx, ny = pool.dims(s)
+            pool.upload(s, unew=np.stack([np.ones((nx, ny)), np.zeros((nx, ny)),
+                                          np.zeros((nx, ny))]),
+                        b=np.zeros((nx + 2, ny + 2)))
+            pool.set_grid_sides(np.array([[s]], dtype=np.int32))
+        pool.step(3)
+        for s in reused:
+            # a lake at rest stays at rest exactly
+            h = pool.download(s)[0]
+            assert np.all(h[0] == 1.0) and np.all(h[1:] == 0.0)
+    finally:
+        pool.close()
+
+
+def test_one_scenario_reaches_its_end_time_while_another_continues():
+    from peanoclaw_amd.scenarios import dam_break_grid
+    pool = _pool()
+    try:
+        a = dam_break_grid(pool, 3, 3, 16, scenario=0)
+        b = dam_break_grid(pool, 3, 3, 16, scenario=1)
+        pool.scenario_set(0, 0.0, 0.0, 0.002)
+        pool.launch_config(fused=2)
+        pool.step(40)
+        t0, dt0, _, n0 = pool.scenario_get(0)
+        t1, dt1, _, n1 = pool.scenario_get(1)
+        assert abs((t0 + dt0) - 0.002) <= 1e-8, (t0, dt0)
+        assert n0 < 40 and n1 == 40 and t1 + dt1 > 0.002
+        # the finished scenario's state no longer changes
+        before = [pool.download(s)[0].copy() for s in a.slots]
+        pool.step(5)
+        after = [pool.download(s)[0] for s in a.slots]
+        assert all(np.array_equal(x, y) for x, y in zip(before, after))
+        assert pool.scenario_get(1)[3] == 45
+        assert b.slots
+    finally:
+        pool.close()
