@@ -645,35 +645,48 @@ int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
                  p->d_smax, nullptr);
   const int nscn = (int)p->scn.size();
   if (nscn > 0) {
-    if (p->nentries > 0) {
-      if (fused) {
-        const int nl = (int)p->h_live.size();
-        const dim3 g((nl + 3) / 4);
-        if (prune)
-          hipLaunchKernelGGL(k_boundary_speed<true>, g, dim3(256), 0, p->stream, p->d_scn,
-                             p->d_bdesc, nl, F, E, p->d_smax_int, p->d_smax, p->prune_slack);
-        else
-          hipLaunchKernelGGL(k_boundary_speed<false>, g, dim3(256), 0, p->stream, p->d_scn,
-                             p->d_bdesc, nl, F, E, p->d_smax_int, p->d_smax, 1.0);
-      }
-      hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
-                         p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
-                         p->d_pd, p->d_smax, p->d_est, prune ? p->d_redo : nullptr, nullptr);
+    if (p->nentries > 0 && fused) {
+      const int nl = (int)p->h_live.size();
+      const dim3 g((nl + 3) / 4);
+      if (prune)
+        hipLaunchKernelGGL(k_boundary_speed<true>, g, dim3(256), 0, p->stream, p->d_scn,
+                           p->d_bdesc, nl, F, E, p->d_smax_int, p->d_smax, p->prune_slack);
+      else
+        hipLaunchKernelGGL(k_boundary_speed<false>, g, dim3(256), 0, p->stream, p->d_scn,
+                           p->d_bdesc, nl, F, E, p->d_smax_int, p->d_smax, 1.0);
     }
-    hipLaunchKernelGGL(k_dt_final, dim3((nscn + 63) / 64), dim3(64), 0, p->stream, p->d_scn,
-                       nscn, p->d_counter, prune ? 1 : 0, p->d_redo, p->prune_slack);
+    // few patch entries: both dt stages in one workgroup
+    const bool block = p->nentries <= kDtBlockEntries;
+    if (block) {
+      hipLaunchKernelGGL(k_dt_block, dim3(1), dim3(1024), 0, p->stream, p->d_scn, nscn,
+                         p->d_plist_scn, p->d_scn_of, p->nentries, p->d_pd, p->d_smax, p->d_est,
+                         p->d_counter, prune ? 1 : 0, p->d_redo, p->prune_slack);
+    } else {
+      if (p->nentries > 0)
+        hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
+                           p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
+                           p->d_pd, p->d_smax, p->d_est, prune ? p->d_redo : nullptr, nullptr);
+      hipLaunchKernelGGL(k_dt_final, dim3((nscn + 63) / 64), dim3(64), 0, p->stream, p->d_scn,
+                         nscn, p->d_counter, prune ? 1 : 0, p->d_redo, p->prune_slack);
+    }
     if (prune) {
       // exact fallback, a no-op unless the pruned minimum was not proven exact
       if (p->ntasks > 0)
         hipLaunchKernelGGL((k_speed<true, 0>), dim3(std::min(p->ntasks, kFallbackWgs)),
                            dim3(64), 0, p->stream, p->d_tasks, p->ntasks, p->d_plist, p->d_pd,
                            F, p->d_scn, sides, E, p->d_smax, p->d_redo, 1);
-      if (p->nentries > 0)
-        hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
-                           p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
-                           p->d_pd, p->d_smax, p->d_est, nullptr, p->d_redo);
-      hipLaunchKernelGGL(k_dt_final, dim3((nscn + 63) / 64), dim3(64), 0, p->stream,
-                         p->d_scn, nscn, p->d_counter, 2, p->d_redo, p->prune_slack);
+      if (block) {
+        hipLaunchKernelGGL(k_dt_block, dim3(1), dim3(1024), 0, p->stream, p->d_scn, nscn,
+                           p->d_plist_scn, p->d_scn_of, p->nentries, p->d_pd, p->d_smax,
+                           p->d_est, p->d_counter, 2, p->d_redo, p->prune_slack);
+      } else {
+        if (p->nentries > 0)
+          hipLaunchKernelGGL(k_dt_partial, dim3((p->nentries + 1023) / 1024), dim3(256), 0,
+                             p->stream, p->d_scn, p->d_plist_scn, p->d_scn_of, p->nentries,
+                             p->d_pd, p->d_smax, p->d_est, nullptr, p->d_redo);
+        hipLaunchKernelGGL(k_dt_final, dim3((nscn + 63) / 64), dim3(64), 0, p->stream,
+                           p->d_scn, nscn, p->d_counter, 2, p->d_redo, p->prune_slack);
+      }
     }
   }
   if (p->ntasks > 0) {

@@ -976,15 +976,10 @@ k_dt_partial(ScnDev* __restrict__ scn, const int* __restrict__ plist_scn,
 //           and set *redo for the exact fallback pass;
 //   mode 2 (after the fallback pass; nothing unless *redo): commit the
 //           flagged scenarios, reset the running minimum of all.
-__global__ void k_dt_final(ScnDev* __restrict__ scn, int nscn, int* __restrict__ counter,
-                           int mode, int* __restrict__ redo, double slack) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (mode == 2 && gl(redo) == 0) return;
-  if (s == 0 && counter) *counter = 0;
-  if (s >= nscn) return;
-  ScnDev& S = scn[s];
-  const double cfl = __longlong_as_double((long long)S.minbits);
-  S.minbits = (unsigned long long)__double_as_longlong(1.7976931348623157e308);
+// The decision for one scenario given its CFL minimum (the running minimum
+// has been read and reset by the caller).
+__device__ __forceinline__ void dt_final_one(ScnDev& S, double cfl, int mode, int* redo,
+                                             double slack) {
   const double maxDt = S.end_time - (S.time + S.dt);
   const bool can_advance = S.pend > S.pbeg && maxDt - 0.0 > kLaEps;
   if (mode == 1 && can_advance && !(cfl <= S.est * (slack * kPruneCheckFrac))) {
@@ -1011,6 +1006,64 @@ __global__ void k_dt_final(ScnDev* __restrict__ scn, int nscn, int* __restrict__
   S.est = cfl;
   S.dt_applied = dt;
   S.advance = adv;
+}
+
+__global__ void k_dt_final(ScnDev* __restrict__ scn, int nscn, int* __restrict__ counter,
+                           int mode, int* __restrict__ redo, double slack) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mode == 2 && gl(redo) == 0) return;
+  if (s == 0 && counter) *counter = 0;
+  if (s >= nscn) return;
+  ScnDev& S = scn[s];
+  const double cfl = __longlong_as_double((long long)S.minbits);
+  S.minbits = (unsigned long long)__double_as_longlong(1.7976931348623157e308);
+  dt_final_one(S, cfl, mode, redo, slack);
+}
+
+// Both stages in one workgroup, for steps with at most kDtBlockEntries
+// patch entries: the per-patch CFL steps and the per-scenario minima
+// (wavefront minima where a wavefront's entries share a scenario, then
+// atomicMin), a workgroup barrier, then the decision per scenario with the
+// minimum read and reset by one atomic exchange.  The same arguments and
+// no-op conditions as k_dt_partial followed by k_dt_final.
+constexpr int kDtBlockEntries = 4096;
+__global__ void __launch_bounds__(1024)
+k_dt_block(ScnDev* __restrict__ scn, int nscn, const int* __restrict__ plist_scn,
+           const int* __restrict__ scn_of_entry, int nentries, const PatchDev* __restrict__ pd,
+           const double* __restrict__ smax, double* __restrict__ est_patch,
+           int* __restrict__ counter, int mode, int* __restrict__ redo, double slack) {
+  if (mode == 2 && gl(redo) == 0) return;
+  if (mode == 1 && threadIdx.x == 0) *redo = 0;
+  const int lane = threadIdx.x & 63;
+  for (int k0 = threadIdx.x - lane; k0 < nentries; k0 += blockDim.x) {
+    const int k = k0 + lane;
+    const bool ok = k < nentries;
+    double v = 1.7976931348623157e308;
+    int sc = -1;
+    if (ok) {
+      const int p = plist_scn[k];
+      v = max_timestep(smax[p], pd[p].dx, pd[p].dy);
+      est_patch[p] = v;
+      sc = scn_of_entry[k];
+    }
+    const int s0 = __shfl(sc, 0);
+    const int last = min(63, nentries - 1 - k0);
+    if (__shfl(sc, last) == s0) {
+      // entries of one scenario (they are grouped by scenario)
+      for (int o = 32; o >= 1; o >>= 1) v = dmin(v, __shfl_xor(v, o));
+      if (lane == 0) atomicMin(&scn[s0].minbits, (unsigned long long)__double_as_longlong(v));
+    } else if (ok) {
+      atomicMin(&scn[sc].minbits, (unsigned long long)__double_as_longlong(v));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && counter) *counter = 0;
+  for (int s = threadIdx.x; s < nscn; s += blockDim.x) {
+    ScnDev& S = scn[s];
+    const unsigned long long bits = atomicExch(
+        &S.minbits, (unsigned long long)__double_as_longlong(1.7976931348623157e308));
+    dt_final_one(S, __longlong_as_double((long long)bits), mode, redo, slack);
+  }
 }
 
 // Ghost fill of the current state's tile ghost cells: one wavefront per strip.
