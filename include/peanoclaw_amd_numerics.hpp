@@ -234,6 +234,15 @@ class SWENumerics {
     return {w[0], w[1]};
   }
 
+  /* The scenario's refinement parameters (BreakingDamSWEScenario's
+   * threshold and minimal / maximal mesh widths) for getDemandedMeshWidth. */
+  void setRefinementCriterion(double threshold, double minimalMeshWidth,
+                              double maximalMeshWidth) {
+    opt_.refinement_threshold = threshold;
+    opt_.minimal_mesh_width = minimalMeshWidth;
+    opt_.maximal_mesh_width = maximalMeshWidth;
+  }
+
   int getNumberOfUnknownsPerCell() const { return pc_get_number_of_unknowns_per_cell(); }
   int getNumberOfParameterFieldsWithoutGhostlayer() const {
     return pc_get_number_of_parameter_fields_without_ghostlayer();

@@ -24,6 +24,7 @@ class RecordingBackend(OracleBackend):
         self.blob = []
         self.nblob = 0
         self.sizes = []
+        self.released = set()
 
     def new_patch(self, pos, size, n, level, virtual, time=0.0, dt=0.0):
         h = super().new_patch(pos, size, n, level, virtual, time, dt)
@@ -34,11 +35,7 @@ class RecordingBackend(OracleBackend):
 
     def set_state(self, s, h, b):
         super().set_state(s, h, b)
-        self.ops.append(f"S {s} {self.nblob}")
-        for a in (h, b):
-            a = np.ascontiguousarray(a, dtype=np.float64).ravel()
-            self.blob.append(a)
-            self.nblob += a.size
+        self.ops.append(f"S {s} {self._blob(h, b)}")
 
     def set_virtual(self, s, virtual):
         super().set_virtual(s, virtual)
@@ -88,11 +85,41 @@ class RecordingBackend(OracleBackend):
         self.ops.append(f"X {s} {hx(max_dt)}")
         return super().solve(s, max_dt)
 
-    # regridding is not part of the recorded sequences
-    def flags(self, *a, **k):
-        raise NotImplementedError("regrid calls are not recorded")
+    def flags(self, slots, threshold, minimal_width, maximal_width):
+        # the demanded mesh width of every patch, which the replay checks
+        import ctypes
 
-    switch_to_leaf = release = set_b = flags
+        from oracle_lib import lib
+        for s in slots:
+            dw = (ctypes.c_double * 2)()
+            lib().or_demanded_mesh_width(self.p[s].ref, threshold, minimal_width, maximal_width,
+                                         dw)
+            self.ops.append(f"D {s} {hx(threshold)} {hx(minimal_width)} {hx(maximal_width)} "
+                            f"{hx(dw[0])} {hx(dw[1])}")
+        return super().flags(slots, threshold, minimal_width, maximal_width)
+
+    def _blob(self, *arrays):
+        off = self.nblob
+        for a in arrays:
+            a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+            self.blob.append(a)
+            self.nblob += a.size
+        return off
+
+    def switch_to_leaf(self, s, time, dt):
+        super().switch_to_leaf(s, time, dt)
+        P = self.p[s]
+        off = self._blob(np.array(P.uold), np.array(P.unew))
+        self.ops.append(f"U {s} {off} {hx(time)} {hx(dt)}")
+
+    def set_b(self, s, b):
+        super().set_b(s, b)
+        self.ops.append(f"B {s} {self._blob(b)}")
+
+    def release(self, s):
+        super().release(s)
+        self.ops.append(f"Q {s}")
+        self.released.add(s)
 
     def write(self, directory):
         trace = os.path.join(directory, "trace.txt")
@@ -107,6 +134,8 @@ class RecordingBackend(OracleBackend):
         raw = np.fromfile(path, dtype=np.float64)
         out, o = {}, 0
         for h, n in enumerate(self.sizes):
+            if h in self.released:
+                continue
             clk = raw[o:o + 3]
             o += 3
             u = raw[o:o + 3 * n * n].reshape(3, n, n)

@@ -18,9 +18,18 @@
 //   F h cfl                                 the CFL step of h; must equal `cfl`
 //   K h time dt                             the host clock of h; must equal
 //   X h maxdt                               solveTimestep(maxdt)
-// OUT receives, per patch in creation order: time, timestep size, estimated
-// next timestep size (3 doubles), then uNew after addPatchToSolution.
-// Exit status 0 when every F and K matched, 1 otherwise, 2 on errors.
+//   D h thr minw maxw w0 w1                 getDemandedMeshWidth with the
+//                                           criterion (thr, minw, maxw); must
+//                                           equal (w0, w1)
+//   U h offset time dt                      switch to a leaf: heap uOld (3
+//                                           (n+2)^2) and uNew (3 n^2) <- blob,
+//                                           clock (time, dt); upload
+//   B h offset                              heap b <- blob ((n+2)^2); upload
+//   Q h                                     destroyPatch (the slot is released)
+// OUT receives, per live (not released) patch in creation order: time,
+// timestep size, estimated next timestep size (3 doubles), then uNew after
+// addPatchToSolution.
+// Exit status 0 when every F, K and D matched, 1 otherwise, 2 on errors.
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -105,6 +114,7 @@ class HeapPatch {
   int getLevel() const { return c_.level; }
   bool isVirtual() const { return c_.isVirtual; }
   void setVirtual(bool v) { c_.isVirtual = v; }
+  void setDemandedMeshWidth(double w0, double w1) { c_.demanded = {w0, w1}; }
   TimeIntervals& getTimeIntervals() { return ti_; }
   Accessor& getAccessor() { return acc_; }
   std::vector<double>& heap() { return heap_; }
@@ -144,6 +154,7 @@ int main(int argc, char** argv) {
     return 2;
   }
   long calls = 0, cflChecks = 0, cflMismatch = 0, clockChecks = 0, clockMismatch = 0;
+  long widthChecks = 0, widthMismatch = 0;
   std::vector<std::unique_ptr<HeapPatch>> patches;
   try {
     peanoclaw_amd::SWENumerics<HeapPatch> num;
@@ -247,6 +258,48 @@ int main(int argc, char** argv) {
         in >> h >> m;
         const std::array<bool, 4> flags{false, false, false, false};
         num.solveTimestep(P(h), hexd(m), false, flags);
+      } else if (op == "D") {
+        int h;
+        std::string thr, minw, maxw, w0, w1;
+        in >> h >> thr >> minw >> maxw >> w0 >> w1;
+        num.setRefinementCriterion(hexd(thr), hexd(minw), hexd(maxw));
+        const std::pair<double, double> w = num.getDemandedMeshWidth(P(h), false);
+        P(h).setDemandedMeshWidth(w.first, w.second);
+        ++widthChecks;
+        if (!(w.first == hexd(w0) && w.second == hexd(w1))) {
+          if (widthMismatch++ < 5)
+            std::fprintf(stderr, "demanded width of patch %d: (%a, %a), expected (%s, %s)\n", h,
+                         w.first, w.second, w0.c_str(), w1.c_str());
+        }
+      } else if (op == "U" || op == "B") {
+        int h;
+        size_t off;
+        in >> h >> off;
+        HeapPatch& p = P(h);
+        const int n = p.n();
+        const size_t a = (size_t)n * n, g = (size_t)(n + 2) * (n + 2);
+        std::vector<double>& u = p.heap();
+        num.downloadToHeap(p);
+        if (op == "U") {
+          std::string t, dt;
+          in >> t >> dt;
+          if (off + 3 * g + 3 * a > blob.size())
+            throw peanoclaw_amd::Error(PC_E_INVALID, "blob too short");
+          std::copy(blob.begin() + off, blob.begin() + off + 3 * g, u.begin() + 3 * a);
+          std::copy(blob.begin() + off + 3 * g, blob.begin() + off + 3 * g + 3 * a, u.begin());
+          p.setVirtual(false);
+          p.getTimeIntervals().setCurrentTime(hexd(t));
+          p.getTimeIntervals().setTimestepSize(hexd(dt));
+        } else {
+          if (off + g > blob.size()) throw peanoclaw_amd::Error(PC_E_INVALID, "blob too short");
+          std::copy(blob.begin() + off, blob.begin() + off + g, u.begin() + 3 * a + 3 * g);
+        }
+        num.uploadFromHeap(p);
+      } else if (op == "Q") {
+        int h;
+        in >> h;
+        num.destroyPatch(P(h));
+        patches[h].reset();
       } else {
         throw peanoclaw_amd::Error(PC_E_INVALID, "unknown trace operation " + op);
       }
@@ -256,6 +309,9 @@ int main(int argc, char** argv) {
     if (!out) throw peanoclaw_amd::Error(PC_E_INVALID, std::string("cannot write ") + argv[3]);
     bool ok = true;
     for (auto& pp : patches) {
+      if (!pp) {  // released: no record (the reader skips it)
+        continue;
+      }
       HeapPatch& p = *pp;
       num.addPatchToSolution(p);
       auto& ti = p.getTimeIntervals();
@@ -271,8 +327,9 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "error (status %d) after %ld calls: %s\n", e.code(), calls, e.what());
     return 2;
   }
-  std::printf("replayed %ld calls on %zu patches; cfl %ld/%ld equal, clocks %ld/%ld equal\n",
+  std::printf("replayed %ld calls on %zu patches; cfl %ld/%ld equal, clocks %ld/%ld equal, "
+              "demanded widths %ld/%ld equal\n",
               calls, patches.size(), cflChecks - cflMismatch, cflChecks,
-              clockChecks - clockMismatch, clockChecks);
-  return (cflMismatch || clockMismatch) ? 1 : 0;
+              clockChecks - clockMismatch, clockChecks, widthChecks - widthMismatch, widthChecks);
+  return (cflMismatch || clockMismatch || widthMismatch) ? 1 : 0;
 }

@@ -4,9 +4,10 @@ CPU: the header compiles against a patch type with a float heap (the
 reference's SWE builds store float) and a custom access policy; the recorded
 call trace of an adaptive-grid run is well formed; the replay program fails
 loudly without a device.  GPU: the replay program runs the recorded calls of
-a two-level grid (global steps, then local time stepping) through the adapter
-on heap-backed patches, and its heap contents and clocks equal the oracle's
-bitwise."""
+a two-level grid (global steps, then local time stepping) and of a grid that
+is refined and coarsened (demanded mesh widths, new patches, switches to
+leaves, released patches) through the adapter on heap-backed patches, and
+its heap contents and clocks equal the oracle's bitwise."""
 import os
 import subprocess
 
@@ -106,7 +107,7 @@ def test_recorded_trace_is_well_formed(tmp_path):
     trace, blob = rec.write(str(tmp_path))
     ops = [l.split() for l in open(trace).read().splitlines()]
     arity = {"N": 9, "S": 2, "V": 2, "Z": 1, "R": 2, "P": 1, "C": 1, "T": 8, "I": 9, "W": 3,
-             "F": 2, "K": 3, "X": 2}
+             "F": 2, "K": 3, "X": 2, "D": 6, "U": 4, "B": 2, "Q": 1}
     assert ops and all(o[0] in arity and len(o) == 1 + arity[o[0]] for o in ops)
     kinds = {o[0] for o in ops}
     # both level transfers, walls, restriction and the LTS clock checks occur
@@ -156,4 +157,50 @@ def test_adapter_replay_two_level_bitwise(tmp_path):
             assert clk[0] == P.s.time and clk[1] == P.s.dt, h
     print(f"adapter replay: {len(rec.ops)} calls, {its} LTS iterations ({leaf_steps} leaf steps); "
           f"bitwise {same}/{total}")
+    assert same == total
+
+
+def _regrid_run(rec):
+    """A 6 x 6 x 9^2 grid without refinement: a regrid that refines the
+    dam-break front, 12 steps, a regrid with a threshold no gradient reaches
+    (coarsens every patch whose children are old enough), 6 more steps."""
+    from peanoclaw_amd.amr import AmrGrid, dam_break_fields
+    fn = dam_break_fields(centre=(0.47, 0.53), radius=0.25, hills=[(0.3, 0.6, 0.1, 0.2)])
+    g = AmrGrid(rec, 6, 6, 9, lambda lev, ij: False, fn, max_level=2)
+    refined = g.regrid()
+    g.run(12)
+    coarsened = g.regrid(threshold=1e9)
+    g.run(6)
+    return g, refined, coarsened
+
+
+def test_recorded_regrid_trace_has_remesh_calls(tmp_path):
+    rec = RecordingBackend()
+    g, refined, coarsened = _regrid_run(rec)
+    assert refined[0] > 0 and coarsened[1] > 0
+    kinds = {l.split()[0] for l in rec.ops}
+    assert {"D", "U", "B", "Q", "V"} <= kinds
+
+
+@pytest.mark.gpu
+def test_adapter_replay_regrid_bitwise(tmp_path):
+    """Remesh through the adapter: demanded mesh widths on the device equal
+    the oracle's, new patches registered with b from the initial data and
+    interpolated from their parent, coarsened patches restricted and
+    switched to leaves, released children destroyed; the heap contents of
+    every live patch equal the oracle's bitwise."""
+    assert os.path.exists(REPLAY), "replay program not built (python -m peanoclaw_amd.build)"
+    rec = RecordingBackend()
+    g, refined, coarsened = _regrid_run(rec)
+    r, out = replay(rec, str(tmp_path))
+    print(r.stdout.strip(), r.stderr.strip()[-2000:])
+    assert r.returncode == 0, r.stderr
+    res = rec.read_result(out)
+    same = total = 0
+    for h, (clk, u) in res.items():
+        ref = np.array(rec.p[h].unew)
+        same += int((u.view(np.uint64) == ref.view(np.uint64)).sum())
+        total += u.size
+    print(f"adapter regrid replay: refined {refined}, coarsened {coarsened}, "
+          f"{len(rec.released)} patches released; bitwise {same}/{total}")
     assert same == total
