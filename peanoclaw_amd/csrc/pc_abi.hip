@@ -113,7 +113,7 @@ struct pc_pool {
   std::vector<int> strip_begin;  // per slot (for the per-patch form)
   int* d_counter = nullptr;  // [0] persistent task counter, [1] pruned-mode redo flag
   int* d_redo = nullptr;     // = d_counter + 1
-  int* d_slot1 = nullptr;  // one-entry patch list of pc_solve_timestep
+  int* d_iota = nullptr;   // d_iota[i] = i for every slot (per-patch kernels' lists)
   double* d_result = nullptr;
   double* d_aux = nullptr;
   size_t aux_cap = 0;
@@ -381,6 +381,12 @@ int rebuild_topology(pc_pool* p) {
       HIPCHK(hipMalloc(arrs[k], sizes[k]));
       HIPCHK(hipMemsetAsync(*arrs[k], 0, sizes[k], p->stream));
     }
+    if (p->d_iota) HIPCHK(hipFree(p->d_iota));
+    p->d_iota = nullptr;
+    HIPCHK(hipMalloc(&p->d_iota, cap * sizeof(int)));
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, p->stream,
+                       p->d_iota, (int)cap);
+    HIPCHK(hipGetLastError());
     p->edges_dirty = true;
     realloc = true;
   }
@@ -741,7 +747,6 @@ int pc_pool_create(int device, size_t hbm_budget_bytes, pc_pool** out) {
   HIPCHK(hipMemsetAsync(p->d_counter, 0, 2 * sizeof(int), p->stream));
   p->d_redo = p->d_counter + 1;
   HIPCHK(hipMalloc(&p->d_result, 8 * sizeof(double)));
-  HIPCHK(hipMalloc(&p->d_slot1, sizeof(int)));
   HIPCHK(hipEventCreate(&p->ev0));
   HIPCHK(hipEventCreate(&p->ev1));
   HIPCHK(hipEventCreate(&p->ev2));
@@ -759,7 +764,7 @@ int pc_pool_destroy(pc_pool* p) {
     if (p->f[k]) hipFree(p->f[k]);
   void* ptrs[] = {p->d_pd, p->d_smax, p->d_smax_int, p->d_est, p->d_scn, p->d_tasks, p->d_utasks,
                   p->d_plist, p->d_plist_scn, p->d_scn_of, p->d_live, p->d_bdesc, p->d_strips,
-                  p->d_counter, p->d_slot1, p->d_result, p->d_aux, p->d_flags, p->d_ebuf[0],
+                  p->d_counter, p->d_iota, p->d_result, p->d_aux, p->d_flags, p->d_ebuf[0],
                   p->d_ebuf[1], p->d_bcol, p->d_sides};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -1014,13 +1019,11 @@ int pc_solve_timestep(pc_pool* p, int slot, double maxDt, const int flags[4],
     slot_clock(p, slot, &s.desc.time, &s.desc.timestep_size);
     s.clock_from_scenario = false;
   }
-  HIPCHK(hipMemcpyAsync(p->d_slot1, &slot, sizeof(int), hipMemcpyHostToDevice, p->stream));
-  HIPCHK(hipStreamSynchronize(p->stream));
   if (p->solver)
-    hipLaunchKernelGGL(k_patch_solve<1>, dim3(1), dim3(64), 0, p->stream, slot, p->d_slot1,
+    hipLaunchKernelGGL(k_patch_solve<1>, dim3(1), dim3(64), 0, p->stream, slot, p->d_iota,
                        p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_smax, p->d_result);
   else
-    hipLaunchKernelGGL(k_patch_solve<0>, dim3(1), dim3(64), 0, p->stream, slot, p->d_slot1,
+    hipLaunchKernelGGL(k_patch_solve<0>, dim3(1), dim3(64), 0, p->stream, slot, p->d_iota,
                        p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_smax, p->d_result);
   HIPCHK(hipGetLastError());
   double res[2];
@@ -1042,13 +1045,11 @@ int pc_patch_cfl(pc_pool* p, int slot, double* out) {
   if (!valid_slot(p, slot) || !out) return fail(PC_E_INVALID, "invalid slot or output");
   hipSetDevice(p->device);
   CHK(prepare(p));
-  HIPCHK(hipMemcpyAsync(p->d_slot1, &slot, sizeof(int), hipMemcpyHostToDevice, p->stream));
-  HIPCHK(hipStreamSynchronize(p->stream));
   if (p->solver)
-    hipLaunchKernelGGL(k_patch_cfl<1>, dim3(1), dim3(64), 0, p->stream, slot, p->d_slot1,
+    hipLaunchKernelGGL(k_patch_cfl<1>, dim3(1), dim3(64), 0, p->stream, slot, p->d_iota,
                        p->d_pd, fields_of(p), cur_of(p, slot), p->d_smax, p->d_result);
   else
-    hipLaunchKernelGGL(k_patch_cfl<0>, dim3(1), dim3(64), 0, p->stream, slot, p->d_slot1,
+    hipLaunchKernelGGL(k_patch_cfl<0>, dim3(1), dim3(64), 0, p->stream, slot, p->d_iota,
                        p->d_pd, fields_of(p), cur_of(p, slot), p->d_smax, p->d_result);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, p->d_result, sizeof(double), hipMemcpyDeviceToHost, p->stream));

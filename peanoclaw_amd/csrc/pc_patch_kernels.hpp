@@ -9,6 +9,12 @@
 
 namespace pcamd {
 
+// a[i] = i: the one-patch lists of the per-patch kernels (plist + slot)
+__global__ void k_iota(int* __restrict__ a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+
 __device__ __forceinline__ long long tile_index(const PatchDev& P, int x, int y) {
   // cell (x, y) with x, y in [-1, n]
   return P.off + (long long)(x + 1) * (P.ny + 2) + (y + 1);
@@ -26,7 +32,7 @@ k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restric
   __shared__ MarchLds L;
   const PatchDev P = pd[slot];
   Task t;
-  t.first = 0; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;
+  t.first = slot; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;  // plist[slot] == slot
   EdgeBufs none{{nullptr, nullptr}, nullptr};
   MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
   march_task<kSpeed, false, 0, SOLVER>(t, A, L);
@@ -53,7 +59,7 @@ k_patch_cfl(int slot, const int* __restrict__ plist, const PatchDev* __restrict_
   __shared__ MarchLds L;
   const PatchDev P = pd[slot];
   Task t;
-  t.first = 0; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;
+  t.first = slot; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;  // plist[slot] == slot
   EdgeBufs none{{nullptr, nullptr}, nullptr};
   MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
   march_task<kSpeed, false, 0, SOLVER>(t, A, L);
