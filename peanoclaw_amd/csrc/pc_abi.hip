@@ -114,6 +114,7 @@ struct pc_pool {
   int* d_counter = nullptr;  // [0] persistent task counter, [1] pruned-mode redo flag
   int* d_redo = nullptr;     // = d_counter + 1
   int* d_iota = nullptr;   // d_iota[i] = i for every slot (per-patch kernels' lists)
+  double* d_segmax = nullptr;  // row-segment speed maxima of the one-patch kernels
   double* d_result = nullptr;
   double* d_aux = nullptr;
   size_t aux_cap = 0;
@@ -747,6 +748,7 @@ int pc_pool_create(int device, size_t hbm_budget_bytes, pc_pool** out) {
   HIPCHK(hipMemsetAsync(p->d_counter, 0, 2 * sizeof(int), p->stream));
   p->d_redo = p->d_counter + 1;
   HIPCHK(hipMalloc(&p->d_result, 8 * sizeof(double)));
+  HIPCHK(hipMalloc(&p->d_segmax, kMaxRowSeg * sizeof(double)));
   HIPCHK(hipEventCreate(&p->ev0));
   HIPCHK(hipEventCreate(&p->ev1));
   HIPCHK(hipEventCreate(&p->ev2));
@@ -764,7 +766,7 @@ int pc_pool_destroy(pc_pool* p) {
     if (p->f[k]) hipFree(p->f[k]);
   void* ptrs[] = {p->d_pd, p->d_smax, p->d_smax_int, p->d_est, p->d_scn, p->d_tasks, p->d_utasks,
                   p->d_plist, p->d_plist_scn, p->d_scn_of, p->d_live, p->d_bdesc, p->d_strips,
-                  p->d_counter, p->d_iota, p->d_result, p->d_aux, p->d_flags, p->d_ebuf[0],
+                  p->d_counter, p->d_iota, p->d_segmax, p->d_result, p->d_aux, p->d_flags, p->d_ebuf[0],
                   p->d_ebuf[1], p->d_bcol, p->d_sides};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -1019,12 +1021,15 @@ int pc_solve_timestep(pc_pool* p, int slot, double maxDt, const int flags[4],
     slot_clock(p, slot, &s.desc.time, &s.desc.timestep_size);
     s.clock_from_scenario = false;
   }
+  const int nseg = patch_row_segments(s.desc.subdivision_factor[0]);
   if (p->solver)
-    hipLaunchKernelGGL(k_patch_solve<1>, dim3(1), dim3(64), 0, p->stream, slot, p->d_iota,
-                       p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_smax, p->d_result);
+    hipLaunchKernelGGL(k_patch_solve<1>, dim3(1), dim3(64 * nseg), 0, p->stream, slot,
+                       p->d_iota, p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_segmax,
+                       p->d_result);
   else
-    hipLaunchKernelGGL(k_patch_solve<0>, dim3(1), dim3(64), 0, p->stream, slot, p->d_iota,
-                       p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_smax, p->d_result);
+    hipLaunchKernelGGL(k_patch_solve<0>, dim3(1), dim3(64 * nseg), 0, p->stream, slot,
+                       p->d_iota, p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_segmax,
+                       p->d_result);
   HIPCHK(hipGetLastError());
   double res[2];
   HIPCHK(hipMemcpyAsync(res, p->d_result, 2 * sizeof(double), hipMemcpyDeviceToHost,
@@ -1045,12 +1050,15 @@ int pc_patch_cfl(pc_pool* p, int slot, double* out) {
   if (!valid_slot(p, slot) || !out) return fail(PC_E_INVALID, "invalid slot or output");
   hipSetDevice(p->device);
   CHK(prepare(p));
+  const int nseg = patch_row_segments(p->slots[slot].desc.subdivision_factor[0]);
   if (p->solver)
-    hipLaunchKernelGGL(k_patch_cfl<1>, dim3(1), dim3(64), 0, p->stream, slot, p->d_iota,
-                       p->d_pd, fields_of(p), cur_of(p, slot), p->d_smax, p->d_result);
+    hipLaunchKernelGGL(k_patch_cfl<1>, dim3(1), dim3(64 * nseg), 0, p->stream, slot,
+                       p->d_iota, p->d_pd, fields_of(p), cur_of(p, slot), p->d_segmax,
+                       p->d_result);
   else
-    hipLaunchKernelGGL(k_patch_cfl<0>, dim3(1), dim3(64), 0, p->stream, slot, p->d_iota,
-                       p->d_pd, fields_of(p), cur_of(p, slot), p->d_smax, p->d_result);
+    hipLaunchKernelGGL(k_patch_cfl<0>, dim3(1), dim3(64 * nseg), 0, p->stream, slot,
+                       p->d_iota, p->d_pd, fields_of(p), cur_of(p, slot), p->d_segmax,
+                       p->d_result);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, p->d_result, sizeof(double), hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));

@@ -21,29 +21,41 @@ __device__ __forceinline__ long long tile_index(const PatchDev& P, int x, int y)
 }
 
 // SWEKernel::solveTimestep on one patch (SWEKernel.cpp:50-88): speed of
-// every edge of uOld, dt = fmin(CFL, maxDt), update, swap.
-// `plist` is a one-entry patch list in global memory holding `slot` (the
-// march reads its patch list with global loads).
+// every edge of uOld, dt = fmin(CFL, maxDt), update, swap.  The speeds run
+// in blockDim/64 row segments, one wavefront each, their maxima meeting in
+// `segmax` (kMaxRowSeg doubles); the first wavefront then updates the whole
+// patch.  `plist` holds plist[slot] == slot in global memory (the march
+// reads its patch list with global loads).
 template <int SOLVER>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64 * kMaxRowSeg)
 k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restrict__ pd,
-              Fields F, int cur, double maxDt, double* __restrict__ smax,
+              Fields F, int cur, double maxDt, double* __restrict__ segmax,
               double* __restrict__ out) {
-  __shared__ MarchLds L;
+  __shared__ MarchLds L[kMaxRowSeg];
+  const int w = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
   const PatchDev P = pd[slot];
   Task t;
-  t.first = slot; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;  // plist[slot] == slot
+  t.first = slot; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;
+  Task ts = t;
+  if (nw > 1) {
+    ts.r0 = seg_first_row(w, P.nx, nw);
+    ts.r1 = seg_first_row(w + 1, P.nx, nw) - 1;
+    ts.sidx = w;
+  }
   EdgeBufs none{{nullptr, nullptr}, nullptr};
-  MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
-  march_task<kSpeed, false, 0, SOLVER>(t, A, L);
+  MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, segmax, nullptr, 1.0};
+  A.smax_stride = 0;  // segment w's maximum to segmax[w]
+  march_task<kSpeed, false, 0, SOLVER>(ts, A, L[w]);
   __syncthreads();
-  const double s = smax[slot];
+  if (w != 0) return;
+  double s = segmax[0];
+  for (int k = 1; k < nw; ++k) s = dmax(s, segmax[k]);
   const double cfl = max_timestep(s, P.dx, P.dy);
   const double dt = fmin(cfl, maxDt);
   A.par = cur;
   A.dt = dt;
   A.smax = nullptr;
-  march_task<kUpdateSwap, false, 0, SOLVER>(t, A, L);
+  march_task<kUpdateSwap, false, 0, SOLVER>(t, A, L[0]);
   if (threadIdx.x == 0) {
     out[0] = dt;
     out[1] = cfl;
@@ -51,20 +63,38 @@ k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restric
 }
 
 // The CFL step of one patch from uOld (state 1 - cur) with its ghost layer,
-// without the update: out[0] = max_timestep(max edge speed).
+// without the update: out[0] = max_timestep(max edge speed); row segments
+// as in k_patch_solve.
 template <int SOLVER>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64 * kMaxRowSeg)
 k_patch_cfl(int slot, const int* __restrict__ plist, const PatchDev* __restrict__ pd,
-            Fields F, int cur, double* __restrict__ smax, double* __restrict__ out) {
-  __shared__ MarchLds L;
+            Fields F, int cur, double* __restrict__ segmax, double* __restrict__ out) {
+  __shared__ MarchLds L[kMaxRowSeg];
+  const int w = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
   const PatchDev P = pd[slot];
   Task t;
-  t.first = slot; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;  // plist[slot] == slot
+  t.first = slot; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;
+  if (nw > 1) {
+    t.r0 = seg_first_row(w, P.nx, nw);
+    t.r1 = seg_first_row(w + 1, P.nx, nw) - 1;
+    t.sidx = w;
+  }
   EdgeBufs none{{nullptr, nullptr}, nullptr};
-  MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, smax, nullptr, 1.0};
-  march_task<kSpeed, false, 0, SOLVER>(t, A, L);
+  MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, segmax, nullptr, 1.0};
+  A.smax_stride = 0;
+  march_task<kSpeed, false, 0, SOLVER>(t, A, L[w]);
   __syncthreads();
-  if (threadIdx.x == 0) out[0] = max_timestep(smax[slot], P.dx, P.dy);
+  if (threadIdx.x == 0) {
+    double s = segmax[0];
+    for (int k = 1; k < nw; ++k) s = dmax(s, segmax[k]);
+    out[0] = max_timestep(s, P.dx, P.dy);
+  }
+}
+
+// Row segments of the one-patch kernels: at least 8 rows each, at most
+// kMaxRowSeg.
+__host__ __device__ __forceinline__ int patch_row_segments(int nx) {
+  return nx >= 8 * kMaxRowSeg ? kMaxRowSeg : (nx >= 16 ? 2 : 1);
 }
 
 // SWEKernel::fillBoundaryLayer (SWEKernel.cpp:176-237) on uOld (state st).
