@@ -152,3 +152,19 @@ def replay(rec, directory, timeout=300):
     r = subprocess.run([REPLAY, trace, blob, out], capture_output=True, text=True,
                        timeout=timeout)
     return r, out
+
+
+if __name__ == "__main__":
+    # Writes the recorded calls of the two-level run (12 global steps, up to
+    # 40 local-time-stepping iterations) to DIR/trace.txt and DIR/blob.bin,
+    # the inputs of: tests/cpp/numerics_replay DIR/trace.txt DIR/blob.bin OUT
+    import sys
+
+    sys.path.insert(0, ROOT)
+    from test_numerics_adapter import _two_level_run
+    d = sys.argv[1]
+    os.makedirs(d, exist_ok=True)
+    rec = RecordingBackend()
+    _two_level_run(rec)
+    rec.write(d)
+    print(f"{len(rec.ops)} calls on {len(rec.sizes)} patches written to {d}")
