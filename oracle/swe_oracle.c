@@ -30,6 +30,10 @@
  *       Region.cpp:174-315; pinned by GridLevelTransferTest overlap areas
  *       (2.25, 0, 0) and GhostLayerCompositorTest region tests, plus
  *       conservation of the area-weighted average.
+ *   or_switch_to_leaf  Patch.cpp:55-76, 793-830; pinned by the closed form
+ *       of restricting a linear-in-time fine leaf and switching the virtual
+ *       patch to a leaf: its uOld / uNew equal the fine data's area average
+ *       at its own times (tests/test_oracle_restriction_time.py).
  *   compositor + traversals  pinned by the traversal call-order tests and the
  *       time-stepping veto test (GhostLayerCompositorTest.cpp:59-124).
  *   BreakingDam, demanded mesh width, refinement decision: BreakingDam.cpp:
@@ -1003,6 +1007,31 @@ void or_post_process_restriction_swe(or_patch *dst, int onlyOverlapped) {
   int z[2] = {0, 0}, dn[2] = {dst->nx, dst->ny};
   or_transform_water_height(dst, z, dn, 1, 1);
   or_transform_water_height(dst, z, dn, 0, 1);
+}
+
+/* Patch::switchToLeaf (Patch.cpp:793-830) with
+ * switchRegionToMinimalFineGridTimeInterval over the whole patch
+ * (Patch.cpp:55-76): a virtual patch holding restricted values on the
+ * [0, 1] time basis becomes a leaf whose uOld / uNew are those values at
+ * its own times `time` and `time + dt` (set in the description before the
+ * call).  Every cell is independent, so the loop order is free. */
+void or_switch_to_leaf(or_patch *p) {
+  p->is_virtual = 0;
+  p->is_leaf = 1;
+  const double time = 0.0, timestepSize = 1.0;
+  double factorForUOld = 1.0, factorForUNew = 1.0;
+  if (!la_equals(timestepSize, 0.0)) {
+    factorForUOld = (p->time - time) / timestepSize;
+    factorForUNew = (p->time + p->dt - time) / timestepSize;
+  }
+  for (int x = 0; x < p->nx; x++)
+    for (int y = 0; y < p->ny; y++)
+      for (int q = 0; q < p->nq; q++) {
+        const int io = or_uold_index(p, q, x, y), in = or_unew_index(p, q, x, y);
+        const double vo = p->uold[io], vn = p->unew[in];
+        p->uold[io] = vo * (1.0 - factorForUOld) + vn * factorForUOld;
+        p->unew[in] = vo * (1.0 - factorForUNew) + vn * factorForUNew;
+      }
 }
 
 /* ------------------------------------------------------------------ */

@@ -122,6 +122,9 @@ void or_interpolate_swe_pure(const int dstSize[2], const int dstOff[2], or_patch
                              or_patch *dst, int toUOld, int toCurrentTime,
                              int useTimeUNewOrUOld);
 void or_post_process_restriction_swe(or_patch *dst, int onlyOverlapped);
+/* Patch::switchToLeaf: restricted [0, 1]-basis values -> the patch's own
+   interval [time, time + dt] (set p->time, p->dt first) */
+void or_switch_to_leaf(or_patch *p);
 void or_transform_water_height(or_patch *p, const int off[2], const int size[2],
                                int modifyUOld, int absoluteToAboveSeaFloor);
 

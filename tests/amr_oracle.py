@@ -47,15 +47,9 @@ class OracleBackend:
         return out
 
     def switch_to_leaf(self, s, time, dt):
-        from peanoclaw_amd.amr import switch_to_leaf_values
         P = self.p[s]
-        uold = np.array(P.uold)
-        a, b = switch_to_leaf_values(uold[:, 1:-1, 1:-1], np.array(P.unew), time, dt)
-        uold[:, 1:-1, 1:-1] = a
-        P.uold[:] = uold
-        P.unew[:] = b
-        self.set_virtual(s, False)
         P.s.time, P.s.dt = time, dt
+        lib().or_switch_to_leaf(P.ref)
 
     def clear(self, s):
         self.p[s].unew[:] = 0.0

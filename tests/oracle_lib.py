@@ -112,6 +112,7 @@ def lib():
             "or_interpolate_swe_pure": (None, [_ip, _ip, P, P, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int]),
             "or_post_process_restriction_swe": (None, [P, ctypes.c_int]),
+            "or_switch_to_leaf": (None, [P]),
             "or_overlapping_region": (ctypes.c_double, [_dp] * 4),
             "or_region_map_to_patch": (None, [_ip, _ip, P, P, ctypes.c_double,
                                        _ip, _ip]),

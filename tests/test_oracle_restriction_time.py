@@ -169,3 +169,35 @@ def test_restriction_zero_timestep_uses_unit_factors():
     lib().or_restrict_default(fine.ref, virt.ref, 0)
     assert virt.unew[0, 1, 1] == pytest.approx(2.0, rel=1e-15)
     assert virt.uold[0, 2, 2] == pytest.approx(2.0, rel=1e-15)
+
+
+@pytest.mark.parametrize("t1,dt1", [(0.41, 0.02), (0.37, 0.0), (3.0, 0.5)])
+def test_switch_to_leaf_evaluates_the_restricted_basis_at_the_patch_times(t1, dt1):
+    """Patch::switchToLeaf (Patch.cpp:793-830): a virtual patch restricted
+    from a linear-in-time leaf over [t0, t0 + dt] becomes a leaf whose uOld
+    and uNew are the leaf data's average at t1 and t1 + dt1."""
+    t0, dt = 0.37, 0.05
+    r = np.random.default_rng(11)
+    f = r.uniform(0.5, 2.0, (9, 9))
+    g = r.uniform(-3.0, 3.0, (9, 9))
+    fine = Patch(1, 0, 9, 1, 1.0 / 3, 1.0 / 3, 1, t0, dt)
+    fine.uold[0, 1:-1, 1:-1] = f
+    fine.unew[0] = f + dt * g
+    virt = Patch(1, 0, 3, 1, 0.0, 1.0, 0, 0.0, 0.0)
+    virt.s.is_virtual = 1
+    virt.s.is_leaf = 0
+    lib().or_restrict_default(fine.ref, virt.ref, 0)
+    before_old, before_new = np.array(virt.uold), np.array(virt.unew)
+    virt.s.time, virt.s.dt = t1, dt1
+    lib().or_switch_to_leaf(virt.ref)
+    assert virt.s.is_leaf == 1 and virt.s.is_virtual == 0
+    tol = 1e-12 * (1.0 + t1 + t0) / dt
+    assert virt.uold[0, 2, 2] == pytest.approx(f.mean() + (t1 - t0) * g.mean(), abs=tol)
+    assert virt.unew[0, 1, 1] == pytest.approx(f.mean() + (t1 + dt1 - t0) * g.mean(), abs=tol)
+    # the ghost ring of uOld is left alone; the driver's host-side helper
+    # (used by the device backend) computes the same values bitwise
+    assert np.array_equal(virt.uold[0, 0], before_old[0, 0])
+    from peanoclaw_amd.amr import switch_to_leaf_values
+    a, b = switch_to_leaf_values(before_old[:, 1:-1, 1:-1], before_new, t1, dt1)
+    assert np.array_equal(a, virt.uold[:, 1:-1, 1:-1])
+    assert np.array_equal(b, virt.unew)
