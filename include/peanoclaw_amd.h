@@ -78,6 +78,9 @@ int pc_pool_synchronize(pc_pool *pool);
  * Patch.cpp:742-769).  Data starts zeroed like the reference's heap vector. */
 int pc_patch_register(pc_pool *pool, const pc_cell_description *desc,
                       int *out_slot);
+/* Release a slot (DataHeap::deleteData of the patch's uIndex).  Grid sides
+ * of other patches that read their ghosts from it become reflective walls
+ * (the grid then has a hole there); its slot id may be reused. */
 int pc_patch_release(pc_pool *pool, int slot);
 int pc_patch_get_description(pc_pool *pool, int slot, pc_cell_description *out);
 int pc_patch_set_description(pc_pool *pool, int slot,

@@ -395,6 +395,23 @@ int rebuild_topology(pc_pool* p) {
   bool any = false;
   for (int i = 0; i < ns; ++i)
     for (int k = 0; k < 4; ++k) any |= p->slots[i].live && p->h_sides[i].kind[k] != kSideTile;
+  // every neighbour side must name a live patch of the same size and
+  // scenario (its rows and edge buffers are read with this patch's stride)
+  for (int i = 0; i < ns; ++i) {
+    if (!p->slots[i].live) continue;
+    const auto& d = p->slots[i].desc;
+    for (int k = 0; k < 4; ++k) {
+      if (p->h_sides[i].kind[k] != kSideNeighbour) continue;
+      const int q = p->h_sides[i].src[k];
+      const bool ok = q >= 0 && q < ns && p->slots[q].live &&
+                      p->slots[q].desc.subdivision_factor[0] == d.subdivision_factor[0] &&
+                      p->slots[q].desc.subdivision_factor[1] == d.subdivision_factor[1] &&
+                      p->slots[q].desc.scenario == d.scenario;
+      if (!ok)
+        return fail(PC_E_INVALID, "a grid side names a released patch or one of another "
+                                  "size or scenario");
+    }
+  }
   p->sides_any = any;
   if (ns > 0)
     HIPCHK(hipMemcpyAsync(p->d_sides, p->h_sides.data(), ns * sizeof(SideDev),
@@ -864,6 +881,17 @@ int pc_patch_release(pc_pool* p, int slot) {
   free_cells(p, p->slots[slot].off, p->slots[slot].cells);
   p->slots[slot].live = false;
   if ((size_t)slot < p->h_sides.size()) p->h_sides[slot] = SideDev{};
+  // A side that read its ghosts from the released patch now faces a hole
+  // in the grid: it becomes a reflective wall, as a grid with that patch
+  // missing would have been given by pc_set_grid_sides.  The slot id may be
+  // reused by a patch of another size, which must never be read as a
+  // neighbour.
+  for (size_t i = 0; i < p->h_sides.size(); ++i)
+    for (int k = 0; k < 4; ++k)
+      if (p->h_sides[i].kind[k] == kSideNeighbour && p->h_sides[i].src[k] == slot) {
+        p->h_sides[i].kind[k] = kSideWall;
+        p->h_sides[i].src[k] = (int)i;
+      }
   p->topo_dirty = true;
   return PC_OK;
 }

@@ -123,6 +123,46 @@ def test_smallest_and_largest_patches_and_slot_reuse():
         pool.close()
 
 
+@pytest.mark.parametrize("fused", [0, 2])
+def test_release_inside_a_grid_turns_neighbour_sides_into_walls(fused):
+    """The centre patch of a 3 x 3 grid with fused ghost sides is released
+    and its slot reused by a 5 x 3 patch of another scenario: the four
+    patches that faced it now see a reflective wall (the grid with a hole),
+    bitwise against the oracle's hole-in-the-grid fill, and the new patch
+    (a lake at rest) is never read as their neighbour."""
+    from peanoclaw_amd.pool import make_description
+    from peanoclaw_amd.scenarios import GridLayout, Workload, dam_break_grid
+    from test_gpu_parity import OracleRun, _compare
+    pool = _pool()
+    try:
+        wl = dam_break_grid(pool, 3, 3, 8)
+        pool.launch_config(fused=fused)
+        pool.step(2)
+        slot_of = np.array(wl.layouts[0].slot_of)
+        centre = int(slot_of[1, 1])
+        pool.release(centre)
+        new = pool.register(make_description((5.0, 5.0), 1.0, 5, ny=3, scenario=1))
+        assert new == centre
+        pool.upload(new, unew=np.stack([np.ones((5, 3)), np.zeros((5, 3)), np.zeros((5, 3))]),
+                    b=np.zeros((7, 5)))
+        pool.scenario_set(1, 0.0, 0.0, np.inf)
+        pool.set_grid_sides(np.array([[new]], dtype=np.int32))
+        slot_of[1, 1] = -1
+        rest = [int(s) for s in slot_of.ravel() if s >= 0]
+        hole = Workload(pool, [GridLayout(8, slot_of)], 0, 8 * 64, rest)
+        orun = OracleRun(pool, hole)
+        for _ in range(4):
+            pool.step(1)
+            orun.step()
+        worst, exact, total = _compare(pool, orun, rest)
+        print(f"grid with a released centre: max rel {worst:.3e}, exact {exact}/{total}")
+        assert exact == total
+        h = pool.download(new)[0]
+        assert np.all(h[0] == 1.0) and np.all(h[1:] == 0.0)
+    finally:
+        pool.close()
+
+
 def test_one_scenario_reaches_its_end_time_while_another_continues():
     from peanoclaw_amd.scenarios import dam_break_grid
     pool = _pool()
