@@ -11,6 +11,14 @@
  *    upload of the heap vector); the heap vector is refreshed from the device
  *    by addPatchToSolution (the plotters read it) and can be pushed again
  *    with uploadFromHeap after host-side edits;
+ *  - device slots are keyed by the cell's CellDescription index, which stays
+ *    fixed for the cell's lifetime.  The heap vector behind it (uIndex) does
+ *    not: Patch::switchToNonVirtual deletes it and Patch::switchToVirtual
+ *    creates a fresh one (Patch.cpp:732-790), once per grid iteration for a
+ *    refined cell (GridLevelTransfer.cpp:108-135, 338-485).  When a call
+ *    sees a new uIndex the slot is refilled from the new heap vector; a
+ *    virtual patch first seen in a call (a cell created refined, never
+ *    initialised as a leaf here) is registered then;
  *  - the CellDescription fields the kernels read (geometry, level, virtual
  *    flag, TimeIntervals, demanded mesh width, restriction bounds) are pushed
  *    to the pool before every call that reads them, so the host's
@@ -74,7 +82,11 @@ inline void check(int rc, const char* call) {
 /* Access to a peanoclaw::Patch by the reference's member names. */
 template <class PatchT>
 struct PeanoPatchAccess {
-  static int key(PatchT& p) { return p.getUIndex(); }
+  /* the cell's identity for its lifetime (CellDescription index) */
+  static int key(PatchT& p) { return p.getCellDescriptionIndex(); }
+  /* the heap vector currently behind it; -1 when it holds no data */
+  static int dataIndex(PatchT& p) { return p.getUIndex(); }
+  static bool isVirtual(PatchT& p) { return p.isVirtual(); }
   static auto& timeIntervals(PatchT& p) { return p.getTimeIntervals(); }
   static auto* uNew(PatchT& p) { return p.getAccessor().getUNewArray(); }
   static auto* uOld(PatchT& p, int unknown) {
@@ -148,10 +160,27 @@ class SWENumerics {
 
   pc_pool* pool() const { return pool_; }
   bool registered(PatchT& p) const { return slots_.count(Access::key(p)) != 0; }
-  int slot(PatchT& p) const {
+  /* The patch's slot, with the device following the host's heap vector:
+   * a new uIndex (a fresh vector from Patch::switchToVirtual) is uploaded
+   * before the slot is used; a virtual patch seen for the first time is
+   * registered and uploaded. */
+  int slot(PatchT& p) {
+    const int data = Access::dataIndex(p);
     auto it = slots_.find(Access::key(p));
-    if (it == slots_.end()) throw Error(PC_E_INVALID, "patch was not initialised on the device");
-    return it->second;
+    if (it == slots_.end()) {
+      if (!Access::isVirtual(p) || data == -1)
+        throw Error(PC_E_INVALID, "patch was not initialised on the device");
+      const int s = registerPatch(p);
+      uploadFromHeap(p);
+      return s;
+    }
+    if (it->second.data != data) {
+      if (data == -1)
+        throw Error(PC_E_INVALID, "patch holds no data (uIndex -1: switched to non-virtual)");
+      it->second.data = data;
+      uploadFromHeap(p);
+    }
+    return it->second.slot;
   }
 
   /* ---- Numerics (Numerics.h:52-183) ---- */
@@ -276,16 +305,18 @@ class SWENumerics {
     d.scenario = 0;
     int s = -1;
     check(pc_patch_register(pool_, &d, &s), "pc_patch_register");
-    slots_[Access::key(p)] = s;
+    slots_[Access::key(p)] = Entry{s, Access::dataIndex(p)};
     cache_[s] = d;
     return s;
   }
 
   /* Remesh::destroyCell: the slot's tile returns to the pool. */
   void destroyPatch(PatchT& p) {
-    const int s = slot(p);
+    auto it = slots_.find(Access::key(p));
+    if (it == slots_.end()) throw Error(PC_E_INVALID, "patch was not initialised on the device");
+    const int s = it->second.slot;
     check(pc_patch_release(pool_, s), "pc_patch_release");
-    slots_.erase(Access::key(p));
+    slots_.erase(it);
     cache_.erase(s);
   }
 
@@ -363,7 +394,11 @@ class SWENumerics {
   SWENumericsOptions opt_;
   std::function<void(PatchT&)> scenario_;
   pc_pool* pool_ = nullptr;
-  std::unordered_map<int, int> slots_;                  // CellDescription.uIndex -> slot
+  struct Entry {
+    int slot;  // pool slot
+    int data;  // uIndex of the heap vector the slot mirrors
+  };
+  std::unordered_map<int, Entry> slots_;                // CellDescription index -> slot
   std::unordered_map<int, pc_cell_description> cache_;  // last description the pool holds
   std::vector<double> stage_;
 };

@@ -5,11 +5,23 @@
 //   numerics_replay TRACE BLOB OUT
 //
 // TRACE holds one call per line (doubles as C99 hex floats, exact):
-//   N h level virtual n x y size time dt   new patch h (initializePatch)
+//   N h level virtual n x y size time dt   new patch h: a leaf is initialised
+//                                           (initializePatch); a virtual one is a
+//                                           cell created refined whose heap vector
+//                                           came from Patch::switchToVirtual (the
+//                                           adapter registers it on first use)
 //   S h offset                              heap uNew h <- blob[offset..] (n*n),
 //                                           b <- next (n+2)^2, uOld <- 0; upload
 //   V h virtual                             virtual/leaf switch of the description
-//   Z h                                     heap uNew, uOld <- 0; upload
+//                                           (a leaf keeps its heap vector)
+//   Z h                                     a virtual patch's next grid iteration:
+//                                           stepUp deletes its heap vector
+//                                           (switchToNonVirtual, uIndex -1), stepDown
+//                                           creates a fresh zeroed one with a new
+//                                           uIndex (switchToVirtual) and the
+//                                           scenario's update puts b back
+//                                           (Patch.cpp:732-790); the adapter sees
+//                                           the new uIndex at its next call
 //   R src dst | P dst                       restrictSolution | postProcessRestriction
 //   C h                                     copyUNewToUOld
 //   T sx sy sox soy dox doy src dst         transferGhostlayer
@@ -47,6 +59,7 @@ namespace harness {
 // The CellDescription fields the Numerics path reads
 // (dastgen/CellDescription.def).
 struct CellDescription {
+  int cellDescriptionIndex = -1;
   int uIndex = -1;
   std::array<double, 2> position{}, size{}, demanded{}, lower{}, upper{};
   std::array<int, 2> subdivision{};
@@ -90,11 +103,12 @@ class Accessor {
 
 class HeapPatch {
  public:
-  HeapPatch(int key, int level, bool virt, int n, double x, double y, double size, double t,
-            double dt)
+  HeapPatch(int key, int uIndex, int level, bool virt, int n, double x, double y, double size,
+            double t, double dt)
       : n_(n), heap_((size_t)3 * n * n + 4 * (size_t)(n + 2) * (n + 2), 0.0), ti_(&c_),
         acc_(&heap_, n) {
-    c_.uIndex = key;
+    c_.cellDescriptionIndex = key;
+    c_.uIndex = uIndex;
     c_.level = level;
     c_.isVirtual = virt;
     c_.position = {x, y};
@@ -103,7 +117,15 @@ class HeapPatch {
     c_.time = t;
     c_.dt = dt;
   }
+  int getCellDescriptionIndex() const { return c_.cellDescriptionIndex; }
   int getUIndex() const { return c_.uIndex; }
+  // switchToNonVirtual + switchToVirtual: a fresh zeroed heap vector under a
+  // new uIndex, b restored (the scenario's update)
+  void replaceHeapVector(int uIndex) {
+    const size_t a = (size_t)n_ * n_, g = (size_t)(n_ + 2) * (n_ + 2);
+    std::fill(heap_.begin(), heap_.begin() + 3 * a + 3 * g, 0.0);
+    c_.uIndex = uIndex;
+  }
   std::array<double, 2> getPosition() const { return c_.position; }
   std::array<double, 2> getSize() const { return c_.size; }
   std::array<int, 2> getSubdivisionFactor() const { return c_.subdivision; }
@@ -155,6 +177,7 @@ int main(int argc, char** argv) {
   }
   long calls = 0, cflChecks = 0, cflMismatch = 0, clockChecks = 0, clockMismatch = 0;
   long widthChecks = 0, widthMismatch = 0;
+  int nextUIndex = 0;  // DataHeap indices handed out
   std::vector<std::unique_ptr<HeapPatch>> patches;
   try {
     peanoclaw_amd::SWENumerics<HeapPatch> num;
@@ -176,9 +199,9 @@ int main(int argc, char** argv) {
         std::string x, y, size, t, dt;
         in >> h >> level >> virt >> n >> x >> y >> size >> t >> dt;
         if (h != (int)patches.size()) throw peanoclaw_amd::Error(PC_E_INVALID, "handles out of order");
-        patches.emplace_back(new HeapPatch(h, level, virt != 0, n, hexd(x), hexd(y), hexd(size),
-                                           hexd(t), hexd(dt)));
-        num.initializePatch(*patches.back());
+        patches.emplace_back(new HeapPatch(h, nextUIndex++, level, virt != 0, n, hexd(x),
+                                           hexd(y), hexd(size), hexd(t), hexd(dt)));
+        if (!virt) num.initializePatch(*patches.back());
       } else if (op == "S") {
         int h;
         size_t off;
@@ -200,11 +223,7 @@ int main(int argc, char** argv) {
       } else if (op == "Z") {
         int h;
         in >> h;
-        HeapPatch& p = P(h);
-        num.downloadToHeap(p);  // keeps b
-        const int n = p.n();
-        std::fill(p.heap().begin(), p.heap().begin() + 3 * n * n + 3 * (n + 2) * (n + 2), 0.0);
-        num.uploadFromHeap(p);
+        P(h).replaceHeapVector(nextUIndex++);
       } else if (op == "R") {
         int s, d;
         in >> s >> d;

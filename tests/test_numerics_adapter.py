@@ -43,6 +43,8 @@ struct FloatPatch {
 // a policy for a patch type without the Peano member names
 struct FloatAccess {
   static int key(FloatPatch& p) { return p.id; }
+  static int dataIndex(FloatPatch& p) { return p.id; }
+  static bool isVirtual(FloatPatch&) { return false; }
   static Clock& timeIntervals(FloatPatch& p) { return p.clock; }
   static float* uNew(FloatPatch& p) { return p.u.data(); }
   static float* uOld(FloatPatch& p, int q) { return p.u.data() + 192 + 100 * q; }
