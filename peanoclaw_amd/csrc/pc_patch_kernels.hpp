@@ -413,13 +413,17 @@ __global__ void k_edge_batch(long long n, const double* __restrict__ in,
 
 constexpr int kMaxHills = 8;
 
-// Initial state of one dam-break request on [0,1]^2: h = column height
-// inside the disc around (cx, cy), `depth` elsewhere, hu = hv = 0, and
-// b = sum of Gaussian hills a_k exp(-|x - c_k|^2 / s2_k), evaluated at
-// every cell centre of the tile including the ghost ring.
+// Initial state of one dam-break request on [0,1]^2, as
+// BreakingDamSWEScenario::initializePatch / getWaterHeight
+// (BreakingDam.cpp:53-83, 165-176) with the request's centre, radius and
+// heights: at the cell centre X = position + (i + 1/2) * subcell size,
+// r = sqrt((X - cx)^2 + (Y - cy)^2), h = column height if r <= radius else
+// `depth`; hu = hv = 0; b = sum of Gaussian hills
+// a_k exp(-|x - c_k|^2 / s2_k) (0 without hills), evaluated at every cell
+// centre of the tile including the ghost ring.
 struct RequestInit {
-  double dx;
-  double cx, cy, r2, col_h, depth;
+  double dx;  // request grid width 1 / cells_per_side (patch positions)
+  double cx, cy, radius, col_h, depth;
   int nh, P, n, cur;
   double hx[kMaxHills], hy[kMaxHills], hs2[kMaxHills], ha[kMaxHills];
 };
@@ -432,20 +436,23 @@ k_init_request(const int* __restrict__ slots, int nslots, const PatchDev* __rest
   if (k >= nslots) return;
   const PatchDev P = pd[slots[k]];
   const int pi = k / a.P, pj = k - (k / a.P) * a.P;
+  // the patch's position as its description holds it (i * n * dx)
+  const double px = (double)(pi * a.n) * a.dx, py = (double)(pj * a.n) * a.dx;
   const int s = a.n + 2;
   for (int c = threadIdx.x; c < s * s; c += blockDim.x) {
     const int x = c / s, y = c - (c / s) * s;
-    const double X = ((double)(pi * a.n + x - 1) + 0.5) * a.dx;
-    const double Y = ((double)(pj * a.n + y - 1) + 0.5) * a.dx;
+    const double X = px + ((double)(x - 1) + 0.5) * P.dx;
+    const double Y = py + ((double)(y - 1) + 0.5) * P.dy;
     double b = 0.0;
     for (int m = 0; m < a.nh; ++m) {
       const double ex = X - a.hx[m], ey = Y - a.hy[m];
       b += a.ha[m] * exp(-(ex * ex + ey * ey) / a.hs2[m]);
     }
     const double ex = X - a.cx, ey = Y - a.cy;
+    const double r = sqrt(ex * ex + ey * ey);
     const long long i = P.off + c;
     F.b[i] = b;
-    F.H(a.cur)[i] = (ex * ex + ey * ey <= a.r2) ? a.col_h : a.depth;
+    F.H(a.cur)[i] = a.col_h * (double)(r <= a.radius) + a.depth * (double)(r > a.radius);
     F.HU(a.cur)[i] = 0.0;
     F.HV(a.cur)[i] = 0.0;
   }

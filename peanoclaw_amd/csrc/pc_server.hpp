@@ -201,7 +201,7 @@ int server_admit(pc_server* s, int id) {
   a.dx = 1.0 / q.req.cells_per_side;
   a.cx = q.req.column_x;
   a.cy = q.req.column_y;
-  a.r2 = q.req.column_radius * q.req.column_radius;
+  a.radius = q.req.column_radius;
   a.col_h = q.req.column_height;
   a.depth = q.req.depth;
   a.nh = q.req.num_hills;
