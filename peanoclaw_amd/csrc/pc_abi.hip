@@ -120,6 +120,14 @@ struct pc_pool {
   size_t aux_cap = 0;
   int* d_flags = nullptr;
   size_t flags_cap = 0;
+  // per-patch task list of pc_step_all_per_patch, kept between calls
+  Task* d_single = nullptr;
+  size_t single_cap = 0;
+  int* d_single_slot = nullptr;
+  size_t single_slot_cap = 0;
+  // staging of pc_edge_batch (6 input and 5 output doubles per edge)
+  double* d_edge_io = nullptr;
+  size_t edge_io_cap = 0;
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -784,7 +792,8 @@ int pc_pool_destroy(pc_pool* p) {
   void* ptrs[] = {p->d_pd, p->d_smax, p->d_smax_int, p->d_est, p->d_scn, p->d_tasks, p->d_utasks,
                   p->d_plist, p->d_plist_scn, p->d_scn_of, p->d_live, p->d_bdesc, p->d_strips,
                   p->d_counter, p->d_iota, p->d_segmax, p->d_result, p->d_aux, p->d_flags, p->d_ebuf[0],
-                  p->d_ebuf[1], p->d_bcol, p->d_sides};
+                  p->d_ebuf[1], p->d_bcol, p->d_sides, p->d_single, p->d_single_slot,
+                  p->d_edge_io};
   for (void* q : ptrs)
     if (q) hipFree(q);
   for (hipEvent_t e : p->upd_ev) hipEventDestroy(e);
@@ -1797,10 +1806,10 @@ int pc_step_all_per_patch(pc_pool* p, int k) {
     single.push_back(Task{0, 1, d.subdivision_factor[0], d.subdivision_factor[1]});
   }
   const int n = (int)single.size();
-  Task* d_single = nullptr;
-  int* d_sl = nullptr;
-  HIPCHK(hipMalloc(&d_single, std::max(n, 1) * sizeof(Task)));
-  HIPCHK(hipMalloc(&d_sl, std::max(n, 1) * sizeof(int)));
+  CHK(ensure_dev(&p->d_single, &p->single_cap, std::max<size_t>(n, 1)));
+  CHK(ensure_dev(&p->d_single_slot, &p->single_slot_cap, std::max<size_t>(n, 1)));
+  Task* const d_single = p->d_single;
+  int* const d_sl = p->d_single_slot;
   for (int i = 0; i < n; ++i) single[i].first = i;
   if (n) {
     HIPCHK(hipMemcpyAsync(d_single, single.data(), n * sizeof(Task), hipMemcpyHostToDevice,
@@ -1842,8 +1851,6 @@ int pc_step_all_per_patch(pc_pool* p, int k) {
   HIPCHK(hipEventSynchronize(p->ev1));
   HIPCHK(hipEventElapsedTime(&p->last_total_ms, p->ev0, p->ev1));
   p->last_update_ms = 0.f;
-  HIPCHK(hipFree(d_single));
-  HIPCHK(hipFree(d_sl));
   p->scn_host_stale = true;
   p->fused_ready = false;
   mark_batched_clock(p);
@@ -1939,9 +1946,9 @@ int pc_edge_batch(pc_pool* p, long long n, const double* in, double* out) {
   if (!p || n < 0 || (n > 0 && (!in || !out))) return fail(PC_E_INVALID, "invalid argument");
   if (n == 0) return PC_OK;
   hipSetDevice(p->device);
-  double *din = nullptr, *dout = nullptr;
-  HIPCHK(hipMalloc(&din, (size_t)n * 6 * sizeof(double)));
-  HIPCHK(hipMalloc(&dout, (size_t)n * 5 * sizeof(double)));
+  CHK(ensure_dev(&p->d_edge_io, &p->edge_io_cap, (size_t)n * 11));
+  double* const din = p->d_edge_io;
+  double* const dout = din + (size_t)n * 6;
   HIPCHK(hipMemcpyAsync(din, in, (size_t)n * 6 * sizeof(double), hipMemcpyHostToDevice,
                         p->stream));
   const long long blocks = std::min<long long>((n + 255) / 256, 65536);
@@ -1955,8 +1962,6 @@ int pc_edge_batch(pc_pool* p, long long n, const double* in, double* out) {
   HIPCHK(hipMemcpyAsync(out, dout, (size_t)n * 5 * sizeof(double), hipMemcpyDeviceToHost,
                         p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
-  HIPCHK(hipFree(din));
-  HIPCHK(hipFree(dout));
   return PC_OK;
 }
 
