@@ -36,14 +36,27 @@ def log(*a):
 
 
 def read_traffic():
-    """HBM bytes per update launch from the committed PMC summary, if any."""
+    """HBM bytes per update launch from the committed PMC summary, and where
+    they come from.  The summary records the source hash of the library it
+    was measured on (peanoclaw_amd.build.source_hash); a summary of another
+    build gives no traffic figure."""
+    from peanoclaw_amd.build import source_hash
+    vals = {}
     try:
         for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1].split()[0])
+            for key in ("traffic_bytes_per_launch:", "library_source_sha256:"):
+                if line.startswith(key):
+                    vals[key] = line.split(":", 1)[1].split()[0]
     except OSError:
-        pass
-    return None
+        return None, "no PMC summary"
+    rel = os.path.relpath(TRAFFIC_FILE, ROOT)
+    if "traffic_bytes_per_launch:" not in vals:
+        return None, f"{rel} holds no traffic figure"
+    if vals.get("library_source_sha256:") != source_hash():
+        return None, f"{rel} was measured on another build of the library"
+    return (float(vals["traffic_bytes_per_launch:"]),
+            f"{rel}: rocprofv3 --pmc FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch (scale "
+            f"factors from k_copy_probe), library source sha256 {source_hash()[:16]}")
 
 
 def cpu_model():
@@ -56,10 +69,11 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(num_patches, seconds, threads):
+def cpu_baseline(num_patches, seconds, threads, runs=5):
     """The oracle (FP64, C, OpenMP over patches) on a sample of the same
     workload generator: ghost fill of every size-class grid + global-dt
-    step of every patch, repeated until `seconds` of wall time."""
+    step of every patch.  One warm-up step, then `runs` timed runs of an
+    equal number of steps sized to `seconds` in total; the median rate."""
     import ctypes
 
     import numpy as np
@@ -89,7 +103,7 @@ def cpu_baseline(num_patches, seconds, threads):
             nxs.append(n)
         for name, arr in (("h", h), ("hu", hu), ("hv", hv), ("b", b)):
             chunks[name].append(arr.ravel())
-        layouts.append((n, slot_of))
+        layouts.append((n, np.ascontiguousarray(slot_of)))
         top += c * (n + 2) ** 2
         slot += c
     H = [np.concatenate(chunks["h"]), np.zeros(top)]
@@ -115,27 +129,29 @@ def cpu_baseline(num_patches, seconds, threads):
     def step():
         c = pl.cur
         for n, so in layouts:
-            so = np.ascontiguousarray(so)
             L.or_grid_fill(so.shape[0], so.shape[1], n,
                            so.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                            off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
                            dptr(H[c]), dptr(HU[c]), dptr(HV[c]))
         L.or_pool_step(ctypes.byref(pl), math.inf, threads)
 
-    step()  # warm-up
-    nsteps = 0
     t0 = time.perf_counter()
-    while True:
-        step()
-        nsteps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or nsteps >= 5000:
-            break
-    return {"value": cells * nsteps / el, "unit": "cell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{num_patches} patches of the same generator ({cells} cells), "
-                      f"{nsteps} global-dt steps incl. ghost fill, {el:.1f} s wall, "
-                      f"FP64 oracle in C (gcc -O2 -ffp-contract=off, OpenMP)"}
+    step()  # warm-up (also sizes the runs)
+    t1 = time.perf_counter() - t0
+    per_run = max(1, int(round(seconds / runs / max(t1, 1e-6))))
+    rates = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        for _ in range(per_run):
+            step()
+        rates.append(cells * per_run / (time.perf_counter() - t0))
+    med = float(np.median(rates))
+    return {"value": med, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "runs": [round(r) for r in rates],
+            "sample": f"{num_patches} patches of the same generator ({cells} cells); median "
+                      f"of {runs} runs of {per_run} global-dt steps each (ghost fill + step) "
+                      f"after one warm-up step; FP64 oracle in C (gcc -O3 -march=x86-64-v3 "
+                      f"-ffp-contract=off, OpenMP over patches)"}
 
 
 def serve_stream(args, tuned):
@@ -187,7 +203,7 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=1.0,
                     help="GiB copied by the plain copy-kernel probe (HBM rate, counter scale)")
     ap.add_argument("--copy-reps", type=int, default=5)
-    ap.add_argument("--serve-requests", type=int, default=32,
+    ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests of the served stream (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-mib", type=int, default=512)
@@ -281,10 +297,11 @@ def main():
     t, dt, est, nsteps = pool.scenario_get(0)
 
     achieved = cells * BYTES_PER_CELL_UPDATE / (upd_ms * 1e-3) / 1e9
-    traffic = read_traffic()
+    traffic, traffic_source = read_traffic()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic,
+                "traffic_source": traffic_source,
                 "kernel": "k_update (row-march f-wave update, all patches)",
                 "algorithmic_bytes_per_launch": cells * BYTES_PER_CELL_UPDATE,
                 "avg_launch_ms": round(upd_ms, 4),

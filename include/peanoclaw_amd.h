@@ -135,6 +135,32 @@ int pc_restrict_solution(pc_pool *pool, int source_slot, int destination_slot,
 /* Numerics::postProcessRestriction (Numerics.h:103-106; SWEKernel.cpp:309-316) */
 int pc_post_process_restriction(pc_pool *pool, int destination_slot,
                                 int restrict_only_overlapped_regions);
+/* Flux correction, off by default in the reference (SolveTimestep's
+ * _correctFluxes; called after solveTimestep, SolveTimestep.cpp:673-679).
+ * Each slot holds the reference's flux array: faces x-lower, x-upper,
+ * y-lower, y-upper, each 3 unknowns x the cells along the face (unknown
+ * slowest), 6 * (nx + ny) doubles.  The present reference code is
+ * reproduced as it is: the upwind term of computeFluxes is dropped by its
+ * ';' (DefaultFluxCorrection.cpph:71-72), Numerics::computeFlux uses g = 1
+ * and scales the momenta by h (Numerics.h:185-203), a source cell reaches
+ * only destination cells overlapping the destination's first tangential
+ * subcell (HyperplaneRegion.cpp:76), and the destination flux is read from
+ * the start of its face (SubgridFaceIterator::restart).  Square patches
+ * only (the face iterator places every upper face at subdivisionFactor[1]).
+ *
+ * Numerics::computeFluxes (Numerics.cpp:93-98; DefaultFluxCorrection.cpph:44-86):
+ * the slot's fluxes from uOld (interior cells next to each face and their
+ * ghost neighbours). */
+int pc_compute_fluxes(pc_pool *pool, int slot);
+/* Numerics::applyFluxCorrection (Numerics.cpp:80-91; DefaultFluxCorrection.cpph:
+ * 88-164): corrects the destination's uNew next to its face (dimension,
+ * -direction) by (F_source - F_destination) * time overlap * overlap area /
+ * subcell area.  PC_E_INVALID when the two patches share their position. */
+int pc_apply_flux_correction(pc_pool *pool, int source_slot, int destination_slot,
+                             int dimension, int direction);
+/* Host <-> device copy of a slot's flux array (to_device = 1 uploads). */
+int pc_patch_fluxes(pc_pool *pool, int slot, int to_device, double *fluxes);
+
 /* DefaultTransfer::copyUNewToUOld / swapUNewAndUOld (DefaultTransfer.cpph:103-127) */
 int pc_copy_unew_to_uold(pc_pool *pool, int slot);
 int pc_swap_unew_and_uold(pc_pool *pool, int slot);
@@ -262,14 +288,16 @@ int pc_edge_batch(pc_pool *pool, long long n, const double *in, double *out);
 
 /* Diagnostic (no reference counterpart): device-to-device copy of `bytes`
  * (rounded down to 16) between two scratch buffers, `reps` timed launches of
- * one plain copy kernel (k_copy_probe) after a warm-up; *best_gbs = (read +
+ * one plain copy kernel (k_copy_probe, one 16-byte element per thread) after
+ * a warm-up; *best_gbs = (read +
  * written bytes) / fastest launch.  Gives the achievable HBM copy rate and
  * the byte scale of hardware traffic counters. */
 int pc_copy_probe(pc_pool *pool, long long bytes, int reps, double *best_gbs);
 
 /* Diagnostic (no reference counterpart): the streaming ceiling of the
  * update's traffic mix.  Seven scratch arrays of `cells` doubles; one flat
- * grid-stride kernel (k_stream_probe) reads four of them and writes three per
+ * kernel (k_stream_probe, a pair of elements per thread in 16-byte lanes)
+ * reads four of them and writes three per
  * element, as the update reads h, hu, hv, b and writes h, hu, hv of a cell,
  * with no neighbour access.  `reps` timed launches after a warm-up;
  * *best_gbs = 56 * cells / fastest launch. */
@@ -308,6 +336,12 @@ int pc_vtu_write(const char *path, int npatches, const pc_vtu_patch *patches);
 /* Download the listed slots (uNew of the current state, b, descriptor) and
  * write them to `path`; age is written as 0. */
 int pc_pool_write_vtu(pc_pool *pool, const char *path, const int32_t *slots, int nslots);
+/* The same cells as legacy ASCII VTK (grid/plotter/VTKSubgridPlotter.cpp:
+ * 9-241): POINTS, CELLS, CELL_TYPES and one SCALARS block per array, in the
+ * VTK plotter's geometry (each patch shrunk by 1 + 0.015 * level around its
+ * centre, VTKSubgridPlotter.cpp:22, 104-106). */
+int pc_vtk_write(const char *path, int npatches, const pc_vtu_patch *patches);
+int pc_pool_write_vtk(pc_pool *pool, const char *path, const int32_t *slots, int nslots);
 
 /* ---- serving: a stream of independent dam-break scenarios ----
  *

@@ -254,6 +254,18 @@ class SWENumerics {
           "pc_post_process_restriction");
   }
 
+  /* Numerics::computeFluxes / applyFluxCorrection (Numerics.cpp:80-98), called
+   * by SolveTimestep when flux correction is enabled (SolveTimestep.cpp:673-679). */
+  void computeFluxes(PatchT& subgrid) {
+    check(pc_compute_fluxes(pool_, push(subgrid)), "pc_compute_fluxes");
+  }
+  void applyFluxCorrection(PatchT& sourceSubgrid, PatchT& destinationSubgrid, int dimension,
+                           int direction) {
+    const int src = push(sourceSubgrid), dst = push(destinationSubgrid);
+    check(pc_apply_flux_correction(pool_, src, dst, dimension, direction),
+          "pc_apply_flux_correction");
+  }
+
   /* BreakingDamSWEScenario::computeDemandedMeshWidth on the current uNew */
   std::pair<double, double> getDemandedMeshWidth(PatchT& p, bool /*isInitializing*/) {
     double w[2];

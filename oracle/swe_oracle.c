@@ -34,6 +34,11 @@
  *       of restricting a linear-in-time fine leaf and switching the virtual
  *       patch to a leaf: its uOld / uNew equal the fine data's area average
  *       at its own times (tests/test_oracle_restriction_time.py).
+ *   or_compute_fluxes, or_apply_flux_correction  DefaultFluxCorrection.cpph:
+ *       13-164, HyperplaneRegion.cpp:25-103, SubgridFaceIterator.cpph,
+ *       Numerics.h:185-203, with the defects of the present code kept;
+ *       pinned by cell-by-cell brute force on 2x2 / 3x3 coarse-fine pairs
+ *       (tests/test_oracle_flux_correction.py).
  *   compositor + traversals  pinned by the traversal call-order tests and the
  *       time-stepping veto test (GhostLayerCompositorTest.cpp:59-124).
  *   BreakingDam, demanded mesh width, refinement decision: BreakingDam.cpp:
@@ -1032,6 +1037,151 @@ void or_switch_to_leaf(or_patch *p) {
         p->uold[io] = vo * (1.0 - factorForUOld) + vn * factorForUOld;
         p->unew[in] = vo * (1.0 - factorForUNew) + vn * factorForUNew;
       }
+}
+
+/* ------------------------------------------------------------------ */
+/* Flux correction (off by default in the reference)                    */
+/* ------------------------------------------------------------------ */
+/* The patch's flux array (Linearization::Linearization, LinearizationZYXQ.h:
+ * flux part): faces (dimension 0 lower, 0 upper, 1 lower, 1 upper), each
+ * nq x (cells along the face), unknown slowest. */
+static int flux_face_cells(const or_patch *p, int dim) { return dim == 0 ? p->ny : p->nx; }
+int or_flux_face_offset(const or_patch *p, int dim, int direction) {
+  int off[4];
+  off[0] = 0;
+  off[1] = off[0] + p->ny * p->nq;
+  off[2] = off[1] + p->ny * p->nq;
+  off[3] = off[2] + p->nx * p->nq;
+  return off[2 * dim + (1 + direction) / 2];
+}
+int or_flux_array_size(const or_patch *p) { return 2 * p->nq * (p->nx + p->ny); }
+
+/* Numerics::computeFlux (Numerics.h:185-203) as the SWE build compiles it:
+ * every unknown times the normal momentum, the momenta times h once more,
+ * and the pressure 0.5 * g * h * h with g = 1.0 on the normal momentum. */
+static void ref_compute_flux(int dim, const double u[3], double f[3]) {
+  for (int i = 0; i < 3; i++) f[i] = u[i] * u[1 + dim];
+  for (int i = 0; i < 2; i++) f[i + 1] *= u[0];
+  const double g = 1.0;
+  const double pr = 0.5 * g * u[0] * u[0];
+  f[1 + dim] += pr;
+}
+
+/* SubgridFaceIterator::restart (SubgridFaceIterator.cpph): the normal index
+ * of the interior cell next to face (dim, direction) and of its ghost
+ * neighbour.  For the upper face the reference reads
+ * getSubdivisionFactor(_direction), i.e. component 1 (ny), whatever `dim`
+ * is; for square patches that is n[dim]. */
+static void face_normal_indices(const or_patch *p, int direction, int *interior, int *ghost) {
+  const int nd = p->ny; /* getSubdivisionFactor(+1) */
+  *interior = direction == -1 ? 0 : nd - 1;
+  *ghost = direction == -1 ? -1 : nd;
+}
+
+/* DefaultFluxCorrectionTemplate::computeFluxes (DefaultFluxCorrection.cpph:
+ * 44-86): per face cell 0.5 * (F(ghost uOld) + F(interior uOld)).  The
+ * statement that would add direction * dt/(2 dx) * (inner - outer) ends
+ * before that term (:71-72), so the flux is the plain average. */
+void or_compute_fluxes(or_patch *p, double *flux) {
+  for (int dim = 0; dim < 2; dim++) {
+    for (int direction = -1; direction < 2; direction += 2) {
+      const int nf = flux_face_cells(p, dim);
+      const int fo = or_flux_face_offset(p, dim, direction);
+      int ni, ng;
+      face_normal_indices(p, direction, &ni, &ng);
+      for (int k = 0; k < nf; k++) {
+        double inner[3], outer[3], fi[3], fg[3];
+        for (int q = 0; q < 3; q++) {
+          const int xi = dim == 0 ? ni : k, yi = dim == 0 ? k : ni;
+          const int xg = dim == 0 ? ng : k, yg = dim == 0 ? k : ng;
+          inner[q] = p->uold[or_uold_index(p, q, xi, yi)];
+          outer[q] = p->uold[or_uold_index(p, q, xg, yg)];
+        }
+        ref_compute_flux(dim, outer, fg);
+        ref_compute_flux(dim, inner, fi);
+        for (int q = 0; q < 3; q++) flux[fo + q * nf + k] = 0.5 * (fg[q] + fi[q]);
+      }
+    }
+  }
+}
+
+static double std_max(double a, double b) { return (a < b) ? b : a; }
+static double std_min(double a, double b) { return (b < a) ? b : a; }
+
+/* DefaultFluxCorrectionTemplate::calculateOverlappingRegion (:13-41) */
+static double flux_overlap(const double p1[2], int c1, const double s1[2], const double p2[2],
+                           int c2, const double s2[2], int axis) {
+  double region = 1.0;
+  for (int d = 0; d < 2; d++) {
+    if (d == axis) continue;
+    const double iv = std_min(p1[d] + ((double)c1 + 1.0) * s1[d], p2[d] + ((double)c2 + 1.0) * s2[d]) -
+                      std_max(p1[d] + (double)c1 * s1[d], p2[d] + (double)c2 * s2[d]);
+    region *= iv;
+    if (region < 0.0) region = 0.0;
+  }
+  return region;
+}
+
+/* DefaultFluxCorrectionTemplate::applyCorrection (:88-164) with
+ * HyperplaneRegion::getInterfaceRegion / getInterfaceRegionForSubcell
+ * (geometry/HyperplaneRegion.cpp:25-103) and the face iterator, as they are:
+ *   - getInterfaceRegion contributes only its dimension and direction (the
+ *     last coordinate in which the positions differ);
+ *   - getInterfaceRegionForSubcell takes the destination's subcell size as
+ *     its size, so a source cell reaches at most the destination cells
+ *     overlapping the destination's first tangential subcell;
+ *   - the destination iterator restarted on a region starts its flux index
+ *     at the face's first entry whatever the region's offset.
+ * Returns 0, or -1 when the two positions are equal in both coordinates
+ * (the reference leaves dimension and direction undefined then). */
+int or_apply_flux_correction(or_patch *src, const double *srcFlux, or_patch *dst,
+                             const double *dstFlux, int dimension, int direction) {
+  const double lower = std_max(src->time, dst->time);
+  const double upper = std_min(src->time + src->dt, dst->time + dst->dt);
+  const double timeOverlap = upper - lower;
+  int rdim = -1;
+  for (int d = 0; d < 2; d++) {
+    if (la_smaller(src->pos[d], dst->pos[d])) rdim = d;
+    else if (la_greater(src->pos[d], dst->pos[d])) rdim = d;
+  }
+  if (rdim < 0) return -1;
+  const int sd = rdim == 0 ? 1 : 0; /* Linearization::getGlobalDimension(0, rdim) */
+  const double eps = 1e-10;         /* HyperplaneRegion::epsilon */
+  const double dvol = dst->subcell[0] * dst->subcell[1];
+  const int nfs = flux_face_cells(src, dimension), nfd = flux_face_cells(dst, dimension);
+  const int fos = or_flux_face_offset(src, dimension, direction);
+  const int fod = or_flux_face_offset(dst, dimension, -direction);
+  int ni, ng;
+  face_normal_indices(dst, -direction, &ni, &ng);
+  (void)ng;
+  for (int ks = 0; ks < nfs; ks++) {
+    double sF[3];
+    for (int q = 0; q < 3; q++) sF[q] = srcFlux[fos + q * nfs + ks];
+    const double pos = dst->pos[sd];
+    const double size = dst->subcell[sd]; /* getSubcellSize() used as the size */
+    const double sub = dst->subcell[sd];
+    double npos = src->pos[sd];
+    npos += src->subcell[sd] * (double)ks;
+    const double lo = std_max(pos, npos);
+    const double hi = std_min(pos + size, npos + src->subcell[sd]);
+    const int loC = (int)((lo - pos + eps) / sub);
+    const int hiC = (int)ceil((hi - pos - eps) / sub);
+    const int off = loC, cnt = hiC - loC;
+    for (int j = 0; j < cnt; j++) {
+      const int t = off + j;
+      if (t < 0 || t >= nfd) continue; /* outside the face: never reached for adjacent patches */
+      double dF[3];
+      for (int q = 0; q < 3; q++) dF[q] = dstFlux[fod + q * nfd + j];
+      const double region = flux_overlap(src->pos, ks, src->subcell, dst->pos, t, dst->subcell,
+                                         dimension);
+      const int x = dimension == 0 ? ni : t, y = dimension == 0 ? t : ni;
+      for (int q = 0; q < 3; q++) {
+        const int k = or_unew_index(dst, q, x, y);
+        dst->unew[k] = dst->unew[k] + (sF[q] - dF[q]) * timeOverlap * region / dvol;
+      }
+    }
+  }
+  return 0;
 }
 
 /* ------------------------------------------------------------------ */

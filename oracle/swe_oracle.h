@@ -125,6 +125,13 @@ void or_post_process_restriction_swe(or_patch *dst, int onlyOverlapped);
 /* Patch::switchToLeaf: restricted [0, 1]-basis values -> the patch's own
    interval [time, time + dt] (set p->time, p->dt first) */
 void or_switch_to_leaf(or_patch *p);
+
+/* ---- flux correction (DefaultFluxCorrection; off by default) ---- */
+int or_flux_face_offset(const or_patch *p, int dim, int direction);
+int or_flux_array_size(const or_patch *p);
+void or_compute_fluxes(or_patch *p, double *flux);
+int or_apply_flux_correction(or_patch *src, const double *srcFlux, or_patch *dst,
+                             const double *dstFlux, int dimension, int direction);
 void or_transform_water_height(or_patch *p, const int off[2], const int size[2],
                                int modifyUOld, int absoluteToAboveSeaFloor);
 

@@ -183,12 +183,19 @@ _SIGS = {
     "pc_scenario_mass": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
     "pc_last_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "pc_edge_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp]),
+    "pc_compute_fluxes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "pc_apply_flux_correction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_int]),
+    "pc_patch_fluxes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]),
     "pc_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, _dp]),
     "pc_vtu_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p]),
+    "pc_vtk_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p]),
     "pc_amr_lts_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "pc_amr_lts_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                        ctypes.POINTER(ctypes.c_longlong)]),
     "pc_pool_write_vtu": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p,
+                                         ctypes.c_int]),
+    "pc_pool_write_vtk": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p,
                                          ctypes.c_int]),
     "pc_stream_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, _dp]),
     "pc_march_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),

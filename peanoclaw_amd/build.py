@@ -38,6 +38,18 @@ def flags():
     ]
 
 
+def source_hash():
+    """SHA-256 over the library's sources and compile flags: identifies the
+    build a committed profile was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in DEPS:
+        with open(f, "rb") as fh:
+            h.update(os.path.basename(f).encode() + b"\0" + fh.read())
+    h.update(" ".join(flags()).encode())
+    return h.hexdigest()
+
+
 def build(verbose=False, out=None, defines=()):
     """Compile the library; `defines` are extra -D tuning macros (e.g.
     PC_UPDATE_WAVES_PER_EU=4) and `out` another output path."""

@@ -55,6 +55,7 @@ struct Slot {
 };
 
 bool la_greater(double a, double b) { return a - b > kLaEps; }
+bool la_smaller(double a, double b) { return b - a > kLaEps; }
 
 }  // namespace
 
@@ -128,6 +129,9 @@ struct pc_pool {
   // staging of pc_edge_batch (6 input and 5 output doubles per edge)
   double* d_edge_io = nullptr;
   size_t edge_io_cap = 0;
+  // flux arrays of the flux correction, kFluxPerSlot doubles per slot
+  double* d_flux = nullptr;
+  size_t flux_slots = 0;
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -793,7 +797,7 @@ int pc_pool_destroy(pc_pool* p) {
                   p->d_plist, p->d_plist_scn, p->d_scn_of, p->d_live, p->d_bdesc, p->d_strips,
                   p->d_counter, p->d_iota, p->d_segmax, p->d_result, p->d_aux, p->d_flags, p->d_ebuf[0],
                   p->d_ebuf[1], p->d_bcol, p->d_sides, p->d_single, p->d_single_slot,
-                  p->d_edge_io};
+                  p->d_edge_io, p->d_flux};
   for (void* q : ptrs)
     if (q) hipFree(q);
   for (hipEvent_t e : p->upd_ev) hipEventDestroy(e);
@@ -1377,6 +1381,110 @@ int pc_restrict_solution(pc_pool* p, int src, int dst, int onlyOverlapped) {
   }
   CHK(transform(p, src, z, sn, 1, 1));
   CHK(transform(p, src, z, sn, 0, 1));
+  return PC_OK;
+}
+
+// ---- flux correction (Numerics::computeFluxes / applyFluxCorrection) ----
+
+// Flux arrays for slots [0, slots.size()), grown with their contents kept
+// (fluxes computed for one patch are read when its neighbours are corrected).
+static int ensure_flux(pc_pool* p) {
+  const size_t need = std::max<size_t>(p->slots.size(), 1);
+  if (p->d_flux && need <= p->flux_slots) return PC_OK;
+  const size_t cap = std::max(need, 2 * p->flux_slots);
+  double* n = nullptr;
+  HIPCHK(hipMalloc(&n, cap * kFluxPerSlot * sizeof(double)));
+  HIPCHK(hipMemsetAsync(n, 0, cap * kFluxPerSlot * sizeof(double), p->stream));
+  if (p->d_flux) {
+    HIPCHK(hipMemcpyAsync(n, p->d_flux, p->flux_slots * kFluxPerSlot * sizeof(double),
+                          hipMemcpyDeviceToDevice, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    HIPCHK(hipFree(p->d_flux));
+  }
+  p->d_flux = n;
+  p->flux_slots = cap;
+  return PC_OK;
+}
+
+static int flux_slot_ok(pc_pool* p, int slot) {
+  if (!valid_slot(p, slot)) return fail(PC_E_INVALID, "invalid slot");
+  const auto& d = p->slots[slot].desc;
+  if (d.subdivision_factor[0] != d.subdivision_factor[1])
+    return fail(PC_E_INVALID, "flux correction needs square patches (the reference's face "
+                              "iterator places every upper face at subdivisionFactor[1])");
+  return PC_OK;
+}
+
+int pc_compute_fluxes(pc_pool* p, int slot) {
+  CHK(flux_slot_ok(p, slot));
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  CHK(ensure_flux(p));
+  const PatchDev P = make_patchdev(p->slots[slot]);
+  const int n = 2 * (P.nx + P.ny);
+  hipLaunchKernelGGL(k_compute_fluxes, dim3((n + 63) / 64), dim3(64), 0, p->stream, P,
+                     fields_of(p), 1 - cur_of(p, slot), p->d_flux + (size_t)slot * kFluxPerSlot);
+  HIPCHK(hipGetLastError());
+  return PC_OK;
+}
+
+int pc_apply_flux_correction(pc_pool* p, int src, int dst, int dimension, int direction) {
+  CHK(flux_slot_ok(p, src));
+  CHK(flux_slot_ok(p, dst));
+  if ((dimension != 0 && dimension != 1) || (direction != -1 && direction != 1))
+    return fail(PC_E_INVALID, "dimension must be 0 or 1 and direction -1 or +1");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  CHK(ensure_flux(p));
+  const auto& sd = p->slots[src].desc;
+  const auto& dd = p->slots[dst].desc;
+  // HyperplaneRegion::getInterfaceRegion: only its dimension is used, the
+  // last coordinate in which the positions differ
+  int rdim = -1;
+  for (int d = 0; d < 2; ++d)
+    if (la_smaller(sd.position[d], dd.position[d]) || la_greater(sd.position[d], dd.position[d]))
+      rdim = d;
+  if (rdim < 0)
+    return fail(PC_E_INVALID, "source and destination share their position (the reference's "
+                              "interface region is undefined then)");
+  double st, sdt, dt0, ddt;
+  slot_clock(p, src, &st, &sdt);
+  slot_clock(p, dst, &dt0, &ddt);
+  // DefaultFluxCorrection.cpph:96-100 (std::max / std::min)
+  const double lower = (st < dt0) ? dt0 : st;
+  const double a = st + sdt, b = dt0 + ddt;
+  const double upper = (b < a) ? b : a;
+  const double overlap = upper - lower;
+  FluxGeom S{}, D{};
+  const PatchDev SP = make_patchdev(p->slots[src]), DP = make_patchdev(p->slots[dst]);
+  for (int d = 0; d < 2; ++d) {
+    S.pos[d] = sd.position[d];
+    D.pos[d] = dd.position[d];
+  }
+  S.sub[0] = SP.dx; S.sub[1] = SP.dy; S.nx = SP.nx; S.ny = SP.ny;
+  D.sub[0] = DP.dx; D.sub[1] = DP.dy; D.nx = DP.nx; D.ny = DP.ny;
+  mark_state_changed(p);
+  hipLaunchKernelGGL(k_apply_flux_correction, dim3(1), dim3(64), 0, p->stream, S, SP,
+                     p->d_flux + (size_t)src * kFluxPerSlot, D, DP,
+                     p->d_flux + (size_t)dst * kFluxPerSlot, fields_of(p), cur_of(p, dst),
+                     dimension, direction, rdim, overlap);
+  HIPCHK(hipGetLastError());
+  return PC_OK;
+}
+
+int pc_patch_fluxes(pc_pool* p, int slot, int to_device, double* fluxes) {
+  CHK(flux_slot_ok(p, slot));
+  if (!fluxes) return fail(PC_E_INVALID, "fluxes is NULL");
+  hipSetDevice(p->device);
+  CHK(ensure_flux(p));
+  const auto& d = p->slots[slot].desc;
+  const size_t n = (size_t)2 * 3 * (d.subdivision_factor[0] + d.subdivision_factor[1]);
+  double* dev = p->d_flux + (size_t)slot * kFluxPerSlot;
+  if (to_device)
+    HIPCHK(hipMemcpyAsync(dev, fluxes, n * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  else
+    HIPCHK(hipMemcpyAsync(fluxes, dev, n * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
   return PC_OK;
 }
 
@@ -1975,9 +2083,7 @@ int pc_copy_probe(pc_pool* p, long long bytes, int reps, double* best_gbs) {
     hipFree(a);
     return fail(PC_E_OOM, "copy probe: hipMalloc failed");
   }
-  int ncu = 256;
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device);
-  const long long blocks = std::min<long long>((n2 + 255) / 256, (long long)ncu * 16);
+  const long long blocks = (n2 + 255) / 256;  // one element per thread
   float best = 1e30f;
   hipError_t e = hipSuccess;
   for (int r = 0; r <= reps && e == hipSuccess; ++r) {  // r = 0 is an untimed warm-up
@@ -2011,9 +2117,7 @@ int pc_stream_probe(pc_pool* p, long long cells, int reps, double* best_gbs) {
   hipError_t e = hipSuccess;
   for (int k = 0; k < 7 && e == hipSuccess; ++k)
     e = hipMemsetAsync(a[k], 0, (size_t)cells * sizeof(double), p->stream);
-  int ncu = 256;
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device);
-  const long long blocks = std::min<long long>((cells + 255) / 256, (long long)ncu * 32);
+  const long long blocks = (cells / 2 + 1 + 255) / 256;  // one pair per thread
   float best = 1e30f;
   for (int r = 0; r <= reps && e == hipSuccess; ++r) {  // r = 0 is an untimed warm-up
     if ((e = hipEventRecord(p->ev0, p->stream)) != hipSuccess) break;

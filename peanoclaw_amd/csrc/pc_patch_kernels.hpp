@@ -409,6 +409,129 @@ __global__ void k_edge_batch(long long n, const double* __restrict__ in,
   }
 }
 
+// ---- flux correction (DefaultFluxCorrection; off by default in the reference) ----
+//
+// Per slot, the reference's flux array (LinearizationZYXQ.h, flux part):
+// faces (x lower, x upper, y lower, y upper), each 3 unknowns x the cells
+// along the face, unknown slowest; kFluxPerSlot doubles reserved per slot.
+constexpr int kFluxPerSlot = 2 * 3 * (64 + 64);
+
+__device__ __forceinline__ int flux_face_offset(int nx, int ny, int dim, int direction) {
+  const int o1 = ny * 3, o2 = o1 + ny * 3, o3 = o2 + nx * 3;
+  const int f = 2 * dim + (1 + direction) / 2;
+  return f == 0 ? 0 : f == 1 ? o1 : f == 2 ? o2 : o3;
+}
+
+// Numerics::computeFlux (Numerics.h:185-203) as the SWE build compiles it:
+// the unknowns times the normal momentum, the momenta times h once more,
+// and 0.5 * g * h * h with g = 1.0 added to the normal momentum.
+__device__ __forceinline__ void ref_face_flux(int dim, const double u[3], double f[3]) {
+  const double un = dim == 0 ? u[1] : u[2];
+  f[0] = u[0] * un;
+  f[1] = u[1] * un;
+  f[2] = u[2] * un;
+  f[1] *= u[0];
+  f[2] *= u[0];
+  const double g = 1.0;
+  const double pr = 0.5 * g * u[0] * u[0];
+  if (dim == 0)
+    f[1] += pr;
+  else
+    f[2] += pr;
+}
+
+// DefaultFluxCorrectionTemplate::computeFluxes (DefaultFluxCorrection.cpph:
+// 44-86): one thread per face cell, 0.5 * (F(ghost uOld) + F(interior uOld));
+// the upwind term of :71-72 is a separate statement the reference discards.
+// The interior / ghost cells of an upper face sit at ny - 1 / ny, as the
+// reference's face iterator reads getSubdivisionFactor(+1) (square patches).
+__global__ void __launch_bounds__(64)
+k_compute_fluxes(PatchDev P, Fields F, int old_state, double* __restrict__ flux) {
+  const int nx = P.nx, ny = P.ny;
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 2 * (nx + ny)) return;
+  int dim, dir;
+  if (e < ny) {
+    dim = 0; dir = -1;
+  } else if (e < 2 * ny) {
+    dim = 0; dir = 1; e -= ny;
+  } else if (e < 2 * ny + nx) {
+    dim = 1; dir = -1; e -= 2 * ny;
+  } else {
+    dim = 1; dir = 1; e -= 2 * ny + nx;
+  }
+  const int nf = dim == 0 ? ny : nx;
+  const int ni = dir == -1 ? 0 : ny - 1, ng = dir == -1 ? -1 : ny;
+  const long long ii = dim == 0 ? tile_index(P, ni, e) : tile_index(P, e, ni);
+  const long long ig = dim == 0 ? tile_index(P, ng, e) : tile_index(P, e, ng);
+  const double inner[3] = {F.H(old_state)[ii], F.HU(old_state)[ii], F.HV(old_state)[ii]};
+  const double outer[3] = {F.H(old_state)[ig], F.HU(old_state)[ig], F.HV(old_state)[ig]};
+  double fi[3], fg[3];
+  ref_face_flux(dim, outer, fg);
+  ref_face_flux(dim, inner, fi);
+  const int fo = flux_face_offset(nx, ny, dim, dir);
+  for (int q = 0; q < 3; ++q) flux[fo + q * nf + e] = 0.5 * (fg[q] + fi[q]);
+}
+
+// What applyCorrection reads of a patch besides its tile.
+struct FluxGeom {
+  double pos[2], sub[2];
+  int nx, ny;
+};
+
+__device__ __forceinline__ double fmax_std(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double fmin_std(double a, double b) { return (b < a) ? b : a; }
+
+// DefaultFluxCorrectionTemplate::applyCorrection (DefaultFluxCorrection.cpph:
+// 88-164) with HyperplaneRegion::getInterfaceRegionForSubcell
+// (HyperplaneRegion.cpp:67-103) and calculateOverlappingRegion (:13-41), as
+// the reference computes them: the destination region of a source cell is
+// taken against the destination's first tangential subcell (the subcell
+// size is used as the patch size), and the destination flux is read from
+// the start of its face whatever the region's offset.  Thread q updates
+// unknown q in the reference's order (source cells, then region cells).
+__global__ void __launch_bounds__(64)
+k_apply_flux_correction(FluxGeom S, PatchDev SP, const double* __restrict__ sflux, FluxGeom D,
+                        PatchDev DP, const double* __restrict__ dflux, Fields F, int dst_cur,
+                        int dimension, int direction, int rdim, double time_overlap) {
+  const int q = threadIdx.x;
+  if (q >= 3) return;
+  const int sd = rdim == 0 ? 1 : 0;
+  const double eps = 1e-10;
+  const double dvol = D.sub[0] * D.sub[1];
+  const int nfs = dimension == 0 ? S.ny : S.nx, nfd = dimension == 0 ? D.ny : D.nx;
+  const int fos = flux_face_offset(S.nx, S.ny, dimension, direction);
+  const int fod = flux_face_offset(D.nx, D.ny, dimension, -direction);
+  const int ni = -direction == -1 ? 0 : D.ny - 1;
+  double* const U = q == 0 ? F.H(dst_cur) : q == 1 ? F.HU(dst_cur) : F.HV(dst_cur);
+  for (int ks = 0; ks < nfs; ++ks) {
+    const double sF = sflux[fos + q * nfs + ks];
+    const double pos = D.pos[sd], size = D.sub[sd], sub = D.sub[sd];
+    double npos = S.pos[sd];
+    npos += S.sub[sd] * (double)ks;
+    const double lo = fmax_std(pos, npos);
+    const double hi = fmin_std(pos + size, npos + S.sub[sd]);
+    const int loC = (int)((lo - pos + eps) / sub);
+    const int hiC = (int)ceil((hi - pos - eps) / sub);
+    for (int j = 0; j < hiC - loC; ++j) {
+      const int t = loC + j;
+      if (t < 0 || t >= nfd) continue;
+      const double dF = dflux[fod + q * nfd + j];
+      // calculateOverlappingRegion over the axis other than `dimension`
+      const int od = dimension == 0 ? 1 : 0;
+      double region = 1.0;
+      const double iv = fmin_std(S.pos[od] + ((double)ks + 1.0) * S.sub[od],
+                                 D.pos[od] + ((double)t + 1.0) * D.sub[od]) -
+                        fmax_std(S.pos[od] + (double)ks * S.sub[od], D.pos[od] + (double)t * D.sub[od]);
+      region *= iv;
+      if (region < 0.0) region = 0.0;
+      const long long i = dimension == 0 ? tile_index(DP, ni, t) : tile_index(DP, t, ni);
+      U[i] = U[i] + (sF - dF) * time_overlap * region / dvol;
+    }
+  }
+  (void)SP;
+}
+
 // ---- served requests ----
 
 constexpr int kMaxHills = 8;
@@ -496,18 +619,19 @@ k_hash_request(const int* __restrict__ slots, int nslots, const PatchDev* __rest
   if (threadIdx.x == 0) atomicAdd(out, red[0]);
 }
 
-// Plain device copy (16-byte lanes, grid-stride): the bandwidth probe whose
-// byte count is known exactly, n2 = number of double2 elements.
+// Plain device copy, one 16-byte element per thread (no grid-stride loop):
+// the bandwidth probe whose byte count is known exactly, n2 = number of
+// double2 elements.
 __global__ void __launch_bounds__(256) k_copy_probe(long long n2,
                                                     const double2* __restrict__ src,
                                                     double2* __restrict__ dst) {
-  for (long long i = threadIdx.x + (long long)blockIdx.x * blockDim.x; i < n2;
-       i += (long long)blockDim.x * gridDim.x)
-    dst[i] = src[i];
+  const long long i = threadIdx.x + (long long)blockIdx.x * blockDim.x;
+  if (i < n2) dst[i] = src[i];
 }
 
 // Flat stream with the update's traffic mix per element: 4 reads (h, hu, hv,
-// b), 3 writes (h, hu, hv).
+// b), 3 writes (h, hu, hv); one pair of elements per thread in 16-byte lanes
+// (the last thread takes an odd element alone).
 __global__ void __launch_bounds__(256) k_stream_probe(long long n, const double* __restrict__ h,
                                                       const double* __restrict__ hu,
                                                       const double* __restrict__ hv,
@@ -515,12 +639,29 @@ __global__ void __launch_bounds__(256) k_stream_probe(long long n, const double*
                                                       double* __restrict__ oh,
                                                       double* __restrict__ ohu,
                                                       double* __restrict__ ohv) {
-  for (long long i = threadIdx.x + (long long)blockIdx.x * blockDim.x; i < n;
-       i += (long long)blockDim.x * gridDim.x) {
-    const double x = h[i] + 0.5 * b[i];
-    oh[i] = x;
-    ohu[i] = hu[i] * x;
-    ohv[i] = hv[i] * x;
+  const long long i = threadIdx.x + (long long)blockIdx.x * blockDim.x;
+  const long long n2 = n / 2;
+  if (i < n2) {
+    const double2 x = reinterpret_cast<const double2*>(h)[i];
+    const double2 y = reinterpret_cast<const double2*>(hu)[i];
+    const double2 z = reinterpret_cast<const double2*>(hv)[i];
+    const double2 w = reinterpret_cast<const double2*>(b)[i];
+    double2 a, c, d;
+    a.x = x.x + 0.5 * w.x;
+    a.y = x.y + 0.5 * w.y;
+    c.x = y.x * a.x;
+    c.y = y.y * a.y;
+    d.x = z.x * a.x;
+    d.y = z.y * a.y;
+    reinterpret_cast<double2*>(oh)[i] = a;
+    reinterpret_cast<double2*>(ohu)[i] = c;
+    reinterpret_cast<double2*>(ohv)[i] = d;
+  } else if (i == n2 && (n & 1)) {
+    const long long k = n - 1;
+    const double x = h[k] + 0.5 * b[k];
+    oh[k] = x;
+    ohu[k] = hu[k] * x;
+    ohv[k] = hv[k] * x;
   }
 }
 

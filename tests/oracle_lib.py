@@ -113,6 +113,11 @@ def lib():
                                                ctypes.c_int]),
             "or_post_process_restriction_swe": (None, [P, ctypes.c_int]),
             "or_switch_to_leaf": (None, [P]),
+            "or_flux_face_offset": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+            "or_flux_array_size": (ctypes.c_int, [P]),
+            "or_compute_fluxes": (None, [P, _dp]),
+            "or_apply_flux_correction": (ctypes.c_int, [P, _dp, P, _dp, ctypes.c_int,
+                                                        ctypes.c_int]),
             "or_overlapping_region": (ctypes.c_double, [_dp] * 4),
             "or_region_map_to_patch": (None, [_ip, _ip, P, P, ctypes.c_double,
                                        _ip, _ip]),
