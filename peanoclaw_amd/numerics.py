@@ -9,7 +9,7 @@ fallback.
 """
 import numpy as np
 
-from ._abi import check, iarr
+from ._abi import check, dptr, iarr
 from .pool import PatchPool, make_description
 
 
@@ -143,6 +143,29 @@ class SWENumerics:
         check(self.lib.pc_demanded_mesh_width(self.pool.h, patch.slot, s.threshold,
                                               s.minimal_mesh_width, s.maximal_mesh_width, out))
         return (out[0], out[1])
+
+    # Numerics.cpp:93-98 / DefaultFluxCorrection.cpph:44-86 (SolveTimestep.cpp:673-679)
+    def computeFluxes(self, subgrid):
+        check(self.lib.pc_compute_fluxes(self.pool.h, subgrid.slot))
+
+    # Numerics.cpp:80-91 / DefaultFluxCorrection.cpph:88-164
+    def applyFluxCorrection(self, sourceSubgrid, destinationSubgrid, dimension, direction):
+        check(self.lib.pc_apply_flux_correction(self.pool.h, sourceSubgrid.slot,
+                                                destinationSubgrid.slot, int(dimension),
+                                                int(direction)))
+
+    def fluxes(self, patch, values=None):
+        """The patch's flux array (6 (nx + ny) doubles, faces x-lower, x-upper,
+        y-lower, y-upper, unknown slowest); uploads `values` when given."""
+        nx, ny = self.pool.dims(patch.slot)
+        if values is not None:
+            v = np.ascontiguousarray(values, dtype=np.float64)
+            assert v.size == 6 * (nx + ny)
+            check(self.lib.pc_patch_fluxes(self.pool.h, patch.slot, 1, dptr(v)))
+            return v
+        out = np.empty(6 * (nx + ny))
+        check(self.lib.pc_patch_fluxes(self.pool.h, patch.slot, 0, dptr(out)))
+        return out
 
     # DefaultTransfer::copyUNewToUOld (called by SolveTimestep.cpp:638)
     def copyUNewToUOld(self, patch):

@@ -1,5 +1,6 @@
 """VTU output of patch data (the content of PeanoClaw's VTUSubgridPlotter,
-grid/plotter/VTUSubgridPlotter.cpp:9-241) and a reader for it.
+grid/plotter/VTUSubgridPlotter.cpp:9-241), legacy VTK output (that of its
+VTKSubgridPlotter, VTKSubgridPlotter.cpp:9-241) and readers for both.
 
 `write_vtu` writes host arrays through the native writer (pc_vtu_write);
 `write_pool_vtu` downloads slots of a device pool and writes them
@@ -30,10 +31,7 @@ class VtuPatch(ctypes.Structure):
     ]
 
 
-def write_vtu(path, patches):
-    """patches: dicts with keys pos (x, y), size (float or (sx, sy)), q
-    (array 3 x nx x ny: h, hu, hv), b (nx x ny) and optionally level, time,
-    dt, demanded (width), age, ghostlayer_width."""
+def _patch_array(patches):
     keep = []
     arr = (VtuPatch * max(len(patches), 1))()
     dp = ctypes.POINTER(ctypes.c_double)
@@ -58,13 +56,34 @@ def write_vtu(path, patches):
             P.q[i] = q[i].ctypes.data_as(dp)
         P.b = b.ctypes.data_as(dp)
         keep += [q, b]
+    return arr, keep
+
+
+def write_vtu(path, patches):
+    """patches: dicts with keys pos (x, y), size (float or (sx, sy)), q
+    (array 3 x nx x ny: h, hu, hv), b (nx x ny) and optionally level, time,
+    dt, demanded (width), age, ghostlayer_width."""
+    arr, keep = _patch_array(patches)
     check(load().pc_vtu_write(str(path).encode(), len(patches), arr))
+
+
+def write_vtk(path, patches):
+    """The same patches as legacy ASCII VTK (VTKSubgridPlotter geometry:
+    each patch shrunk by 1 + 0.015 * level around its centre)."""
+    arr, keep = _patch_array(patches)
+    check(load().pc_vtk_write(str(path).encode(), len(patches), arr))
 
 
 def write_pool_vtu(pool, path, slots):
     """Current state (uNew), b and descriptor of device slots to `path`."""
     s = (ctypes.c_int32 * max(len(slots), 1))(*slots)
     check(load().pc_pool_write_vtu(pool.h, str(path).encode(), s, len(slots)))
+
+
+def write_pool_vtk(pool, path, slots):
+    """As write_pool_vtu, in legacy ASCII VTK."""
+    s = (ctypes.c_int32 * max(len(slots), 1))(*slots)
+    check(load().pc_pool_write_vtk(pool.h, str(path).encode(), s, len(slots)))
 
 
 _TYPES = {"Float64": np.float64, "Float32": np.float32, "Int32": np.int32, "UInt8": np.uint8}
@@ -93,4 +112,34 @@ def read_vtu(path):
     return pts, conn, types, arrays
 
 
-__all__ = ["write_vtu", "write_pool_vtu", "read_vtu", "VtuPatch"]
+def read_vtk(path):
+    """(points (n, 3), connectivity (m, 4), types (m,), cell arrays by name)
+    of a legacy ASCII VTK unstructured grid as write_vtk writes it."""
+    tok = open(path).read().split("\n")
+    if not tok[0].startswith("# vtk DataFile") or tok[2] != "ASCII" or \
+            tok[3] != "DATASET UNSTRUCTURED_GRID":
+        raise ValueError("not a legacy ASCII unstructured grid")
+    k = 4
+    nv = int(tok[k].split()[1])
+    pts = np.array([[float(v) for v in tok[k + 1 + i].split()] for i in range(nv)])
+    k += 1 + nv
+    nc = int(tok[k].split()[1])
+    cells = np.array([[int(v) for v in tok[k + 1 + i].split()] for i in range(nc)])
+    if (cells[:, 0] != 4).any():
+        raise ValueError("expected quadrilaterals")
+    k += 1 + nc
+    types = np.array([int(tok[k + 1 + i]) for i in range(nc)])
+    k += 1 + nc
+    if tok[k] != f"CELL_DATA {nc}":
+        raise ValueError("expected cell data")
+    k += 1
+    arrays = {}
+    while k < len(tok) and tok[k].startswith("SCALARS"):
+        name = tok[k].split()[1]
+        arrays[name] = np.array([float(tok[k + 2 + i]) for i in range(nc)])
+        k += 2 + nc
+    return pts, cells[:, 1:], types, arrays
+
+
+__all__ = ["write_vtu", "write_vtk", "write_pool_vtu", "write_pool_vtk", "read_vtu", "read_vtk",
+           "VtuPatch"]

@@ -67,3 +67,35 @@ def test_invalid_arguments(tmp_path):
     with pytest.raises(PeanoClawAmdError):
         write_vtu(tmp_path / "missing_dir" / "x.vtu", [_patch((0, 0), 1.0, 2, 1,
                                                               np.random.default_rng(2))])
+
+
+def test_legacy_vtk_carries_the_vtu_cells_in_the_vtk_plotter_geometry(tmp_path):
+    """Legacy ASCII VTK (VTKSubgridPlotter): the same cell arrays as the VTU
+    file, exactly (17 significant digits), and every patch of level l shrunk
+    by 1 + 0.015 l around its centre (VTKSubgridPlotter.cpp:22, 104-106)."""
+    from peanoclaw_amd.vtu import read_vtk, write_vtk
+    rng = np.random.default_rng(4)
+    patches = [_patch((0.0, 0.0), 0.5, 4, 1, rng), _patch((0.5, 0.0), 0.5, 4, 1, rng),
+               _patch((0.0, 0.5), 0.5 / 3, 3, 2, rng)]
+    write_vtu(tmp_path / "a.vtu", patches)
+    write_vtk(tmp_path / "a.vtk", patches)
+    pu, cu, tu, au = read_vtu(tmp_path / "a.vtu")
+    pk, ck, tk, ak = read_vtk(tmp_path / "a.vtk")
+    assert set(au) == set(ak) and (tk == 9).all() and len(ck) == len(cu)
+    for name in au:
+        assert np.array_equal(au[name], ak[name]), name
+    o = 0
+    for p in patches:
+        n = p["q"].shape[1]
+        g = 0.015 * p["level"]
+        s = slice(o, o + n * n)
+        o += n * n
+        xu, yu = pu[cu[s], 0].mean(axis=1), pu[cu[s], 1].mean(axis=1)
+        xk, yk = pk[ck[s], 0].mean(axis=1), pk[ck[s], 1].mean(axis=1)
+        size = p["size"]
+        np.testing.assert_allclose(xk, p["pos"][0] + size * g / 2 + (xu - p["pos"][0]) / (1 + g),
+                                   rtol=0, atol=1e-14)
+        np.testing.assert_allclose(yk, p["pos"][1] + size * g / 2 + (yu - p["pos"][1]) / (1 + g),
+                                   rtol=0, atol=1e-14)
+    # patches of one level share no vertices across the gap
+    assert len(pk) == 2 * 25 + 16
