@@ -247,10 +247,10 @@ class AmrGrid:
         raise RuntimeError("face region not covered")
 
     def _ghost_plan(self, changed=None):
-        """Face entries and walls of every leaf.  With `changed` (handles of
-        the patches a regrid created, refined, coarsened or released), only
-        the leaves that are new or whose face sources changed are looked up
-        again; the others keep their entries."""
+        """Face and corner entries and walls of every leaf.  With `changed`
+        (handles of the patches a regrid created, refined, coarsened or
+        released), only the leaves that are new or whose sources changed are
+        looked up again; the others keep their entries."""
         n = self.n
         # region geometry per face: (size, source offset, destination offset)
         geom = {(0, 0): ((1, n), (n - 1, 0), (-1, 0)), (0, 1): ((1, n), (0, 0), (n, 0)),
@@ -280,6 +280,20 @@ class AmrGrid:
                     res.append(("transfer", same) if same is not None else face(p, dim, upper))
                 pe = [(kind, src, p) + geom[f] for f, (kind, src) in zip(FACES, res)
                       if kind != "wall"]
+                # corner ghosts from the diagonal neighbour, as the edge
+                # functor fills them (GhostlayerCompositorFunctors.cpp:87-141):
+                # the patch of this level there (transfer), else the coarser
+                # leaf covering it (interpolation); outside the domain the
+                # walls fill them
+                for di, dj in ((-1, -1), (-1, 1), (1, -1), (1, 1)):
+                    qi, qj = i + di, j + dj
+                    if not (0 <= qi < nx_ and 0 <= qj < ny_):
+                        continue
+                    src = self._covering(p.level, (qi, qj))
+                    kind = "transfer" if src.level == p.level else "interp"
+                    soff = (n - 1 if di == -1 else 0, n - 1 if dj == -1 else 0)
+                    doff = (-1 if di == -1 else n, -1 if dj == -1 else n)
+                    pe.append((kind, src, p, (1, 1), soff, doff))
                 we = [(p, FACES[k][0], FACES[k][1]) for k in wall_rank if res[k][0] == "wall"]
                 # the same entries packed as the rows of the device plan
                 # (pc_amr_ghost: kind, src, dst, size, src_offset, dst_offset;

@@ -10,8 +10,9 @@ GhostLayerCompositor.cpp:280-320, and the face functor,
 GhostlayerCompositorFunctors.cpp:23-77), where a quadrant without a patch
 of that level holds the coarser patch covering it (Peano's hanging-vertex
 adjacency).  Here the stencils are built from the geometry alone and run
-through the oracle's compositor; the face ghost cells of every leaf must
-equal the driver's bitwise.  Grids: the C1 and C3 refinement structures
+through the oracle's compositor (faces, and corners by the edge functor,
+:87-141); the whole ghost ring of every leaf must equal the driver's
+bitwise.  Grids: the C1 and C3 refinement structures
 and random three-level trees, with random data, b = 0 (so the SWE h <-> h+b
 round trips are exact and the order of the calls cannot matter).
 """
@@ -94,7 +95,8 @@ def _compositor_fill(g, B):
 
 
 def _faces(u):
-    return (u[:, 0, 1:-1], u[:, -1, 1:-1], u[:, 1:-1, 0], u[:, 1:-1, -1])
+    """The ghost ring: both ghost rows and columns, corners included."""
+    return (u[:, 0, :], u[:, -1, :], u[:, 1:-1, 0], u[:, 1:-1, -1])
 
 
 def _check(build, seed):

@@ -201,3 +201,54 @@ def test_switch_to_leaf_evaluates_the_restricted_basis_at_the_patch_times(t1, dt
     a, b = switch_to_leaf_values(before_old[:, 1:-1, 1:-1], before_new, t1, dt1)
     assert np.array_equal(a, virt.uold[:, 1:-1, 1:-1])
     assert np.array_equal(b, virt.unew)
+
+
+@pytest.mark.parametrize("sides", [("left",), ("right", "top"), ("left", "bottom", "top")])
+def test_only_overlapped_restriction_equals_full_restriction_where_read(sides):
+    """GridLevelTransfer restricts only the regions of a leaf that coarse
+    neighbours' ghost layers overlap (restrictSolution(onlyOverlapped),
+    DefaultRestriction.cpph:108-150, Region.cpp:244-315); the batched step
+    restricts whole leaves.  With refinement factor 3 the regions align with
+    coarse cells, so every coarse cell a neighbour's ghost layer reads holds
+    bitwise the same value either way."""
+    r = np.random.default_rng(len(sides))
+    n = 6
+    sub = 1.0 / n
+    # bounds of the region overlapped by ghost layers (Patch.cpp:916-941):
+    # `upper` from the lower sides, `lower` from the upper sides
+    BIG = np.finfo(np.float64).max
+    upper, lower = [-BIG, -BIG], [BIG, BIG]
+    read = np.zeros((n, n), dtype=bool)
+    if "left" in sides:
+        upper[0] = sub
+        read[0, :] = True
+    if "bottom" in sides:
+        upper[1] = sub
+        read[:, 0] = True
+    if "right" in sides:
+        lower[0] = 1.0 - sub
+        read[-1, :] = True
+    if "top" in sides:
+        lower[1] = 1.0 - sub
+        read[:, -1] = True
+    results = []
+    for only in (1, 0):
+        virt = Patch(1, 0, n, 1, 0.0, 1.0, 0, 0.0, 0.0)
+        virt.s.is_virtual = 1
+        virt.s.is_leaf = 0
+        for d in range(2):
+            virt.s.lower_ghost_bounds[d] = lower[d]
+            virt.s.upper_ghost_bounds[d] = upper[d]
+        rr = np.random.default_rng(5)
+        for a in range(3):
+            for c in range(3):
+                leaf = Patch(1, 0, n, 1, (a / 3.0, c / 3.0), 1.0 / 3.0, 1, 0.25, 0.05)
+                leaf.unew[0] = rr.uniform(0.5, 2.0, (n, n))
+                leaf.uold[0, 1:-1, 1:-1] = rr.uniform(0.5, 2.0, (n, n))
+                lib().or_restrict_default(leaf.ref, virt.ref, only)
+        results.append((np.array(virt.unew[0]), np.array(virt.uold[0, 1:-1, 1:-1])))
+    (pn, po), (fn, fo) = results
+    assert read.any()
+    assert np.array_equal(pn[read].view(np.uint64), fn[read].view(np.uint64))
+    assert np.array_equal(po[read].view(np.uint64), fo[read].view(np.uint64))
+    assert (pn[~read] == 0.0).all()  # the rest is left alone with onlyOverlapped
