@@ -202,7 +202,6 @@ def test_request_initial_state_is_the_breaking_dam():
             ref, _ = breaking_dam((d.position[0], d.position[1]), d.size[0], 32)
             assert np.array_equal(unew, ref), sl
             assert not b.any()
-        srv.release(rid)
 
 
 def test_stream_requests_up_to_2048_alone_and_in_mix():
