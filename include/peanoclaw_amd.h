@@ -82,6 +82,13 @@ int pc_patch_register(pc_pool *pool, const pc_cell_description *desc,
  * of other patches that read their ghosts from it become reflective walls
  * (the grid then has a hole there); its slot id may be reused. */
 int pc_patch_release(pc_pool *pool, int slot);
+/* The slot's description.  While the batched step drives the slot's clock,
+ * time and timestep size are its scenario's, and estimated_next_timestep_size
+ * is the patch's own CFL estimate (SWEKernel.cpp:88) - except in the pruned
+ * launch mode (pc_set_launch_config fused = 2) on a grid with side sources,
+ * where only the scenario minimum is computed exactly and is reported for
+ * every patch (a lower bound of the patch's own estimate); fused = 0 or 1
+ * gives exact per-patch estimates at the same dt. */
 int pc_patch_get_description(pc_pool *pool, int slot, pc_cell_description *out);
 int pc_patch_set_description(pc_pool *pool, int slot,
                              const pc_cell_description *desc);
