@@ -293,6 +293,15 @@ int pc_last_timing(pc_pool *pool, double *total_ms, double *update_ms);
  * tools/SWE_WaveAccumulationBlock.cpp:154-159. */
 int pc_edge_batch(pc_pool *pool, long long n, const double *in, double *out);
 
+/* pc_edge_batch without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The edges are staged through a static device array of at most 65,536
+ * edges (5.8 MB per GPU, held until the process ends); larger n fails with
+ * PC_E_INVALID.  Every call on one GPU shares that array: calls that overlap
+ * on the same GPU (other threads, other pools) race. */
+int pc_edge_batch_static(pc_pool *pool, long long n, const double *in, double *out);
+
 /* Diagnostic (no reference counterpart): device-to-device copy of `bytes`
  * (rounded down to 16) between two scratch buffers, `reps` timed launches of
  * one plain copy kernel (k_copy_probe, one 16-byte element per thread) after

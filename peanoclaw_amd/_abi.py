@@ -183,6 +183,7 @@ _SIGS = {
     "pc_scenario_mass": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
     "pc_last_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "pc_edge_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp]),
+    "pc_edge_batch_static": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp]),
     "pc_compute_fluxes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pc_apply_flux_correction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_int, ctypes.c_int]),

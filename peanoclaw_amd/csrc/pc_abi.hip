@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -755,6 +756,32 @@ extern "C" {
 
 const char* pc_last_error_string(void) { return g_err.c_str(); }
 
+// Staging of pc_edge_batch_static: a file-scope device array (one copy per
+// GPU, held until the process ends) of kStaticEdgeMax edges, 6 input and 5
+// output doubles each.  Host code reaches it only through the device
+// address hipGetSymbolAddress returns on that GPU, kept per device.
+constexpr long long kStaticEdgeMax = 65536;
+static __device__ __align__(256) double scratch[kStaticEdgeMax * 11];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Once per device, with `device` current: the device address of `scratch`.
+static int scratch_setup(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) return fail(PC_E_INVALID, "device index too large");
+  if (g_scratch_ptr[device]) return PC_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(PC_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = static_cast<double*>(ptr);
+  return PC_OK;
+}
+
 int pc_pool_create(int device, size_t hbm_budget_bytes, pc_pool** out) {
   if (!out) return fail(PC_E_INVALID, "out is NULL");
   int n = 0;
@@ -766,6 +793,7 @@ int pc_pool_create(int device, size_t hbm_budget_bytes, pc_pool** out) {
     return fail(PC_E_NODEVICE, std::string("device is ") + prop.gcnArchName +
                                    ", this library is built for gfx950");
   HIPCHK(hipSetDevice(device));
+  CHK(scratch_setup(device));
   pc_pool* p = new pc_pool();
   p->device = device;
   p->budget = hbm_budget_bytes ? hbm_budget_bytes : (size_t)prop.totalGlobalMem / 10 * 9;
@@ -2057,6 +2085,33 @@ int pc_edge_batch(pc_pool* p, long long n, const double* in, double* out) {
   CHK(ensure_dev(&p->d_edge_io, &p->edge_io_cap, (size_t)n * 11));
   double* const din = p->d_edge_io;
   double* const dout = din + (size_t)n * 6;
+  HIPCHK(hipMemcpyAsync(din, in, (size_t)n * 6 * sizeof(double), hipMemcpyHostToDevice,
+                        p->stream));
+  const long long blocks = std::min<long long>((n + 255) / 256, 65536);
+  if (p->solver)
+    hipLaunchKernelGGL(k_edge_batch<1>, dim3((unsigned)blocks), dim3(256), 0, p->stream, n,
+                       din, dout);
+  else
+    hipLaunchKernelGGL(k_edge_batch<0>, dim3((unsigned)blocks), dim3(256), 0, p->stream, n,
+                       din, dout);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout, (size_t)n * 5 * sizeof(double), hipMemcpyDeviceToHost,
+                        p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return PC_OK;
+}
+
+int pc_edge_batch_static(pc_pool* p, long long n, const double* in, double* out) {
+  if (!p || n < 0 || (n > 0 && (!in || !out))) return fail(PC_E_INVALID, "invalid argument");
+  if (n > kStaticEdgeMax)
+    return fail(PC_E_INVALID, "pc_edge_batch_static: at most 65536 edges per call "
+                              "(the size of its static staging array)");
+  if (n == 0) return PC_OK;
+  hipSetDevice(p->device);
+  double* const base = g_scratch_ptr[p->device];
+  if (!base) return fail(PC_E_HIP, "pc_edge_batch_static: staging not set up on this device");
+  double* const din = base;
+  double* const dout = base + kStaticEdgeMax * 6;
   HIPCHK(hipMemcpyAsync(din, in, (size_t)n * 6 * sizeof(double), hipMemcpyHostToDevice,
                         p->stream));
   const long long blocks = std::min<long long>((n + 255) / 256, 65536);
