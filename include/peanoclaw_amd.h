@@ -321,11 +321,12 @@ int pc_stream_probe(pc_pool *pool, long long cells, int reps, double *best_gbs);
 
 /* Diagnostic (no reference counterpart): the memory-pattern ceiling of the
  * batched update in the pool's own layout.  One launch of k_march_probe
- * walks the batched step's task list exactly as the update does (persistent
- * grid, a wavefront per task, rows marched in order, four rows prefetched), reading
+ * walks the batched step's task list as the update does (its launch shape,
+ * a wavefront per task, rows marched in order, four rows prefetched), reading
  * h, hu, hv, b of every tile row of the current state and writing h, hu, hv
- * of the interior rows to scratch arrays, without the edge arithmetic; the
- * pool's state is not changed.  `reps` timed launches after a warm-up;
+ * of the interior rows (and, as the update does, of the ghost columns next
+ * to them) to scratch arrays, without the edge arithmetic; the pool's state
+ * is not changed.  `reps` timed launches after a warm-up;
  * *best_gbs = 56 * interior cells / fastest launch. */
 int pc_march_probe(pc_pool *pool, int reps, double *best_gbs);
 

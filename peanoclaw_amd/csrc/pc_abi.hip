@@ -2213,7 +2213,10 @@ int pc_march_probe(pc_pool* p, int reps, double* best_gbs) {
     return fail(PC_E_OOM, "march probe: hipMalloc failed");
   }
   const Fields F = fields_of(p);
-  const int grid = p->persistent_wgs > 0 ? p->persistent_wgs : 4096;
+  // the update's launch shape (pc_step_all): a persistent grid only when the
+  // list has more tasks than it has workgroups
+  const int pers = (p->persistent_wgs > 0 && p->ntasks > p->persistent_wgs) ? 1 : 0;
+  const int grid = pers ? p->persistent_wgs : p->ntasks;
   long long cells = 0;
   for (int s : p->h_live)
     cells += (long long)p->slots[s].desc.subdivision_factor[0] * p->slots[s].desc.subdivision_factor[1];
@@ -2224,7 +2227,7 @@ int pc_march_probe(pc_pool* p, int reps, double* best_gbs) {
     if ((e = hipEventRecord(p->ev0, p->stream)) != hipSuccess) break;
     hipLaunchKernelGGL(k_march_probe, dim3(grid), dim3(64), 0, p->stream, p->d_tasks, p->ntasks,
                        p->d_plist, p->d_pd, F.h[par], F.hu[par], F.hv[par], F.b, o[0], o[1], o[2],
-                       ctr);
+                       ctr, pers);
     if ((e = hipGetLastError()) != hipSuccess) break;
     if ((e = hipEventRecord(p->ev1, p->stream)) != hipSuccess) break;
     if ((e = hipEventSynchronize(p->ev1)) != hipSuccess) break;

@@ -9,7 +9,7 @@
 // 64/W patches of the same size (W = ny rounded up to a power of two); lane
 // (seg, col) walks down column `col` of patch `seg`, keeps the previous row
 // in registers, solves the x-edge between the two rows and the y-edge to its
-// left neighbour lane (exchanged through LDS inside the wavefront), and
+// left neighbour lane (cell terms handed over by DPP wavefront shifts), and
 // finalises the previous row once both of its x-edges are known.  Each cell
 // is read from HBM once and written once.  The two y-edges per row that
 // touch the ghost columns are solved up front by a pre-pass in which the
@@ -193,12 +193,35 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// LDS scratch of one wavefront (64 lanes).
+// Lane exchange of the march's y-sweep through DPP wavefront shifts:
+// from_left(v) is lane-1's v (lane 0 keeps its own), from_right(v) lane+1's
+// (lane 63 keeps its own).  One v_mov_b32_dpp per dword, no LDS round trip.
+__device__ __forceinline__ int dpp_shr1(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false);  // wave_shr:1
+}
+__device__ __forceinline__ int dpp_shl1(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false);  // wave_shl:1
+}
+__device__ __forceinline__ double from_left(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_shr1((int)(unsigned)(b & 0xffffffffLL));
+  const int hi = dpp_shr1((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double from_right(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_shl1((int)(unsigned)(b & 0xffffffffLL));
+  const int hi = dpp_shl1((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ float from_left(float v) {
+  return __int_as_float(dpp_shr1(__float_as_int(v)));
+}
+
+// LDS scratch of one wavefront (64 lanes): the pre-pass's boundary y-edges,
+// [side*2 + k][seg*nx + row-1].
 struct MarchLds {
-  double yc[7][64];      // y-sweep cell terms of the current row
-  double nc[5][64];      // y-sweep speed terms of the new row (fused mode)
-  double ydow[2][64];    // y-edge results handed to the lower neighbour
-  double pre[4][64];     // boundary y-edges: [side*2 + k][seg*nx + row-1]
+  double pre[4][64];
 };
 
 struct MarchArgs {
@@ -273,6 +296,8 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   const int nrows = rb - ra + 1;
 
   // ---- pre-pass: the y-edges (row, 0|1) and (row, ny|ny+1) ----
+  // (after the previous task's last reads of L.pre on this wavefront)
+  wave_sync();
   const int npre = t.nseg * nrows * 2;
   for (int e = lane; e < npre; e += 64) {
     const int s = e / (2 * nrows);
@@ -398,7 +423,6 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   // pruned: bound terms and raw h, hu of the latest new row
   BoundTerms pbt = {0.0f, 0.0f, 0.0f};
   double pnh = 1.0, pnhu = 0.0;
-  float2* const nb = reinterpret_cast<float2*>(&L.nc[0][0]);
 
   double h, hu, hv, b;
   load_row(ra - 1, h, hu, hv, b);
@@ -491,85 +515,62 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
     if (r >= ra && r <= rb) {
       // y-edges of row r: lane col solves (col-1 | col) for col >= 2
       const CellQ cy = cell_dir(cb, chv);
-      L.yc[0][lane] = cy.h;
-      L.yc[1][lane] = cy.q;
-      L.yc[2][lane] = cy.b;
-      L.yc[3][lane] = cy.u;
-      L.yc[4][lane] = cy.sq;
-      L.yc[5][lane] = cy.c;
-      L.yc[6][lane] = cy.f;
-      if (PRUNE && have_new) {
-        nb[lane] = make_float2(pbt.ay, pbt.c);
-        L.nc[1][lane] = pnh;
-        L.nc[2][lane] = nhv;
-      }
-      CellQ ncy = {};
-      if (FUSED && !PRUNE && have_new) {
-        ncy = cell_dir(nbase, nhv);
-        L.nc[0][lane] = ncy.h;
-        L.nc[1][lane] = ncy.q;
-        L.nc[2][lane] = ncy.u;
-        L.nc[3][lane] = ncy.sq;
-        L.nc[4][lane] = ncy.c;
-      }
-      wave_sync();
-      const int slot = seg * nx + r - 1;
-      // left neighbour lane; a lane without one (col 1) reads its own terms,
-      // which makes its y-edge a dummy whose results are not used
-      const int lm = col >= 2 ? lane - 1 : lane;
+      // left neighbour lane; on a lane without one (col 1) the y-edge is a
+      // dummy whose results are not used
       CellQ cl;
-      cl.h = L.yc[0][lm]; cl.q = L.yc[1][lm]; cl.b = L.yc[2][lm];
-      cl.u = L.yc[3][lm]; cl.sq = L.yc[4][lm]; cl.c = L.yc[5][lm]; cl.f = L.yc[6][lm];
+      cl.h = from_left(cy.h); cl.q = from_left(cy.q); cl.b = from_left(cy.b);
+      cl.u = from_left(cy.u); cl.sq = from_left(cy.sq); cl.c = from_left(cy.c);
+      cl.f = from_left(cy.f);
+      CellQ ncy = {};
+      if (FUSED && !PRUNE && have_new) ncy = cell_dir(nbase, nhv);
+      const int slot = seg * nx + r - 1;
       if (MODE == kSpeed) {
         if (!SPEED_B) cl.b = 0.0;
         const double sp = ES::speed(cl, cy);
         if (col >= 2 && active) smaxl = dmax(smaxl, sp);
         if (active && col == 1) smaxl = dmax(smaxl, L.pre[0][slot]);
         if (active && col == ny) smaxl = dmax(smaxl, L.pre[1][slot]);
-        wave_sync();
       } else {
         double dhL, dhR, dqL, dqR;
         ES::solve(cl, cy, dhL, dhR, dqL, dqR);
         double up_h, up_hv;
+        // the lower neighbour's share, handed down one lane
+        const double ydh = from_right(dyinv * dhL), ydq = from_right(dyinv * dqL);
         if (col >= 2) {
           up_h = dyinv * dhR;
           up_hv = dyinv * dqR;
-          if (active) {
-            L.ydow[0][lane - 1] = dyinv * dhL;
-            L.ydow[1][lane - 1] = dyinv * dqL;
-          }
         } else {
           up_h = segok ? L.pre[0][slot] : 0.0;
           up_hv = segok ? L.pre[1][slot] : 0.0;
         }
         if (PRUNE && have_new) {
-          const float2 lb = nb[lm];
+          const float2 lb = make_float2(from_left(pbt.ay), from_left(pbt.c));
+          const double lnh = from_left(pnh), lnhv = from_left(nhv);
           const bool cand = col >= 2 && active && bound_candidate(lb.x, lb.y, pbt.ay, pbt.c, thr);
           if (__any(cand ? 1 : 0)) {
             const CellQ nr = cell_dir(cell_base(pnh, 0.0), nhv);
-            CellQ nl = cell_dir(cell_base(L.nc[1][lm], 0.0), L.nc[2][lm]);
+            CellQ nl = cell_dir(cell_base(lnh, 0.0), lnhv);
             const double sp = ES::speed(nl, nr);
             if (col >= 2 && active) smaxl = dmax(smaxl, sp);
           }
         }
         if (FUSED && !PRUNE && have_new) {
           CellQ nl;
-          nl.h = L.nc[0][lm]; nl.q = L.nc[1][lm]; nl.b = 0.0;
-          nl.u = L.nc[2][lm]; nl.sq = L.nc[3][lm]; nl.c = L.nc[4][lm]; nl.f = 0.0;
+          nl.h = from_left(ncy.h); nl.q = from_left(ncy.q); nl.b = 0.0;
+          nl.u = from_left(ncy.u); nl.sq = from_left(ncy.sq); nl.c = from_left(ncy.c);
+          nl.f = 0.0;
           const double sp = ES::speed(nl, ncy);
           if (col >= 2 && active) smaxl = dmax(smaxl, sp);
         }
-        wave_sync();
         double dn_h = 0.0, dn_hv = 0.0;
         if (col < ny) {
-          dn_h = L.ydow[0][lane];
-          dn_hv = L.ydow[1][lane];
+          dn_h = ydh;
+          dn_hv = ydq;
         } else if (segok) {
           dn_h = L.pre[2][slot];
           dn_hv = L.pre[3][slot];
         }
         yUpH = up_h; yUpHv = up_hv; yDnH = dn_h; yDnHv = dn_hv;
-        wave_sync();
       }
     }
     px = cx;
@@ -578,37 +579,24 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
 
   if (PRUNE && MODE == kUpdate) {
     // the y-edges of the last new row (row nx) are still open
-    nb[lane] = make_float2(pbt.ay, pbt.c);
-    L.nc[1][lane] = pnh;
-    L.nc[2][lane] = nhv;
-    wave_sync();
-    const int lm = col >= 2 ? lane - 1 : lane;
-    const float2 lb = nb[lm];
+    const float2 lb = make_float2(from_left(pbt.ay), from_left(pbt.c));
+    const double lnh = from_left(pnh), lnhv = from_left(nhv);
     const bool cand = col >= 2 && active && bound_candidate(lb.x, lb.y, pbt.ay, pbt.c, thr);
     if (__any(cand ? 1 : 0)) {
       const CellQ nr = cell_dir(cell_base(pnh, 0.0), nhv);
-      CellQ nl = cell_dir(cell_base(L.nc[1][lm], 0.0), L.nc[2][lm]);
+      CellQ nl = cell_dir(cell_base(lnh, 0.0), lnhv);
       const double sp = ES::speed(nl, nr);
       if (col >= 2 && active) smaxl = dmax(smaxl, sp);
     }
-    wave_sync();
   }
   if (FUSED && !PRUNE && MODE == kUpdate) {
     // the y-edges of the last new row (row nx) are still open
     const CellQ ncy = cell_dir(nbase, nhv);
-    L.nc[0][lane] = ncy.h;
-    L.nc[1][lane] = ncy.q;
-    L.nc[2][lane] = ncy.u;
-    L.nc[3][lane] = ncy.sq;
-    L.nc[4][lane] = ncy.c;
-    wave_sync();
-    const int lm = col >= 2 ? lane - 1 : lane;
     CellQ nl;
-    nl.h = L.nc[0][lm]; nl.q = L.nc[1][lm]; nl.b = 0.0;
-    nl.u = L.nc[2][lm]; nl.sq = L.nc[3][lm]; nl.c = L.nc[4][lm]; nl.f = 0.0;
+    nl.h = from_left(ncy.h); nl.q = from_left(ncy.q); nl.b = 0.0;
+    nl.u = from_left(ncy.u); nl.sq = from_left(ncy.sq); nl.c = from_left(ncy.c); nl.f = 0.0;
     const double sp = ES::speed(nl, ncy);
     if (col >= 2 && active) smaxl = dmax(smaxl, sp);
-    wave_sync();
   }
 
   if (MODE == kSpeed || FUSED) {
@@ -837,26 +825,30 @@ k_boundary_speed(const ScnDev* __restrict__ scn, const BoundaryDesc* __restrict_
 }
 
 // Memory-pattern probe of the row march: the update kernel's traversal
-// (same tasks, persistent grid, one wavefront per task, lanes on columns,
+// (same tasks and launch shape, one wavefront per task, lanes on columns,
 // rows marched in order) reading h, hu, hv, b of every row of the tiles and
-// writing h, hu, hv of the interior rows to scratch arrays, with no edge
-// arithmetic.  Without arithmetic to hide behind, the loads run four rows
-// ahead (rows in blocks of four).  The rate it reaches is the memory rate of
-// this traversal of the layout.
+// writing h, hu, hv of the interior rows to scratch arrays, the edge lanes
+// also into the row's ghost columns as the update does (whole cache lines),
+// with no edge arithmetic.  Without arithmetic to hide behind, the loads run
+// four rows ahead (rows in blocks of four).  The rate it reaches is the
+// memory rate of this traversal of the layout.
 __global__ void __launch_bounds__(64)
 k_march_probe(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plist,
               const PatchDev* __restrict__ pd, const double* __restrict__ h,
               const double* __restrict__ hu, const double* __restrict__ hv,
               const double* __restrict__ b, double* __restrict__ oh, double* __restrict__ ohu,
-              double* __restrict__ ohv, int* __restrict__ counter) {
+              double* __restrict__ ohv, int* __restrict__ counter, int persistent) {
   constexpr int K = 4;
   __shared__ int next;
   const int lane = threadIdx.x & 63;
+  int ti = blockIdx.x;
   for (;;) {
-    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
-    __syncthreads();
-    const int ti = next;
-    __syncthreads();
+    if (persistent) {
+      if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+      __syncthreads();
+      ti = next;
+      __syncthreads();
+    }
     if (ti >= ntasks) return;
     const Task t = gl(tasks + ti);
     const int W = pow2ceil(t.ny);
@@ -891,6 +883,12 @@ k_march_probe(const Task* __restrict__ tasks, int ntasks, const int* __restrict_
           gs(oh + o, x);
           gs(ohu + o, chu[k] * x);
           gs(ohv + o, chv[k] * x);
+          if (col == 1 || col == t.ny) {
+            const long long og = o + (col == 1 ? -1 : 1);
+            gs(oh + og, x);
+            gs(ohu + og, chu[k] * x);
+            gs(ohv + og, chv[k] * x);
+          }
         }
       }
 #pragma unroll
@@ -898,6 +896,7 @@ k_march_probe(const Task* __restrict__ tasks, int ntasks, const int* __restrict_
         ch[k] = nh[k]; chu[k] = nhu[k]; chv[k] = nhv[k]; cb[k] = nbv[k];
       }
     }
+    if (!persistent) return;
   }
 }
 
