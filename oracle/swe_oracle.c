@@ -809,19 +809,20 @@ void or_interpolate_default(const int dstSize[2], const int dstOff[2],
  * the source is not transformed back (the reference's round trip
  * (h + b) - b can move h by an ulp; here the source keeps its bits).  The
  * destination region is transformed back to h with its own b. */
-void or_interpolate_swe_pure(const int dstSize[2], const int dstOff[2], or_patch *src,
-                             or_patch *dst, int toUOld, int toCurrentTime,
-                             int useTimeUNewOrUOld) {
-  interpolate_values(dstSize, dstOff, src, dst, toUOld, toCurrentTime, useTimeUNewOrUOld, 1);
-  or_transform_water_height(dst, dstOff, dstSize, toUOld, 1);
-}
-
-void or_interpolate_swe(const int dstSize[2], const int dstOff[2],
-                        or_patch *src, or_patch *dst, int toUOld,
-                        int toCurrentTime, int useTimeUNewOrUOld) {
-  int so[2], ss[2];
+/* The destination region of an interpolation must lie in the destination
+ * (the ghost ring included for uOld), and the source region it maps to,
+ * grown by one cell, in the source's cells 0..n (SWEKernel.cpp:252-263 reach
+ * index n).  The reference does not check either; a grid whose geometry
+ * breaks them would make it transform memory outside the patch.  -1 when
+ * violated (nothing is changed), else 0. */
+static int interpolation_regions(const int dstSize[2], const int dstOff[2], or_patch *src,
+                                 or_patch *dst, int toUOld, int so[2], int ss[2]) {
+  int dn[2] = {dst->nx, dst->ny}, sn[2] = {src->nx, src->ny};
+  for (int d = 0; d < 2; d++) {
+    if (dstOff[d] < -1 || dstSize[d] < 0 || dstOff[d] + dstSize[d] > dn[d] + 1) return -1;
+    if (!toUOld && (dstOff[d] < 0 || dstOff[d] + dstSize[d] > dn[d])) return -1;
+  }
   or_region_map_to_patch(dstOff, dstSize, dst, src, 1e-12, so, ss);
-  int sn[2] = {src->nx, src->ny};
   for (int d = 0; d < 2; d++) {
     if (so[d] > 0) {
       so[d] = so[d] - 1;
@@ -829,7 +830,26 @@ void or_interpolate_swe(const int dstSize[2], const int dstOff[2],
     } else {
       ss[d] = imin(sn[d], ss[d] + 1);
     }
+    if (so[d] < 0 || so[d] + ss[d] > sn[d] + 1) return -1;
   }
+  return 0;
+}
+
+int or_interpolate_swe_pure(const int dstSize[2], const int dstOff[2], or_patch *src,
+                            or_patch *dst, int toUOld, int toCurrentTime,
+                            int useTimeUNewOrUOld) {
+  int so[2], ss[2];
+  if (interpolation_regions(dstSize, dstOff, src, dst, toUOld, so, ss)) return -1;
+  interpolate_values(dstSize, dstOff, src, dst, toUOld, toCurrentTime, useTimeUNewOrUOld, 1);
+  or_transform_water_height(dst, dstOff, dstSize, toUOld, 1);
+  return 0;
+}
+
+int or_interpolate_swe(const int dstSize[2], const int dstOff[2],
+                       or_patch *src, or_patch *dst, int toUOld,
+                       int toCurrentTime, int useTimeUNewOrUOld) {
+  int so[2], ss[2];
+  if (interpolation_regions(dstSize, dstOff, src, dst, toUOld, so, ss)) return -1;
   or_transform_water_height(src, so, ss, 1, 0);
   or_transform_water_height(src, so, ss, 0, 0);
   or_interpolate_default(dstSize, dstOff, src, dst, toUOld, toCurrentTime,
@@ -837,6 +857,7 @@ void or_interpolate_swe(const int dstSize[2], const int dstOff[2],
   or_transform_water_height(src, so, ss, 1, 1);
   or_transform_water_height(src, so, ss, 0, 1);
   or_transform_water_height(dst, dstOff, dstSize, toUOld, 1);
+  return 0;
 }
 
 /* DefaultRestriction::calculateOverlappingRegion (DefaultRestriction.h:44-62) */

@@ -110,17 +110,19 @@ void or_transfer_ghostlayer(const int size[2], const int srcOff[2],
 void or_interpolate_default(const int dstSize[2], const int dstOff[2],
                             or_patch *src, or_patch *dst, int toUOld,
                             int toCurrentTime, int useTimeUNewOrUOld);
-void or_interpolate_swe(const int dstSize[2], const int dstOff[2],
-                        or_patch *src, or_patch *dst, int toUOld,
-                        int toCurrentTime, int useTimeUNewOrUOld);
+/* 0, or -1 (nothing changed) when the destination region or the grown
+ * source region it maps to is outside its patch */
+int or_interpolate_swe(const int dstSize[2], const int dstOff[2],
+                       or_patch *src, or_patch *dst, int toUOld,
+                       int toCurrentTime, int useTimeUNewOrUOld);
 void or_restrict_default(or_patch *src, or_patch *dst, int onlyOverlapped);
 void or_restrict_swe(or_patch *src, or_patch *dst, int onlyOverlapped);
 /* the SWE transfers between levels with the h <-> h+b transform applied to
    the values read, the source left unchanged (batched AMR semantics) */
 void or_restrict_swe_pure(or_patch *src, or_patch *dst);
-void or_interpolate_swe_pure(const int dstSize[2], const int dstOff[2], or_patch *src,
-                             or_patch *dst, int toUOld, int toCurrentTime,
-                             int useTimeUNewOrUOld);
+int or_interpolate_swe_pure(const int dstSize[2], const int dstOff[2], or_patch *src,
+                            or_patch *dst, int toUOld, int toCurrentTime,
+                            int useTimeUNewOrUOld);
 void or_post_process_restriction_swe(or_patch *dst, int onlyOverlapped);
 /* Patch::switchToLeaf: restricted [0, 1]-basis values -> the patch's own
    interval [time, time + dt] (set p->time, p->dt first) */

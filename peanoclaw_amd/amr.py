@@ -1,6 +1,7 @@
 """Adaptive patch grids (refinement factor 3) with global time steps.
 
-The grid is px x py coarse patches of n x n cells on [0,1]^2; a refined
+The grid is px x py square coarse patches of n x n cells, size 1/px (the
+domain [0, 1] x [0, py/px]); a refined
 patch becomes virtual and holds 3 x 3 patches of n x n cells one level
 finer, to any depth.  One step is the sequence of Numerics calls
 PeanoClaw's traversal makes for a grid whose leaves all share one clock
@@ -57,7 +58,7 @@ class AmrPatch:
 
 class AmrGrid:
     """Patch tree with refinement factor 3: level 1 is a px x py grid of
-    patches of n x n cells on [0,1]^2; a patch at level l that `refine(l,
+    square patches of n x n cells, size 1/px (the domain [0, 1] x [0, py/px]); a patch at level l that `refine(l,
     (i, j))` selects becomes virtual and holds 3 x 3 patches at level l+1
     (indices (3i+a, 3j+c)), down to `max_level`.  `regrid` refines and
     coarsens during a run."""
@@ -71,7 +72,8 @@ class AmrGrid:
         self.max_level = max_level
         self.fn = fn
         self.levels = {}
-        todo = [(1, (i, j), (i / px, j / py), 1.0 / px) for i in range(px) for j in range(py)]
+        # square patches of size 1/px: the domain is [0, 1] x [0, py/px]
+        todo = [(1, (i, j), (i / px, j / px), 1.0 / px) for i in range(px) for j in range(py)]
         while todo:
             nxt = []
             for lev, ij, pos, size in todo:

@@ -62,8 +62,9 @@ class OracleBackend:
         lib().or_restrict_swe_pure(self.p[src].ref, self.p[dst].ref)
 
     def interpolate_pure(self, size, doff, src, dst, to_uold, to_current, use_time):
-        lib().or_interpolate_swe_pure(iarr(*size), iarr(*doff), self.p[src].ref, self.p[dst].ref,
-                                      to_uold, to_current, use_time)
+        if lib().or_interpolate_swe_pure(iarr(*size), iarr(*doff), self.p[src].ref,
+                                         self.p[dst].ref, to_uold, to_current, use_time):
+            raise RuntimeError("interpolation region outside its patch")
 
     def post_restrict(self, dst):
         lib().or_post_process_restriction_swe(self.p[dst].ref, 0)
@@ -76,8 +77,9 @@ class OracleBackend:
                                      self.p[dst].ref)
 
     def interpolate(self, size, doff, src, dst, to_uold, to_current, use_time):
-        lib().or_interpolate_swe(iarr(*size), iarr(*doff), self.p[src].ref, self.p[dst].ref,
-                                 to_uold, to_current, use_time)
+        if lib().or_interpolate_swe(iarr(*size), iarr(*doff), self.p[src].ref, self.p[dst].ref,
+                                    to_uold, to_current, use_time):
+            raise RuntimeError("interpolation region outside its patch")
 
     def fill_boundary(self, s, dim, upper):
         lib().or_fill_boundary_layer(self.p[s].ref, dim, upper)

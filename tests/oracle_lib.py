@@ -104,12 +104,12 @@ def lib():
             "or_transfer_ghostlayer": (None, [_ip, _ip, _ip, P, P]),
             "or_interpolate_default": (None, [_ip, _ip, P, P, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int]),
-            "or_interpolate_swe": (None, [_ip, _ip, P, P, ctypes.c_int,
+            "or_interpolate_swe": (ctypes.c_int, [_ip, _ip, P, P, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int]),
             "or_restrict_default": (None, [P, P, ctypes.c_int]),
             "or_restrict_swe": (None, [P, P, ctypes.c_int]),
             "or_restrict_swe_pure": (None, [P, P]),
-            "or_interpolate_swe_pure": (None, [_ip, _ip, P, P, ctypes.c_int, ctypes.c_int,
+            "or_interpolate_swe_pure": (ctypes.c_int, [_ip, _ip, P, P, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int]),
             "or_post_process_restriction_swe": (None, [P, ctypes.c_int]),
             "or_switch_to_leaf": (None, [P]),

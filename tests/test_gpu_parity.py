@@ -161,11 +161,11 @@ def test_interpolation_and_restriction_swe(amd):
     of = _oracle_patch_from(fu, fo, fb, (1.0 / 3, 1.0 / 3), 1.0 / 3, 0.3, 0.01)
     # fine left ghost layer from the coarse patch (face functor arguments)
     num.interpolateSolution((1, nf), (-1, 0), Patch(amd, sc), Patch(amd, sf), True, False, True)
-    olib().or_interpolate_swe(iarr(1, nf), iarr(-1, 0), oc.ref, of.ref, 1, 0, 1)
+    assert olib().or_interpolate_swe(iarr(1, nf), iarr(-1, 0), oc.ref, of.ref, 1, 0, 1) == 0
     # whole fine patch into uNew and uOld (Remesh::createCell, Remesh.cpp:399-418)
     num.interpolateSolution((nf, nf), (0, 0), Patch(amd, sc), Patch(amd, sf), False, False,
                             False)
-    olib().or_interpolate_swe(iarr(nf, nf), iarr(0, 0), oc.ref, of.ref, 0, 0, 0)
+    assert olib().or_interpolate_swe(iarr(nf, nf), iarr(0, 0), oc.ref, of.ref, 0, 0, 0) == 0
     for s, o in ((sc, oc), (sf, of)):
         gn, go, gb = amd.download(s)
         assert relmax(gn, o.unew) == 0.0
