@@ -142,6 +142,39 @@ int pc_restrict_solution(pc_pool *pool, int source_slot, int destination_slot,
 /* Numerics::postProcessRestriction (Numerics.h:103-106; SWEKernel.cpp:309-316) */
 int pc_post_process_restriction(pc_pool *pool, int destination_slot,
                                 int restrict_only_overlapped_regions);
+
+/* Batched level transfers for regridding (Remesh::createCell /
+ * destroyCell's interpolateSolution and restrictSolution sequences,
+ * Remesh.cpp:331-540).  A batch has exactly the effect of its calls made one
+ * after another in the order given: calls that share no patch run together,
+ * one launch per step for all of them.  Every call is checked before any
+ * runs; an invalid one fails the batch with nothing changed.
+ *
+ * pc_interpolate_batch: `count` pc_interpolate_solution calls; call i has
+ * destination_size {dsize[2i], dsize[2i+1]}, destination_offset
+ * {doff[2i], doff[2i+1]}, source src[i], destination dst[i], and
+ * opts[3i..3i+2] = interpolate_to_uold, interpolate_to_current_time,
+ * use_time_unew_or_uold. */
+int pc_interpolate_batch(pc_pool *pool, int count, const int *dsize, const int *doff,
+                         const int *src, const int *dst, const int *opts);
+/* `count` pc_restrict_solution(src[i], dst[i], restrict_only_overlapped_regions)
+ * calls (the restrictions of several fine patches into one virtual patch
+ * keep their order). */
+int pc_restrict_batch(pc_pool *pool, int count, const int *src, const int *dst,
+                      int restrict_only_overlapped_regions);
+/* Patch::switchToLeaf (Patch.cpp:793-830) of `count` distinct patches: the
+ * patch becomes a leaf with interval [time[i], time[i] + dt[i]], and its
+ * restricted values, which hold the [0, 1] time basis, are evaluated at the
+ * interval's ends (uOld at time[i], uNew at time[i] + dt[i]). */
+int pc_switch_to_leaf(pc_pool *pool, int count, const int *slots, const double *time,
+                      const double *dt);
+/* Zero uNew and uOld (ghost ring included) of `count` distinct patches, as
+ * a virtual patch is cleared before the restrictions into it; b is kept. */
+int pc_patches_clear(pc_pool *pool, int count, const int *slots);
+/* Set b (ghost ring included, the layout of pc_patch_upload's b) of `count`
+ * distinct patches from one array: patch i takes the next
+ * (nx_i + 2)(ny_i + 2) values. */
+int pc_patches_set_b(pc_pool *pool, int count, const int *slots, const double *b);
 /* Flux correction, off by default in the reference (SolveTimestep's
  * _correctFluxes; called after solveTimestep, SolveTimestep.cpp:673-679).
  * Each slot holds the reference's flux array: faces x-lower, x-upper,

@@ -150,6 +150,12 @@ _SIGS = {
                                             ctypes.c_int]),
     "pc_post_process_restriction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
                                                    ctypes.c_int]),
+    "pc_interpolate_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip, _ip, _ip,
+                                            _ip]),
+    "pc_restrict_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip, ctypes.c_int]),
+    "pc_switch_to_leaf": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp]),
+    "pc_patches_clear": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip]),
+    "pc_patches_set_b": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _dp]),
     "pc_copy_unew_to_uold": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pc_swap_unew_and_uold": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pc_demanded_mesh_width": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,

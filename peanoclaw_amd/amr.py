@@ -1,9 +1,8 @@
 """Adaptive patch grids (refinement factor 3) with global time steps.
 
 The grid is px x py square coarse patches of n x n cells, size 1/px (the
-domain [0, 1] x [0, py/px]); a refined
-patch becomes virtual and holds 3 x 3 patches of n x n cells one level
-finer, to any depth.  One step is the sequence of Numerics calls
+domain [0, 1] x [0, py/px]); a refined patch becomes virtual and holds
+3 x 3 patches of n x n cells one level finer, to any depth.  One step is the sequence of Numerics calls
 PeanoClaw's traversal makes for a grid whose leaves all share one clock
 (maximumTimestepSize = the global CFL minimum, SolveTimestep.cpp:591-751):
 
@@ -168,14 +167,17 @@ class AmrGrid:
                    if all(not c.virtual and flag.get(c.handle) == 0 and c.age >= 5
                           for c in v.children)]
         changed = set()
+        # the coarsened patches share no patch: their clear / restrict /
+        # post-process / switch sequences run as batches, in that order
+        if coarsen:
+            B.clear_batch([v.handle for v in coarsen])
+            B.restrict_batch([(c.handle, v.handle) for v in coarsen for c in v.children])
+            for v in coarsen:
+                B.post_restrict(v.handle)
+            B.switch_to_leaf_batch([v.handle for v in coarsen], self.time, self.dt)
         for v in coarsen:
             changed.add(v.handle)
             changed.update(c.handle for c in v.children)
-            B.clear(v.handle)
-            for c in v.children:
-                B.restrict(c.handle, v.handle)
-            B.post_restrict(v.handle)
-            B.switch_to_leaf(v.handle, self.time, self.dt)
             for c in v.children:
                 del self.levels[c.level][c.ij]
                 B.release(c.handle)
@@ -196,11 +198,13 @@ class AmrGrid:
                     created.append((p, ch))
                     changed.add(ch.handle)
                     self.levels.setdefault(lev, {})[ij] = ch
-        for p, ch in created:
-            B.set_b(ch.handle, self._fields(ch)[1])
-        for p, ch in created:
-            B.interpolate((n, n), (0, 0), p.handle, ch.handle, 0, 0, 0)
-            B.interpolate((n, n), (0, 0), p.handle, ch.handle, 1, 1, 0)
+        # b of the new patches in one upload, then their interpolations
+        # (uNew at the new time, uOld at the current time; the nine children
+        # of a parent in order) as one batch
+        B.set_b_batch([ch.handle for _, ch in created], [self._fields(ch)[1] for _, ch in created])
+        B.interpolate_batch([call for p, ch in created
+                             for call in (((n, n), (0, 0), p.handle, ch.handle, 0, 0, 0),
+                                          ((n, n), (0, 0), p.handle, ch.handle, 1, 1, 0))])
         for p in refine:
             B.set_virtual(p.handle, True)
             p.virtual = True
@@ -547,14 +551,7 @@ class DeviceBackend:
         return [int(f[s]) for s in slots]
 
     def switch_to_leaf(self, s, time, dt):
-        unew, uold, _ = self.pool.download(s)
-        a, b = switch_to_leaf_values(uold[:, 1:-1, 1:-1], unew, time, dt)
-        uold[:, 1:-1, 1:-1] = a
-        self.pool.upload(s, unew=b, uold=uold)
-        d = self.pool.description(s)
-        d.is_virtual = 0
-        d.time, d.timestep_size = time, dt
-        self.pool.set_description(s, d)
+        self.switch_to_leaf_batch([s], time, dt)
 
     def set_state(self, s, h, b):
         n = h.shape[0]
@@ -583,6 +580,44 @@ class DeviceBackend:
         check(self.lib.pc_interpolate_solution(self.pool.h, iarr(*size), iarr(*doff), src, dst,
                                                to_uold, to_current, use_time))
 
+    # batched forms: the effect of the per-call methods in the order given
+    def interpolate_batch(self, calls):
+        """calls: (size, doff, src, dst, to_uold, to_current, use_time)."""
+        if not calls:
+            return
+        size = _ints([c[0] for c in calls])
+        doff = _ints([c[1] for c in calls])
+        src, dst = _ints([c[2] for c in calls]), _ints([c[3] for c in calls])
+        opts = _ints([c[4:7] for c in calls])
+        check(self.lib.pc_interpolate_batch(self.pool.h, len(calls), _ip(size), _ip(doff),
+                                            _ip(src), _ip(dst), _ip(opts)))
+
+    def restrict_batch(self, pairs):
+        """pairs: (src, dst), restricted into the virtual dst in order."""
+        if not pairs:
+            return
+        src, dst = _ints([a for a, _ in pairs]), _ints([b for _, b in pairs])
+        check(self.lib.pc_restrict_batch(self.pool.h, len(pairs), _ip(src), _ip(dst), 0))
+
+    def clear_batch(self, handles):
+        if handles:
+            check(self.lib.pc_patches_clear(self.pool.h, len(handles), _ip(_ints(handles))))
+
+    def set_b_batch(self, handles, bs):
+        if handles:
+            b = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64).ravel()
+                                                     for x in bs]))
+            check(self.lib.pc_patches_set_b(self.pool.h, len(handles), _ip(_ints(handles)),
+                                            b.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def switch_to_leaf_batch(self, handles, time, dt):
+        if handles:
+            t = np.full(len(handles), float(time))
+            d = np.full(len(handles), float(dt))
+            dp = ctypes.POINTER(ctypes.c_double)
+            check(self.lib.pc_switch_to_leaf(self.pool.h, len(handles), _ip(_ints(handles)),
+                                             t.ctypes.data_as(dp), d.ctypes.data_as(dp)))
+
     def fill_boundary(self, s, dim, upper):
         check(self.lib.pc_fill_boundary_layer(self.pool.h, s, dim, upper))
 
@@ -605,12 +640,12 @@ class DeviceBackend:
         return self.pool.download(s)[0]
 
 
-def switch_to_leaf_values(uold, unew, time, dt):
-    """Patch::switchToLeaf (Patch.cpp:793-830): values restricted on the
-    [0, 1] time basis evaluated at time (uOld) and time + dt (uNew)."""
-    f_old = (time - 0.0) / 1.0
-    f_new = (time + dt - 0.0) / 1.0
-    return uold * (1.0 - f_old) + unew * f_old, uold * (1.0 - f_new) + unew * f_new
+def _ints(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.int32).ravel())
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
 def refined_set_from_flags(pool, px, n, fn, threshold=0.05):
@@ -822,5 +857,5 @@ def dam_break_fields(centre=(0.5, 0.5), radius=0.25, h_in=2.0, h_out=1.0, hills=
     return fn
 
 
-__all__ = ["AmrGrid", "BatchedAmr", "switch_to_leaf_values", "TwoLevelGrid", "DeviceBackend", "dam_break_fields",
+__all__ = ["AmrGrid", "BatchedAmr", "TwoLevelGrid", "DeviceBackend", "dam_break_fields",
            "front_refinement", "refined_set_from_flags", "wet_dry_fields"]

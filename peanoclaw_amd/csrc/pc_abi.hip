@@ -133,6 +133,9 @@ struct pc_pool {
   // flux arrays of the flux correction, kFluxPerSlot doubles per slot
   double* d_flux = nullptr;
   size_t flux_slots = 0;
+  // staging of the batched level transfers (pc_*_batch, pc_patches_*)
+  char* d_batch = nullptr;
+  size_t batch_cap = 0;
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -825,7 +828,7 @@ int pc_pool_destroy(pc_pool* p) {
                   p->d_plist, p->d_plist_scn, p->d_scn_of, p->d_live, p->d_bdesc, p->d_strips,
                   p->d_counter, p->d_iota, p->d_segmax, p->d_result, p->d_aux, p->d_flags, p->d_ebuf[0],
                   p->d_ebuf[1], p->d_bcol, p->d_sides, p->d_single, p->d_single_slot,
-                  p->d_edge_io, p->d_flux};
+                  p->d_edge_io, p->d_flux, p->d_batch};
   for (void* q : ptrs)
     if (q) hipFree(q);
   for (hipEvent_t e : p->upd_ev) hipEventDestroy(e);
@@ -1182,10 +1185,14 @@ int pc_transfer_ghostlayer(pc_pool* p, const int size[2], const int so[2],
   return PC_OK;
 }
 
-static int transform(pc_pool* p, int slot, const int off[2], const int size[2], int uold,
-                     int to_above_sea_floor) {
-  const int n = size[0] * size[1];
-  if (n <= 0) return PC_OK;
+// SWEKernel::transformWaterHeight over a region of one state (uold: the
+// previous state, else uNew with its layout), planned and checked before
+// anything runs; `out.sx * out.sy == 0` is a no-op.
+static int plan_transform(pc_pool* p, int slot, const int off[2], const int size[2], int uold,
+                          int to_above_sea_floor, TransformItem& out) {
+  out = TransformItem{};
+  out.slot = slot;
+  if (size[0] * size[1] <= 0) return PC_OK;
   const int cur = cur_of(p, slot);
   const auto& d = p->slots[slot].desc;
   const int nx = d.subdivision_factor[0], ny = d.subdivision_factor[1];
@@ -1195,11 +1202,31 @@ static int transform(pc_pool* p, int slot, const int off[2], const int size[2], 
   if (!uold && (off[0] < 0 || off[1] < 0 ||
                 (long long)(off[0] + size[0] - 1) * ny + (off[1] + size[1] - 1) >= 3LL * nx * ny))
     return fail(PC_E_INVALID, "uNew transform region outside the unknowns");
+  out.st = uold ? 1 - cur : cur;
+  out.ox = off[0];
+  out.oy = off[1];
+  out.sx = size[0];
+  out.sy = size[1];
+  out.sign = to_above_sea_floor ? -1.0 : 1.0;
+  out.unew_layout = uold ? 0 : 1;
+  return PC_OK;
+}
+
+static int launch_transform(pc_pool* p, const TransformItem& it) {
+  const int n = it.sx * it.sy;
+  if (n <= 0) return PC_OK;
   hipLaunchKernelGGL(k_transform, dim3((n + 255) / 256), dim3(256), 0, p->stream, p->d_pd,
-                     slot, fields_of(p), uold ? 1 - cur : cur, off[0], off[1], size[0],
-                     size[1], to_above_sea_floor ? -1.0 : 1.0, uold ? 0 : 1);
+                     it.slot, fields_of(p), it.st, it.ox, it.oy, it.sx, it.sy, it.sign,
+                     it.unew_layout);
   HIPCHK(hipGetLastError());
   return PC_OK;
+}
+
+static int transform(pc_pool* p, int slot, const int off[2], const int size[2], int uold,
+                     int to_above_sea_floor) {
+  TransformItem it;
+  CHK(plan_transform(p, slot, off, size, uold, to_above_sea_floor, it));
+  return launch_transform(p, it);
 }
 
 // Region::mapToPatch (Region.cpp:174-206)
@@ -1223,13 +1250,21 @@ static void map_to_patch(const int off[2], const int size[2], const PatchDev& sr
   }
 }
 
-int pc_interpolate_solution(pc_pool* p, const int dsize[2], const int doff[2], int src,
-                            int dst, int toUOld, int toCurrentTime, int useTimeUNewOrUOld) {
+// One interpolateSolution call (SWEKernel.cpp:239-292 around
+// DefaultInterpolation.cpph:29-270), planned and checked before anything
+// runs: the source region (grown by one cell) to h + b in both states, the
+// interpolation, both back to h, the destination region back to h.
+struct InterpPlan {
+  InterpArgs a;
+  int n;  // destination cells
+  TransformItem fwd[2], back[3];
+};
+
+static int plan_interpolation(pc_pool* p, const int dsize[2], const int doff[2], int src,
+                              int dst, int toUOld, int toCurrentTime, int useTimeUNewOrUOld,
+                              InterpPlan& out) {
   if (!valid_slot(p, src) || !valid_slot(p, dst) || !dsize || !doff)
     return fail(PC_E_INVALID, "invalid argument");
-  hipSetDevice(p->device);
-  CHK(prepare(p));
-  mark_state_changed(p);
   const auto& sd = p->slots[src].desc;
   const auto& dd = p->slots[dst].desc;
   const PatchDev S = make_patchdev(p->slots[src]), D = make_patchdev(p->slots[dst]);
@@ -1267,9 +1302,13 @@ int pc_interpolate_solution(pc_pool* p, const int dsize[2], const int doff[2], i
     tf = toCurrentTime ? (dt_ - st) / (st + sdt - st) : (dt_ + ddt - st) / (st + sdt - st);
   }
   tf = std::max(0.0, std::min(1.0, tf));
-  CHK(transform(p, src, so, ss, 1, 0));
-  CHK(transform(p, src, so, ss, 0, 0));
-  InterpArgs a{};
+  CHK(plan_transform(p, src, so, ss, 1, 0, out.fwd[0]));
+  CHK(plan_transform(p, src, so, ss, 0, 0, out.fwd[1]));
+  CHK(plan_transform(p, src, so, ss, 1, 1, out.back[0]));
+  CHK(plan_transform(p, src, so, ss, 0, 1, out.back[1]));
+  CHK(plan_transform(p, dst, doff, dsize, toUOld, 1, out.back[2]));
+  InterpArgs& a = out.a;
+  a = InterpArgs{};
   a.srcSlot = src; a.dstSlot = dst;
   a.srcCur = cur_of(p, src); a.dstCur = cur_of(p, dst);
   a.sx = dsize[0]; a.sy = dsize[1]; a.ox = doff[0]; a.oy = doff[1];
@@ -1281,15 +1320,28 @@ int pc_interpolate_solution(pc_pool* p, const int dsize[2], const int doff[2], i
   a.dpos[0] = dd.position[0]; a.dpos[1] = dd.position[1];
   a.dsub[0] = D.dx; a.dsub[1] = D.dy;
   a.sn[0] = sn[0]; a.sn[1] = sn[1];
-  const int n = dsize[0] * dsize[1];
-  if (n > 0) {
-    hipLaunchKernelGGL(k_interpolate, dim3((n + 127) / 128), dim3(128), 0, p->stream,
-                       p->d_pd, fields_of(p), a);
+  out.n = dsize[0] * dsize[1];
+  return PC_OK;
+}
+
+int pc_interpolate_solution(pc_pool* p, const int dsize[2], const int doff[2], int src,
+                            int dst, int toUOld, int toCurrentTime, int useTimeUNewOrUOld) {
+  if (!valid_slot(p, src) || !valid_slot(p, dst) || !dsize || !doff)
+    return fail(PC_E_INVALID, "invalid argument");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  InterpPlan pl;
+  CHK(plan_interpolation(p, dsize, doff, src, dst, toUOld, toCurrentTime, useTimeUNewOrUOld,
+                         pl));
+  mark_state_changed(p);
+  CHK(launch_transform(p, pl.fwd[0]));
+  CHK(launch_transform(p, pl.fwd[1]));
+  if (pl.n > 0) {
+    hipLaunchKernelGGL(k_interpolate, dim3((pl.n + 127) / 128), dim3(128), 0, p->stream,
+                       p->d_pd, fields_of(p), pl.a);
     HIPCHK(hipGetLastError());
   }
-  CHK(transform(p, src, so, ss, 1, 1));
-  CHK(transform(p, src, so, ss, 0, 1));
-  CHK(transform(p, dst, doff, dsize, toUOld, 1));
+  for (const TransformItem& it : pl.back) CHK(launch_transform(p, it));
   return PC_OK;
 }
 
@@ -1341,12 +1393,20 @@ static int overlapped_regions(const double lower[2], const double upper[2],
   return 4;
 }
 
-int pc_restrict_solution(pc_pool* p, int src, int dst, int onlyOverlapped) {
+// One restrictSolution call (SWEKernel.cpp:294-310 around
+// DefaultRestriction.cpph:16-142), planned and checked before anything runs:
+// the source to h + b in both states, its (up to four) regions accumulated
+// into the destination in order, the source back to h.
+struct RestrictPlan {
+  TransformItem fwd[2], back[2];
+  RestrictArgs r[4];
+  int nr;  // non-empty regions, in the reference's order
+};
+
+static int plan_restriction(pc_pool* p, int src, int dst, int onlyOverlapped,
+                            RestrictPlan& out) {
   if (!valid_slot(p, src) || !valid_slot(p, dst))
     return fail(PC_E_INVALID, "invalid slot");
-  hipSetDevice(p->device);
-  CHK(prepare(p));
-  mark_state_changed(p);
   const auto& sd = p->slots[src].desc;
   const auto& dd = p->slots[dst].desc;
   if (!dd.is_virtual) return fail(PC_E_INVALID, "restriction destination must be virtual");
@@ -1355,8 +1415,10 @@ int pc_restrict_solution(pc_pool* p, int src, int dst, int onlyOverlapped) {
   const int dn[2] = {dd.subdivision_factor[0], dd.subdivision_factor[1]};
   const int z[2] = {0, 0};
   // SWEKernel.cpp:294-297
-  CHK(transform(p, src, z, sn, 1, 0));
-  CHK(transform(p, src, z, sn, 0, 0));
+  CHK(plan_transform(p, src, z, sn, 1, 0, out.fwd[0]));
+  CHK(plan_transform(p, src, z, sn, 0, 0, out.fwd[1]));
+  CHK(plan_transform(p, src, z, sn, 1, 1, out.back[0]));
+  CHK(plan_transform(p, src, z, sn, 0, 1, out.back[1]));
   RestrictArgs a{};
   a.srcSlot = src; a.dstSlot = dst;
   a.srcCur = cur_of(p, src); a.dstCur = cur_of(p, dst);
@@ -1392,6 +1454,7 @@ int pc_restrict_solution(pc_pool* p, int src, int dst, int onlyOverlapped) {
     rsz[1] = sn[1];
     nr = 1;
   }
+  out.nr = 0;
   for (int r = 0; r < nr; ++r) {
     if (rsz[2 * r] <= 0 || rsz[2 * r + 1] <= 0) continue;
     a.rOff[0] = roff[2 * r]; a.rOff[1] = roff[2 * r + 1];
@@ -1400,15 +1463,318 @@ int pc_restrict_solution(pc_pool* p, int src, int dst, int onlyOverlapped) {
     for (int d = 0; d < 2; ++d)
       if (a.dOff[d] < 0 || a.dOff[d] + a.dSize[d] > dn[d])
         return fail(PC_E_INVALID, "restriction source is not inside the destination");
-    const int n = a.dSize[0] * a.dSize[1];
-    if (n > 0) {
-      hipLaunchKernelGGL(k_restrict, dim3((n + 63) / 64), dim3(64), 0, p->stream, p->d_pd,
-                         fields_of(p), a);
+    if (a.dSize[0] * a.dSize[1] > 0) out.r[out.nr++] = a;
+  }
+  return PC_OK;
+}
+
+int pc_restrict_solution(pc_pool* p, int src, int dst, int onlyOverlapped) {
+  if (!valid_slot(p, src) || !valid_slot(p, dst))
+    return fail(PC_E_INVALID, "invalid slot");
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  RestrictPlan pl;
+  CHK(plan_restriction(p, src, dst, onlyOverlapped, pl));
+  mark_state_changed(p);
+  CHK(launch_transform(p, pl.fwd[0]));
+  CHK(launch_transform(p, pl.fwd[1]));
+  for (int r = 0; r < pl.nr; ++r) {
+    const int n = pl.r[r].dSize[0] * pl.r[r].dSize[1];
+    hipLaunchKernelGGL(k_restrict, dim3((n + 63) / 64), dim3(64), 0, p->stream, p->d_pd,
+                       fields_of(p), pl.r[r]);
+    HIPCHK(hipGetLastError());
+  }
+  CHK(launch_transform(p, pl.back[0]));
+  CHK(launch_transform(p, pl.back[1]));
+  return PC_OK;
+}
+
+}  // extern "C"
+
+// ---- batched level transfers ----
+//
+// A batch is exactly the sequence of its per-call entry points.  Calls that
+// share no patch commute, so each call goes to the first wave after every
+// earlier call it shares a source or destination with; a wave runs as a few
+// launches with one item per call (blockIdx.y).  Everything is checked
+// before anything runs.
+
+// Wave of every call: one more than the latest wave of an earlier call that
+// touches either of its two patches.
+static std::vector<int> conflict_waves(int count, const int* a, const int* b, int* nwaves) {
+  std::vector<int> wave(count);
+  std::map<int, int> last;
+  int top = -1;
+  for (int i = 0; i < count; ++i) {
+    int w = 0;
+    for (int s : {a[i], b[i]}) {
+      auto it = last.find(s);
+      if (it != last.end()) w = std::max(w, it->second + 1);
+    }
+    wave[i] = w;
+    last[a[i]] = w;
+    last[b[i]] = w;
+    top = std::max(top, w);
+  }
+  *nwaves = top + 1;
+  return wave;
+}
+
+// Host image of the staging buffer: arrays of items appended at 16-byte
+// aligned offsets, uploaded in one copy.
+struct Staging {
+  std::vector<char> bytes;
+  template <class T>
+  size_t add(const std::vector<T>& v) {
+    const size_t off = (bytes.size() + 15) / 16 * 16;
+    bytes.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+  }
+};
+
+static int upload_staging(pc_pool* p, const Staging& st) {
+  if (st.bytes.empty()) return PC_OK;
+  CHK(ensure_dev(&p->d_batch, &p->batch_cap, st.bytes.size()));
+  HIPCHK(hipMemcpyAsync(p->d_batch, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice,
+                        p->stream));
+  return PC_OK;
+}
+
+constexpr int kMaxBatchY = 65535;  // grid y limit: items per launch
+
+static int launch_transform_batch(pc_pool* p, size_t off, const std::vector<TransformItem>& v) {
+  int most = 0;
+  for (const auto& it : v) most = std::max(most, it.sx * it.sy);
+  if (most == 0) return PC_OK;
+  const TransformItem* d = reinterpret_cast<const TransformItem*>(p->d_batch + off);
+  for (size_t b = 0; b < v.size(); b += kMaxBatchY) {
+    const unsigned ny = (unsigned)std::min<size_t>(kMaxBatchY, v.size() - b);
+    hipLaunchKernelGGL(k_transform_batch, dim3((most + 255) / 256, ny), dim3(256), 0,
+                       p->stream, p->d_pd, fields_of(p), d + b);
+    HIPCHK(hipGetLastError());
+  }
+  return PC_OK;
+}
+
+// Slots of a one-launch patch batch: live, each at most once, at most
+// kMaxBatchY of them.
+static int check_batch_slots(pc_pool* p, int count, const int* slots) {
+  if (count > kMaxBatchY) return fail(PC_E_INVALID, "at most 65535 patches per call");
+  std::vector<char> seen(p->slots.size(), 0);
+  for (int i = 0; i < count; ++i) {
+    if (!valid_slot(p, slots[i])) return fail(PC_E_INVALID, "invalid slot");
+    if (seen[slots[i]]) return fail(PC_E_INVALID, "a slot appears twice in the batch");
+    seen[slots[i]] = 1;
+  }
+  return PC_OK;
+}
+
+extern "C" {
+
+int pc_interpolate_batch(pc_pool* p, int count, const int* dsize, const int* doff,
+                         const int* src, const int* dst, const int* opts) {
+  if (!p || count < 0 || (count > 0 && (!dsize || !doff || !src || !dst || !opts)))
+    return fail(PC_E_INVALID, "invalid argument");
+  if (count == 0) return PC_OK;
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  std::vector<InterpPlan> pl(count);
+  for (int i = 0; i < count; ++i)
+    CHK(plan_interpolation(p, dsize + 2 * i, doff + 2 * i, src[i], dst[i], opts[3 * i],
+                           opts[3 * i + 1], opts[3 * i + 2], pl[i]));
+  int nw = 0;
+  const std::vector<int> wave = conflict_waves(count, src, dst, &nw);
+  std::vector<std::vector<TransformItem>> fwd(nw), back(nw);
+  std::vector<std::vector<InterpArgs>> interp(nw);
+  std::vector<int> most(nw, 0);
+  for (int i = 0; i < count; ++i) {
+    const int w = wave[i];
+    for (const auto& it : pl[i].fwd)
+      if (it.sx * it.sy > 0) fwd[w].push_back(it);
+    if (pl[i].n > 0) {
+      interp[w].push_back(pl[i].a);
+      most[w] = std::max(most[w], pl[i].n);
+    }
+    for (const auto& it : pl[i].back)
+      if (it.sx * it.sy > 0) back[w].push_back(it);
+  }
+  Staging st;
+  std::vector<size_t> ofw(nw), oin(nw), obk(nw);
+  for (int w = 0; w < nw; ++w) {
+    ofw[w] = st.add(fwd[w]);
+    oin[w] = st.add(interp[w]);
+    obk[w] = st.add(back[w]);
+  }
+  mark_state_changed(p);
+  CHK(upload_staging(p, st));
+  for (int w = 0; w < nw; ++w) {
+    CHK(launch_transform_batch(p, ofw[w], fwd[w]));
+    const InterpArgs* d = reinterpret_cast<const InterpArgs*>(p->d_batch + oin[w]);
+    for (size_t b = 0; b < interp[w].size(); b += kMaxBatchY) {
+      const unsigned ny = (unsigned)std::min<size_t>(kMaxBatchY, interp[w].size() - b);
+      hipLaunchKernelGGL(k_interpolate_batch, dim3((most[w] + 127) / 128, ny), dim3(128), 0,
+                         p->stream, p->d_pd, fields_of(p), d + b);
       HIPCHK(hipGetLastError());
     }
+    CHK(launch_transform_batch(p, obk[w], back[w]));
   }
-  CHK(transform(p, src, z, sn, 1, 1));
-  CHK(transform(p, src, z, sn, 0, 1));
+  // the host image of the items lives until the copy has been consumed
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return PC_OK;
+}
+
+int pc_restrict_batch(pc_pool* p, int count, const int* src, const int* dst,
+                      int onlyOverlapped) {
+  if (!p || count < 0 || (count > 0 && (!src || !dst)))
+    return fail(PC_E_INVALID, "invalid argument");
+  if (count == 0) return PC_OK;
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  std::vector<RestrictPlan> pl(count);
+  for (int i = 0; i < count; ++i) CHK(plan_restriction(p, src[i], dst[i], onlyOverlapped, pl[i]));
+  int nw = 0;
+  const std::vector<int> wave = conflict_waves(count, src, dst, &nw);
+  // per wave: forward transforms, region k of every call (k = 0..3, in the
+  // order of each call's regions), back transforms
+  std::vector<std::vector<TransformItem>> fwd(nw), back(nw);
+  std::vector<std::vector<RestrictArgs>> reg(4 * nw);
+  std::vector<int> most(4 * nw, 0);
+  for (int i = 0; i < count; ++i) {
+    const int w = wave[i];
+    for (const auto& it : pl[i].fwd)
+      if (it.sx * it.sy > 0) fwd[w].push_back(it);
+    for (int r = 0; r < pl[i].nr; ++r) {
+      reg[4 * w + r].push_back(pl[i].r[r]);
+      most[4 * w + r] = std::max(most[4 * w + r], pl[i].r[r].dSize[0] * pl[i].r[r].dSize[1]);
+    }
+    for (const auto& it : pl[i].back)
+      if (it.sx * it.sy > 0) back[w].push_back(it);
+  }
+  Staging st;
+  std::vector<size_t> ofw(nw), obk(nw), org(4 * nw);
+  for (int w = 0; w < nw; ++w) {
+    ofw[w] = st.add(fwd[w]);
+    for (int r = 0; r < 4; ++r) org[4 * w + r] = st.add(reg[4 * w + r]);
+    obk[w] = st.add(back[w]);
+  }
+  mark_state_changed(p);
+  CHK(upload_staging(p, st));
+  for (int w = 0; w < nw; ++w) {
+    CHK(launch_transform_batch(p, ofw[w], fwd[w]));
+    for (int r = 0; r < 4; ++r) {
+      const auto& v = reg[4 * w + r];
+      const RestrictArgs* d = reinterpret_cast<const RestrictArgs*>(p->d_batch + org[4 * w + r]);
+      for (size_t b = 0; b < v.size(); b += kMaxBatchY) {
+        const unsigned ny = (unsigned)std::min<size_t>(kMaxBatchY, v.size() - b);
+        hipLaunchKernelGGL(k_restrict_batch, dim3((most[4 * w + r] + 63) / 64, ny), dim3(64), 0,
+                           p->stream, p->d_pd, fields_of(p), d + b);
+        HIPCHK(hipGetLastError());
+      }
+    }
+    CHK(launch_transform_batch(p, obk[w], back[w]));
+  }
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return PC_OK;
+}
+
+// Patch::switchToLeaf (Patch.cpp:793-830) for a batch of virtual patches:
+// the new interval [time[i], time[i] + dt[i]] and the restricted [0, 1]-basis
+// values evaluated at its ends.
+int pc_switch_to_leaf(pc_pool* p, int count, const int* slots, const double* time,
+                      const double* dt) {
+  if (!p || count < 0 || (count > 0 && (!slots || !time || !dt)))
+    return fail(PC_E_INVALID, "invalid argument");
+  if (count == 0) return PC_OK;
+  CHK(check_batch_slots(p, count, slots));
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  std::vector<LeafSwitch> v(count);
+  int most = 0;
+  for (int i = 0; i < count; ++i) {
+    const auto& d = p->slots[slots[i]].desc;
+    // Patch.cpp:55-76 with the [0, 1] basis (time 0, step 1)
+    v[i].slot = slots[i];
+    v[i].cur = cur_of(p, slots[i]);
+    v[i].fOld = (time[i] - 0.0) / 1.0;
+    v[i].fNew = (time[i] + dt[i] - 0.0) / 1.0;
+    most = std::max(most, d.subdivision_factor[0] * d.subdivision_factor[1]);
+  }
+  Staging st;
+  const size_t off = st.add(v);
+  mark_state_changed(p);
+  CHK(upload_staging(p, st));
+  hipLaunchKernelGGL(k_switch_to_leaf, dim3((most + 255) / 256, (unsigned)count), dim3(256), 0,
+                     p->stream, p->d_pd, fields_of(p),
+                     reinterpret_cast<const LeafSwitch*>(p->d_batch + off));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(p->stream));
+  for (int i = 0; i < count; ++i) {
+    Slot& s = p->slots[slots[i]];
+    s.desc.is_virtual = 0;
+    s.desc.time = time[i];
+    s.desc.timestep_size = dt[i];
+    s.clock_from_scenario = false;
+  }
+  return PC_OK;
+}
+
+int pc_patches_clear(pc_pool* p, int count, const int* slots) {
+  if (!p || count < 0 || (count > 0 && !slots)) return fail(PC_E_INVALID, "invalid argument");
+  if (count == 0) return PC_OK;
+  CHK(check_batch_slots(p, count, slots));
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  std::vector<int> sl(slots, slots + count), cur(count);
+  int most = 0;
+  for (int i = 0; i < count; ++i) {
+    const auto& d = p->slots[slots[i]].desc;
+    cur[i] = cur_of(p, slots[i]);
+    most = std::max(most, (d.subdivision_factor[0] + 2) * (d.subdivision_factor[1] + 2));
+  }
+  Staging st;
+  const size_t o1 = st.add(sl), o2 = st.add(cur);
+  mark_state_changed(p);
+  CHK(upload_staging(p, st));
+  hipLaunchKernelGGL(k_clear_patches, dim3((most + 255) / 256, (unsigned)count), dim3(256), 0,
+                     p->stream, p->d_pd, fields_of(p),
+                     reinterpret_cast<const int*>(p->d_batch + o1),
+                     reinterpret_cast<const int*>(p->d_batch + o2));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return PC_OK;
+}
+
+int pc_patches_set_b(pc_pool* p, int count, const int* slots, const double* b) {
+  if (!p || count < 0 || (count > 0 && (!slots || !b)))
+    return fail(PC_E_INVALID, "invalid argument");
+  if (count == 0) return PC_OK;
+  CHK(check_batch_slots(p, count, slots));
+  hipSetDevice(p->device);
+  CHK(prepare(p));
+  std::vector<int> sl(slots, slots + count);
+  std::vector<long long> soff(count);
+  long long total = 0;
+  int most = 0;
+  for (int i = 0; i < count; ++i) {
+    const auto& d = p->slots[slots[i]].desc;
+    const int area = (d.subdivision_factor[0] + 2) * (d.subdivision_factor[1] + 2);
+    soff[i] = total;
+    total += area;
+    most = std::max(most, area);
+  }
+  Staging st;
+  const size_t o1 = st.add(sl), o2 = st.add(soff);
+  const size_t o3 = st.add(std::vector<double>(b, b + total));
+  mark_state_changed(p);
+  CHK(upload_staging(p, st));
+  hipLaunchKernelGGL(k_scatter_b, dim3((most + 255) / 256, (unsigned)count), dim3(256), 0,
+                     p->stream, p->d_pd, fields_of(p),
+                     reinterpret_cast<const int*>(p->d_batch + o1),
+                     reinterpret_cast<const long long*>(p->d_batch + o2),
+                     reinterpret_cast<const double*>(p->d_batch + o3));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(p->stream));
   return PC_OK;
 }
 

@@ -164,25 +164,47 @@ __global__ void k_copy_swap(const PatchDev* __restrict__ pd, int slot, Fields F,
 // addresses element x*ny + y of that array: unknown q = idx / (nx ny) at
 // (x', y') = ((idx mod nx ny) / ny, idx mod ny).  With unew_layout set the
 // kernel writes that element, as the reference does; b is read at (x, y).
+__device__ __forceinline__ void transform_cell(const PatchDev& P, const Fields& F, int st,
+                                               int ox, int oy, int sy, double sign,
+                                               int unew_layout, int k) {
+  const int x = ox + k / sy, y = oy + (k - (k / sy) * sy);
+  const long long ib = tile_index(P, x, y);
+  double* f = F.H(st);
+  long long i = ib;
+  if (unew_layout && (x >= P.nx || y >= P.ny)) {
+    const int lin = x * P.ny + y;
+    const int area = P.nx * P.ny;
+    const int q = lin / area, r = lin - q * area;
+    f = q == 0 ? F.H(st) : (q == 1 ? F.HU(st) : F.HV(st));
+    i = tile_index(P, r / P.ny, r - (r / P.ny) * P.ny);
+  }
+  f[i] = f[i] + sign * F.b[ib];
+}
+
 __global__ void k_transform(const PatchDev* __restrict__ pd, int slot, Fields F,
                             int st, int ox, int oy, int sx, int sy, double sign,
                             int unew_layout) {
   const PatchDev P = pd[slot];
   for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < sx * sy;
-       k += blockDim.x * gridDim.x) {
-    const int x = ox + k / sy, y = oy + (k - (k / sy) * sy);
-    const long long ib = tile_index(P, x, y);
-    double* f = F.H(st);
-    long long i = ib;
-    if (unew_layout && (x >= P.nx || y >= P.ny)) {
-      const int lin = x * P.ny + y;
-      const int area = P.nx * P.ny;
-      const int q = lin / area, r = lin - q * area;
-      f = q == 0 ? F.H(st) : (q == 1 ? F.HU(st) : F.HV(st));
-      i = tile_index(P, r / P.ny, r - (r / P.ny) * P.ny);
-    }
-    f[i] = f[i] + sign * F.b[ib];
-  }
+       k += blockDim.x * gridDim.x)
+    transform_cell(P, F, st, ox, oy, sy, sign, unew_layout, k);
+}
+
+// One region transform of a batch (the arguments of one k_transform launch).
+struct TransformItem {
+  int slot, st, ox, oy, sx, sy, unew_layout, pad;
+  double sign;
+};
+
+// Batched k_transform: item blockIdx.y, its region cells over blockIdx.x.
+// The items of one launch touch disjoint memory.
+__global__ void k_transform_batch(const PatchDev* __restrict__ pd, Fields F,
+                                  const TransformItem* __restrict__ items) {
+  const TransformItem it = items[blockIdx.y];
+  const PatchDev P = pd[it.slot];
+  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < it.sx * it.sy;
+       k += blockDim.x * gridDim.x)
+    transform_cell(P, F, it.st, it.ox, it.oy, it.sy, it.sign, it.unew_layout, k);
 }
 
 struct InterpArgs {
@@ -195,59 +217,75 @@ struct InterpArgs {
 };
 
 // DefaultInterpolationTemplate::interpolateSolution
-// (DefaultInterpolation.cpph:29-270): one thread per destination cell.
+// (DefaultInterpolation.cpph:29-270): destination cell k of the region.
+__device__ __forceinline__ void interp_cell(const PatchDev& S, const PatchDev& D,
+                                            const Fields& F, const InterpArgs& a, int k) {
+  const int x = a.ox + k / a.sy, y = a.oy + (k - (k / a.sy) * a.sy);
+  const long long di = tile_index(D, x, y);
+  const int dOld = 1 - a.dstCur;
+  double* dq[3];
+  if (a.toUOld) {
+    dq[0] = F.H(dOld); dq[1] = F.HU(dOld); dq[2] = F.HV(dOld);
+  } else {
+    dq[0] = F.H(a.dstCur); dq[1] = F.HU(a.dstCur); dq[2] = F.HV(a.dstCur);
+  }
+  F.H(dOld)[di] = 0.0;
+  F.HU(dOld)[di] = 0.0;
+  F.HV(dOld)[di] = 0.0;
+  double c[2];
+  c[0] = a.dpos[0] + a.dsub[0] * (double)x + a.dsub[0] * 0.5;
+  c[1] = a.dpos[1] + a.dsub[1] * (double)y + a.dsub[1] * 0.5;
+  int si[2];
+  for (int d = 0; d < 2; ++d) {
+    const double base = a.spos[d] + a.ssub[d] * 0.5;
+    const double loc = c[d] - base;
+    int v = (int)floor(loc * a.sinv[d]);
+    v = v < a.sn[d] - 2 ? v : a.sn[d] - 2;
+    v = v > 0 ? v : 0;
+    si[d] = v;
+  }
+  const double tf = a.tf;
+  for (int o = 0; o < 4; ++o) {
+    const int off[2] = {o & 1, (o >> 1) & 1};
+    const int nb[2] = {si[0] + off[0], si[1] + off[1]};
+    double np[2];
+    np[0] = a.spos[0] + a.ssub[0] * (double)nb[0] + a.ssub[0] * 0.5;
+    np[1] = a.spos[1] + a.ssub[1] * (double)nb[1] + a.ssub[1] * 0.5;
+    double sf = 1.0;
+    for (int d = 0; d < 2; ++d) {
+      const double sign = (double)(1 - 2 * off[d]);
+      sf *= 1.0 - (sign * (c[d] - np[d])) * a.sinv[d];
+    }
+    const long long si_ = tile_index(S, nb[0], nb[1]);
+    const double* so[3] = {F.H(1 - a.srcCur), F.HU(1 - a.srcCur), F.HV(1 - a.srcCur)};
+    const double* sn[3] = {F.H(a.srcCur), F.HU(a.srcCur), F.HV(a.srcCur)};
+    for (int q = 0; q < 3; ++q) {
+      const double vo = so[q][si_], vn = sn[q][si_];
+      if (a.toUOld)
+        dq[q][di] = dq[q][di] + sf * (vo * (1.0 - tf) + vn * tf);
+      else
+        dq[q][di] = dq[q][di] + sf * vo * (1.0 - tf) + sf * vn * tf;
+    }
+  }
+}
+
+// One thread per destination cell.
 __global__ void k_interpolate(const PatchDev* __restrict__ pd, Fields F, InterpArgs a) {
   const PatchDev S = pd[a.srcSlot], D = pd[a.dstSlot];
   for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < a.sx * a.sy;
-       k += blockDim.x * gridDim.x) {
-    const int x = a.ox + k / a.sy, y = a.oy + (k - (k / a.sy) * a.sy);
-    const long long di = tile_index(D, x, y);
-    const int dOld = 1 - a.dstCur;
-    double* dq[3];
-    if (a.toUOld) {
-      dq[0] = F.H(dOld); dq[1] = F.HU(dOld); dq[2] = F.HV(dOld);
-    } else {
-      dq[0] = F.H(a.dstCur); dq[1] = F.HU(a.dstCur); dq[2] = F.HV(a.dstCur);
-    }
-    F.H(dOld)[di] = 0.0;
-    F.HU(dOld)[di] = 0.0;
-    F.HV(dOld)[di] = 0.0;
-    double c[2];
-    c[0] = a.dpos[0] + a.dsub[0] * (double)x + a.dsub[0] * 0.5;
-    c[1] = a.dpos[1] + a.dsub[1] * (double)y + a.dsub[1] * 0.5;
-    int si[2];
-    for (int d = 0; d < 2; ++d) {
-      const double base = a.spos[d] + a.ssub[d] * 0.5;
-      const double loc = c[d] - base;
-      int v = (int)floor(loc * a.sinv[d]);
-      v = v < a.sn[d] - 2 ? v : a.sn[d] - 2;
-      v = v > 0 ? v : 0;
-      si[d] = v;
-    }
-    const double tf = a.tf;
-    for (int o = 0; o < 4; ++o) {
-      const int off[2] = {o & 1, (o >> 1) & 1};
-      const int nb[2] = {si[0] + off[0], si[1] + off[1]};
-      double np[2];
-      np[0] = a.spos[0] + a.ssub[0] * (double)nb[0] + a.ssub[0] * 0.5;
-      np[1] = a.spos[1] + a.ssub[1] * (double)nb[1] + a.ssub[1] * 0.5;
-      double sf = 1.0;
-      for (int d = 0; d < 2; ++d) {
-        const double sign = (double)(1 - 2 * off[d]);
-        sf *= 1.0 - (sign * (c[d] - np[d])) * a.sinv[d];
-      }
-      const long long si_ = tile_index(S, nb[0], nb[1]);
-      const double* so[3] = {F.H(1 - a.srcCur), F.HU(1 - a.srcCur), F.HV(1 - a.srcCur)};
-      const double* sn[3] = {F.H(a.srcCur), F.HU(a.srcCur), F.HV(a.srcCur)};
-      for (int q = 0; q < 3; ++q) {
-        const double vo = so[q][si_], vn = sn[q][si_];
-        if (a.toUOld)
-          dq[q][di] = dq[q][di] + sf * (vo * (1.0 - tf) + vn * tf);
-        else
-          dq[q][di] = dq[q][di] + sf * vo * (1.0 - tf) + sf * vn * tf;
-      }
-    }
-  }
+       k += blockDim.x * gridDim.x)
+    interp_cell(S, D, F, a, k);
+}
+
+// Batched k_interpolate: call blockIdx.y, its destination cells over
+// blockIdx.x; the calls of one launch touch disjoint patches.
+__global__ void k_interpolate_batch(const PatchDev* __restrict__ pd, Fields F,
+                                    const InterpArgs* __restrict__ items) {
+  const InterpArgs a = items[blockIdx.y];
+  const PatchDev S = pd[a.srcSlot], D = pd[a.dstSlot];
+  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < a.sx * a.sy;
+       k += blockDim.x * gridDim.x)
+    interp_cell(S, D, F, a, k);
 }
 
 struct RestrictArgs {
@@ -260,59 +298,133 @@ struct RestrictArgs {
 };
 
 // DefaultRestrictionTemplate::restrictRegion (DefaultRestriction.cpph:16-106):
-// one thread per destination cell, source cells in the reference's order.
+// destination cell k of the mapped region, source cells in the reference's
+// order.
+__device__ __forceinline__ void restrict_cell(const PatchDev& S, const PatchDev& D,
+                                              const Fields& F, const RestrictArgs& a, int k) {
+  const int di0 = a.dOff[0] + k / a.dSize[1];
+  const int di1 = a.dOff[1] + (k - (k / a.dSize[1]) * a.dSize[1]);
+  double dp[2] = {a.dpos[0] + (double)di0 * a.dstSub[0],
+                  a.dpos[1] + (double)di1 * a.dstSub[1]};
+  int cOff[2], cSize[2];
+  for (int d = 0; d < 2; ++d) {
+    double ot = (dp[d] - a.spos[d] + a.eps);
+    ot /= a.srcSub[d];
+    const int o = (int)ot;
+    double st = dp[d] + a.dstSub[d] - a.spos[d] - a.eps;
+    st /= a.srcSub[d];
+    st -= ot;
+    st += 1.0;
+    const int s = (int)st;
+    const int upper = o + s, rUpper = a.rOff[d] + a.rSize[d];
+    cOff[d] = o > a.rOff[d] ? o : a.rOff[d];
+    cSize[d] = (upper < rUpper ? upper : rUpper) - cOff[d];
+  }
+  const long long di = tile_index(D, di0, di1);
+  const int dOldSt = 1 - a.dstCur, sOldSt = 1 - a.srcCur;
+  double accO[3] = {F.H(dOldSt)[di], F.HU(dOldSt)[di], F.HV(dOldSt)[di]};
+  double accN[3] = {F.H(a.dstCur)[di], F.HU(a.dstCur)[di], F.HV(a.dstCur)[di]};
+  for (int sx = 0; sx < cSize[0]; ++sx)
+    for (int sy = 0; sy < cSize[1]; ++sy) {
+      const int s0 = cOff[0] + sx, s1 = cOff[1] + sy;
+      const double sp[2] = {a.spos[0] + (double)s0 * a.srcSub[0],
+                            a.spos[1] + (double)s1 * a.srcSub[1]};
+      double region = 1.0;
+      for (int d = 0; d < 2; ++d) {
+        const double hi = dmin(dp[d] + a.dstSub[d], sp[d] + a.srcSub[d]);
+        const double lo = dmax(dp[d], sp[d]);
+        region *= hi - lo;
+        region = dmax(region, 0.0);
+      }
+      const long long si = tile_index(S, s0, s1);
+      const double vo[3] = {F.H(sOldSt)[si], F.HU(sOldSt)[si], F.HV(sOldSt)[si]};
+      const double vn[3] = {F.H(a.srcCur)[si], F.HU(a.srcCur)[si], F.HV(a.srcCur)[si]};
+      for (int q = 0; q < 3; ++q) {
+        const double dO = (vo[q] * (1.0 - a.tfOld) + vn[q] * a.tfOld) * region / a.dstRegion;
+        const double dN = (vo[q] * (1.0 - a.tfNew) + vn[q] * a.tfNew) * region / a.dstRegion;
+        accO[q] = accO[q] + dO;
+        accN[q] = accN[q] + dN;
+      }
+    }
+  F.H(dOldSt)[di] = accO[0]; F.HU(dOldSt)[di] = accO[1]; F.HV(dOldSt)[di] = accO[2];
+  F.H(a.dstCur)[di] = accN[0]; F.HU(a.dstCur)[di] = accN[1]; F.HV(a.dstCur)[di] = accN[2];
+}
+
+// One thread per destination cell.
 __global__ void k_restrict(const PatchDev* __restrict__ pd, Fields F, RestrictArgs a) {
   const PatchDev S = pd[a.srcSlot], D = pd[a.dstSlot];
   const int n = a.dSize[0] * a.dSize[1];
-  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < n;
+  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < n; k += blockDim.x * gridDim.x)
+    restrict_cell(S, D, F, a, k);
+}
+
+// Batched k_restrict: region blockIdx.y, its destination cells over
+// blockIdx.x; the regions of one launch write disjoint destination patches.
+__global__ void k_restrict_batch(const PatchDev* __restrict__ pd, Fields F,
+                                 const RestrictArgs* __restrict__ items) {
+  const RestrictArgs a = items[blockIdx.y];
+  const PatchDev S = pd[a.srcSlot], D = pd[a.dstSlot];
+  const int n = a.dSize[0] * a.dSize[1];
+  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < n; k += blockDim.x * gridDim.x)
+    restrict_cell(S, D, F, a, k);
+}
+
+// Patch::switchToLeaf (Patch.cpp:793-830) of a batch of patches (patch
+// blockIdx.y): the [0, 1]-basis values of the restriction evaluated at the
+// leaf's interval, uOld = vo (1 - fOld) + vn fOld and uNew = vo (1 - fNew)
+// + vn fNew over the interior cells.
+struct LeafSwitch {
+  int slot, cur;
+  double fOld, fNew;
+};
+__global__ void k_switch_to_leaf(const PatchDev* __restrict__ pd, Fields F,
+                                 const LeafSwitch* __restrict__ items) {
+  const LeafSwitch it = items[blockIdx.y];
+  const PatchDev P = pd[it.slot];
+  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < P.nx * P.ny;
        k += blockDim.x * gridDim.x) {
-    const int di0 = a.dOff[0] + k / a.dSize[1];
-    const int di1 = a.dOff[1] + (k - (k / a.dSize[1]) * a.dSize[1]);
-    double dp[2] = {a.dpos[0] + (double)di0 * a.dstSub[0],
-                    a.dpos[1] + (double)di1 * a.dstSub[1]};
-    int cOff[2], cSize[2];
-    for (int d = 0; d < 2; ++d) {
-      double ot = (dp[d] - a.spos[d] + a.eps);
-      ot /= a.srcSub[d];
-      const int o = (int)ot;
-      double st = dp[d] + a.dstSub[d] - a.spos[d] - a.eps;
-      st /= a.srcSub[d];
-      st -= ot;
-      st += 1.0;
-      const int s = (int)st;
-      const int upper = o + s, rUpper = a.rOff[d] + a.rSize[d];
-      cOff[d] = o > a.rOff[d] ? o : a.rOff[d];
-      cSize[d] = (upper < rUpper ? upper : rUpper) - cOff[d];
+    const long long i = tile_index(P, k / P.ny, k - (k / P.ny) * P.ny);
+    double* o[3] = {F.H(1 - it.cur), F.HU(1 - it.cur), F.HV(1 - it.cur)};
+    double* n[3] = {F.H(it.cur), F.HU(it.cur), F.HV(it.cur)};
+    for (int q = 0; q < 3; ++q) {
+      const double vo = o[q][i], vn = n[q][i];
+      o[q][i] = vo * (1.0 - it.fOld) + vn * it.fOld;
+      n[q][i] = vo * (1.0 - it.fNew) + vn * it.fNew;
     }
-    const long long di = tile_index(D, di0, di1);
-    const int dOldSt = 1 - a.dstCur, sOldSt = 1 - a.srcCur;
-    double accO[3] = {F.H(dOldSt)[di], F.HU(dOldSt)[di], F.HV(dOldSt)[di]};
-    double accN[3] = {F.H(a.dstCur)[di], F.HU(a.dstCur)[di], F.HV(a.dstCur)[di]};
-    for (int sx = 0; sx < cSize[0]; ++sx)
-      for (int sy = 0; sy < cSize[1]; ++sy) {
-        const int s0 = cOff[0] + sx, s1 = cOff[1] + sy;
-        const double sp[2] = {a.spos[0] + (double)s0 * a.srcSub[0],
-                              a.spos[1] + (double)s1 * a.srcSub[1]};
-        double region = 1.0;
-        for (int d = 0; d < 2; ++d) {
-          const double hi = dmin(dp[d] + a.dstSub[d], sp[d] + a.srcSub[d]);
-          const double lo = dmax(dp[d], sp[d]);
-          region *= hi - lo;
-          region = dmax(region, 0.0);
-        }
-        const long long si = tile_index(S, s0, s1);
-        const double vo[3] = {F.H(sOldSt)[si], F.HU(sOldSt)[si], F.HV(sOldSt)[si]};
-        const double vn[3] = {F.H(a.srcCur)[si], F.HU(a.srcCur)[si], F.HV(a.srcCur)[si]};
-        for (int q = 0; q < 3; ++q) {
-          const double dO = (vo[q] * (1.0 - a.tfOld) + vn[q] * a.tfOld) * region / a.dstRegion;
-          const double dN = (vo[q] * (1.0 - a.tfNew) + vn[q] * a.tfNew) * region / a.dstRegion;
-          accO[q] = accO[q] + dO;
-          accN[q] = accN[q] + dN;
-        }
-      }
-    F.H(dOldSt)[di] = accO[0]; F.HU(dOldSt)[di] = accO[1]; F.HV(dOldSt)[di] = accO[2];
-    F.H(a.dstCur)[di] = accN[0]; F.HU(a.dstCur)[di] = accN[1]; F.HV(a.dstCur)[di] = accN[2];
   }
+}
+
+// Zero uNew's interior and all of uOld (ghost ring included) of a batch of
+// patches (patch blockIdx.y, state `cur[blockIdx.y]` current); b is kept.
+__global__ void k_clear_patches(const PatchDev* __restrict__ pd, Fields F,
+                                const int* __restrict__ slots, const int* __restrict__ curs) {
+  const int slot = slots[blockIdx.y], cur = curs[blockIdx.y];
+  const PatchDev P = pd[slot];
+  const int w = P.ny + 2, area = (P.nx + 2) * w;
+  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < area; k += blockDim.x * gridDim.x) {
+    const long long i = P.off + k;
+    F.H(1 - cur)[i] = 0.0;
+    F.HU(1 - cur)[i] = 0.0;
+    F.HV(1 - cur)[i] = 0.0;
+    const int x = k / w, y = k - x * w;
+    if (x >= 1 && x <= P.nx && y >= 1 && y <= P.ny) {
+      F.H(cur)[i] = 0.0;
+      F.HU(cur)[i] = 0.0;
+      F.HV(cur)[i] = 0.0;
+    }
+  }
+}
+
+// b of a batch of tiles from a staging array: tile blockIdx.y takes
+// (nx+2)(ny+2) values from src + soff[blockIdx.y].
+__global__ void k_scatter_b(const PatchDev* __restrict__ pd, Fields F,
+                            const int* __restrict__ slots, const long long* __restrict__ soff,
+                            const double* __restrict__ src) {
+  const PatchDev P = pd[slots[blockIdx.y]];
+  const double* s = src + soff[blockIdx.y];
+  const int area = (P.nx + 2) * (P.ny + 2);
+  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < area; k += blockDim.x * gridDim.x)
+    F.b[P.off + k] = s[k];
 }
 
 // BreakingDamSWEScenario::computeDemandedMeshWidth (BreakingDam.cpp:85-139)

@@ -81,6 +81,27 @@ class OracleBackend:
                                     to_uold, to_current, use_time):
             raise RuntimeError("interpolation region outside its patch")
 
+    # batched forms: the per-call methods one after another
+    def interpolate_batch(self, calls):
+        for c in calls:
+            self.interpolate(*c)
+
+    def restrict_batch(self, pairs):
+        for src, dst in pairs:
+            self.restrict(src, dst)
+
+    def clear_batch(self, handles):
+        for s in handles:
+            self.clear(s)
+
+    def set_b_batch(self, handles, bs):
+        for s, b in zip(handles, bs):
+            self.set_b(s, b)
+
+    def switch_to_leaf_batch(self, handles, time, dt):
+        for s in handles:
+            self.switch_to_leaf(s, time, dt)
+
     def fill_boundary(self, s, dim, upper):
         lib().or_fill_boundary_layer(self.p[s].ref, dim, upper)
 
