@@ -117,7 +117,13 @@ struct pc_pool {
   int* d_redo = nullptr;     // = d_counter + 1
   int* d_iota = nullptr;   // d_iota[i] = i for every slot (per-patch kernels' lists)
   double* d_segmax = nullptr;  // row-segment speed maxima of the one-patch kernels
+  double* d_keep = nullptr;    // k_patch_solve: uNew of the patch while it is updated
   double* d_result = nullptr;
+  // results of the synchronous one-patch calls, written by their kernels
+  // straight into pinned host memory (h_result; d_result_host is its device
+  // address): no copy back after the kernel
+  double* h_result = nullptr;
+  double* d_result_host = nullptr;
   double* d_aux = nullptr;
   size_t aux_cap = 0;
   int* d_flags = nullptr;
@@ -808,7 +814,11 @@ int pc_pool_create(int device, size_t hbm_budget_bytes, pc_pool** out) {
   HIPCHK(hipMemsetAsync(p->d_counter, 0, 2 * sizeof(int), p->stream));
   p->d_redo = p->d_counter + 1;
   HIPCHK(hipMalloc(&p->d_result, 8 * sizeof(double)));
-  HIPCHK(hipMalloc(&p->d_segmax, kMaxRowSeg * sizeof(double)));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&p->h_result), 8 * sizeof(double),
+                       hipHostMallocMapped));
+  HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&p->d_result_host), p->h_result, 0));
+  HIPCHK(hipMalloc(&p->d_segmax, kPatchSeg * sizeof(double)));
+  HIPCHK(hipMalloc(&p->d_keep, 3 * 64 * 64 * sizeof(double)));
   HIPCHK(hipEventCreate(&p->ev0));
   HIPCHK(hipEventCreate(&p->ev1));
   HIPCHK(hipEventCreate(&p->ev2));
@@ -826,11 +836,12 @@ int pc_pool_destroy(pc_pool* p) {
     if (p->f[k]) hipFree(p->f[k]);
   void* ptrs[] = {p->d_pd, p->d_smax, p->d_smax_int, p->d_est, p->d_scn, p->d_tasks, p->d_utasks,
                   p->d_plist, p->d_plist_scn, p->d_scn_of, p->d_live, p->d_bdesc, p->d_strips,
-                  p->d_counter, p->d_iota, p->d_segmax, p->d_result, p->d_aux, p->d_flags, p->d_ebuf[0],
+                  p->d_counter, p->d_iota, p->d_segmax, p->d_keep, p->d_result, p->d_aux, p->d_flags, p->d_ebuf[0],
                   p->d_ebuf[1], p->d_bcol, p->d_sides, p->d_single, p->d_single_slot,
                   p->d_edge_io, p->d_flux, p->d_batch};
   for (void* q : ptrs)
     if (q) hipFree(q);
+  if (p->h_result) hipHostFree(p->h_result);
   for (hipEvent_t e : p->upd_ev) hipEventDestroy(e);
   hipEventDestroy(p->ev0);
   hipEventDestroy(p->ev1);
@@ -1097,16 +1108,14 @@ int pc_solve_timestep(pc_pool* p, int slot, double maxDt, const int flags[4],
   if (p->solver)
     hipLaunchKernelGGL(k_patch_solve<1>, dim3(1), dim3(64 * nseg), 0, p->stream, slot,
                        p->d_iota, p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_segmax,
-                       p->d_result);
+                       p->d_result_host, p->d_keep);
   else
     hipLaunchKernelGGL(k_patch_solve<0>, dim3(1), dim3(64 * nseg), 0, p->stream, slot,
                        p->d_iota, p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_segmax,
-                       p->d_result);
+                       p->d_result_host, p->d_keep);
   HIPCHK(hipGetLastError());
-  double res[2];
-  HIPCHK(hipMemcpyAsync(res, p->d_result, 2 * sizeof(double), hipMemcpyDeviceToHost,
-                        p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
+  const double res[2] = {p->h_result[0], p->h_result[1]};
   // SWEKernel.cpp:83-87
   if (la_greater(res[0], 0.0)) {
     s.desc.time = s.desc.time + s.desc.timestep_size;
@@ -1126,14 +1135,14 @@ int pc_patch_cfl(pc_pool* p, int slot, double* out) {
   if (p->solver)
     hipLaunchKernelGGL(k_patch_cfl<1>, dim3(1), dim3(64 * nseg), 0, p->stream, slot,
                        p->d_iota, p->d_pd, fields_of(p), cur_of(p, slot), p->d_segmax,
-                       p->d_result);
+                       p->d_result_host);
   else
     hipLaunchKernelGGL(k_patch_cfl<0>, dim3(1), dim3(64 * nseg), 0, p->stream, slot,
                        p->d_iota, p->d_pd, fields_of(p), cur_of(p, slot), p->d_segmax,
-                       p->d_result);
+                       p->d_result_host);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, p->d_result, sizeof(double), hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
+  *out = p->h_result[0];
   return PC_OK;
 }
 

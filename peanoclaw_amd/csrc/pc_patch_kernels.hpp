@@ -20,34 +20,51 @@ __device__ __forceinline__ long long tile_index(const PatchDev& P, int x, int y)
   return P.off + (long long)(x + 1) * (P.ny + 2) + (y + 1);
 }
 
+// Row segments of the one-patch kernels (one wavefront each): at least 2
+// rows each, at most kPatchSeg.  A single call is latency-bound, so the
+// rows are spread over as many wavefronts as the patch allows.
+constexpr int kPatchSeg = 8;
+__host__ __device__ __forceinline__ int patch_row_segments(int nx) {
+  const int s = nx / 2;
+  return s < 1 ? 1 : (s > kPatchSeg ? kPatchSeg : s);
+}
+
 // SWEKernel::solveTimestep on one patch (SWEKernel.cpp:50-88): speed of
-// every edge of uOld, dt = fmin(CFL, maxDt), update, swap.  The speeds run
-// in blockDim/64 row segments, one wavefront each, their maxima meeting in
-// `segmax` (kMaxRowSeg doubles); the first wavefront then updates the whole
-// patch.  `plist` holds plist[slot] == slot in global memory (the march
-// reads its patch list with global loads).
+// every edge of uOld, dt = fmin(CFL, maxDt), update, swap.  Both the speeds
+// and the update run in blockDim/64 row segments, one wavefront each: the
+// speeds' maxima meet in `segmax` (kPatchSeg doubles), every wavefront then
+// takes dt from them.  The update reads uOld and writes uNew only, so the
+// segments run side by side; the old uNew, which becomes uOld, waits in
+// `keep` (3 n^2 doubles) until every segment has read uOld.  `plist` holds
+// plist[slot] == slot in global memory (the march reads its patch list with
+// global loads).
 template <int SOLVER>
-__global__ void __launch_bounds__(64 * kMaxRowSeg)
+__global__ void __launch_bounds__(64 * kPatchSeg)
 k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restrict__ pd,
               Fields F, int cur, double maxDt, double* __restrict__ segmax,
-              double* __restrict__ out) {
-  __shared__ MarchLds L[kMaxRowSeg];
+              double* __restrict__ out, double* __restrict__ keep) {
+  __shared__ MarchLds L[kPatchSeg];
   const int w = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
   const PatchDev P = pd[slot];
   Task t;
   t.first = slot; t.nseg = 1; t.nx = P.nx; t.ny = P.ny;
-  Task ts = t;
   if (nw > 1) {
-    ts.r0 = seg_first_row(w, P.nx, nw);
-    ts.r1 = seg_first_row(w + 1, P.nx, nw) - 1;
-    ts.sidx = w;
+    t.r0 = seg_first_row(w, P.nx, nw);
+    t.r1 = seg_first_row(w + 1, P.nx, nw) - 1;
+    t.sidx = w;
   }
   EdgeBufs none{{nullptr, nullptr}, nullptr};
   MarchArgs A{plist, pd, F, nullptr, none, 1 - cur, 0.0, segmax, nullptr, 1.0};
   A.smax_stride = 0;  // segment w's maximum to segmax[w]
-  march_task<kSpeed, false, 0, SOLVER>(ts, A, L[w]);
+  march_task<kSpeed, false, 0, SOLVER>(t, A, L[w]);
+  const int n = P.nx * P.ny;
+  for (int k = (int)threadIdx.x; k < n; k += (int)blockDim.x) {
+    const long long i = tile_index(P, k / P.ny, k - (k / P.ny) * P.ny);
+    keep[k] = F.H(cur)[i];
+    keep[n + k] = F.HU(cur)[i];
+    keep[2 * n + k] = F.HV(cur)[i];
+  }
   __syncthreads();
-  if (w != 0) return;
   double s = segmax[0];
   for (int k = 1; k < nw; ++k) s = dmax(s, segmax[k]);
   const double cfl = max_timestep(s, P.dx, P.dy);
@@ -55,7 +72,14 @@ k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restric
   A.par = cur;
   A.dt = dt;
   A.smax = nullptr;
-  march_task<kUpdateSwap, false, 0, SOLVER>(t, A, L[0]);
+  march_task<kUpdateFromOld, false, 0, SOLVER>(t, A, L[w]);
+  __syncthreads();
+  for (int k = (int)threadIdx.x; k < n; k += (int)blockDim.x) {
+    const long long i = tile_index(P, k / P.ny, k - (k / P.ny) * P.ny);
+    F.H(1 - cur)[i] = keep[k];
+    F.HU(1 - cur)[i] = keep[n + k];
+    F.HV(1 - cur)[i] = keep[2 * n + k];
+  }
   if (threadIdx.x == 0) {
     out[0] = dt;
     out[1] = cfl;
@@ -66,10 +90,10 @@ k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restric
 // without the update: out[0] = max_timestep(max edge speed); row segments
 // as in k_patch_solve.
 template <int SOLVER>
-__global__ void __launch_bounds__(64 * kMaxRowSeg)
+__global__ void __launch_bounds__(64 * kPatchSeg)
 k_patch_cfl(int slot, const int* __restrict__ plist, const PatchDev* __restrict__ pd,
             Fields F, int cur, double* __restrict__ segmax, double* __restrict__ out) {
-  __shared__ MarchLds L[kMaxRowSeg];
+  __shared__ MarchLds L[kPatchSeg];
   const int w = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
   const PatchDev P = pd[slot];
   Task t;
@@ -89,12 +113,6 @@ k_patch_cfl(int slot, const int* __restrict__ plist, const PatchDev* __restrict_
     for (int k = 1; k < nw; ++k) s = dmax(s, segmax[k]);
     out[0] = max_timestep(s, P.dx, P.dy);
   }
-}
-
-// Row segments of the one-patch kernels: at least 8 rows each, at most
-// kMaxRowSeg.
-__host__ __device__ __forceinline__ int patch_row_segments(int nx) {
-  return nx >= 8 * kMaxRowSeg ? kMaxRowSeg : (nx >= 16 ? 2 : 1);
 }
 
 // SWEKernel::fillBoundaryLayer (SWEKernel.cpp:176-237) on uOld (state st).
