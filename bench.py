@@ -353,7 +353,7 @@ def main():
         "per_patch_launch_value": per_patch_value,
         "batched_vs_per_patch": value / per_patch_value if per_patch_value else None,
         "autotune": None if tuned is None else {k: v for k, v in tuned.items()
-                                                if k != "candidates"},
+                                                if k not in ("candidates", "stage2")},
         "mass_rel_drift": abs(mass1 - mass0) / mass0,
         "p50_solve_latency_s": serving["p50_s"] if serving else None,
         "p99_solve_latency_s": serving["p99_s"] if serving else None,
