@@ -742,8 +742,18 @@ int launch_step(pc_pool* p, bool timed_update, hipEvent_t eb = nullptr,
     // counter (contended atomics) instead of its own block index
     const int pers = (p->persistent_wgs > 0 && p->nutasks > p->persistent_wgs) ? 1 : 0;
     const int grid = pers ? p->persistent_wgs : p->nutasks;
-    if (prune)
+    // field arrays below 2^29 doubles: 32-bit tile offsets (TileMem)
+    const bool o32 = p->cap_cells <= (1LL << 29);
+    if (prune && o32)
+      hipLaunchKernelGGL((k_update<2, true, 0, true>), dim3(grid), dim3(64), 0, p->stream,
+                         p->d_utasks, p->nutasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
+                         p->d_smax_int, p->d_counter, pers, p->prune_slack);
+    else if (prune)
       hipLaunchKernelGGL((k_update<2, true, 0>), dim3(grid), dim3(64), 0, p->stream,
+                         p->d_utasks, p->nutasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
+                         p->d_smax_int, p->d_counter, pers, p->prune_slack);
+    else if (fused && o32)
+      hipLaunchKernelGGL((k_update<1, true, 0, true>), dim3(grid), dim3(64), 0, p->stream,
                          p->d_utasks, p->nutasks, p->d_plist, p->d_pd, F, p->d_scn, sides, E,
                          p->d_smax_int, p->d_counter, pers, p->prune_slack);
     else if (fused)

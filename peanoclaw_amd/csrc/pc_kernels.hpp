@@ -245,11 +245,36 @@ struct MarchArgs {
   int smax_stride = 1;
 };
 
+// Tile accesses of the march.  With O32 the field arrays hold fewer than
+// 2^29 doubles, so an element index fits 32 bits with its byte offset: every
+// access is then a uniform array base plus one 32-bit byte offset, which the
+// loads and stores take as their saddr / voffset operands, instead of a
+// 64-bit address computed per access.
+template <bool O32>
+struct TileMem {
+  using I = long long;
+  __device__ static __forceinline__ double ld(const double* b, I i) { return gl(b + i); }
+  __device__ static __forceinline__ void st(double* b, I i, double v) { gs(b + i, v); }
+};
+template <>
+struct TileMem<true> {
+  using I = unsigned;
+  __device__ static __forceinline__ double ld(const double* b, I i) {
+    return gl(reinterpret_cast<const double*>(reinterpret_cast<const char*>(b) + (i << 3)));
+  }
+  __device__ static __forceinline__ void st(double* b, I i, double v) {
+    gs(reinterpret_cast<double*>(reinterpret_cast<char*>(b) + (i << 3)), v);
+  }
+};
+
 // FUSE: 0 = no next-state speeds, 1 = exact speeds of every interior edge of
 // the new state, 2 = pruned (exact speeds only for bound candidates).
 // PERSEG: every patch of the task has its own step and advance flag.
-template <int MODE, bool SIDES, int FUSE, int SOLVER, bool PERSEG = false>
+// O32: 32-bit tile offsets (TileMem).
+template <int MODE, bool SIDES, int FUSE, int SOLVER, bool PERSEG = false, bool O32 = false>
 __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, MarchLds& L) {
+  using M = TileMem<O32>;
+  using I = typename M::I;
   constexpr bool FUSED = FUSE != 0;
   constexpr bool PRUNE = FUSE == 2;
   // the f-wave's speeds do not depend on b; the augmented solver's do (the
@@ -355,7 +380,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   }
   wave_sync();
 
-  const long long base = P.off + col;
+  const I base = (I)(P.off + col);
   const double dxinv = P.dxinv, dyinv = P.dyinv;
   const double dt = PERSEG ? gl(A.seg_dt + p) : A.dt;
   double smaxl = 0.0;
@@ -367,28 +392,28 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   }
   // ghost-row sources (x sides), kept in scalars
   int kx0 = kSideTile, kx1 = kSideTile;
-  long long gx0 = 0, gx1 = 0;  // index of the first interior cell of the ghost row source
+  I gx0 = 0, gx1 = 0;  // index of the first interior cell of the ghost row source
   bool neg0 = false, neg1 = false;
   if (SIDES) {
     kx0 = gl(&A.sides[p].kind[0]);
     kx1 = gl(&A.sides[p].kind[1]);
     if (kx0 == kSideNeighbour) {
       const PatchDev Q = gl(A.pd + gl(&A.sides[p].src[0]));
-      gx0 = Q.off + (long long)Q.nx * stride + col;
+      gx0 = (I)(Q.off + (long long)Q.nx * stride + col);
     } else if (kx0 == kSideWall) {
-      gx0 = base + stride;
+      gx0 = base + (I)stride;
       neg0 = true;
     } else {
       gx0 = base;
     }
     if (kx1 == kSideNeighbour) {
       const PatchDev Q = gl(A.pd + gl(&A.sides[p].src[1]));
-      gx1 = Q.off + stride + col;
+      gx1 = (I)(Q.off + stride + col);
     } else if (kx1 == kSideWall) {
-      gx1 = base + (long long)nx * stride;
+      gx1 = base + (I)nx * (I)stride;
       neg1 = true;
     } else {
-      gx1 = base + (long long)(nx + 1) * stride;
+      gx1 = base + (I)(nx + 1) * (I)stride;
     }
   }
 
@@ -396,18 +421,18 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   auto load_row = [&](int r, double& h, double& hu, double& hv, double& b) {
     h = 1.0; hu = 0.0; hv = 0.0; b = 0.0;
     if (!active) return;
-    const long long i = base + (long long)r * stride;
-    if (MODE != kSpeed || SPEED_B) b = gl(Bs + i);
+    const I i = base + (I)r * (I)stride;
+    if (MODE != kSpeed || SPEED_B) b = M::ld(Bs, i);
     if (SIDES && (r == 0 || r == nx + 1)) {
-      const long long j = r == 0 ? gx0 : gx1;
-      h = gl(Hs + j);
-      hu = gl(HUs + j);
+      const I j = r == 0 ? gx0 : gx1;
+      h = M::ld(Hs, j);
+      hu = M::ld(HUs, j);
       if (r == 0 ? neg0 : neg1) hu = -hu;
       return;
     }
-    h = gl(Hs + i);
-    hu = gl(HUs + i);
-    hv = gl(HVs + i);
+    h = M::ld(Hs, i);
+    hu = M::ld(HUs, i);
+    hv = M::ld(HVs, i);
   };
 
   // previous row (x-sweep terms + raw values)
@@ -453,25 +478,25 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
           if (h1 < kClampH) { hu1 = 0.0; hv1 = 0.0; }
           if (h1 < 0.0) h1 = 0.0;
           if (active) {
-            const long long o = base + (long long)(r - 1) * stride;
+            const I o = base + (I)(r - 1) * (I)stride;
             if (MODE == kUpdateSwap) {
               // uOld interior <- old uNew, uNew <- new values
-              Hsrc[o] = Hdst[o];
-              HUsrc[o] = HUdst[o];
-              HVsrc[o] = HVdst[o];
+              M::st(Hsrc, o, M::ld(Hdst, o));
+              M::st(HUsrc, o, M::ld(HUdst, o));
+              M::st(HVsrc, o, M::ld(HVdst, o));
             }
-            gs(Hdst + o, h1);
-            gs(HUdst + o, hu1);
-            gs(HVdst + o, hv1);
+            M::st(Hdst, o, h1);
+            M::st(HUdst, o, hu1);
+            M::st(HVdst, o, hv1);
             if (MODE == kUpdate && (col == 1 || col == ny)) {
               // Also store into the row's ghost column (0 or ny+1) of the new
               // state, which no one reads before it is refilled: the row's
               // cache lines are then written completely (no masked partial
               // writebacks at the row boundaries).
-              const long long og = o + (col == 1 ? -1 : 1);
-              gs(Hdst + og, h1);
-              gs(HUdst + og, hu1);
-              gs(HVdst + og, hv1);
+              const I og = col == 1 ? o - 1 : o + 1;
+              M::st(Hdst, og, h1);
+              M::st(HUdst, og, hu1);
+              M::st(HVdst, og, hv1);
             }
             if (SIDES && (col == 1 || col == ny)) {
               double* Ed = Edst + (long long)((p * 2 + (col == 1 ? 0 : 1)) * 2) * kERows + r - 2;
@@ -545,9 +570,10 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
         }
         if (PRUNE && have_new) {
           const float2 lb = make_float2(from_left(pbt.ay), from_left(pbt.c));
-          const double lnh = from_left(pnh), lnhv = from_left(nhv);
           const bool cand = col >= 2 && active && bound_candidate(lb.x, lb.y, pbt.ay, pbt.c, thr);
           if (__any(cand ? 1 : 0)) {
+            // (a wave-uniform branch: every lane takes part in the shifts)
+            const double lnh = from_left(pnh), lnhv = from_left(nhv);
             const CellQ nr = cell_dir(cell_base(pnh, 0.0), nhv);
             CellQ nl = cell_dir(cell_base(lnh, 0.0), lnhv);
             const double sp = ES::speed(nl, nr);
@@ -580,9 +606,9 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   if (PRUNE && MODE == kUpdate) {
     // the y-edges of the last new row (row nx) are still open
     const float2 lb = make_float2(from_left(pbt.ay), from_left(pbt.c));
-    const double lnh = from_left(pnh), lnhv = from_left(nhv);
     const bool cand = col >= 2 && active && bound_candidate(lb.x, lb.y, pbt.ay, pbt.c, thr);
     if (__any(cand ? 1 : 0)) {
+      const double lnh = from_left(pnh), lnhv = from_left(nhv);
       const CellQ nr = cell_dir(cell_base(pnh, 0.0), nhv);
       CellQ nl = cell_dir(cell_base(lnh, 0.0), lnhv);
       const double sp = ES::speed(nl, nr);
@@ -645,7 +671,7 @@ k_speed(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plis
 
 // One task per workgroup, or a persistent grid pulling tasks from a
 // device-side counter (reset by k_dt_final before every update launch).
-template <int FUSE, bool SIDES, int SOLVER>
+template <int FUSE, bool SIDES, int SOLVER, bool O32 = false>
 __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(PC_UPDATE_WAVES_PER_EU, 8)))
 k_update(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ plist,
@@ -670,7 +696,7 @@ k_update(const Task* __restrict__ tasks, int ntasks, const int* __restrict__ pli
       MarchArgs A{plist, pd, F, sides, E, (gl(&S->stepc) - 1) & 1, gl(&S->dt_applied),
                   smax_next, scn, slack};
       A.smax_stride = kMaxRowSeg;
-      march_task<kUpdate, SIDES, SIDES ? FUSE : 0, SOLVER>(t, A, L);
+      march_task<kUpdate, SIDES, SIDES ? FUSE : 0, SOLVER, false, O32>(t, A, L);
     }
     if (!persistent) return;
   }

@@ -174,21 +174,35 @@ __device__ __forceinline__ void edge_solve(CellQ L, CellQ R, double& dhL,
 // Wave-uniform fast paths.  When every lane of the wavefront has two wet
 // cells (h >= dryTol) the dry substitution is the identity, and when every
 // lane's waves are subsonic (s1 < -zeroTol, s2 > zeroTol) the distribution
-// is "wave 1 left, wave 2 right".  The operations that remain are exactly
-// those the general path performs in that case, so results are unchanged
-// bit for bit; only the selects of the unused branches are skipped.
+// is "wave 1 left, wave 2 right".  The operations that remain are those the
+// general path performs in that case (with the exceptions noted where they
+// occur, which cannot change a cell's result), so results are unchanged bit
+// for bit; only the selects of the unused branches are skipped.
 __device__ __forceinline__ bool wave_all(bool c) { return __all(c ? 1 : 0) != 0; }
+
+// wave_speeds for two wet cells with the hardware min / max: they differ
+// from the selects of wave_speeds only for a NaN or for +0 against -0, and
+// neither occurs here (c >= sqrt(g dryTol) > 0, so u - c and uRoe - cRoe
+// are zero only as x - x = +0, and u + c likewise).
+__device__ __forceinline__ void wave_speeds_wet(const CellQ& L, const CellQ& R,
+                                                double& s1, double& s2) {
+  const double hRoe = 0.5 * (L.h + R.h);
+  const double uRoe = div_nr(L.u * L.sq + R.u * R.sq, L.sq + R.sq);
+  const double cRoe = sqrt_nr(kGravity * hRoe);
+  s1 = __builtin_fmin(L.u - L.c, uRoe - cRoe);
+  s2 = __builtin_fmax(R.u + R.c, uRoe + cRoe);
+}
 
 __device__ __forceinline__ double edge_speed_wet(const CellQ& L, const CellQ& R) {
   double s1, s2;
-  wave_speeds(L, R, s1, s2);
-  return dmax(fabs(s1), fabs(s2));
+  wave_speeds_wet(L, R, s1, s2);
+  return __builtin_fmax(fabs(s1), fabs(s2));
 }
 
 __device__ __forceinline__ void edge_solve_wet(const CellQ& L, const CellQ& R, double& dhL,
                                                double& dhR, double& dqL, double& dqR) {
   double s1, s2;
-  wave_speeds(L, R, s1, s2);
+  wave_speeds_wet(L, R, s1, s2);
   const double d0 = R.q - L.q;
   const double d1 = (R.f - L.f) + (kHalfG * (L.h + R.h)) * (R.b - L.b);
   const double inv = div_nr(1.0, s2 - s1);
@@ -196,8 +210,13 @@ __device__ __forceinline__ void edge_solve_wet(const CellQ& L, const CellQ& R, d
   const double beta2 = (d1 - s1 * d0) * inv;
   const double z1 = beta1 * s1, z2 = beta2 * s2;
   if (wave_all(s1 < -kZeroTol && s2 > kZeroTol)) {
-    dhL = 0.0 + beta1; dqL = 0.0 + z1;
-    dhR = 0.0 + beta2; dqR = 0.0 + z2;
+    // The general path's sums start from +0, so they return +0 where these
+    // are -0.  The march (the only caller) uses the four values only in
+    // sums that start from +0 or whose other term is never -0 (the cell's
+    // accumulators begin as 0.0 + x), so the cell results are the same bits
+    // either way, and the additions are left out.
+    dhL = beta1; dqL = z1;
+    dhR = beta2; dqR = z2;
     return;
   }
   double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
