@@ -217,6 +217,20 @@ __device__ __forceinline__ double from_right(double v) {
 __device__ __forceinline__ float from_left(float v) {
   return __int_as_float(dpp_shr1(__float_as_int(v)));
 }
+// from_left for values whose lane-0 copy is never used: lane 0 reads 0
+// (DPP bound_ctrl), so v need not be copied into the destination first.
+__device__ __forceinline__ int dpp_shr1_z(int v) {
+  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true);  // wave_shr:1, bound_ctrl
+}
+__device__ __forceinline__ double from_left_z(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_shr1_z((int)(unsigned)(b & 0xffffffffLL));
+  const int hi = dpp_shr1_z((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ float from_left_z(float v) {
+  return __int_as_float(dpp_shr1_z(__float_as_int(v)));
+}
 
 // LDS scratch of one wavefront (64 lanes): the pre-pass's boundary y-edges,
 // [side*2 + k][seg*nx + row-1].
@@ -282,14 +296,19 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   constexpr bool SPEED_B = SOLVER != 0;
   static_assert(SOLVER == 0 || FUSE == 0, "next-step speeds are fused for the f-wave only");
   using ES = EdgeSolver<SOLVER>;
+  constexpr bool DUMMY_FREE = MODE != kSpeed && ES::kDummyLanesFree;
   const int lane = threadIdx.x & 63;
   const int nx = t.nx, ny = t.ny;
   const int W = pow2ceil(ny);
   const int seg = lane / W;
   const int col = lane - seg * W + 1;  // 1-based interior column
   const bool segok = seg < t.nseg;
+  // the update's y-edges on col-1 lanes are dummies (see the y-sweep)
+  const unsigned long long col1_lanes = __builtin_amdgcn_ballot_w64(col == 1);
   const int p = gl(A.plist + t.first + (segok ? seg : 0));
   const bool active = segok && (col <= ny) && (!PERSEG || gl(A.seg_adv + p) != 0);
+  // lane masks for the pruned mode's candidate votes
+  const unsigned long long act_lanes = __builtin_amdgcn_ballot_w64(active);
   const PatchDev P = gl(A.pd + p);
   const int stride = ny + 2;
   const int par = A.par;
@@ -451,11 +470,15 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
 
   double h, hu, hv, b;
   load_row(ra - 1, h, hu, hv, b);
+  // wet lanes of the previous row (DUMMY_FREE: the update takes the wetness
+  // votes of its edges from the rows' masks)
+  unsigned long long wet_prev = 0;
   for (int r = ra - 1; r <= rb + 1; ++r) {
     const double ch = h, chu = hu, chv = hv, cbv = b;
     if (r < rb + 1) load_row(r + 1, h, hu, hv, b);  // prefetch the next row
     const CellBase cb = cell_base(ch, cbv);
     const CellQ cx = cell_dir(cb, chu);
+    const unsigned long long wet_row = __builtin_amdgcn_ballot_w64(ch >= kDryTol);
 
     if (r >= ra) {
       // x-edge (r-1 | r)
@@ -463,7 +486,11 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
         smaxl = dmax(smaxl, ES::speed(px, cx));
       } else {
         double dhL, dhR, dqL, dqR;
-        ES::solve(px, cx, dhL, dhR, dqL, dqR);
+        if (DUMMY_FREE)
+          ES::solve_voted((wet_prev & wet_row) == __builtin_amdgcn_read_exec(), px, cx, 0,
+                          dhL, dhR, dqL, dqR);
+        else
+          ES::solve(px, cx, dhL, dhR, dqL, dqR);
         if (r >= ra + 1) {
           // finalise row r-1: x-left (already in acc), x-right, y-up, y-down
           double nH = accH + dxinv * dhL;
@@ -507,8 +534,8 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
           if (PRUNE) {
             const BoundTerms bt = bound_terms(h1, hu1, hv1);
             if (r >= ra + 2) {
-              const bool cand = active && bound_candidate(pbt.ax, pbt.c, bt.ax, bt.c, thr);
-              if (__any(cand ? 1 : 0)) {
+              const bool cand = bound_candidate(pbt.ax, pbt.c, bt.ax, bt.c, thr);
+              if ((__builtin_amdgcn_ballot_w64(cand) & act_lanes) != 0) {
                 const CellQ ea = cell_dir(cell_base(pnh, 0.0), pnhu);
                 const CellQ eb = cell_dir(cell_base(h1, 0.0), hu1);
                 const double sp = ES::speed(ea, eb);
@@ -541,11 +568,19 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
       // y-edges of row r: lane col solves (col-1 | col) for col >= 2
       const CellQ cy = cell_dir(cb, chv);
       // left neighbour lane; on a lane without one (col 1) the y-edge is a
-      // dummy whose results are not used
+      // dummy whose results are not used.  Where the solver takes any
+      // operands on such lanes (DUMMY_FREE), they are left out of its choice
+      // of path and lane 0 reads zeros.
       CellQ cl;
-      cl.h = from_left(cy.h); cl.q = from_left(cy.q); cl.b = from_left(cy.b);
-      cl.u = from_left(cy.u); cl.sq = from_left(cy.sq); cl.c = from_left(cy.c);
-      cl.f = from_left(cy.f);
+      if (DUMMY_FREE) {
+        cl.h = from_left_z(cy.h); cl.q = from_left_z(cy.q); cl.b = from_left_z(cy.b);
+        cl.u = from_left_z(cy.u); cl.sq = from_left_z(cy.sq); cl.c = from_left_z(cy.c);
+        cl.f = from_left_z(cy.f);
+      } else {
+        cl.h = from_left(cy.h); cl.q = from_left(cy.q); cl.b = from_left(cy.b);
+        cl.u = from_left(cy.u); cl.sq = from_left(cy.sq); cl.c = from_left(cy.c);
+        cl.f = from_left(cy.f);
+      }
       CellQ ncy = {};
       if (FUSED && !PRUNE && have_new) ncy = cell_dir(nbase, nhv);
       const int slot = seg * nx + r - 1;
@@ -557,7 +592,14 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
         if (active && col == ny) smaxl = dmax(smaxl, L.pre[1][slot]);
       } else {
         double dhL, dhR, dqL, dqR;
-        ES::solve(cl, cy, dhL, dhR, dqL, dqR);
+        if (DUMMY_FREE)
+          // lane k's left cell is lane k-1's: wet where the row mask shifted
+          // by one lane is set
+          ES::solve_voted((((wet_row << 1) & wet_row) | col1_lanes) == __builtin_amdgcn_read_exec(),
+                          cl, cy,
+                          col1_lanes, dhL, dhR, dqL, dqR);
+        else
+          ES::solve(cl, cy, dhL, dhR, dqL, dqR);
         double up_h, up_hv;
         // the lower neighbour's share, handed down one lane
         const double ydh = from_right(dyinv * dhL), ydq = from_right(dyinv * dqL);
@@ -569,9 +611,9 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
           up_hv = segok ? L.pre[1][slot] : 0.0;
         }
         if (PRUNE && have_new) {
-          const float2 lb = make_float2(from_left(pbt.ay), from_left(pbt.c));
-          const bool cand = col >= 2 && active && bound_candidate(lb.x, lb.y, pbt.ay, pbt.c, thr);
-          if (__any(cand ? 1 : 0)) {
+          const float2 lb = make_float2(from_left_z(pbt.ay), from_left_z(pbt.c));
+          const bool cand = bound_candidate(lb.x, lb.y, pbt.ay, pbt.c, thr);
+          if ((__builtin_amdgcn_ballot_w64(cand) & act_lanes & ~col1_lanes) != 0) {
             // (a wave-uniform branch: every lane takes part in the shifts)
             const double lnh = from_left(pnh), lnhv = from_left(nhv);
             const CellQ nr = cell_dir(cell_base(pnh, 0.0), nhv);
@@ -601,13 +643,14 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
     }
     px = cx;
     phv = chv;
+    wet_prev = wet_row;
   }
 
   if (PRUNE && MODE == kUpdate) {
     // the y-edges of the last new row (row nx) are still open
-    const float2 lb = make_float2(from_left(pbt.ay), from_left(pbt.c));
-    const bool cand = col >= 2 && active && bound_candidate(lb.x, lb.y, pbt.ay, pbt.c, thr);
-    if (__any(cand ? 1 : 0)) {
+    const float2 lb = make_float2(from_left_z(pbt.ay), from_left_z(pbt.c));
+    const bool cand = bound_candidate(lb.x, lb.y, pbt.ay, pbt.c, thr);
+    if ((__builtin_amdgcn_ballot_w64(cand) & act_lanes & ~col1_lanes) != 0) {
       const double lnh = from_left(pnh), lnhv = from_left(nhv);
       const CellQ nr = cell_dir(cell_base(pnh, 0.0), nhv);
       CellQ nl = cell_dir(cell_base(lnh, 0.0), lnhv);
@@ -804,7 +847,7 @@ k_boundary_speed(const ScnDev* __restrict__ scn, const BoundaryDesc* __restrict_
       const BoundTerms bl = bound_terms(hL, qL, 0.0), br = bound_terms(hR, qR, 0.0);
       need = valid && bound_candidate(bl.ax, bl.c, br.ax, br.c, thr);
     }
-    if (__any(need ? 1 : 0)) {
+    if (wave_any(need)) {
       const CellQ cl = cell_dir(cell_base(hL, 0.0), qL);
       const CellQ cr = cell_dir(cell_base(hR, 0.0), qR);
       const double sp = edge_speed_any(cl, cr);
@@ -833,7 +876,7 @@ k_boundary_speed(const ScnDev* __restrict__ scn, const BoundaryDesc* __restrict_
         const BoundTerms bl = bound_terms(hL, qL, 0.0), br = bound_terms(hR, qR, 0.0);
         need = valid && bound_candidate(bl.ax, bl.c, br.ax, br.c, thr);
       }
-      if (__any(need ? 1 : 0)) {
+      if (wave_any(need)) {
         const CellQ cl = cell_dir(cell_base(hL, 0.0), qL);
         const CellQ cr = cell_dir(cell_base(hR, 0.0), qR);
         const double sp = edge_speed_any(cl, cr);

@@ -178,7 +178,18 @@ __device__ __forceinline__ void edge_solve(CellQ L, CellQ R, double& dhL,
 // general path performs in that case (with the exceptions noted where they
 // occur, which cannot change a cell's result), so results are unchanged bit
 // for bit; only the selects of the unused branches are skipped.
-__device__ __forceinline__ bool wave_all(bool c) { return __all(c ? 1 : 0) != 0; }
+// Wave votes over the active lanes, read straight from the compare mask
+// (no per-lane 0/1 value and second compare, as __all / __any make).
+__device__ __forceinline__ bool wave_all(bool c) {
+  return __builtin_amdgcn_ballot_w64(c) == __builtin_amdgcn_read_exec();
+}
+__device__ __forceinline__ bool wave_any(bool c) { return __builtin_amdgcn_ballot_w64(c) != 0; }
+// Both conditions on every active lane outside the mask `ign` (one mask per
+// compare, combined as masks; `ign` holds active lanes only).
+__device__ __forceinline__ bool wave_all2(bool c0, bool c1, unsigned long long ign = 0) {
+  return ((__builtin_amdgcn_ballot_w64(c0) & __builtin_amdgcn_ballot_w64(c1)) | ign) ==
+         __builtin_amdgcn_read_exec();
+}
 
 // wave_speeds for two wet cells with the hardware min / max: they differ
 // from the selects of wave_speeds only for a NaN or for +0 against -0, and
@@ -199,8 +210,11 @@ __device__ __forceinline__ double edge_speed_wet(const CellQ& L, const CellQ& R)
   return __builtin_fmax(fabs(s1), fabs(s2));
 }
 
+// `ign`: lanes holding dummy edges whose results are discarded; they take
+// no part in the wave's choice of path (their values may be anything).
 __device__ __forceinline__ void edge_solve_wet(const CellQ& L, const CellQ& R, double& dhL,
-                                               double& dhR, double& dqL, double& dqR) {
+                                               double& dhR, double& dqL, double& dqR,
+                                               unsigned long long ign = 0) {
   double s1, s2;
   wave_speeds_wet(L, R, s1, s2);
   const double d0 = R.q - L.q;
@@ -209,7 +223,7 @@ __device__ __forceinline__ void edge_solve_wet(const CellQ& L, const CellQ& R, d
   const double beta1 = (s2 * d0 - d1) * inv;
   const double beta2 = (d1 - s1 * d0) * inv;
   const double z1 = beta1 * s1, z2 = beta2 * s2;
-  if (wave_all(s1 < -kZeroTol && s2 > kZeroTol)) {
+  if (wave_all2(s1 < -kZeroTol, s2 > kZeroTol, ign)) {
     // The general path's sums start from +0, so they return +0 where these
     // are -0.  The march (the only caller) uses the four values only in
     // sums that start from +0 or whose other term is never -0 (the cell's
@@ -239,17 +253,19 @@ __device__ __forceinline__ void edge_solve_wet(const CellQ& L, const CellQ& R, d
   dhL = a; dhR = b; dqL = c; dqR = d;
 }
 
-// Dispatch on wave-wide wetness (all lanes must call it together).
+// Dispatch on wave-wide wetness (all lanes must call it together); the
+// lanes of `ign` hold dummy edges, as in edge_solve_wet.
 __device__ __forceinline__ void edge_solve_any(const CellQ& L, const CellQ& R, double& dhL,
-                                               double& dhR, double& dqL, double& dqR) {
-  if (wave_all(L.h >= kDryTol && R.h >= kDryTol))
-    edge_solve_wet(L, R, dhL, dhR, dqL, dqR);
+                                               double& dhR, double& dqL, double& dqR,
+                                               unsigned long long ign = 0) {
+  if (wave_all2(L.h >= kDryTol, R.h >= kDryTol, ign))
+    edge_solve_wet(L, R, dhL, dhR, dqL, dqR, ign);
   else
     edge_solve(L, R, dhL, dhR, dqL, dqR);
 }
 
 __device__ __forceinline__ double edge_speed_any(const CellQ& L, const CellQ& R) {
-  if (wave_all(L.h >= kDryTol && R.h >= kDryTol)) return edge_speed_wet(L, R);
+  if (wave_all2(L.h >= kDryTol, R.h >= kDryTol)) return edge_speed_wet(L, R);
   return edge_speed(L, R);
 }
 
@@ -481,6 +497,19 @@ struct EdgeSolver {
                                                double& dhR, double& dqL, double& dqR) {
     edge_solve_any(L, R, dhL, dhR, dqL, dqR);
   }
+  // Lanes of `ign` hold dummy edges (any operands, results discarded), and
+  // the caller has taken the wetness vote: `all_wet` says that every lane
+  // outside `ign` has two wet cells.
+  static constexpr bool kDummyLanesFree = true;
+  __device__ static __forceinline__ void solve_voted(bool all_wet, const CellQ& L,
+                                                     const CellQ& R, unsigned long long ign,
+                                                     double& dhL, double& dhR, double& dqL,
+                                                     double& dqR) {
+    if (all_wet)
+      edge_solve_wet(L, R, dhL, dhR, dqL, dqR, ign);
+    else
+      edge_solve(L, R, dhL, dhR, dqL, dqR);
+  }
 };
 
 template <>
@@ -490,6 +519,14 @@ struct EdgeSolver<1> {
   }
   __device__ static __forceinline__ void solve(const CellQ& L, const CellQ& R, double& dhL,
                                                double& dhR, double& dqL, double& dqR) {
+    aug_solve(L.h, R.h, L.q, R.q, L.b, R.b, dhL, dhR, dqL, dqR);
+  }
+  // Dummy edges get real (if meaningless) operands: the solver is not
+  // exercised on arbitrary states.  It takes no wetness vote.
+  static constexpr bool kDummyLanesFree = false;
+  __device__ static __forceinline__ void solve_voted(bool, const CellQ& L, const CellQ& R,
+                                                     unsigned long long, double& dhL,
+                                                     double& dhR, double& dqL, double& dqR) {
     aug_solve(L.h, R.h, L.q, R.q, L.b, R.b, dhL, dhR, dqL, dqR);
   }
 };
@@ -520,10 +557,20 @@ __device__ __forceinline__ BoundTerms bound_terms(double h, double hu, double hv
 
 constexpr float kBoundMargin = 1.0f + 1.0f / 1024.0f;
 
+// max of two bound terms.  The terms are +0, positive, +inf or a NaN with a
+// clear sign bit (|hu| and the hardware square root of a positive value, or
+// 0 for a dry cell); for those the IEEE order is the order of the bit
+// patterns as signed integers, and a NaN wins, so a NaN term still makes the
+// edge a candidate.  One integer max instead of fmaxf's two operand
+// canonicalisations and the max.
+__device__ __forceinline__ float fmax_nonneg(float a, float b) {
+  return __int_as_float(max(__float_as_int(a), __float_as_int(b)));
+}
+
 // True unless the bound proves the edge slower than `thr` (NaN -> true).
 __device__ __forceinline__ bool bound_candidate(float aL, float cL, float aR, float cR,
                                                 float thr) {
-  const float b = (fmaxf(aL, aR) + fmaxf(cL, cR)) * kBoundMargin;
+  const float b = (fmax_nonneg(aL, aR) + fmax_nonneg(cL, cR)) * kBoundMargin;
   return !(b < thr);
 }
 
