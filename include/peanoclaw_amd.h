@@ -69,8 +69,9 @@ const char *pc_last_error_string(void);
 int pc_pool_create(int device, size_t hbm_budget_bytes, pc_pool **out);
 int pc_pool_destroy(pc_pool *pool);
 int pc_pool_bytes_in_use(pc_pool *pool, size_t *out);
-/* Pre-allocate field storage for `cells` tile cells (sum of (nx+2)(ny+2)),
- * 56 bytes each; fails with PC_E_OOM beyond the budget. */
+/* Pre-allocate field storage for `cells` tile cells (sum of
+ * (nx+2)*pc_tile_stride(ny), plus up to 15 alignment cells per tile), 56
+ * bytes each; fails with PC_E_OOM beyond the budget. */
 int pc_pool_reserve(pc_pool *pool, long long cells);
 int pc_pool_synchronize(pc_pool *pool);
 
@@ -99,10 +100,17 @@ int pc_patch_download(pc_pool *pool, int slot, double *unew,
                       double *uold_with_ghost, double *b_with_ghost);
 
 /* Bulk access for large pools: the tile offset of a slot in the field
- * arrays, the state (0/1) holding a scenario's current values, and a copy of
- * cells [begin, begin+count) of state `state` (and of b when b != NULL)
- * between host arrays and the pool (to_device = 1 uploads). */
+ * arrays and its row stride (row x of the tile, ghost rows included, starts
+ * at offset + x*stride; stride = pc_tile_stride(ny): ny + 2, padded to a
+ * multiple of 16 for ny >= 8 so that every row's interior starts on a
+ * 128-byte line), the state (0/1) holding a scenario's current values, and a
+ * copy of cells [begin, begin+count) of state `state` (and of b when
+ * b != NULL) between host arrays and the pool (to_device = 1 uploads).  No
+ * counterpart in the reference (its patches live in Peano's heap); these
+ * serve hosts that fill or read a whole pool at once. */
 int pc_patch_offset(pc_pool *pool, int slot, long long *offset);
+int pc_patch_stride(pc_pool *pool, int slot, int *stride);
+int pc_tile_stride(int ny); /* -1 outside [2, 64] */
 int pc_scenario_current_state(pc_pool *pool, int scenario, int *state);
 int pc_pool_copy_fields(pc_pool *pool, int to_device, int state,
                         long long begin, long long count, double *h,
@@ -219,7 +227,8 @@ int pc_get_ghostlayer_width(void);
 /* ---- batched step (all patches, one scenario clock per scenario) ---- */
 /* One ghost strip: `length` cells copied from source cells
  * (src_slot, src_cell + k*src_stride) to (dst_slot, dst_cell + k*dst_stride),
- * cell indices into the slot's (nx+2)*(ny+2) tile; negate is a bit mask:
+ * cell indices into the slot's tile (cell (x, y) of the tile, ghost ring
+ * included, is x*pc_patch_stride + y); negate is a bit mask:
  * bit 0 negates hu, bit 1 negates hv (reflective walls; a corner reflected
  * by two walls has both).  Strips must be independent (no strip reads a
  * ghost cell written by another).                                         */

@@ -132,6 +132,8 @@ _SIGS = {
     "pc_patch_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]),
     "pc_patch_offset": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_longlong)]),
+    "pc_patch_stride": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip]),
+    "pc_tile_stride": (ctypes.c_int, [ctypes.c_int]),
     "pc_scenario_current_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip]),
     "pc_pool_copy_fields": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_longlong, ctypes.c_longlong,

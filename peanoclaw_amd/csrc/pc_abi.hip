@@ -51,7 +51,8 @@ struct Slot {
   bool live = false;
   pc_cell_description desc{};
   long long off = 0;
-  long long cells = 0;
+  long long cells = 0;  // (nx + 2) * stride
+  int stride = 0;       // row stride of the tile (tile_stride(ny))
   bool clock_from_scenario = false;
 };
 
@@ -269,20 +270,31 @@ int grow_fields(pc_pool* p, long long need_cells) {
   return PC_OK;
 }
 
-int alloc_cells(pc_pool* p, long long n, long long* off) {
+// First fit of n cells; with phase >= 0 the offset is phase modulo
+// kTileAlign (the part of a free range before it stays free).
+int alloc_cells(pc_pool* p, long long n, long long* off, int phase = -1) {
+  auto first_at = [&](long long a) {
+    return phase < 0 ? a : a + ((phase - a) % kTileAlign + kTileAlign) % kTileAlign;
+  };
   for (size_t i = 0; i < p->free_ranges.size(); ++i) {
-    auto& r = p->free_ranges[i];
-    if (r.second >= n) {
-      *off = r.first;
-      r.first += n;
-      r.second -= n;
-      if (r.second == 0) p->free_ranges.erase(p->free_ranges.begin() + i);
-      return PC_OK;
-    }
+    const long long a = p->free_ranges[i].first, len = p->free_ranges[i].second;
+    const long long st = first_at(a);
+    if (st + n > a + len) continue;
+    p->free_ranges.erase(p->free_ranges.begin() + i);
+    if (st + n < a + len) p->free_ranges.push_back({st + n, a + len - st - n});
+    if (st > a) p->free_ranges.push_back({a, st - a});
+    std::sort(p->free_ranges.begin(), p->free_ranges.end());
+    *off = st;
+    return PC_OK;
   }
-  CHK(grow_fields(p, p->top + n));
-  *off = p->top;
-  p->top += n;
+  const long long st = first_at(p->top);
+  CHK(grow_fields(p, st + n));
+  if (st > p->top) {
+    p->free_ranges.push_back({p->top, st - p->top});
+    std::sort(p->free_ranges.begin(), p->free_ranges.end());
+  }
+  *off = st;
+  p->top = st + n;
   return PC_OK;
 }
 
@@ -368,6 +380,7 @@ PatchDev make_patchdev(const Slot& s) {
   d.off = s.off;
   d.nx = s.desc.subdivision_factor[0];
   d.ny = s.desc.subdivision_factor[1];
+  d.stride = s.stride;
   d.scn = s.desc.scenario;
   d.dx = s.desc.size[0] / (double)d.nx;
   d.dy = s.desc.size[1] / (double)d.ny;
@@ -568,9 +581,10 @@ int rebuild_topology(pc_pool* p) {
       const int i = live[k];
       const PatchDev& P = pd[i];
       const SideDev& sd = p->h_sides[i];
-      const long long stride = P.ny + 2;
+      const long long stride = P.stride;  // a same-size neighbour's too
       BoundaryDesc& D = bdv[k];
       D.off = P.off;
+      D.stride = P.stride;
       D.nx = P.nx;
       D.ny = P.ny;
       D.scn = P.scn;
@@ -900,8 +914,9 @@ int pc_patch_register(pc_pool* p, const pc_cell_description* d, int* out_slot) {
   Slot s;
   s.live = true;
   s.desc = *d;
-  s.cells = (long long)(nx + 2) * (ny + 2);
-  CHK(alloc_cells(p, s.cells, &s.off));
+  s.stride = tile_stride(ny);
+  s.cells = (long long)(nx + 2) * s.stride;
+  CHK(alloc_cells(p, s.cells, &s.off, tile_phase(ny)));
   // cells above the high-water mark are still zero from the allocation
   if (s.off < p->hwm)
     for (int k = 0; k < kNumFields; ++k)
@@ -1010,23 +1025,25 @@ int pc_patch_upload(pc_pool* p, int slot, const double* unew, const double* uold
   const Slot& s = p->slots[slot];
   const int nx = s.desc.subdivision_factor[0], ny = s.desc.subdivision_factor[1];
   const int cur = cur_of(p, slot);
-  const size_t area = (size_t)(nx + 2) * (ny + 2);
+  // host arrays in the reference layout (rows of ny, or ny + 2 with the
+  // ghost ring), tile rows of s.stride
+  const size_t w = (size_t)(ny + 2), area = (size_t)(nx + 2) * w, pitch = s.stride * sizeof(double);
   if (unew) {
     for (int q = 0; q < 3; ++q) {
-      double* dst = p->f[2 * q + cur] + s.off + (ny + 2) + 1;
-      HIPCHK(hipMemcpy2DAsync(dst, (ny + 2) * sizeof(double), unew + (size_t)q * nx * ny,
-                              ny * sizeof(double), ny * sizeof(double), nx,
-                              hipMemcpyHostToDevice, p->stream));
+      double* dst = p->f[2 * q + cur] + s.off + s.stride + 1;
+      HIPCHK(hipMemcpy2DAsync(dst, pitch, unew + (size_t)q * nx * ny, ny * sizeof(double),
+                              ny * sizeof(double), nx, hipMemcpyHostToDevice, p->stream));
     }
   }
   if (uold) {
     for (int q = 0; q < 3; ++q)
-      HIPCHK(hipMemcpyAsync(p->f[2 * q + 1 - cur] + s.off, uold + q * area,
-                            area * sizeof(double), hipMemcpyHostToDevice, p->stream));
+      HIPCHK(hipMemcpy2DAsync(p->f[2 * q + 1 - cur] + s.off, pitch, uold + q * area,
+                              w * sizeof(double), w * sizeof(double), nx + 2,
+                              hipMemcpyHostToDevice, p->stream));
   }
   if (b)
-    HIPCHK(hipMemcpyAsync(p->f[6] + s.off, b, area * sizeof(double), hipMemcpyHostToDevice,
-                          p->stream));
+    HIPCHK(hipMemcpy2DAsync(p->f[6] + s.off, pitch, b, w * sizeof(double), w * sizeof(double),
+                            nx + 2, hipMemcpyHostToDevice, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
   mark_state_changed(p);
   return PC_OK;
@@ -1039,23 +1056,23 @@ int pc_patch_download(pc_pool* p, int slot, double* unew, double* uold, double* 
   const Slot& s = p->slots[slot];
   const int nx = s.desc.subdivision_factor[0], ny = s.desc.subdivision_factor[1];
   const int cur = cur_of(p, slot);
-  const size_t area = (size_t)(nx + 2) * (ny + 2);
+  const size_t w = (size_t)(ny + 2), area = (size_t)(nx + 2) * w, pitch = s.stride * sizeof(double);
   if (unew) {
     for (int q = 0; q < 3; ++q) {
-      const double* src = p->f[2 * q + cur] + s.off + (ny + 2) + 1;
-      HIPCHK(hipMemcpy2DAsync(unew + (size_t)q * nx * ny, ny * sizeof(double), src,
-                              (ny + 2) * sizeof(double), ny * sizeof(double), nx,
-                              hipMemcpyDeviceToHost, p->stream));
+      const double* src = p->f[2 * q + cur] + s.off + s.stride + 1;
+      HIPCHK(hipMemcpy2DAsync(unew + (size_t)q * nx * ny, ny * sizeof(double), src, pitch,
+                              ny * sizeof(double), nx, hipMemcpyDeviceToHost, p->stream));
     }
   }
   if (uold) {
     for (int q = 0; q < 3; ++q)
-      HIPCHK(hipMemcpyAsync(uold + q * area, p->f[2 * q + 1 - cur] + s.off,
-                            area * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+      HIPCHK(hipMemcpy2DAsync(uold + q * area, w * sizeof(double), p->f[2 * q + 1 - cur] + s.off,
+                              pitch, w * sizeof(double), nx + 2, hipMemcpyDeviceToHost,
+                              p->stream));
   }
   if (b)
-    HIPCHK(hipMemcpyAsync(b, p->f[6] + s.off, area * sizeof(double), hipMemcpyDeviceToHost,
-                          p->stream));
+    HIPCHK(hipMemcpy2DAsync(b, w * sizeof(double), p->f[6] + s.off, pitch, w * sizeof(double),
+                            nx + 2, hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
   return PC_OK;
 }
@@ -1065,6 +1082,14 @@ int pc_patch_offset(pc_pool* p, int slot, long long* off) {
   *off = p->slots[slot].off;
   return PC_OK;
 }
+
+int pc_patch_stride(pc_pool* p, int slot, int* stride) {
+  if (!valid_slot(p, slot) || !stride) return fail(PC_E_INVALID, "invalid slot");
+  *stride = p->slots[slot].stride;
+  return PC_OK;
+}
+
+int pc_tile_stride(int ny) { return ny >= 2 && ny <= 64 ? tile_stride(ny) : -1; }
 
 int pc_scenario_current_state(pc_pool* p, int scenario, int* state) {
   if (!p || !state || scenario < 0) return fail(PC_E_INVALID, "invalid argument");
@@ -1749,7 +1774,7 @@ int pc_patches_clear(pc_pool* p, int count, const int* slots) {
   for (int i = 0; i < count; ++i) {
     const auto& d = p->slots[slots[i]].desc;
     cur[i] = cur_of(p, slots[i]);
-    most = std::max(most, (d.subdivision_factor[0] + 2) * (d.subdivision_factor[1] + 2));
+    most = std::max(most, (d.subdivision_factor[0] + 2) * p->slots[slots[i]].stride);
   }
   Staging st;
   const size_t o1 = st.add(sl), o2 = st.add(cur);
@@ -2038,8 +2063,7 @@ int pc_set_ghost_strips(pc_pool* p, const pc_ghost_strip* strips, int n) {
     const auto& sd = p->slots[g.src_slot].desc;
     if (dd.scenario != sd.scenario)
       return fail(PC_E_INVALID, "ghost strips must stay within one scenario");
-    const long long dcells = (long long)(dd.subdivision_factor[0] + 2) * (dd.subdivision_factor[1] + 2);
-    const long long scells = (long long)(sd.subdivision_factor[0] + 2) * (sd.subdivision_factor[1] + 2);
+    const long long dcells = p->slots[g.dst_slot].cells, scells = p->slots[g.src_slot].cells;
     if (g.length > 0) {
       const long long d0 = g.dst_cell, d1 = g.dst_cell + (long long)(g.length - 1) * g.dst_stride;
       const long long s0 = g.src_cell, s1 = g.src_cell + (long long)(g.length - 1) * g.src_stride;
@@ -2098,8 +2122,9 @@ int pc_grid_strips(pc_pool* p, int px, int py, const int* slot_of, pc_ghost_stri
     return slot_of[i * py + j];
   };
   std::vector<pc_ghost_strip> res;
-  const int sy = ny + 2;
-  auto cell = [&](int x, int y) { return (x + 1) * sy + (y + 1); };
+  const int sy = ny + 2;          // rows of the symbolic ghost ring below
+  const int ts = tile_stride(ny);  // rows of the tiles (strip cells)
+  auto cell = [&](int x, int y) { return (x + 1) * ts + (y + 1); };
   for (int i = 0; i < px; ++i)
     for (int j = 0; j < py; ++j) {
       const int me = at(i, j);

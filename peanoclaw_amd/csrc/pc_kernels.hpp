@@ -2,8 +2,13 @@
 //
 // Patch pool layout in HBM (structure of arrays): every field (h, hu, hv in
 // two states for ping-pong, b once) is one device array; patch p owns the
-// contiguous tile [off_p, off_p + (nx+2)(ny+2)) in each of them, stored in
-// the reference's order (x rows, y contiguous; grid/LinearizationZYXQ.h).
+// contiguous tile [off_p, off_p + (nx+2)·stride_p) in each of them, stored in
+// the reference's order (x rows, y contiguous; grid/LinearizationZYXQ.h),
+// row x of the tile (ghost rows included) at off_p + x·stride_p.  For ny >= 8
+// the rows are padded to a multiple of 16 doubles and off_p = 15 mod 16, so
+// the interior of every row (columns 1..ny) starts on a 128-byte cache line:
+// a wavefront's load of a row's interior touches ny/16 lines, not one more.
+// Narrower patches keep stride ny + 2.
 //
 // The update is a row march: one 64-lane wavefront owns a task of up to
 // 64/W patches of the same size (W = ny rounded up to a power of two); lane
@@ -35,6 +40,14 @@ namespace pcamd {
 
 constexpr int kERows = 64;  // edge-buffer rows per side (max nx)
 
+// Row stride of a tile with ny interior columns, and the residue of its
+// offset modulo kTileAlign (interior rows on 128-byte lines for ny >= 8).
+constexpr int kTileAlign = 16;
+__host__ __device__ constexpr int tile_stride(int ny) {
+  return ny >= 8 ? (ny + 2 + kTileAlign - 1) / kTileAlign * kTileAlign : ny + 2;
+}
+__host__ __device__ constexpr int tile_phase(int ny) { return ny >= 8 ? kTileAlign - 1 : -1; }
+
 
 // Global-memory accessors: loads and stores through address space 1, so the
 // compiler emits global_* instead of flat_* instructions (flat accesses
@@ -61,7 +74,7 @@ struct PatchDev {
   long long off;  // tile base in the field arrays
   int nx, ny;
   int scn;        // scenario of the patch
-  int pad;
+  int stride;     // row stride of the tile (tile_stride(ny))
   double dx, dy;  // subcell size (Patch.cpp:38)
   double dxinv, dyinv;
 };
@@ -109,6 +122,7 @@ struct BoundaryDesc {
   int xwall[2];
   int shared[2];
   int rseg;  // row segments of the update (seams: seg_first_row(k >= 1))
+  int stride;  // row stride of the tile
   double dx, dy;
 };
 
@@ -310,7 +324,7 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
   // lane masks for the pruned mode's candidate votes
   const unsigned long long act_lanes = __builtin_amdgcn_ballot_w64(active);
   const PatchDev P = gl(A.pd + p);
-  const int stride = ny + 2;
+  const int stride = P.stride;  // the same for every patch of the task
   const int par = A.par;
   constexpr bool FROM_OLD = MODE == kUpdateSwap || MODE == kUpdateFromOld;
   const int src = FROM_OLD ? 1 - par : par;
@@ -515,11 +529,12 @@ __device__ __forceinline__ void march_task(const Task& t, const MarchArgs& A, Ma
             M::st(Hdst, o, h1);
             M::st(HUdst, o, hu1);
             M::st(HVdst, o, hv1);
-            if (MODE == kUpdate && (col == 1 || col == ny)) {
-              // Also store into the row's ghost column (0 or ny+1) of the new
-              // state, which no one reads before it is refilled: the row's
-              // cache lines are then written completely (no masked partial
-              // writebacks at the row boundaries).
+            if (MODE == kUpdate && stride == ny + 2 && (col == 1 || col == ny)) {
+              // Packed rows (ny < 8): also store into the row's ghost column
+              // (0 or ny+1) of the new state, which no one reads before it is
+              // refilled: the row's cache lines are then written completely
+              // (no masked partial writebacks at the row boundaries).  Padded
+              // rows are whole lines without them.
               const I og = col == 1 ? o - 1 : o + 1;
               M::st(Hdst, og, h1);
               M::st(HUdst, og, hu1);
@@ -768,7 +783,7 @@ k_boundary_speed(const ScnDev* __restrict__ scn, const BoundaryDesc* __restrict_
     const double est = gl(&scn[D.scn].est);
     if (est > 0.0) thr = (float)(kCfl * dmin(D.dx, D.dy) / (est * slack));
   }
-  const int nx = D.nx, ny = D.ny, stride = ny + 2;
+  const int nx = D.nx, ny = D.ny, stride = D.stride;
   // edge list: x-lower (ny), x-upper (ny), y-lower (nx), y-upper (nx).  In
   // pruned mode only the scenario minimum of the CFL steps is used, so an
   // edge shared with a same-size neighbour is evaluated once, by the patch
@@ -896,8 +911,8 @@ k_boundary_speed(const ScnDev* __restrict__ scn, const BoundaryDesc* __restrict_
 // Memory-pattern probe of the row march: the update kernel's traversal
 // (same tasks and launch shape, one wavefront per task, lanes on columns,
 // rows marched in order) reading h, hu, hv, b of every row of the tiles and
-// writing h, hu, hv of the interior rows to scratch arrays, the edge lanes
-// also into the row's ghost columns as the update does (whole cache lines),
+// writing h, hu, hv of the interior rows to scratch arrays, on packed rows
+// the edge lanes also into the row's ghost columns as the update does,
 // with no edge arithmetic.  Without arithmetic to hide behind, the loads run
 // four rows ahead (rows in blocks of four).  The rate it reaches is the
 // memory rate of this traversal of the layout.
@@ -926,7 +941,7 @@ k_march_probe(const Task* __restrict__ tasks, int ntasks, const int* __restrict_
     const bool segok = seg < t.nseg;
     const bool active = segok && col <= t.ny;
     const PatchDev P = gl(pd + gl(plist + t.first + (segok ? seg : 0)));
-    const long long stride = t.ny + 2;
+    const long long stride = P.stride;
     const long long base = P.off + (active ? col : 1);
     const int last = t.nx + 1;  // rows 0 .. nx+1
     double ch[K], chu[K], chv[K], cb[K];
@@ -952,7 +967,7 @@ k_march_probe(const Task* __restrict__ tasks, int ntasks, const int* __restrict_
           gs(oh + o, x);
           gs(ohu + o, chu[k] * x);
           gs(ohv + o, chv[k] * x);
-          if (col == 1 || col == t.ny) {
+          if (stride == t.ny + 2 && (col == 1 || col == t.ny)) {
             const long long og = o + (col == 1 ? -1 : 1);
             gs(oh + og, x);
             gs(ohu + og, chu[k] * x);
@@ -979,7 +994,7 @@ __global__ void k_edges_init(const int* __restrict__ slots, int nslots,
   const int p = slots[k];
   const PatchDev P = pd[p];
   const int par = scn[P.scn].stepc & 1;
-  const int stride = P.ny + 2;
+  const int stride = P.stride;
   for (int r = threadIdx.x + 1; r <= P.nx; r += blockDim.x) {
     const long long row = P.off + (long long)r * stride;
     for (int side = 0; side < 2; ++side) {

@@ -17,7 +17,7 @@ __global__ void k_iota(int* __restrict__ a, int n) {
 
 __device__ __forceinline__ long long tile_index(const PatchDev& P, int x, int y) {
   // cell (x, y) with x, y in [-1, n]
-  return P.off + (long long)(x + 1) * (P.ny + 2) + (y + 1);
+  return P.off + (long long)(x + 1) * P.stride + (y + 1);
 }
 
 // Row segments of the one-patch kernels (one wavefront each): at least 2
@@ -418,7 +418,7 @@ __global__ void k_clear_patches(const PatchDev* __restrict__ pd, Fields F,
                                 const int* __restrict__ slots, const int* __restrict__ curs) {
   const int slot = slots[blockIdx.y], cur = curs[blockIdx.y];
   const PatchDev P = pd[slot];
-  const int w = P.ny + 2, area = (P.nx + 2) * w;
+  const int w = P.stride, area = (P.nx + 2) * w;  // row padding included
   for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < area; k += blockDim.x * gridDim.x) {
     const long long i = P.off + k;
     F.H(1 - cur)[i] = 0.0;
@@ -434,15 +434,17 @@ __global__ void k_clear_patches(const PatchDev* __restrict__ pd, Fields F,
 }
 
 // b of a batch of tiles from a staging array: tile blockIdx.y takes
-// (nx+2)(ny+2) values from src + soff[blockIdx.y].
+// (nx+2)(ny+2) values (rows of ny+2) from src + soff[blockIdx.y].
 __global__ void k_scatter_b(const PatchDev* __restrict__ pd, Fields F,
                             const int* __restrict__ slots, const long long* __restrict__ soff,
                             const double* __restrict__ src) {
   const PatchDev P = pd[slots[blockIdx.y]];
   const double* s = src + soff[blockIdx.y];
-  const int area = (P.nx + 2) * (P.ny + 2);
-  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < area; k += blockDim.x * gridDim.x)
-    F.b[P.off + k] = s[k];
+  const int w = P.ny + 2, area = (P.nx + 2) * w;
+  for (int k = threadIdx.x + blockIdx.x * blockDim.x; k < area; k += blockDim.x * gridDim.x) {
+    const int x = k / w, y = k - x * w;
+    F.b[P.off + (long long)x * P.stride + y] = s[k];
+  }
 }
 
 // BreakingDamSWEScenario::computeDemandedMeshWidth (BreakingDam.cpp:85-139)
@@ -703,7 +705,7 @@ k_init_request(const int* __restrict__ slots, int nslots, const PatchDev* __rest
     }
     const double ex = X - a.cx, ey = Y - a.cy;
     const double r = sqrt(ex * ex + ey * ey);
-    const long long i = P.off + c;
+    const long long i = P.off + (long long)x * P.stride + y;
     F.b[i] = b;
     F.H(a.cur)[i] = a.col_h * (double)(r <= a.radius) + a.depth * (double)(r > a.radius);
     F.HU(a.cur)[i] = 0.0;
@@ -733,7 +735,7 @@ k_hash_request(const int* __restrict__ slots, int nslots, const PatchDev* __rest
   unsigned long long acc = 0;
   for (int c = threadIdx.x; c < nx * ny; c += blockDim.x) {
     const int x = c / ny, y = c - (c / ny) * ny;
-    const long long i = P.off + (long long)(x + 1) * (ny + 2) + (y + 1);
+    const long long i = tile_index(P, x, y);
     unsigned long long v = mix64((unsigned long long)k * (unsigned long long)(nx * ny) + c);
     v = mix64(v ^ (unsigned long long)__double_as_longlong(F.H(cur)[i]));
     v = mix64(v ^ (unsigned long long)__double_as_longlong(F.HU(cur)[i]));

@@ -95,10 +95,11 @@ int server_upload_slots(pc_server* s, const std::vector<int>& slots) {
 int server_register(pc_server* s, ServerReq& q) {
   pc_pool* p = s->pool;
   const int n = q.req.patch_cells, P = q.P;
-  const long long per = (long long)(n + 2) * (n + 2);
+  const long long per = (long long)(n + 2) * tile_stride(n);
   const long long total = per * P * P;
   long long off = 0;
-  CHK(alloc_cells(p, total, &off));
+  // per is a multiple of kTileAlign for padded rows: every tile keeps the phase
+  CHK(alloc_cells(p, total, &off, tile_phase(n)));
   // the other state buffer starts zeroed like a fresh heap vector; the
   // current one and b are written by k_init_request
   if (off < p->hwm) {
@@ -124,6 +125,7 @@ int server_register(pc_server* s, ServerReq& q) {
       sl.live = true;
       sl.off = off + per * k;
       sl.cells = per;
+      sl.stride = tile_stride(n);
       sl.clock_from_scenario = true;
       pc_cell_description& d = sl.desc;
       d.position[0] = i * n * dx;
@@ -302,7 +304,7 @@ int pc_server_submit(pc_server* s, const pc_request* req, int* out_id) {
   ServerReq q;
   q.req = *req;
   q.P = req->cells_per_side / req->patch_cells;
-  q.tile_cells = (long long)(req->patch_cells + 2) * (req->patch_cells + 2) * q.P * q.P;
+  q.tile_cells = (long long)(req->patch_cells + 2) * tile_stride(req->patch_cells) * q.P * q.P;
   if ((size_t)q.tile_cells * kBytesPerCell > s->pool->budget)
     return fail(PC_E_OOM, "request exceeds the pool's HBM budget");
   *out_id = (int)s->reqs.size();

@@ -33,6 +33,13 @@ def make_description(position, size, n, scenario=0, level=0, time=0.0,
     return d
 
 
+def tile_stride(ny):
+    """Row stride (doubles) of the pool's tiles with ny interior columns:
+    ny + 2, padded to a multiple of 16 for ny >= 8 so that every row's
+    interior starts on a 128-byte line."""
+    return load().pc_tile_stride(int(ny))
+
+
 class PatchPool:
     """A pool of SWE patches resident in HBM on one gfx950 device."""
 
@@ -85,6 +92,13 @@ class PatchPool:
     def offset(self, slot):
         o = ctypes.c_longlong()
         check(self.lib.pc_patch_offset(self.h, slot, ctypes.byref(o)))
+        return o.value
+
+    def stride(self, slot):
+        """Row stride of the slot's tile in the field arrays (row x of the
+        tile, ghost rows included, starts at offset(slot) + x * stride)."""
+        o = ctypes.c_int()
+        check(self.lib.pc_patch_stride(self.h, slot, ctypes.byref(o)))
         return o.value
 
     # ---- data ----

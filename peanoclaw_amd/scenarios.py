@@ -19,7 +19,7 @@ import math
 import numpy as np
 
 from ._abi import GHOST_STRIP_DTYPE
-from .pool import make_description
+from .pool import make_description, tile_stride
 
 SIZES = (8, 16, 32, 64)
 
@@ -183,7 +183,7 @@ def uneven_patches(pool, num_patches=50000, seed=2, field_seed=3, scenario=0, dx
     rng = np.random.default_rng(field_seed)
     layouts, slots = [], []
     cells = 0
-    pool.reserve(sum(c * (n + 2) ** 2 for n, c in counts.items()))
+    pool.reserve(sum(c * ((n + 2) * tile_stride(n) + 16) for n, c in counts.items()))
     pool.scenario_set(scenario, 0.0, 0.0, np.inf)
     state = pool.current_state(scenario)
     # large classes first: their tiles come first in the pool
@@ -197,16 +197,21 @@ def uneven_patches(pool, num_patches=50000, seed=2, field_seed=3, scenario=0, dx
         # ghost-ring values of h, hu, hv are rewritten by the ghost fill
         order = [int(slot_of[i // rows, i % rows]) for i in range(c)]
         offs = np.array([pool.offset(s) for s in order])
-        if np.all(np.diff(offs) == (n + 2) ** 2):
-            pool.copy_fields(True, state, int(offs[0]), np.ascontiguousarray(h.ravel()),
-                             np.ascontiguousarray(hu.ravel()), np.ascontiguousarray(hv.ravel()),
-                             np.ascontiguousarray(b.ravel()))
+        ts = tile_stride(n)
+
+        def rows(a):
+            # tiles with rows padded to the pool's stride (padding zero)
+            out = np.zeros(a.shape[:-1] + (ts,))
+            out[..., :n + 2] = a
+            return out
+
+        if np.all(np.diff(offs) == (n + 2) * ts):
+            pool.copy_fields(True, state, int(offs[0]), rows(h).ravel(), rows(hu).ravel(),
+                             rows(hv).ravel(), rows(b).ravel())
         else:
             for t, s in enumerate(order):
-                pool.copy_fields(True, state, int(offs[t]), np.ascontiguousarray(h[t].ravel()),
-                                 np.ascontiguousarray(hu[t].ravel()),
-                                 np.ascontiguousarray(hv[t].ravel()),
-                                 np.ascontiguousarray(b[t].ravel()))
+                pool.copy_fields(True, state, int(offs[t]), rows(h[t]).ravel(),
+                                 rows(hu[t]).ravel(), rows(hv[t]).ravel(), rows(b[t]).ravel())
         layouts.append(GridLayout(n, slot_of))
         slots.extend(order)
         cells += c * n * n

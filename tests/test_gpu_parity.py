@@ -192,30 +192,28 @@ def test_interpolation_and_restriction_swe(amd):
         amd.release(s)
 
 
-def _oracle_pool_from_device(pool, wl):
-    """Oracle arrays in pool layout, from the device's current state."""
-    top = max(pool.offset(s) + (pool.dims(s)[0] + 2) * (pool.dims(s)[1] + 2) for s in wl.slots)
-    st = pool.current_state(wl.scenario)
-    H = [np.zeros(top), np.zeros(top)]
-    HU = [np.zeros(top), np.zeros(top)]
-    HV = [np.zeros(top), np.zeros(top)]
-    B = np.zeros(top)
-    pool.copy_fields(False, st, 0, H[0], HU[0], HV[0], B)
-    return H, HU, HV, B
-
-
 class OracleRun:
+    """The oracle's pool: the workload's tiles packed in slot order with rows
+    of ny + 2 (the oracle's layout), seeded from the device's current state;
+    the device tiles have rows of pool.stride(slot)."""
+
     def __init__(self, pool, wl):
         self.wl = wl
-        self.H, self.HU, self.HV, self.B = _oracle_pool_from_device(pool, wl)
         nslots = max(wl.slots) + 1
         self.off = np.zeros(nslots, dtype=np.int64)
         nx = np.zeros(nslots, dtype=np.int32)
         ny = np.zeros(nslots, dtype=np.int32)
         dx = np.zeros(nslots)
         dy = np.zeros(nslots)
+        top = 0
+        for s in sorted(wl.slots):
+            self.off[s] = top
+            top += (pool.dims(s)[0] + 2) * (pool.dims(s)[1] + 2)
+        self.top = top
+        h, hu, hv, b = self.from_device(pool, with_b=True)
+        self.H, self.HU, self.HV = [h, np.zeros(top)], [hu, np.zeros(top)], [hv, np.zeros(top)]
+        self.B = b
         for s in wl.slots:
-            self.off[s] = pool.offset(s)
             d = pool.description(s)
             nx[s], ny[s] = d.subdivision_factor[0], d.subdivision_factor[1]
             dx[s] = d.size[0] / d.subdivision_factor[0]
@@ -234,6 +232,30 @@ class OracleRun:
             self.pl.hv[k] = odptr(self.HV[k])
         self.pl.b = odptr(self.B)
         self.pl.cur = 0
+
+    def from_device(self, pool, with_b=False):
+        """h, hu, hv (and b) of the device's current state in the oracle's
+        layout."""
+        slots = sorted(self.wl.slots)
+        dev_top = max(pool.offset(s) + (pool.dims(s)[0] + 2) * pool.stride(s) for s in slots)
+        raw = [np.zeros(dev_top) for _ in range(4)]
+        pool.copy_fields(False, pool.current_state(self.wl.scenario), 0, *raw)
+        out = [np.zeros(self.top) for _ in range(4 if with_b else 3)]
+        groups = {}
+        for s in slots:
+            groups.setdefault((pool.dims(s), pool.stride(s)), []).append(s)
+        for ((nx, ny), st), ss in groups.items():
+            for k0 in range(0, len(ss), 2048):
+                part = ss[k0:k0 + 2048]
+                doff = np.array([pool.offset(s) for s in part], dtype=np.int64)
+                poff = self.off[part]
+                x = np.arange(nx + 2)[None, :, None]
+                y = np.arange(ny + 2)[None, None, :]
+                di = doff[:, None, None] + x * st + y
+                pi = poff[:, None, None] + x * (ny + 2) + y
+                for o, r in zip(out, raw):
+                    o[pi] = r[di]
+        return out
 
     def fill(self):
         c = self.pl.cur
@@ -416,9 +438,7 @@ def test_uneven_50k_full_size_bitwise():
     print(f"50k workload set up in {time.perf_counter() - t0:.1f} s", flush=True)
 
     def compare():
-        st = pool.current_state(0)
-        got = [np.empty(top) for _ in range(3)]
-        pool.copy_fields(False, st, 0, *got)
+        got = orun.from_device(pool)
         c = orun.pl.cur
         same = 0
         for g, ref in zip(got, (orun.H[c], orun.HU[c], orun.HV[c])):

@@ -243,7 +243,8 @@ def test_admission_waits_for_budget():
     from peanoclaw_amd.pool import PatchPool
     from peanoclaw_amd.serving import Server, make_request
     from peanoclaw_amd._abi import PeanoClawAmdError
-    tile_cells = 64 * 34 * 34  # N = 256 in 32^2 patches
+    from peanoclaw_amd.pool import tile_stride
+    tile_cells = 64 * 34 * tile_stride(32)  # N = 256 in 32^2 patches (padded rows)
     budget = int(tile_cells * 56 * 1.5)  # room for one such request, not two
     reqs = [make_request(256, 32, end_time=0.004, column=(0.5, 0.5), column_height=2.0)
             for _ in range(2)]
