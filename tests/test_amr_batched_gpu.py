@@ -121,6 +121,38 @@ def test_batched_amr_three_level_wet_dry_bitwise(solver):
         pool.close()
 
 
+@pytest.mark.timeout(900)
+def test_batched_amr_c3_1000_steps_against_the_oracle():
+    """C3 itself (9 x 9 x 27^2 basin, 3 levels, 729 leaves, 531,441 leaf
+    cells) for its full 1000 steps, batched on the device, against the
+    oracle's AMR driver step by step: the same dt at every step and, after
+    the last, every leaf value bitwise equal (the bar is 1e-10 relative).
+    The oracle side takes about two minutes."""
+    from peanoclaw_amd.amr import BatchedAmr
+    from peanoclaw_amd.pool import PatchPool
+    pool = PatchPool(0)
+    try:
+        dev, ora = _c3(pool)
+        amr = BatchedAmr(dev)
+        t0 = time.perf_counter()
+        for it in range(1000):
+            assert amr.step(1) == ora.step(), f"dt differs at step {it}"
+            if it % 200 == 199:
+                print(f"step {it + 1}: {time.perf_counter() - t0:.1f} s", flush=True)
+        same, total = _equal(dev, ora)
+        a, b = dev.leaf_states(), ora.leaf_states()
+        worst = max(float(np.max(np.abs(a[k] - b[k]) / np.maximum(np.abs(b[k]), 1e-300)))
+                    for k in a)
+        t, dt, k = amr.clock()
+        print(f"C3 batched vs oracle, 1000 steps to t={t + dt:.4e}: bitwise {same}/{total}, "
+              f"worst relative difference {worst:.2e}")
+        assert k == 1000 and worst <= 1e-10
+        assert same == total
+        amr.close()
+    finally:
+        pool.close()
+
+
 def _mass(grid, px, n):
     return sum(float(v[0].sum()) * (1.0 / (px * n * 3 ** (k[0] - 1))) ** 2
                for k, v in grid.leaf_states().items())
