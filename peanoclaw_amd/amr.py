@@ -108,6 +108,24 @@ class AmrGrid:
         return (np.ascontiguousarray(h, dtype=np.float64),
                 np.ascontiguousarray(bb, dtype=np.float64))
 
+    def _b_many(self, pts):
+        """b of the initial data of several patches (as _fields, one
+        evaluation of the field function over all of them; the same
+        operations per cell, so the same values)."""
+        if not pts:
+            return []
+        n = self.n
+        ii = np.arange(-1, n + 1)
+        px = np.array([pt.pos[0] for pt in pts])[:, None, None]
+        py = np.array([pt.pos[1] for pt in pts])[:, None, None]
+        sub = np.array([pt.size / n for pt in pts])[:, None, None]
+        X = px + (ii[None, :, None] + 0.5) * sub
+        Y = py + (ii[None, None, :] + 0.5) * sub
+        X, Y = np.broadcast_arrays(X, Y)
+        _, bb = self.fn(X, Y)
+        bb = np.broadcast_to(bb, X.shape)
+        return [np.ascontiguousarray(bb[k], dtype=np.float64) for k in range(len(pts))]
+
     def _rebuild(self, changed=None):
         """Children, leaves, virtual patches, restriction and ghost plans
         from the current tree."""
@@ -201,7 +219,7 @@ class AmrGrid:
         # b of the new patches in one upload, then their interpolations
         # (uNew at the new time, uOld at the current time; the nine children
         # of a parent in order) as one batch
-        B.set_b_batch([ch.handle for _, ch in created], [self._fields(ch)[1] for _, ch in created])
+        B.set_b_batch([ch.handle for _, ch in created], self._b_many([ch for _, ch in created]))
         B.interpolate_batch([call for p, ch in created
                              for call in (((n, n), (0, 0), p.handle, ch.handle, 0, 0, 0),
                                           ((n, n), (0, 0), p.handle, ch.handle, 1, 1, 0))])
@@ -821,7 +839,10 @@ class BatchedAmr:
 
 
 def iarr_list(v):
-    return (ctypes.c_int * max(len(v), 1))(*v)
+    """int32 array of v (at least one element) as an int pointer; the pointer
+    keeps the array alive."""
+    a = np.ascontiguousarray(np.asarray(v if len(v) else [0], dtype=np.int32))
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
 def front_refinement(px, py, n, fn, max_level, jump=0.5):

@@ -143,6 +143,13 @@ struct pc_pool {
   // staging of the batched level transfers (pc_*_batch, pc_patches_*)
   char* d_batch = nullptr;
   size_t batch_cap = 0;
+  // the table buffer of a destroyed AMR plan, kept for the next
+  // pc_amr_create (a regrid replaces the plan)
+  char* amr_tables = nullptr;
+  size_t amr_tables_cap = 0;
+  // pinned host staging for large synchronous uploads (plan tables)
+  char* h_stage = nullptr;
+  size_t h_stage_cap = 0;
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -862,10 +869,11 @@ int pc_pool_destroy(pc_pool* p) {
                   p->d_plist, p->d_plist_scn, p->d_scn_of, p->d_live, p->d_bdesc, p->d_strips,
                   p->d_counter, p->d_iota, p->d_segmax, p->d_keep, p->d_result, p->d_aux, p->d_flags, p->d_ebuf[0],
                   p->d_ebuf[1], p->d_bcol, p->d_sides, p->d_single, p->d_single_slot,
-                  p->d_edge_io, p->d_flux, p->d_batch};
+                  p->d_edge_io, p->d_flux, p->d_batch, p->amr_tables};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (p->h_result) hipHostFree(p->h_result);
+  if (p->h_stage) hipHostFree(p->h_stage);
   for (hipEvent_t e : p->upd_ev) hipEventDestroy(e);
   hipEventDestroy(p->ev0);
   hipEventDestroy(p->ev1);
@@ -1574,6 +1582,48 @@ struct Staging {
     bytes.resize(off + v.size() * sizeof(T));
     if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
     return off;
+  }
+};
+
+// Tables gathered in the pool's pinned staging buffer (kept between calls,
+// so its pages are resident) and uploaded with one DMA: add() places a
+// host array (or zeros) at a 256-byte aligned offset; upload() copies
+// everything to dst, synchronously.
+struct PinnedTables {
+  struct Piece {
+    const void* src;  // nullptr: zeros
+    size_t bytes, off;
+  };
+  std::vector<Piece> pieces;
+  size_t total = 0;
+  size_t add(const void* src, size_t bytes) {
+    const size_t off = (total + 255) / 256 * 256;
+    pieces.push_back({src, bytes, off});
+    total = off + std::max<size_t>(bytes, 16);  // an empty table still gets an address
+    return off;
+  }
+  template <class T>
+  size_t add(const std::vector<T>& v) { return add(v.data(), v.size() * sizeof(T)); }
+  int upload(pc_pool* p, char* dst) const {
+    if (total == 0) return PC_OK;
+    if (p->h_stage_cap < total) {
+      HIPCHK(hipStreamSynchronize(p->stream));
+      if (p->h_stage) HIPCHK(hipHostFree(p->h_stage));
+      p->h_stage = nullptr;
+      p->h_stage_cap = 0;
+      const size_t cap = std::max<size_t>(2 * total, size_t(4) << 20);
+      HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&p->h_stage), cap, hipHostMallocDefault));
+      p->h_stage_cap = cap;
+    }
+    for (const Piece& c : pieces) {
+      if (c.src)
+        std::memcpy(p->h_stage + c.off, c.src, c.bytes);
+      else
+        std::memset(p->h_stage + c.off, 0, c.bytes);
+    }
+    HIPCHK(hipMemcpyAsync(dst, p->h_stage, total, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return PC_OK;
   }
 };
 

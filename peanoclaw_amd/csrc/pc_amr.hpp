@@ -737,6 +737,8 @@ struct pc_amr {
   int* d_wall_dims = nullptr;
   double* d_smax = nullptr;
   AmrClock* d_clock = nullptr;
+  char* d_tables = nullptr;  // one allocation holding every d_* table above
+  size_t tables_cap = 0;
   double last_dt = 0.0;
   bool shared_clock = true;  // all leaves on one clock (global steps allowed)
   // local time stepping: per-slot clocks and flags, neighbour lists of the
@@ -778,21 +780,15 @@ int amr_upload(T** d, const std::vector<T>& h) {
 }
 
 void amr_free(pc_amr* a) {
-  hipFree(a->d_tasks);
-  hipFree(a->d_utasks);
-  hipFree(a->d_rseg);
-  hipFree(a->d_smax_seg);
-  hipFree(a->d_plist);
-  hipFree(a->d_leaves);
-  hipFree(a->d_vblocks);
-  hipFree(a->d_cptr);
-  hipFree(a->d_cpairs);
-  hipFree(a->d_pairs);
-  hipFree(a->d_ghosts);
-  hipFree(a->d_walls);
-  hipFree(a->d_wall_dims);
-  hipFree(a->d_smax);
-  hipFree(a->d_clock);
+  // the table buffer goes back to the pool for the next plan (the caller
+  // has synchronised the stream)
+  if (a->d_tables && !a->pool->amr_tables) {
+    a->pool->amr_tables = a->d_tables;
+    a->pool->amr_tables_cap = a->tables_cap;
+  } else if (a->d_tables) {
+    hipFree(a->d_tables);
+  }
+  a->d_tables = nullptr;
   if (a->gexec) hipGraphExecDestroy(a->gexec);
   if (a->graph) hipGraphDestroy(a->graph);
   void* lts[] = {a->lts.t,    a->lts.dt,   a->lts.adv, a->lts.stepdt, a->lts.iter, a->lts.steps,
@@ -927,13 +923,28 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
   std::vector<AmrVBlock> vblocks;
   std::vector<int> cptr{0}, cpairs;
   std::vector<AmrRestrict> pairs;
+  // the restrictions of each destination slot, in the given order
+  std::vector<int> rbeg(role.size() + 1, 0), rlist(nrestrictions);
+  for (int r = 0; r < nrestrictions; ++r) {
+    const int v = restrictions[2 * r + 1];
+    if (v >= 0 && v < (int)role.size()) ++rbeg[v + 1];
+  }
+  for (size_t k = 1; k < rbeg.size(); ++k) rbeg[k] += rbeg[k - 1];
+  {
+    std::vector<int> fill(rbeg.begin(), rbeg.end() - 1);
+    for (int r = 0; r < nrestrictions; ++r) {
+      const int v = restrictions[2 * r + 1];
+      if (v >= 0 && v < (int)role.size()) rlist[fill[v]++] = r;
+    }
+  }
+  std::vector<int> ccount;
   for (int vi = 0; vi < nvirtuals; ++vi) {
     const int v = virtuals[vi];
     const int pbeg = (int)pairs.size();
     const auto& dd = p->slots[v].desc;
     const PatchDev D = make_patchdev(p->slots[v]);
-    for (int r = 0; r < nrestrictions; ++r) {
-      if (restrictions[2 * r + 1] != v) continue;
+    for (int ri = rbeg[v]; ri < rbeg[v + 1]; ++ri) {
+      const int r = rlist[ri];
       const int s = restrictions[2 * r];
       if (s < 0 || s >= (int)role.size() || role[s] != 1)
         return bail("a restriction source is not a leaf of the plan");
@@ -963,15 +974,23 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
     const int base = (int)cptr.size() - 1;
     for (int c0 = 0; c0 < nx * ny; c0 += kRestrictCells)
       vblocks.push_back(AmrVBlock{v, c0, base, 0});
-    for (int c = 0; c < nx * ny; ++c) {
-      const int x = c / ny, y = c % ny;
-      for (int pi = pbeg; pi < (int)pairs.size(); ++pi) {
-        const AmrRestrict& R = pairs[pi];
-        if (x >= R.dOff[0] && x < R.dOff[0] + R.dSize[0] && y >= R.dOff[1] &&
-            y < R.dOff[1] + R.dSize[1])
-          cpairs.push_back(pi);
-      }
-      cptr.push_back((int)cpairs.size());
+    // per cell the pairs whose region covers it, in pair order: count, then
+    // fill pair by pair (each pair marks its region's cells)
+    ccount.assign((size_t)nx * ny, 0);
+    for (int pi = pbeg; pi < (int)pairs.size(); ++pi) {
+      const AmrRestrict& R = pairs[pi];
+      for (int x = R.dOff[0]; x < R.dOff[0] + R.dSize[0]; ++x)
+        for (int y = R.dOff[1]; y < R.dOff[1] + R.dSize[1]; ++y) ++ccount[(size_t)x * ny + y];
+    }
+    const size_t cbeg = cpairs.size();
+    for (int c = 0; c < nx * ny; ++c) cptr.push_back(cptr.back() + ccount[c]);
+    cpairs.resize(cptr.back());
+    for (int c = 0; c < nx * ny; ++c) ccount[c] = cptr[base + c] - (int)cbeg;  // cursors
+    for (int pi = pbeg; pi < (int)pairs.size(); ++pi) {
+      const AmrRestrict& R = pairs[pi];
+      for (int x = R.dOff[0]; x < R.dOff[0] + R.dSize[0]; ++x)
+        for (int y = R.dOff[1]; y < R.dOff[1] + R.dSize[1]; ++y)
+          cpairs[cbeg + ccount[(size_t)x * ny + y]++] = pi;
     }
   }
   for (int r = 0; r < nrestrictions; ++r) {
@@ -1075,27 +1094,56 @@ int pc_amr_create(pc_pool* p, const int32_t* leaves, int nleaves, const int32_t*
   AmrClock c{};
   c.time = t0;
   c.dt = dt0;
-  int rc = PC_OK;
-  if ((rc = amr_upload(&a->d_tasks, tasks)) != PC_OK ||
-      (rc = amr_upload(&a->d_utasks, utasks)) != PC_OK ||
-      (rc = amr_upload(&a->d_rseg, rseg)) != PC_OK ||
-      (rc = amr_upload(&a->d_smax_seg,
-                       std::vector<double>(p->slots.size() * kMaxRowSeg, 0.0))) != PC_OK ||
-      (rc = amr_upload(&a->d_plist, plist)) != PC_OK ||
-      (rc = amr_upload(&a->d_leaves, a->leaves)) != PC_OK ||
-      (rc = amr_upload(&a->d_vblocks, vblocks)) != PC_OK ||
-      (rc = amr_upload(&a->d_cptr, cptr)) != PC_OK ||
-      (rc = amr_upload(&a->d_cpairs, cpairs)) != PC_OK ||
-      (rc = amr_upload(&a->d_pairs, pairs)) != PC_OK ||
-      (rc = amr_upload(&a->d_ghosts, gh)) != PC_OK ||
-      (rc = amr_upload(&a->d_walls, wl)) != PC_OK ||
-      (rc = amr_upload(&a->d_wall_dims, wdims)) != PC_OK ||
-      (rc = amr_upload(&a->d_smax, std::vector<double>(p->slots.size(), 0.0))) != PC_OK ||
-      (rc = amr_upload(&a->d_clock, std::vector<AmrClock>{c})) != PC_OK) {
+  // every table in one device allocation, filled by one copy (a regrid
+  // makes a new plan: one allocation, usually the previous plan's, and one
+  // DMA instead of fifteen of each)
+  PinnedTables st;
+  const size_t o_tasks = st.add(tasks), o_utasks = st.add(utasks), o_rseg = st.add(rseg);
+  const size_t o_smax_seg = st.add(nullptr, p->slots.size() * kMaxRowSeg * sizeof(double));
+  const size_t o_plist = st.add(plist), o_leaves = st.add(a->leaves);
+  const size_t o_vblocks = st.add(vblocks), o_cptr = st.add(cptr), o_cpairs = st.add(cpairs);
+  const size_t o_pairs = st.add(pairs), o_ghosts = st.add(gh), o_walls = st.add(wl);
+  const size_t o_wall_dims = st.add(wdims);
+  const size_t o_smax = st.add(nullptr, p->slots.size() * sizeof(double));
+  const std::vector<AmrClock> clk{c};
+  const size_t o_clock = st.add(clk);
+  hipError_t e = hipSuccess;
+  if (p->amr_tables && p->amr_tables_cap >= st.total) {
+    a->d_tables = p->amr_tables;  // idle: its plan was destroyed after a sync
+    a->tables_cap = p->amr_tables_cap;
+    p->amr_tables = nullptr;
+    p->amr_tables_cap = 0;
+  } else {
+    a->tables_cap = 2 * st.total;  // room for a grown grid
+    e = hipMalloc(&a->d_tables, a->tables_cap);
+  }
+  if (e != hipSuccess) {
     amr_free(a);
     delete a;
-    return rc;
+    return fail(PC_E_HIP, std::string("plan tables: ") + hipGetErrorString(e));
   }
+  const int urc = st.upload(p, a->d_tables);
+  if (urc != PC_OK) {
+    amr_free(a);
+    delete a;
+    return urc;
+  }
+  char* const base = a->d_tables;
+  a->d_tasks = reinterpret_cast<Task*>(base + o_tasks);
+  a->d_utasks = reinterpret_cast<Task*>(base + o_utasks);
+  a->d_rseg = reinterpret_cast<int*>(base + o_rseg);
+  a->d_smax_seg = reinterpret_cast<double*>(base + o_smax_seg);
+  a->d_plist = reinterpret_cast<int*>(base + o_plist);
+  a->d_leaves = reinterpret_cast<int*>(base + o_leaves);
+  a->d_vblocks = reinterpret_cast<AmrVBlock*>(base + o_vblocks);
+  a->d_cptr = reinterpret_cast<int*>(base + o_cptr);
+  a->d_cpairs = reinterpret_cast<int*>(base + o_cpairs);
+  a->d_pairs = reinterpret_cast<AmrRestrict*>(base + o_pairs);
+  a->d_ghosts = reinterpret_cast<AmrGhost*>(base + o_ghosts);
+  a->d_walls = reinterpret_cast<AmrWalls*>(base + o_walls);
+  a->d_wall_dims = reinterpret_cast<int*>(base + o_wall_dims);
+  a->d_smax = reinterpret_cast<double*>(base + o_smax);
+  a->d_clock = reinterpret_cast<AmrClock*>(base + o_clock);
   *out = a;
   return PC_OK;
 }
