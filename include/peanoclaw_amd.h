@@ -489,6 +489,22 @@ typedef struct {
 typedef struct {
   int32_t slot, dimension, upper;
 } pc_amr_wall;
+/* The face, corner and wall entries of every leaf of a patch tree with
+ * refinement factor 3, as pc_amr_create takes them (host only, no pool):
+ * patch k has level[k] >= 1 and index (ij[2k], ij[2k+1]) on its level
+ * (level 1: a px x py grid of n x n cells); leaves[] lists the leaves in
+ * plan order (indices into the patch arrays); entries name patches by
+ * handle[k].  Per leaf: faces x-, x+, y-, y+ (outside the domain: a wall),
+ * corners (-,-), (-,+), (+,-), (+,+) (skipped outside the domain), each a
+ * transfer from the patch of the leaf's level there, else an interpolation
+ * from the nearest coarser patch covering it (the compositor's choice,
+ * GhostLayerCompositor.cpp:280-320, GhostlayerCompositorFunctors.cpp:87-141);
+ * then the walls left, top, right, bottom (SolveTimestep.cpp:94-131).  With
+ * NULL outputs only the counts are returned. */
+int pc_amr_face_plan(int px, int py, int n, int npatch, const int32_t *level,
+                     const int32_t *ij, const int32_t *handle, const int32_t *leaves,
+                     int nleaves, pc_amr_ghost *ghosts, int cap_ghosts, int *nghosts,
+                     pc_amr_wall *walls, int cap_walls, int *nwalls);
 /* restrictions: n pairs (leaf, virtual) in call order. */
 int pc_amr_create(pc_pool *pool, const int32_t *leaves, int nleaves,
                   const int32_t *virtuals, int nvirtuals, const int32_t *restrictions,

@@ -220,6 +220,10 @@ _SIGS = {
     "pc_server_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pc_pool_device_fields": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] +
                               [ctypes.POINTER(_dp)] * 4),
+    "pc_amr_face_plan": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_int, _ip, ctypes.c_void_p, ctypes.c_int, _ip]),
     "pc_amr_create": (ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.c_int, _ip, ctypes.c_int,
                                      _ip, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                      ctypes.c_void_p, ctypes.c_int,

@@ -149,7 +149,22 @@ class AmrGrid:
         for i in range(self.px):
             for j in range(self.py):
                 walk(self.levels[1][(i, j)], [])
-        self._plan = self._ghost_plan(changed)
+        # the face plan is derived when first used (the batched driver takes
+        # the library's packed rows instead): the changes since the last
+        # plan accumulate for its incremental update (None: everything)
+        self._plan_value = None
+        if changed is None or getattr(self, "_plan_changed", None) is None:
+            self._plan_changed = None  # a full derivation is pending
+        else:
+            self._plan_changed |= set(changed)
+
+    @property
+    def _plan(self):
+        """(face and corner entries, walls) of every leaf (_ghost_plan)."""
+        if self._plan_value is None:
+            self._plan_value = self._ghost_plan(self._plan_changed)
+            self._plan_changed = set()
+        return self._plan_value
 
     def regrid(self, threshold=0.05, minimal_width=None, maximal_width=None):
         """One remeshing pass on the current state with the BreakingDam
@@ -333,9 +348,33 @@ class AmrGrid:
         self._face_cache = cache
         return plan, walls
 
+    def packed_plan_native(self):
+        """The rows of packed_plan() built by the library (pc_amr_face_plan)
+        from the tree alone: the same rules, without deriving the Python
+        plan.  Patch handles must be ints."""
+        lib = load()
+        pts = [pt for lev in sorted(self.levels) for pt in self.levels[lev].values()]
+        index = {id(pt): k for k, pt in enumerate(pts)}
+        level = np.ascontiguousarray([pt.level for pt in pts], dtype=np.int32)
+        ij = np.ascontiguousarray([pt.ij for pt in pts], dtype=np.int32).reshape(-1)
+        handle = np.ascontiguousarray([pt.handle for pt in pts], dtype=np.int32)
+        leaves = np.ascontiguousarray([index[id(p)] for p in self.leaves], dtype=np.int32)
+        gh = np.zeros((8 * len(leaves) + 1, 9), np.int32)
+        wl = np.zeros((4 * len(leaves) + 1, 3), np.int32)
+        ng, nw = ctypes.c_int(), ctypes.c_int()
+        vp = ctypes.c_void_p
+        check(lib.pc_amr_face_plan(self.px, self.py, self.n, len(pts),
+                                   level.ctypes.data_as(vp), ij.ctypes.data_as(vp),
+                                   handle.ctypes.data_as(vp), leaves.ctypes.data_as(vp),
+                                   len(leaves), gh.ctypes.data_as(vp), len(gh),
+                                   ctypes.byref(ng), wl.ctypes.data_as(vp), len(wl),
+                                   ctypes.byref(nw)))
+        return gh[:ng.value].copy(), wl[:nw.value].copy()
+
     def packed_plan(self):
         """The face entries and walls of the current plan as int32 rows in
         the layout of pc_amr_ghost and pc_amr_wall (plan order)."""
+        self._plan  # derive the plan (and its per-leaf rows) if stale
         cache = self._face_cache
         rows = [cache[p.handle] for p in self.leaves]
         gh = np.concatenate([c[4] for c in rows]) if rows else np.zeros((0, 9), np.int32)
@@ -738,19 +777,18 @@ class BatchedAmr:
         pairs = [x for leaf, anc in grid._restrictions for v in anc for x in (leaf.handle,
                                                                                 v.handle)]
         rest = iarr_list(pairs)
-        plan, walls = grid._plan
         # the face entries and walls as int32 rows in the layout of
-        # pc_amr_ghost and pc_amr_wall
+        # pc_amr_ghost and pc_amr_wall, from the library
         assert ctypes.sizeof(AmrGhost) == 9 * 4 and ctypes.sizeof(AmrWall) == 3 * 4
-        gh, wl = grid.packed_plan()
-        assert len(gh) == len(plan) and len(wl) == len(walls)
-        gh = np.ascontiguousarray(gh if len(gh) else np.zeros((1, 9), np.int32))
-        wl = np.ascontiguousarray(wl if len(wl) else np.zeros((1, 3), np.int32))
+        gh, wl = grid.packed_plan_native()
+        ng, nw = len(gh), len(wl)
+        gh = np.ascontiguousarray(gh if ng else np.zeros((1, 9), np.int32))
+        wl = np.ascontiguousarray(wl if nw else np.zeros((1, 3), np.int32))
         h = ctypes.c_void_p()
         check(self.lib.pc_amr_create(self.pool.h, leaves, len(grid.leaves), virtuals,
                                      len(grid.virtuals), rest, len(pairs) // 2,
-                                     gh.ctypes.data_as(ctypes.c_void_p), len(plan),
-                                     wl.ctypes.data_as(ctypes.c_void_p), len(walls),
+                                     gh.ctypes.data_as(ctypes.c_void_p), ng,
+                                     wl.ctypes.data_as(ctypes.c_void_p), nw,
                                      ctypes.byref(h)))
         self.h = h
 

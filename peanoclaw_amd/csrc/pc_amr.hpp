@@ -39,6 +39,8 @@
 // arithmetic in the same order.
 #pragma once
 
+#include <unordered_map>
+
 namespace {
 
 struct AmrClock {
@@ -1447,6 +1449,113 @@ int pc_amr_clock(pc_amr* a, double* time, double* timestep_size, long long* step
   if (time) *time = c.time;
   if (timestep_size) *timestep_size = c.dt;
   if (steps) *steps = c.steps;
+  return PC_OK;
+}
+
+// Face, corner and wall entries of every leaf of a tree of patches with
+// refinement factor 3 (host only, no device work): the rows pc_amr_create
+// takes.  Patch k has level[k] >= 1 and index (ij[2k], ij[2k+1]) on its
+// level; level 1 is a px x py grid.  For each leaf in `leaves` order
+// (indices into the patch arrays): its four faces x-, x+, y-, y+ (a face
+// outside the domain is a wall), then its four corners (-,-), (-,+), (+,-),
+// (+,+) (skipped outside the domain); a face or corner is a transfer from
+// the patch of the leaf's level there, else an interpolation from the
+// nearest coarser patch covering it (the compositor's choice,
+// GhostLayerCompositor.cpp:280-320, GhostlayerCompositorFunctors.cpp:87-141);
+// then its walls in the order left, top, right, bottom
+// (SolveTimestep.cpp:94-131).  Entries name patches by handle[k].
+int pc_amr_face_plan(int px, int py, int n, int npatch, const int32_t* level,
+                     const int32_t* ij, const int32_t* handle, const int32_t* leaves,
+                     int nleaves, pc_amr_ghost* ghosts, int cap_ghosts, int* nghosts,
+                     pc_amr_wall* walls, int cap_walls, int* nwalls) {
+  if (px < 1 || py < 1 || n < 1 || npatch < 0 || nleaves < 0 || !nghosts || !nwalls ||
+      (npatch && (!level || !ij || !handle)) || (nleaves && !leaves))
+    return fail(PC_E_INVALID, "invalid argument");
+  auto key = [](long long l, long long i, long long j) { return (l << 50) | (i << 25) | j; };
+  std::unordered_map<long long, int> at;
+  at.reserve((size_t)npatch * 2);
+  for (int k = 0; k < npatch; ++k) {
+    if (level[k] < 1 || level[k] > 12 || ij[2 * k] < 0 || ij[2 * k + 1] < 0)
+      return fail(PC_E_INVALID, "invalid patch level or index");
+    at[key(level[k], ij[2 * k], ij[2 * k + 1])] = k;
+  }
+  auto width = [&](int lev, long long& wx, long long& wy) {
+    long long f = 1;
+    for (int l = 1; l < lev; ++l) f *= 3;
+    wx = px * f;
+    wy = py * f;
+  };
+  // the patch at (lev, i, j), else the nearest coarser one covering it;
+  // -1 outside the domain or where nothing covers it
+  auto covering = [&](int lev, long long i, long long j) {
+    long long wx, wy;
+    width(lev, wx, wy);
+    if (i < 0 || j < 0 || i >= wx || j >= wy) return -1;
+    for (; lev >= 1; --lev, i /= 3, j /= 3) {
+      auto it = at.find(key(lev, i, j));
+      if (it != at.end()) return it->second;
+    }
+    return -1;
+  };
+  const int steps[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};  // x-, x+, y-, y+
+  const int geom[4][6] = {  // size, source offset, destination offset per face
+      {1, n, n - 1, 0, -1, 0}, {1, n, 0, 0, n, 0}, {n, 1, 0, n - 1, 0, -1}, {n, 1, 0, 0, 0, n}};
+  const int corners[4][2] = {{-1, -1}, {-1, 1}, {1, -1}, {1, 1}};
+  const int wall_face[4] = {0, 3, 1, 2};  // left, top, right, bottom as face indices
+  int ng = 0, nw = 0;
+  for (int li = 0; li < nleaves; ++li) {
+    const int p = leaves[li];
+    if (p < 0 || p >= npatch) return fail(PC_E_INVALID, "invalid leaf index");
+    const int lev = level[p];
+    const long long i = ij[2 * p], j = ij[2 * p + 1];
+    long long wx, wy;
+    width(lev, wx, wy);
+    bool wall[4];
+    auto emit = [&](int src, int sx, int sy, int sox, int soy, int dox, int doy) {
+      if (ghosts && ng < cap_ghosts) {
+        pc_amr_ghost& g = ghosts[ng];
+        g.kind = level[src] == lev ? 0 : 1;
+        g.src = handle[src];
+        g.dst = handle[p];
+        g.size[0] = sx;
+        g.size[1] = sy;
+        g.src_offset[0] = sox;
+        g.src_offset[1] = soy;
+        g.dst_offset[0] = dox;
+        g.dst_offset[1] = doy;
+      }
+      ++ng;
+    };
+    for (int f = 0; f < 4; ++f) {
+      const long long qi = i + steps[f][0], qj = j + steps[f][1];
+      wall[f] = qi < 0 || qj < 0 || qi >= wx || qj >= wy;
+      if (wall[f]) continue;
+      const int src = covering(lev, qi, qj);
+      if (src < 0) return fail(PC_E_INVALID, "face region not covered");
+      const int* g = geom[f];
+      emit(src, g[0], g[1], g[2], g[3], g[4], g[5]);
+    }
+    for (const auto& c : corners) {
+      const int src = covering(lev, i + c[0], j + c[1]);
+      if (src < 0) continue;  // outside the domain: the walls fill it
+      emit(src, 1, 1, c[0] == -1 ? n - 1 : 0, c[1] == -1 ? n - 1 : 0, c[0] == -1 ? -1 : n,
+           c[1] == -1 ? -1 : n);
+    }
+    for (int w = 0; w < 4; ++w) {
+      const int f = wall_face[w];
+      if (!wall[f]) continue;
+      if (walls && nw < cap_walls) {
+        walls[nw].slot = handle[p];
+        walls[nw].dimension = f / 2;
+        walls[nw].upper = f % 2;
+      }
+      ++nw;
+    }
+  }
+  *nghosts = ng;
+  *nwalls = nw;
+  if ((ghosts && ng > cap_ghosts) || (walls && nw > cap_walls))
+    return fail(PC_E_INVALID, "output capacity too small");
   return PC_OK;
 }
 

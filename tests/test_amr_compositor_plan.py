@@ -105,6 +105,10 @@ def _check(build, seed):
     _random_state(ga, A, seed)
     _random_state(gb, B, seed)
     before = {p.handle: A.uold(p.handle) for p in ga.leaves}
+    # the library's rows of the same plan (pc_amr_face_plan, used by the
+    # batched driver) equal the driver's
+    for x, y in zip(ga.packed_plan_native(), ga.packed_plan()):
+        assert np.array_equal(x, y)
     ga.fill_ghosts()
     calls = _compositor_fill(gb, B)
     n_int = n_interp = 0

@@ -207,6 +207,28 @@ def test_regrid_follows_the_front_and_conserves_mass_roughly():
     print("refined fractions:", [round(f, 3) for f in [first] + fr])
 
 
+def test_native_face_plan_follows_regrids():
+    """The library's face rows (pc_amr_face_plan, what the batched driver
+    uploads) equal the driver's incrementally updated plan after every
+    regrid of a front that refines and coarsens."""
+    grid = _front_grid()
+    grid.regrid()
+    for k in range(6):
+        grid.run(10)
+        grid.regrid()
+        for x, y in zip(grid.packed_plan_native(), grid.packed_plan()):
+            assert np.array_equal(x, y), k
+    # coarsening as well: a lake at rest, fully refined, coarsens back
+    def fn(X, Y):
+        b = -1.0 + 0.004 * np.exp(-((X - 0.5) ** 2 + (Y - 0.5) ** 2) / 0.01)
+        return 0.0 - b, b
+    lake = AmrGrid(OracleBackend(), 3, 3, 6, lambda lev, ij: True, fn, max_level=2)
+    lake.run(5)
+    assert lake.regrid() == (0, 9)
+    for x, y in zip(lake.packed_plan_native(), lake.packed_plan()):
+        assert np.array_equal(x, y)
+
+
 def test_refined_patches_reproduce_a_linear_field():
     """New patches are interpolated from their parent: a linear surface
     (Gaussian b) is reproduced at every new cell centre."""
