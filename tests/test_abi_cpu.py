@@ -55,3 +55,17 @@ def test_constants_and_no_device_error(libpath):
     rc = lib.pc_pool_create(0, 0, ctypes.byref(h))
     assert rc == _abi.PC_E_NODEVICE
     assert b"device" in lib.pc_last_error_string()
+
+
+def test_tile_stride_rule(libpath):
+    """Rows of ny + 2 doubles below ny = 8; from 8 on, padded to a multiple
+    of 16 doubles (so that the interior can start on a 128-byte line)."""
+    from peanoclaw_amd import _abi
+    lib = _abi.load()
+    for ny in range(2, 65):
+        st = lib.pc_tile_stride(ny)
+        if ny < 8:
+            assert st == ny + 2
+        else:
+            assert st % 16 == 0 and ny + 2 <= st < ny + 18
+    assert lib.pc_tile_stride(1) == -1 and lib.pc_tile_stride(65) == -1

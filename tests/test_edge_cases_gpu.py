@@ -64,6 +64,47 @@ def test_invalid_arguments_fail_loudly_and_leave_the_pool_usable():
         pool.close()
 
 
+def test_padded_tiles_placement_and_pitched_copies():
+    """Tiles of patches with ny >= 8 start at offset 15 mod 16 with rows of
+    pc_tile_stride(ny) doubles, narrower ones keep rows of ny + 2; uploads
+    and downloads in the reference layout round-trip through the padded
+    rows, and the pool's raw arrays hold row x of the tile at
+    offset + x * stride."""
+    from peanoclaw_amd.pool import make_description, tile_stride
+    pool = _pool()
+    try:
+        rng = np.random.default_rng(9)
+        shapes = [(4, 3), (9, 8), (5, 7), (64, 64), (12, 31), (8, 16), (3, 2), (33, 47)]
+        slots = []
+        for k, (nx, ny) in enumerate(shapes):
+            s = pool.register(make_description((2.0 * k, 0.0), 1.0, nx, ny=ny))
+            st, off = pool.stride(s), pool.offset(s)
+            assert st == tile_stride(ny)
+            if ny >= 8:
+                assert st % 16 == 0 and off % 16 == 15
+            else:
+                assert st == ny + 2
+            unew = rng.uniform(-1, 1, (3, nx, ny))
+            uold = rng.uniform(-1, 1, (3, nx + 2, ny + 2))
+            b = rng.uniform(-1, 1, (nx + 2, ny + 2))
+            pool.upload(s, unew=unew, uold=uold, b=b)
+            slots.append((s, unew, uold, b))
+        top = max(pool.offset(s) + (pool.dims(s)[0] + 2) * pool.stride(s) for s, *_ in slots)
+        raw = [np.zeros(top) for _ in range(4)]
+        pool.copy_fields(False, pool.current_state(0), 0, *raw)
+        for s, unew, uold, b in slots:
+            gn, go, gb = pool.download(s)
+            assert np.array_equal(gn, unew) and np.array_equal(go, uold)
+            assert np.array_equal(gb, b)
+            nx, ny = pool.dims(s)
+            o, st = pool.offset(s), pool.stride(s)
+            tile = lambda a: a[o:o + (nx + 2) * st].reshape(nx + 2, st)
+            assert np.array_equal(tile(raw[0])[1:-1, 1:ny + 1], unew[0])
+            assert np.array_equal(tile(raw[3])[:, :ny + 2], b)
+    finally:
+        pool.close()
+
+
 def test_smallest_and_largest_patches_and_slot_reuse():
     """A 2 x 2 and a 64 x 64 patch, each alone with reflective walls: five
     batched steps bitwise against the per-patch solveTimestep sequence of
