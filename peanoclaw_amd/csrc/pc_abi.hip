@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -125,6 +126,7 @@ struct pc_pool {
   // address): no copy back after the kernel
   double* h_result = nullptr;
   double* d_result_host = nullptr;
+  unsigned long long call_seq = 0;  // completion words of the per-call kernels
   double* d_aux = nullptr;
   size_t aux_cap = 0;
   int* d_flags = nullptr;
@@ -845,8 +847,10 @@ int pc_pool_create(int device, size_t hbm_budget_bytes, pc_pool** out) {
   HIPCHK(hipMemsetAsync(p->d_counter, 0, 2 * sizeof(int), p->stream));
   p->d_redo = p->d_counter + 1;
   HIPCHK(hipMalloc(&p->d_result, 8 * sizeof(double)));
+  // fine-grained (coherent), so a kernel's stores are visible while it runs
   HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&p->h_result), 8 * sizeof(double),
-                       hipHostMallocMapped));
+                       hipHostMallocMapped | hipHostMallocCoherent));
+  std::memset(p->h_result, 0, 8 * sizeof(double));
   HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&p->d_result_host), p->h_result, 0));
   HIPCHK(hipMalloc(&p->d_segmax, kPatchSeg * sizeof(double)));
   HIPCHK(hipMalloc(&p->d_keep, 3 * 64 * 64 * sizeof(double)));
@@ -1135,6 +1139,33 @@ int pc_get_number_of_parameter_fields_without_ghostlayer(void) { return 0; }
 int pc_get_number_of_parameter_fields_with_ghostlayer(void) { return 1; }
 int pc_get_ghostlayer_width(void) { return 1; }
 
+}  // extern "C"
+
+namespace {
+// Wait for a synchronous per-call kernel: spin on its completion word in
+// pinned host memory (visible as soon as the kernel has stored its results,
+// without waiting for the kernel's retirement to reach the stream), for at
+// most 2 ms; then, or if the word does not come, synchronise the stream,
+// which also reports any error.
+int wait_call(pc_pool* p, unsigned long long seq) {
+  const unsigned long long* word =
+      reinterpret_cast<const unsigned long long*>(p->h_result + kDoneWord);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned it = 0;; ++it) {
+    if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return PC_OK;
+    if ((it & 255) == 255 &&
+        std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2))
+      break;
+  }
+  HIPCHK(hipStreamSynchronize(p->stream));
+  if (__atomic_load_n(word, __ATOMIC_ACQUIRE) != seq)
+    return fail(PC_E_HIP, "a per-call kernel did not signal its completion");
+  return PC_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int pc_solve_timestep(pc_pool* p, int slot, double maxDt, const int flags[4],
                       double* out_dt, double* out_est) {
   (void)flags;
@@ -1148,16 +1179,17 @@ int pc_solve_timestep(pc_pool* p, int slot, double maxDt, const int flags[4],
     s.clock_from_scenario = false;
   }
   const int nseg = patch_row_segments(s.desc.subdivision_factor[0]);
+  const unsigned long long seq = ++p->call_seq;
   if (p->solver)
     hipLaunchKernelGGL(k_patch_solve<1>, dim3(1), dim3(64 * nseg), 0, p->stream, slot,
                        p->d_iota, p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_segmax,
-                       p->d_result_host, p->d_keep);
+                       p->d_result_host, p->d_keep, seq);
   else
     hipLaunchKernelGGL(k_patch_solve<0>, dim3(1), dim3(64 * nseg), 0, p->stream, slot,
                        p->d_iota, p->d_pd, fields_of(p), cur_of(p, slot), maxDt, p->d_segmax,
-                       p->d_result_host, p->d_keep);
+                       p->d_result_host, p->d_keep, seq);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(p->stream));
+  CHK(wait_call(p, seq));
   const double res[2] = {p->h_result[0], p->h_result[1]};
   // SWEKernel.cpp:83-87
   if (la_greater(res[0], 0.0)) {
@@ -1175,16 +1207,17 @@ int pc_patch_cfl(pc_pool* p, int slot, double* out) {
   hipSetDevice(p->device);
   CHK(prepare(p));
   const int nseg = patch_row_segments(p->slots[slot].desc.subdivision_factor[0]);
+  const unsigned long long seq = ++p->call_seq;
   if (p->solver)
     hipLaunchKernelGGL(k_patch_cfl<1>, dim3(1), dim3(64 * nseg), 0, p->stream, slot,
                        p->d_iota, p->d_pd, fields_of(p), cur_of(p, slot), p->d_segmax,
-                       p->d_result_host);
+                       p->d_result_host, seq);
   else
     hipLaunchKernelGGL(k_patch_cfl<0>, dim3(1), dim3(64 * nseg), 0, p->stream, slot,
                        p->d_iota, p->d_pd, fields_of(p), cur_of(p, slot), p->d_segmax,
-                       p->d_result_host);
+                       p->d_result_host, seq);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(p->stream));
+  CHK(wait_call(p, seq));
   *out = p->h_result[0];
   return PC_OK;
 }

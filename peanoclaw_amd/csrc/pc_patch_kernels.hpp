@@ -29,6 +29,15 @@ __host__ __device__ __forceinline__ int patch_row_segments(int nx) {
   return s < 1 ? 1 : (s > kPatchSeg ? kPatchSeg : s);
 }
 
+// Completion word of a synchronous per-call kernel: `seq` stored to
+// out[kDoneWord] (fine-grained host memory) after the results, with release
+// ordering at system scope, so the host sees the results once it sees seq.
+constexpr int kDoneWord = 4;
+__device__ __forceinline__ void signal_done(double* out, unsigned long long seq) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(out + kDoneWord), seq,
+                     __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // SWEKernel::solveTimestep on one patch (SWEKernel.cpp:50-88): speed of
 // every edge of uOld, dt = fmin(CFL, maxDt), update, swap.  Both the speeds
 // and the update run in blockDim/64 row segments, one wavefront each: the
@@ -42,7 +51,7 @@ template <int SOLVER>
 __global__ void __launch_bounds__(64 * kPatchSeg)
 k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restrict__ pd,
               Fields F, int cur, double maxDt, double* __restrict__ segmax,
-              double* __restrict__ out, double* __restrict__ keep) {
+              double* __restrict__ out, double* __restrict__ keep, unsigned long long seq) {
   __shared__ MarchLds L[kPatchSeg];
   const int w = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
   const PatchDev P = pd[slot];
@@ -69,6 +78,13 @@ k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restric
   for (int k = 1; k < nw; ++k) s = dmax(s, segmax[k]);
   const double cfl = max_timestep(s, P.dx, P.dy);
   const double dt = fmin(cfl, maxDt);
+  // the host needs only dt and the estimate: it is released here, and the
+  // update below finishes on the stream before any later work on the pool
+  if (threadIdx.x == 0) {
+    out[0] = dt;
+    out[1] = cfl;
+    signal_done(out, seq);
+  }
   A.par = cur;
   A.dt = dt;
   A.smax = nullptr;
@@ -80,10 +96,6 @@ k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restric
     F.HU(1 - cur)[i] = keep[n + k];
     F.HV(1 - cur)[i] = keep[2 * n + k];
   }
-  if (threadIdx.x == 0) {
-    out[0] = dt;
-    out[1] = cfl;
-  }
 }
 
 // The CFL step of one patch from uOld (state 1 - cur) with its ghost layer,
@@ -92,7 +104,8 @@ k_patch_solve(int slot, const int* __restrict__ plist, const PatchDev* __restric
 template <int SOLVER>
 __global__ void __launch_bounds__(64 * kPatchSeg)
 k_patch_cfl(int slot, const int* __restrict__ plist, const PatchDev* __restrict__ pd,
-            Fields F, int cur, double* __restrict__ segmax, double* __restrict__ out) {
+            Fields F, int cur, double* __restrict__ segmax, double* __restrict__ out,
+            unsigned long long seq) {
   __shared__ MarchLds L[kPatchSeg];
   const int w = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
   const PatchDev P = pd[slot];
@@ -112,6 +125,7 @@ k_patch_cfl(int slot, const int* __restrict__ plist, const PatchDev* __restrict_
     double s = segmax[0];
     for (int k = 1; k < nw; ++k) s = dmax(s, segmax[k]);
     out[0] = max_timestep(s, P.dx, P.dy);
+    signal_done(out, seq);
   }
 }
 
