@@ -2055,7 +2055,8 @@ static int refine_flags_slots(pc_pool* p, const std::vector<int>& slots, double 
   HIPCHK(hipMemcpyAsync(d_sl, slots.data(), n * sizeof(int), hipMemcpyHostToDevice,
                         p->stream));
   hipLaunchKernelGGL(k_refine_flags, dim3(n), dim3(64), 0, p->stream, d_sl, n, p->d_pd,
-                     fields_of(p), p->d_scn, thr, minW, maxW, p->d_flags, p->d_aux);
+                     fields_of(p), p->d_scn, thr, minW, maxW, p->d_flags, p->d_aux,
+                     static_cast<double*>(nullptr), 0ull);
   HIPCHK(hipGetLastError());
   if (flags)
     HIPCHK(hipMemcpyAsync(flags, p->d_flags, n * sizeof(int), hipMemcpyDeviceToHost,
@@ -2072,9 +2073,17 @@ int pc_demanded_mesh_width(pc_pool* p, int slot, double thr, double minW, double
   if (!valid_slot(p, slot) || !out) return fail(PC_E_INVALID, "invalid slot");
   hipSetDevice(p->device);
   CHK(prepare(p));
-  int32_t f;
-  std::vector<int> one{slot};
-  CHK(refine_flags_slots(p, one, thr, minW, maxW, &f, out));
+  // one launch: the slot through the identity table, the result straight
+  // into pinned host memory, the completion word after it
+  const unsigned long long seq = ++p->call_seq;
+  int* const flag_host = reinterpret_cast<int*>(p->d_result_host + 6);
+  hipLaunchKernelGGL(k_refine_flags, dim3(1), dim3(64), 0, p->stream, p->d_iota + slot, 1,
+                     p->d_pd, fields_of(p), p->d_scn, thr, minW, maxW, flag_host,
+                     p->d_result_host, p->d_result_host, seq);
+  HIPCHK(hipGetLastError());
+  CHK(wait_call(p, seq));
+  out[0] = p->h_result[0];
+  out[1] = p->h_result[1];
   return PC_OK;
 }
 

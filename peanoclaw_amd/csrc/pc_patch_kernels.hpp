@@ -468,7 +468,8 @@ __global__ void __launch_bounds__(64)
 k_refine_flags(const int* __restrict__ slots, int nslots, const PatchDev* __restrict__ pd,
                Fields F, const ScnDev* __restrict__ scn, double threshold,
                double minW, double maxW, int* __restrict__ flags,
-               double* __restrict__ demanded) {
+               double* __restrict__ demanded, double* __restrict__ done = nullptr,
+               unsigned long long seq = 0) {
   const int k = blockIdx.x;
   if (k >= nslots) return;
   const int slot = slots[k];
@@ -507,6 +508,7 @@ k_refine_flags(const int* __restrict__ slots, int nslots, const PatchDev* __rest
     flags[k] = f;
     demanded[2 * k] = dw0;
     demanded[2 * k + 1] = dw1;
+    if (done) signal_done(done, seq);  // a single-patch call (one workgroup)
   }
 }
 

@@ -607,6 +607,10 @@ def test_refinement_flags_bitexact():
         olib().or_demanded_mesh_width(op.ref, 0.05, 1.0 / 1296, 1.0 / 432, out)
         sub = (ctypes.c_double * 2)(*op.s.subcell)
         assert flags[s] == olib().or_refinement_decision(sub, out)
+        # the per-call form (Numerics::getDemandedMeshWidth) of the same patch
+        got = (ctypes.c_double * 2)()
+        assert pool.lib.pc_demanded_mesh_width(pool.h, s, 0.05, 1.0 / 1296, 1.0 / 432, got) == 0
+        assert list(got) == list(out)
     refined = float((flags[wl.slots] == 1).mean())
     print("refined fraction of coarse patches:", refined)
     assert 0.0 < refined < 1.0
