@@ -119,7 +119,10 @@ int pc_pool_copy_fields(pc_pool *pool, int to_device, int state,
 /* ---- Numerics-mirroring per-patch calls ---- */
 /* Numerics::solveTimestep (Numerics.h:147-152; SWEKernel.cpp:125-167).
  * boundary_flags[4] is accepted for interface parity and unused, as in the
- * reference.  Outputs the applied dt and estimatedNextTimestepSize. */
+ * reference.  Outputs the applied dt and estimatedNextTimestepSize.  Returns
+ * as soon as both are known; the update itself finishes on the pool's
+ * stream, before any later call's work on the pool (an error of it is
+ * reported by a later call that synchronises). */
 int pc_solve_timestep(pc_pool *pool, int slot, double maximum_timestep_size,
                       const int boundary_flags[4], double *out_dt,
                       double *out_estimated_next_dt);
@@ -535,7 +538,12 @@ int pc_amr_lts_init(pc_amr *amr, const int32_t *nptr, const int32_t *nbr);
 int pc_amr_lts_step(pc_amr *amr, int k, double end_time, long long *leaf_steps);
 int pc_amr_destroy(pc_amr *amr);
 
-/* Raw device pointers of the pool (for zero-copy interop); state = 0/1. */
+/* Raw device pointers of the pool (for zero-copy interop); state = 0/1.
+ * The library's own work runs on the pool's stream and some calls return
+ * before it has finished (pc_solve_timestep returns once dt is known, while
+ * the update still runs; the batched steps queue): call
+ * pc_pool_synchronize before touching these arrays from another stream or
+ * from the host. */
 int pc_pool_device_fields(pc_pool *pool, int state, double **h, double **hu,
                           double **hv, double **b);
 
