@@ -1475,7 +1475,9 @@ int pc_amr_face_plan(int px, int py, int n, int npatch, const int32_t* level,
   std::unordered_map<long long, int> at;
   at.reserve((size_t)npatch * 2);
   for (int k = 0; k < npatch; ++k) {
-    if (level[k] < 1 || level[k] > 12 || ij[2 * k] < 0 || ij[2 * k + 1] < 0)
+    // the hash key holds 25 bits per index
+    if (level[k] < 1 || level[k] > 12 || ij[2 * k] < 0 || ij[2 * k + 1] < 0 ||
+        ij[2 * k] >= (1 << 25) || ij[2 * k + 1] >= (1 << 25))
       return fail(PC_E_INVALID, "invalid patch level or index");
     at[key(level[k], ij[2 * k], ij[2 * k + 1])] = k;
   }

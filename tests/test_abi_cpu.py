@@ -69,3 +69,48 @@ def test_tile_stride_rule(libpath):
         else:
             assert st % 16 == 0 and ny + 2 <= st < ny + 18
     assert lib.pc_tile_stride(1) == -1 and lib.pc_tile_stride(65) == -1
+
+
+def test_face_plan_counts_and_rejects_bad_input(libpath):
+    """pc_amr_face_plan on a 2x1 level-1 grid (host only): counts with NULL
+    outputs, the rows themselves, and PC_E_INVALID for an index beyond the
+    hash key's 25 bits, an uncovered face and too small an output."""
+    import numpy as np
+    from peanoclaw_amd import _abi
+    lib = _abi.load()
+    vp = ctypes.c_void_p
+
+    def plan(level, ij, leaves, px=2, py=1, cap_g=None, cap_w=None, outputs=True):
+        level = np.ascontiguousarray(level, np.int32)
+        ij = np.ascontiguousarray(ij, np.int32).reshape(-1)
+        handle = np.arange(len(level), dtype=np.int32) + 10
+        leaves = np.ascontiguousarray(leaves, np.int32)
+        gh = np.zeros((8 * len(leaves) + 1, 9), np.int32)
+        wl = np.zeros((4 * len(leaves) + 1, 3), np.int32)
+        ng, nw = ctypes.c_int(), ctypes.c_int()
+        rc = lib.pc_amr_face_plan(px, py, 4, len(level), level.ctypes.data_as(vp),
+                                  ij.ctypes.data_as(vp), handle.ctypes.data_as(vp),
+                                  leaves.ctypes.data_as(vp), len(leaves),
+                                  gh.ctypes.data_as(vp) if outputs else None,
+                                  len(gh) if cap_g is None else cap_g, ctypes.byref(ng),
+                                  wl.ctypes.data_as(vp) if outputs else None,
+                                  len(wl) if cap_w is None else cap_w, ctypes.byref(nw))
+        return rc, gh[:ng.value], wl[:nw.value], ng.value, nw.value
+
+    # two level-1 patches side by side: each has one face neighbour and
+    # three walls; no corner lies inside the domain
+    rc, gh, wl, ng, nw = plan([1, 1], [[0, 0], [1, 0]], [0, 1], outputs=False)
+    assert rc == 0 and (ng, nw) == (2, 6)
+    rc, gh, wl, ng, nw = plan([1, 1], [[0, 0], [1, 0]], [0, 1])
+    assert rc == 0
+    # x+ face of patch 0 from patch 1 (same level: kind 0), x- face of patch 1
+    assert gh[0].tolist() == [0, 11, 10, 1, 4, 0, 0, 4, 0]
+    assert gh[1].tolist() == [0, 10, 11, 1, 4, 3, 0, -1, 0]
+    # walls left, top, right, bottom as (slot, dimension, upper)
+    assert wl[:3].tolist() == [[10, 0, 0], [10, 1, 1], [10, 1, 0]]
+    assert wl[3:].tolist() == [[11, 1, 1], [11, 0, 1], [11, 1, 0]]
+    assert plan([1, 1], [[0, 0], [1 << 25, 0]], [0])[0] == _abi.PC_E_INVALID
+    assert plan([1, 13], [[0, 0], [1, 0]], [0])[0] == _abi.PC_E_INVALID
+    assert plan([1], [[0, 0]], [0])[0] == _abi.PC_E_INVALID      # x+ face uncovered
+    assert plan([1, 1], [[0, 0], [1, 0]], [0, 1], cap_g=1)[0] == _abi.PC_E_INVALID
+    assert plan([1, 1], [[0, 0], [1, 0]], [0, 1], cap_w=5)[0] == _abi.PC_E_INVALID
