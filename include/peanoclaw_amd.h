@@ -494,9 +494,9 @@ typedef struct {
 } pc_amr_wall;
 /* The face, corner and wall entries of every leaf of a patch tree with
  * refinement factor 3, as pc_amr_create takes them (host only, no pool):
- * patch k has level[k] in 1..12 and index (ij[2k], ij[2k+1]), each below
- * 2^25, on its level (level 1: a px x py grid of n x n cells); leaves[]
- * lists the leaves in
+ * patch k has level[k] in 1..12 and index (ij[2k], ij[2k+1]) inside its
+ * level's grid of px 3^(l-1) x py 3^(l-1) patches, each index below 2^25
+ * (level 1: a px x py grid of n x n cells); leaves[] lists the leaves in
  * plan order (indices into the patch arrays); entries name patches by
  * handle[k].  Per leaf: faces x-, x+, y-, y+ (outside the domain: a wall),
  * corners (-,-), (-,+), (+,-), (+,+) (skipped outside the domain), each a

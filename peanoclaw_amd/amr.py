@@ -53,6 +53,7 @@ class AmrPatch:
         self.ij = ij  # index on its level
         self.children = []
         self.age = 0  # global steps since creation
+        self.k = -1  # index in the packed patch arrays (packed_plan_native)
 
 
 class AmrGrid:
@@ -129,26 +130,36 @@ class AmrGrid:
     def _rebuild(self, changed=None):
         """Children, leaves, virtual patches, restriction and ghost plans
         from the current tree."""
-        for lev in sorted(self.levels):
-            for ij, pt in self.levels[lev].items():
-                pt.children = ([self.levels[lev + 1][(3 * ij[0] + a, 3 * ij[1] + c)]
-                                for a in range(3) for c in range(3)] if pt.virtual else [])
-        self.virtuals = [pt for lev in sorted(self.levels)
-                         for pt in self.levels[lev].values() if pt.virtual]
-        self.leaves = [pt for lev in sorted(self.levels)
-                       for pt in self.levels[lev].values() if not pt.virtual]
-        # (leaf, virtual ancestors nearest first) in depth-first tree order
-        self._restrictions = []
+        levels = self.levels
+        order = sorted(levels)
+        for lev in order:
+            finer = levels.get(lev + 1)
+            for (i, j), pt in levels[lev].items():
+                if pt.virtual:
+                    i3, j3 = 3 * i, 3 * j
+                    pt.children = [finer[(i3 + a, j3 + c)] for a in (0, 1, 2) for c in (0, 1, 2)]
+                elif pt.children:
+                    pt.children = []
+        every = [pt for lev in order for pt in levels[lev].values()]
+        self.virtuals = [pt for pt in every if pt.virtual]
+        self.leaves = [pt for pt in every if not pt.virtual]
+        # (leaf, virtual ancestors nearest first) in depth-first tree order;
+        # the leaves below one virtual patch share its (read-only) list
+        restrictions = self._restrictions = []
 
         def walk(pt, ancestors):
-            if pt.virtual:
-                for c in pt.children:
-                    walk(c, [pt] + ancestors)
-            elif ancestors:
-                self._restrictions.append((pt, ancestors))
+            nearest = [pt] + ancestors
+            for c in pt.children:
+                if c.virtual:
+                    walk(c, nearest)
+                else:
+                    restrictions.append((c, nearest))
+        top = levels[1]
         for i in range(self.px):
             for j in range(self.py):
-                walk(self.levels[1][(i, j)], [])
+                pt = top[(i, j)]
+                if pt.virtual:
+                    walk(pt, [])
         # the face plan is derived when first used (the batched driver takes
         # the library's packed rows instead): the changes since the last
         # plan accumulate for its incremental update (None: everything)
@@ -354,13 +365,15 @@ class AmrGrid:
         plan.  Patch handles must be ints."""
         lib = load()
         pts = [pt for lev in sorted(self.levels) for pt in self.levels[lev].values()]
-        index = {id(pt): k for k, pt in enumerate(pts)}
-        level = np.ascontiguousarray([pt.level for pt in pts], dtype=np.int32)
-        ij = np.ascontiguousarray([pt.ij for pt in pts], dtype=np.int32).reshape(-1)
-        handle = np.ascontiguousarray([pt.handle for pt in pts], dtype=np.int32)
-        leaves = np.ascontiguousarray([index[id(p)] for p in self.leaves], dtype=np.int32)
-        gh = np.zeros((8 * len(leaves) + 1, 9), np.int32)
-        wl = np.zeros((4 * len(leaves) + 1, 3), np.int32)
+        m, nl = len(pts), len(self.leaves)
+        for k, pt in enumerate(pts):
+            pt.k = k
+        level = np.fromiter((pt.level for pt in pts), np.int32, m)
+        ij = np.fromiter((x for pt in pts for x in pt.ij), np.int32, 2 * m)
+        handle = np.fromiter((pt.handle for pt in pts), np.int32, m)
+        leaves = np.fromiter((p.k for p in self.leaves), np.int32, nl)
+        gh = np.empty((8 * nl + 1, 9), np.int32)
+        wl = np.empty((4 * nl + 1, 3), np.int32)
         ng, nw = ctypes.c_int(), ctypes.c_int()
         vp = ctypes.c_void_p
         check(lib.pc_amr_face_plan(self.px, self.py, self.n, len(pts),

@@ -1471,31 +1471,54 @@ int pc_amr_face_plan(int px, int py, int n, int npatch, const int32_t* level,
   if (px < 1 || py < 1 || n < 1 || npatch < 0 || nleaves < 0 || !nghosts || !nwalls ||
       (npatch && (!level || !ij || !handle)) || (nleaves && !leaves))
     return fail(PC_E_INVALID, "invalid argument");
+  // index grid of every level: px 3^(l-1) x py 3^(l-1) patches
+  constexpr int kMaxLevel = 12;
+  long long gw[kMaxLevel + 1][2];
+  for (int l = 1; l <= kMaxLevel; ++l) {
+    gw[l][0] = l == 1 ? px : 3 * gw[l - 1][0];
+    gw[l][1] = l == 1 ? py : 3 * gw[l - 1][1];
+  }
+  // patch lookup per level: a dense table where the level's grid is small
+  // against the patch count, else a hash map (25 bits per index in its key)
   auto key = [](long long l, long long i, long long j) { return (l << 50) | (i << 25) | j; };
   std::unordered_map<long long, int> at;
-  at.reserve((size_t)npatch * 2);
+  std::vector<int> dense[kMaxLevel + 1];
+  int nat = 0;
   for (int k = 0; k < npatch; ++k) {
-    // the hash key holds 25 bits per index
-    if (level[k] < 1 || level[k] > 12 || ij[2 * k] < 0 || ij[2 * k + 1] < 0 ||
-        ij[2 * k] >= (1 << 25) || ij[2 * k + 1] >= (1 << 25))
+    const int l = level[k];
+    if (l < 1 || l > kMaxLevel || ij[2 * k] < 0 || ij[2 * k + 1] < 0 ||
+        ij[2 * k] >= gw[l][0] || ij[2 * k + 1] >= gw[l][1] || ij[2 * k] >= (1 << 25) ||
+        ij[2 * k + 1] >= (1 << 25))
       return fail(PC_E_INVALID, "invalid patch level or index");
-    at[key(level[k], ij[2 * k], ij[2 * k + 1])] = k;
   }
-  auto width = [&](int lev, long long& wx, long long& wy) {
-    long long f = 1;
-    for (int l = 1; l < lev; ++l) f *= 3;
-    wx = px * f;
-    wy = py * f;
+  const long long dense_cap = 16LL * npatch + 65536;
+  for (int k = 0; k < npatch; ++k) {
+    const int l = level[k];
+    if (dense[l].empty() && gw[l][0] <= dense_cap / gw[l][1])
+      dense[l].assign(gw[l][0] * gw[l][1], -1);
+    if (!dense[l].empty())
+      dense[l][ij[2 * k] * gw[l][1] + ij[2 * k + 1]] = k;
+    else
+      ++nat;
+  }
+  if (nat) {
+    at.reserve((size_t)nat * 2);
+    for (int k = 0; k < npatch; ++k)
+      if (dense[level[k]].empty()) at[key(level[k], ij[2 * k], ij[2 * k + 1])] = k;
+  }
+  auto find = [&](int lev, long long i, long long j) {
+    if (!dense[lev].empty()) return dense[lev][i * gw[lev][1] + j];
+    if (!nat) return -1;
+    auto it = at.find(key(lev, i, j));
+    return it != at.end() ? it->second : -1;
   };
   // the patch at (lev, i, j), else the nearest coarser one covering it;
   // -1 outside the domain or where nothing covers it
   auto covering = [&](int lev, long long i, long long j) {
-    long long wx, wy;
-    width(lev, wx, wy);
-    if (i < 0 || j < 0 || i >= wx || j >= wy) return -1;
+    if (i < 0 || j < 0 || i >= gw[lev][0] || j >= gw[lev][1]) return -1;
     for (; lev >= 1; --lev, i /= 3, j /= 3) {
-      auto it = at.find(key(lev, i, j));
-      if (it != at.end()) return it->second;
+      const int k = find(lev, i, j);
+      if (k >= 0) return k;
     }
     return -1;
   };
@@ -1510,8 +1533,7 @@ int pc_amr_face_plan(int px, int py, int n, int npatch, const int32_t* level,
     if (p < 0 || p >= npatch) return fail(PC_E_INVALID, "invalid leaf index");
     const int lev = level[p];
     const long long i = ij[2 * p], j = ij[2 * p + 1];
-    long long wx, wy;
-    width(lev, wx, wy);
+    const long long wx = gw[lev][0], wy = gw[lev][1];
     bool wall[4];
     auto emit = [&](int src, int sx, int sy, int sox, int soy, int dox, int doy) {
       if (ghosts && ng < cap_ghosts) {

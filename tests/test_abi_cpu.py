@@ -73,8 +73,9 @@ def test_tile_stride_rule(libpath):
 
 def test_face_plan_counts_and_rejects_bad_input(libpath):
     """pc_amr_face_plan on a 2x1 level-1 grid (host only): counts with NULL
-    outputs, the rows themselves, and PC_E_INVALID for an index beyond the
-    hash key's 25 bits, an uncovered face and too small an output."""
+    outputs, the rows themselves, and PC_E_INVALID for an index outside its
+    level's grid or beyond the hash key's 25 bits, an uncovered face and too
+    small an output."""
     import numpy as np
     from peanoclaw_amd import _abi
     lib = _abi.load()
@@ -111,6 +112,7 @@ def test_face_plan_counts_and_rejects_bad_input(libpath):
     assert wl[3:].tolist() == [[11, 1, 1], [11, 0, 1], [11, 1, 0]]
     assert plan([1, 1], [[0, 0], [1 << 25, 0]], [0])[0] == _abi.PC_E_INVALID
     assert plan([1, 13], [[0, 0], [1, 0]], [0])[0] == _abi.PC_E_INVALID
+    assert plan([1, 1], [[0, 0], [2, 0]], [0])[0] == _abi.PC_E_INVALID   # outside the grid
     assert plan([1], [[0, 0]], [0])[0] == _abi.PC_E_INVALID      # x+ face uncovered
     assert plan([1, 1], [[0, 0], [1, 0]], [0, 1], cap_g=1)[0] == _abi.PC_E_INVALID
     assert plan([1, 1], [[0, 0], [1, 0]], [0, 1], cap_w=5)[0] == _abi.PC_E_INVALID
