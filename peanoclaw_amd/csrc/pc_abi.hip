@@ -2054,7 +2054,7 @@ static int refine_flags_slots(pc_pool* p, const std::vector<int>& slots, double 
   int* d_sl = p->d_flags + n;
   HIPCHK(hipMemcpyAsync(d_sl, slots.data(), n * sizeof(int), hipMemcpyHostToDevice,
                         p->stream));
-  hipLaunchKernelGGL(k_refine_flags, dim3(n), dim3(64), 0, p->stream, d_sl, n, p->d_pd,
+  hipLaunchKernelGGL(k_refine_flags, dim3(n), dim3(kRefineThreads), 0, p->stream, d_sl, n, p->d_pd,
                      fields_of(p), p->d_scn, thr, minW, maxW, p->d_flags, p->d_aux,
                      static_cast<double*>(nullptr), 0ull);
   HIPCHK(hipGetLastError());
@@ -2077,7 +2077,7 @@ int pc_demanded_mesh_width(pc_pool* p, int slot, double thr, double minW, double
   // into pinned host memory, the completion word after it
   const unsigned long long seq = ++p->call_seq;
   int* const flag_host = reinterpret_cast<int*>(p->d_result_host + 6);
-  hipLaunchKernelGGL(k_refine_flags, dim3(1), dim3(64), 0, p->stream, p->d_iota + slot, 1,
+  hipLaunchKernelGGL(k_refine_flags, dim3(1), dim3(kRefineThreads), 0, p->stream, p->d_iota + slot, 1,
                      p->d_pd, fields_of(p), p->d_scn, thr, minW, maxW, flag_host,
                      p->d_result_host, p->d_result_host, seq);
   HIPCHK(hipGetLastError());

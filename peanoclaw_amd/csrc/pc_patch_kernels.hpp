@@ -463,8 +463,10 @@ __global__ void k_scatter_b(const PatchDev* __restrict__ pd, Fields F,
 
 // BreakingDamSWEScenario::computeDemandedMeshWidth (BreakingDam.cpp:85-139)
 // and SolveTimestep::leaveCell's decision (SolveTimestep.cpp:765-786):
-// one wavefront per patch over uNew.
-__global__ void __launch_bounds__(64)
+// one workgroup of kRefineThreads per patch over uNew (a maximum, so the
+// order of the reduction does not change the result).
+constexpr int kRefineThreads = 256;
+__global__ void __launch_bounds__(kRefineThreads)
 k_refine_flags(const int* __restrict__ slots, int nslots, const PatchDev* __restrict__ pd,
                Fields F, const ScnDev* __restrict__ scn, double threshold,
                double minW, double maxW, int* __restrict__ flags,
@@ -476,9 +478,10 @@ k_refine_flags(const int* __restrict__ slots, int nslots, const PatchDev* __rest
   const PatchDev P = pd[slot];
   const int cur = scn[P.scn].stepc & 1;
   const double* H = F.H(cur);
+  __shared__ double wave_max[kRefineThreads / 64];
   double g = 0.0;
   const int nxy = (P.nx - 1) * (P.ny - 1);
-  for (int e = threadIdx.x; e < nxy; e += 64) {
+  for (int e = threadIdx.x; e < nxy; e += kRefineThreads) {
     const int xi = e / (P.ny - 1), yi = e - (e / (P.ny - 1)) * (P.ny - 1);
     const double q0 = H[tile_index(P, xi, yi)];
     const double q0x = (H[tile_index(P, xi + 1, yi)] - q0) / P.dx;
@@ -486,7 +489,10 @@ k_refine_flags(const int* __restrict__ slots, int nslots, const PatchDev* __rest
     g = fmax(g, sqrt(fabs(q0x) * fabs(q0x) + fabs(q0y) * fabs(q0y)));
   }
   for (int m = 32; m >= 1; m >>= 1) g = fmax(g, __shfl_xor(g, m));
+  if ((threadIdx.x & 63) == 0) wave_max[threadIdx.x >> 6] = g;
+  __syncthreads();
   if (threadIdx.x == 0) {
+    for (int w = 1; w < kRefineThreads / 64; ++w) g = fmax(g, wave_max[w]);
     double dw0, dw1;
     if (fabs(minW - maxW) <= kLaEps) {
       dw0 = dw1 = minW;
