@@ -84,3 +84,16 @@ def test_tree_lists_after_regrids(seed):
         refined += r
         coarsened += c
     assert refined > 0 and coarsened > 0
+
+
+def test_deep_corner_tree_face_plan_with_dense_and_hashed_levels():
+    """A tree refined at the domain corner down to level 12: the library's
+    face plan looks the coarse levels up in dense tables and the fine ones
+    (grids of up to 3^11 x 3^11 patches) in its hash map; its rows equal the
+    Python driver's."""
+    grid = AmrGrid(_TreeOnly(0), 1, 1, 2, lambda lev, ij: ij == (0, 0), _flat_field,
+                   max_level=12)
+    assert max(grid.levels) == 12 and len(grid.leaves) == 1 + 8 * 11
+    gh_n, wl_n = grid.packed_plan_native()
+    gh_p, wl_p = grid.packed_plan()
+    assert len(gh_n) > 0 and np.array_equal(gh_n, gh_p) and np.array_equal(wl_n, wl_p)
